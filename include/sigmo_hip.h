@@ -1,0 +1,213 @@
+/*
+ * sigmo_hip.h -- C ABI of the MI355X (gfx950) implementation of SIGMo's batched
+ * subgraph-isomorphism hot path: signatures -> candidate filter/refine -> GMCR -> join.
+ *
+ * Every entry point returns an int status (SIGMO_OK == 0, negative on failure) and
+ * records a message retrievable with sigmo_last_error().  All device buffers belong to
+ * opaque handles created here; host buffers are caller-owned.  A context binds one
+ * device and one HIP stream; it is externally synchronised (one host thread per
+ * context).  Calls enqueue on the context stream and return; calls that deliver data
+ * to host memory synchronise the stream first.
+ *
+ * Reference interfaces replaced (paths relative to the SIGMo repository):
+ *   createDeviceCSRGraph / destroyDeviceCSRGraph   library/include/graph.hpp:255-322
+ *   getDeviceGraphAllocSize                          library/include/graph.hpp:324-332,423-432
+ *   signature::Signature<PowerGraph,4>               library/include/signature.hpp:20-212,336-390
+ *   candidates::Candidates                           library/include/candidates.hpp:14-204
+ *   isomorphism::filter::filterCandidates            library/include/isomorphism.hpp:22-62
+ *   isomorphism::filter::refineCandidates            library/include/isomorphism.hpp:64-127
+ *   isomorphism::mapping::GMCR::generateGMCR         library/include/gmcr.hpp:34-128
+ *   isomorphism::join::joinCandidates                library/include/isomorphism.hpp:270-381
+ *   io::loadCSRGraphsFromFile / FromLines            library/include/io.hpp:37-55, graph.hpp:151-247
+ *   utils::BatchedEvent::getProfilingInfo            library/include/utils.hpp:127-161
+ */
+#ifndef SIGMO_HIP_H
+#define SIGMO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------ */
+#define SIGMO_OK 0
+#define SIGMO_ERR_INVALID (-1)     /* bad argument / malformed input                    */
+#define SIGMO_ERR_NOMEM (-2)       /* device or host allocation failed                  */
+#define SIGMO_ERR_HIP (-3)         /* HIP runtime error                                 */
+#define SIGMO_ERR_UNSUPPORTED (-4) /* shape outside the supported envelope              */
+#define SIGMO_ERR_OVERFLOW (-5)    /* a counter exceeded its documented width           */
+#define SIGMO_ERR_IO (-6)          /* file read/write failure                           */
+
+/* Limits of this implementation (the reference assumes k <= 29 and n <= 64 silently). */
+#define SIGMO_MAX_QUERY_NODES 32     /* nodes per query graph  (reference: stack[30])    */
+#define SIGMO_MAX_DATA_GRAPH_NODES 128 /* nodes per data graph (reference: 64, ZINC filter) */
+#define SIGMO_WILDCARD_LABEL 255     /* types.hpp:22 WILDCARD_NODE                        */
+
+/* Signature scope, signature.hpp:18 SignatureScope{Data, Query}. */
+#define SIGMO_SCOPE_DATA 0
+#define SIGMO_SCOPE_QUERY 1
+
+/* Phase ids for sigmo_phase_ms (per-phase accumulated kernel time, like BatchedEvent). */
+#define SIGMO_PHASE_SIG_DATA 0
+#define SIGMO_PHASE_SIG_QUERY 1
+#define SIGMO_PHASE_FILTER 2 /* filter + every refine iteration (sigmo.cpp filter_times) */
+#define SIGMO_PHASE_GMCR 3
+#define SIGMO_PHASE_JOIN 4
+#define SIGMO_PHASE_COUNT 5
+
+typedef struct sigmo_ctx sigmo_ctx;
+typedef struct sigmo_host_graphs sigmo_host_graphs;
+typedef struct sigmo_graph sigmo_graph;
+typedef struct sigmo_signatures sigmo_signatures;
+typedef struct sigmo_candidates sigmo_candidates;
+typedef struct sigmo_gmcr sigmo_gmcr;
+
+/* ---- errors ------------------------------------------------------------------------ */
+/* Message of the last failing call on this host thread ("" if none). */
+const char* sigmo_last_error(void);
+
+/* ---- context (device + stream); replaces the sycl::queue every reference class holds -- */
+int sigmo_ctx_create(int device, sigmo_ctx** out);
+int sigmo_ctx_destroy(sigmo_ctx* ctx);
+int sigmo_ctx_sync(sigmo_ctx* ctx);
+/* Accumulated kernel milliseconds of one phase since the last reset (hipEvent timing,
+ * the analogue of utils.hpp:149-157 BatchedEvent::getProfilingInfo). */
+int sigmo_phase_ms(sigmo_ctx* ctx, int phase, double* ms);
+int sigmo_phase_reset(sigmo_ctx* ctx);
+/* Device name and global memory size (sigmo.cpp:20-21). */
+int sigmo_device_info(sigmo_ctx* ctx, char* name, size_t name_len, uint64_t* global_mem);
+
+/* ---- host-side batched CSR (the arrays createDeviceCSRGraph concatenates) ------------ */
+/* A host batch holds graph_offsets[G+1], row_offsets[N+1] (global), column_indices[E]
+ * (global node ids), node_labels[N], edge_labels[E] exactly as graph.hpp:264-294 builds. */
+int sigmo_host_graphs_from_arrays(uint32_t num_graphs, const uint32_t* graph_offsets, const uint32_t* row_offsets,
+                                  const uint32_t* column_indices, const uint8_t* node_labels, const uint8_t* edge_labels,
+                                  sigmo_host_graphs** out);
+/* .dat loader, one graph per line "n#N l#L (id label)*N e#E (u v [label])*E"
+ * (io.hpp:46-55 loadCSRGraphsFromFile via graph.hpp:154-247). */
+int sigmo_host_graphs_load_dat(const char* path, sigmo_host_graphs** out);
+/* Same parser over an in-memory buffer of '\n'-separated lines (io.hpp:37-44). */
+int sigmo_host_graphs_parse_dat(const char* text, size_t len, sigmo_host_graphs** out);
+int sigmo_host_graphs_save_dat(const sigmo_host_graphs* g, const char* path);
+int sigmo_host_graphs_info(const sigmo_host_graphs* g, uint32_t* num_graphs, uint64_t* total_nodes, uint64_t* total_edges);
+int sigmo_host_graphs_arrays(const sigmo_host_graphs* g, const uint32_t** graph_offsets, const uint32_t** row_offsets,
+                             const uint32_t** column_indices, const uint8_t** node_labels, const uint8_t** edge_labels);
+/* Graphs [first, first+count) as a new batch (ids rebased to 0). */
+int sigmo_host_graphs_slice(const sigmo_host_graphs* g, uint32_t first, uint32_t count, sigmo_host_graphs** out);
+/* Concatenate b after a. */
+int sigmo_host_graphs_concat(const sigmo_host_graphs* a, const sigmo_host_graphs* b, sigmo_host_graphs** out);
+int sigmo_host_graphs_free(sigmo_host_graphs* g);
+
+/* ---- deterministic synthetic workload generator (no reference counterpart) ---------- */
+/* size_dist: 0 = clamp(round(Normal(24,8)),6,64); 1 = clamp(round(LogNormal(ln 24,0.45)),6,128).
+ * Molecules [first, first+count) of the sequence seeded by `seed`: any slice equals the
+ * same slice of a longer run (per-molecule PRNG streams). */
+#define SIGMO_SIZE_NORMAL 0
+#define SIGMO_SIZE_LOGNORMAL 1
+int sigmo_gen_molecules(uint64_t seed, uint64_t first, uint64_t count, int size_dist, int threads,
+                        sigmo_host_graphs** out);
+/* BFS-grown induced query graphs with k ~ U[kmin,kmax].  perturb_frac: exact fraction of
+ * queries with one relabelled node; ring_frac < 0: no ring constraint, else the exact
+ * fraction of queries containing a cycle (the rest acyclic). */
+int sigmo_gen_queries(uint64_t seed, uint32_t count, uint32_t kmin, uint32_t kmax, int size_dist, double perturb_frac,
+                      double ring_frac, sigmo_host_graphs** out);
+/* Poisson arrivals (exponential gaps at `qps`) and per-request query counts U[qmin,qmax]. */
+int sigmo_gen_arrivals(uint64_t seed, uint32_t count, double qps, uint32_t qmin, uint32_t qmax, double* arrival_s,
+                       uint32_t* queries_per_request);
+
+/* ---- device batched CSR graph (createDeviceCSRGraph, graph.hpp:255-314) --------------- */
+int sigmo_graph_upload(sigmo_ctx* ctx, const sigmo_host_graphs* host, sigmo_graph** out);
+int sigmo_graph_free(sigmo_graph* g); /* destroyDeviceCSRGraph, graph.hpp:316-322 */
+int sigmo_graph_info(const sigmo_graph* g, uint32_t* num_graphs, uint64_t* total_nodes, uint64_t* total_edges);
+/* Bytes of the reference CSR arrays (graph.hpp:324-328) and of the whole device image. */
+int sigmo_graph_alloc_size(const sigmo_graph* g, uint64_t* csr_bytes, uint64_t* device_bytes);
+
+/* ---- signatures (signature::Signature<PowerGraph,4>, signature.hpp:157-212) ---------- */
+int sigmo_signatures_create(sigmo_ctx* ctx, uint64_t data_nodes, uint64_t query_nodes, sigmo_signatures** out);
+int sigmo_signatures_free(sigmo_signatures* s);
+/* generateCSRSignatures (signature.hpp:125-152). */
+int sigmo_signatures_generate(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope);
+/* refineCSRSignatures<PowerGraph>(view_size) (signature.hpp:336-390). */
+int sigmo_signatures_refine(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope, uint32_t view_size);
+/* Copy the packed 16x4-bit signatures (SignatureDevice::signature) of a scope. */
+int sigmo_signatures_download(sigmo_ctx* ctx, const sigmo_signatures* s, int scope, uint64_t* out, uint64_t count);
+/* getDataSignatureAllocationSize / getQuerySignatureAllocationSize (signature.hpp:181-198). */
+int sigmo_signatures_alloc_size(const sigmo_signatures* s, int scope, uint64_t* bytes);
+
+/* ---- candidates (candidates::Candidates, candidates.hpp:165-204) ---------------------- */
+/* Zero-initialised source_nodes x target_nodes bitset (candidates.hpp:165-179). */
+int sigmo_candidates_create(sigmo_ctx* ctx, uint64_t source_nodes, uint64_t target_nodes, sigmo_candidates** out);
+int sigmo_candidates_free(sigmo_candidates* c);
+/* filterCandidates<Query> (isomorphism.hpp:22-62). */
+int sigmo_candidates_filter(sigmo_ctx* ctx, sigmo_candidates* c, const sigmo_graph* query, const sigmo_graph* data,
+                            const sigmo_signatures* s);
+/* refineCandidates<Query> (isomorphism.hpp:64-127) with the current-level signatures. */
+int sigmo_candidates_refine(sigmo_ctx* ctx, sigmo_candidates* c, const sigmo_graph* query, const sigmo_graph* data,
+                            const sigmo_signatures* s);
+/* Download in the reference layout: uint32 words, row stride ceil(target/32)
+ * (getHostCandidates, candidates.hpp:185-190).  `words` holds rows*stride words. */
+int sigmo_candidates_download(sigmo_ctx* ctx, const sigmo_candidates* c, uint32_t* words, uint64_t n_words);
+/* getCandidatesCount(source) for every source row (candidates.hpp:77-81). */
+int sigmo_candidates_counts(sigmo_ctx* ctx, const sigmo_candidates* c, uint32_t* counts, uint64_t n);
+/* getAllocationSize (candidates.hpp:196) in the reference's bytes, and device bytes. */
+int sigmo_candidates_alloc_size(const sigmo_candidates* c, uint64_t* ref_bytes, uint64_t* device_bytes);
+
+/* ---- GMCR (isomorphism::mapping::GMCR, gmcr.hpp:18-135) ------------------------------- */
+int sigmo_gmcr_create(sigmo_ctx* ctx, sigmo_gmcr** out);
+int sigmo_gmcr_free(sigmo_gmcr* m);
+/* generateGMCR (gmcr.hpp:34-128).  Query graphs with fewer than 2 nodes are rejected. */
+int sigmo_gmcr_generate(sigmo_ctx* ctx, sigmo_gmcr* m, const sigmo_graph* query, const sigmo_graph* data,
+                        const sigmo_candidates* c);
+int sigmo_gmcr_total(const sigmo_gmcr* m, uint64_t* total_query_indices);
+/* data_graph_offsets[Gd+1] and query_graph_indices[P], ascending query id per segment. */
+int sigmo_gmcr_download(sigmo_ctx* ctx, const sigmo_gmcr* m, uint32_t* data_graph_offsets, uint64_t n_offsets,
+                        uint32_t* query_graph_indices, uint64_t n_indices);
+
+/* ---- join (isomorphism::join::joinCandidates, isomorphism.hpp:270-381) ---------------- */
+/* Adds the match count to *num_matches (the reference accumulates, sigmo.cpp:145-146).
+ * per_pair_counts (nullable, host, GMCR order, length total_query_indices) receives each
+ * pair's count: 0/1 in find-first mode, leaf hits in find-all mode. */
+int sigmo_join(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data, const sigmo_candidates* c,
+               const sigmo_gmcr* m, int find_first, uint64_t* num_matches, int32_t* per_pair_counts);
+
+/* ---- whole plain pipeline (sigmo.cpp:102-159 phase order) ---------------------------- */
+typedef struct sigmo_pipeline_result {
+  uint64_t num_matches;    /* "# Matches" (sigmo.cpp:215)                               */
+  uint64_t gmcr_pairs;     /* total_query_indices                                         */
+  uint64_t matched_pairs;  /* pairs with count > 0 (== num_matches in find-first mode)    */
+  double phase_ms[SIGMO_PHASE_COUNT];
+  double total_ms;         /* device time from first signature kernel to join completion */
+} sigmo_pipeline_result;
+/* Runs gen(D), gen(Q), filter, R x {refsig(D,r), refsig(Q,r), refine}, GMCR, join on
+ * device-resident graphs, with one launch per phase; allocates and frees its buffers. */
+int sigmo_pipeline_run(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data, uint32_t refinement_steps,
+                       int find_first, sigmo_pipeline_result* result);
+
+/* The same plain pipeline with its signature/candidate/GMCR buffers allocated once
+ * (the reference's setup phase, sigmo.cpp:75-91) and reused by every execute. */
+typedef struct sigmo_pipeline sigmo_pipeline;
+int sigmo_pipeline_create(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data, sigmo_pipeline** out);
+int sigmo_pipeline_execute(sigmo_pipeline* p, uint32_t refinement_steps, int find_first, sigmo_pipeline_result* result);
+/* Per-pair counts of the last execute (GMCR order) and the GMCR arrays. */
+int sigmo_pipeline_pairs(sigmo_pipeline* p, uint32_t* data_graph_offsets, uint64_t n_offsets, uint32_t* query_graph_indices,
+                         int32_t* counts, uint64_t n_pairs);
+int sigmo_pipeline_free(sigmo_pipeline* p);
+
+/* ---- per-kernel timing (hipEvents on the context stream) ----------------------------- */
+#define SIGMO_KERNEL_SIG_GENERATE 0
+#define SIGMO_KERNEL_SIG_REFINE 1
+#define SIGMO_KERNEL_FILTER 2
+#define SIGMO_KERNEL_REFINE 3
+#define SIGMO_KERNEL_GMCR 4
+#define SIGMO_KERNEL_JOIN 5
+#define SIGMO_KERNEL_COUNT 6
+/* Launch count, summed milliseconds and summed algorithmic HBM bytes of one kernel family
+ * since the last sigmo_phase_reset. */
+int sigmo_kernel_stats(sigmo_ctx* ctx, int kernel, uint64_t* launches, double* ms, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SIGMO_HIP_H */

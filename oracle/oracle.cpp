@@ -1,0 +1,426 @@
+// oracle.cpp -- CPU restatement of SIGMo's hot path.  TEST INFRASTRUCTURE ONLY.
+//
+// This file is the parity checker for the gfx950 implementation.  Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg load it, and only to check or
+// time; no product code path calls it.
+//
+// It restates the reference SYCL kernels literally, in the reference's own data layout
+// (uint32 candidate words, row stride ceil(D/32)), one loop per reference loop, with the
+// conventions that make the reference's undefined behaviour definite:
+//   * every variable shift count is masked to width-1 (uint64 & 63, uint32 & 31);
+//   * Bitset<uint64_t> indices pass through uint8_t and subtract the offset in uint32;
+//   * PowerGraph reachables start at zero;
+//   * refineCandidates is applied per row (no cross-row race);
+//   * GMCR segments list query graphs in ascending id; query graphs with < 2 nodes are
+//     rejected;
+//   * the join starts every (query, molecule) pair from fresh state with every mapping[]
+//     entry equal to the molecule's first node id (local id 0).
+// Reference files are cited per function (paths relative to the SIGMo repository).
+//
+// Planted-mistake switches (`mutation` arguments) exist so tests can show that the
+// pinning checks reject plausible errors; 0 is the faithful restatement.
+#include <omp.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+using u8 = uint8_t;
+using u16 = uint16_t;
+using u32 = uint32_t;
+using u64 = uint64_t;
+
+constexpr u8 kWildcard = 255;  // types.hpp:22
+constexpr int kMaxQueryNodes = 33;  // reference: 30 (isomorphism.hpp:288); k+1 frames needed
+
+inline int ctz64(u64 x) { return x ? __builtin_ctzll(x) : 64; }
+inline int ctz32(u32 x) { return x ? __builtin_ctz(x) : 32; }
+
+// ---- SignatureDevice, signature.hpp:23-54 (Bits = 4, 16 labels) -------------------------
+inline u8 sig_get(u64 sig, u8 label) {
+  if (label < 16) return (sig >> (label * 4)) & 15;  // :39-42
+  return 0;
+}
+inline void sig_set(u64& sig, u8 label, u8 count) {  // :32-37
+  if (label < 16 && count < 16) {
+    sig &= ~(u64(15) << (label * 4));
+    sig |= u64(count) << (label * 4);
+  }
+}
+inline void sig_inc(u64& sig, u8 label, u8 add = 1) {  // :44-51
+  if (label < 16) {
+    u8 count = sig_get(sig, label);
+    if (count < 15) sig_set(sig, label, static_cast<u8>(count + add));
+  }
+}
+
+// ---- Bitset<uint64_t>, utils.hpp:64-96 -------------------------------------------------
+struct Bitset64 {
+  u64 data = 0;
+  u32 offset = 0;
+  Bitset64() = default;
+  explicit Bitset64(u32 off) : offset(off) {}
+  static u32 amount(u8 idx, u32 off) { return (static_cast<u32>(idx) - off) & 63u; }  // A1/A2
+  void set(u8 idx) { data |= u64(1) << amount(idx, offset); }
+  void unset(u8 idx) { data &= ~(u64(1) << amount(idx, offset)); }
+  bool get(u8 idx) const { return (data & (u64(1) << amount(idx, offset))) != 0; }
+  void clear() { data = 0; }
+  void merge(const Bitset64& o) { data |= o.data; }
+  void difference(const Bitset64& o) { data &= ~o.data; }
+  bool empty() const { return data == 0; }
+  u16 size() const { return static_cast<u16>(__builtin_popcountll(data)); }
+  u16 getSetBit(u8 idx) const {  // :85-95
+    idx = static_cast<u8>(idx - offset);
+    u16 ret = 0xFFFF;
+    u64 tmp = data;
+    for (int count = 0; count <= idx; ++count) {
+      ret = static_cast<u16>(ctz64(tmp));
+      tmp &= ~(u64(1) << (ret & 63));
+    }
+    return ret;
+  }
+};
+
+// ---- DeviceBatchedCSRGraph accessors, graph.hpp:62-112 ---------------------------------
+struct Graph {
+  u32 G;
+  const u32* go;  // graph_offsets
+  const u32* ro;  // row_offsets
+  const u32* ci;  // column_indices
+  const u8* nl;   // node_labels
+  u64 N() const { return go[G]; }
+  bool isNeighbor(u32 a, u32 b) const {  // :72-77
+    for (u32 i = ro[a]; i < ro[a + 1]; ++i)
+      if (ci[i] == b) return true;
+    return false;
+  }
+  u32 nodes(u32 g) const { return go[g + 1] - go[g]; }  // :84
+  u32 prev(u32 g) const { return go[g]; }              // :86
+  u64 graphID(u32 v) const {                           // :100-111
+    u64 lo = 0, hi = G;
+    while (lo < hi) {
+      u64 mid = lo + (hi - lo) / 2;
+      if (v >= go[mid]) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo > 0 ? lo - 1 : static_cast<u64>(-1);
+  }
+};
+
+// ---- CandidatesDevice, candidates.hpp:16-162 ------------------------------------------
+struct Cands {
+  u32* w;
+  u64 stride;  // single_node_size = ceil(target/32)
+  void insert(u32 s, u32 c) const { w[s * stride + c / 32] |= u32(1) << (c % 32); }      // :32-36
+  void remove(u32 s, u32 c) const { w[s * stride + c / 32] &= ~(u32(1) << (c % 32)); }   // :60-64
+  bool contains(u32 s, u32 c) const { return (w[s * stride + c / 32] & (u32(1) << (c % 32))) != 0; }
+  u32 count(u32 s) const {  // :77-81
+    u32 n = 0;
+    for (u64 i = 0; i < stride; ++i) n += __builtin_popcount(w[s * stride + i]);
+    return n;
+  }
+  u32 mask_for(u64 i, u32 si, u32 ei, u32 gs, u32 ge) const {  // :92-99 / :138-145, A1
+    u32 mask = ~u32(0);
+    if (si == ei - 1) mask = (mask >> ((32u - (ge - gs)) & 31u)) << ((gs % 32u) & 31u);
+    else if (i == si) mask <<= (gs % 32u);
+    else if (i == ei - 1) mask >>= ((32u - (ge % 32u)) & 31u);
+    return mask;
+  }
+  u32 count(u32 s, u32 gs, u32 ge) const {  // :86-104
+    u32 n = 0;
+    const u32 si = gs / 32, ei = (ge + 31) / 32;
+    for (u64 i = si; i < ei; ++i) n += __builtin_popcount(w[s * stride + i] & mask_for(i, si, ei, gs, ge));
+    return n;
+  }
+  u32 at(u32 s, u32 idx, u32 gs, u32 ge) const {  // :132-162
+    u32 n = 0;
+    const u32 si = gs / 32, ei = (ge + 31) / 32;
+    for (u64 i = si; i < ei; ++i) {
+      u32 block = w[s * stride + i] & mask_for(i, si, ei, gs, ge);
+      u32 bc = __builtin_popcount(block);
+      if (n + bc > idx) {
+        while (block != 0) {
+          u32 t = ctz32(block);
+          if (n == idx) return static_cast<u32>(i * 32 + t);
+          block ^= u32(1) << t;
+          ++n;
+        }
+      } else {
+        n += bc;
+      }
+    }
+    return 0xFFFFFFFFu;
+  }
+};
+
+Graph make_graph(u32 G, const u32* go, const u32* ro, const u32* ci, const u8* nl) { return Graph{G, go, ro, ci, nl}; }
+
+}  // namespace
+
+extern "C" {
+
+void oracle_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+}
+int oracle_get_threads(void) { return omp_get_max_threads(); }
+
+// SignatureDevice known-answer hooks (signature.hpp:32-51).
+uint64_t oracle_sig_set(uint64_t sig, uint8_t label, uint8_t count) {
+  sig_set(sig, label, count);
+  return sig;
+}
+uint64_t oracle_sig_inc(uint64_t sig, uint8_t label, uint8_t add) {
+  sig_inc(sig, label, add);
+  return sig;
+}
+uint8_t oracle_sig_get(uint64_t sig, uint8_t label) { return sig_get(sig, label); }
+
+// CandidatesDevice single-word hooks (candidates.hpp:32-64,77-162).
+void oracle_cand_insert(uint32_t* words, uint64_t stride, uint32_t s, uint32_t c) { Cands{words, stride}.insert(s, c); }
+void oracle_cand_remove(uint32_t* words, uint64_t stride, uint32_t s, uint32_t c) { Cands{words, stride}.remove(s, c); }
+uint32_t oracle_cand_count(uint32_t* words, uint64_t stride, uint32_t s) { return Cands{words, stride}.count(s); }
+uint32_t oracle_cand_count_range(uint32_t* words, uint64_t stride, uint32_t s, uint32_t gs, uint32_t ge) {
+  return Cands{words, stride}.count(s, gs, ge);
+}
+uint32_t oracle_cand_at(uint32_t* words, uint64_t stride, uint32_t s, uint32_t idx, uint32_t gs, uint32_t ge) {
+  return Cands{words, stride}.at(s, idx, gs, ge);
+}
+
+// generateCSRSignatures, signature.hpp:125-152 (range over every node, increments onto the
+// existing values; the constructor zero-fills them, :166-167).
+void oracle_generate_signatures(uint32_t G, const uint32_t* go, const uint32_t* ro, const uint32_t* ci, const uint8_t* nl,
+                                uint64_t* sigs) {
+  const Graph g = make_graph(G, go, ro, ci, nl);
+  const int64_t N = static_cast<int64_t>(g.N());
+#pragma omp parallel for schedule(static)
+  for (int64_t v = 0; v < N; ++v) {
+    for (u32 i = ro[v]; i < ro[v + 1]; ++i) sig_inc(sigs[v], nl[ci[i]]);
+  }
+}
+
+// refineCSRSignatures<PowerGraph>, signature.hpp:336-390.  `reach` holds the per-node
+// old_reachables data words (zero before the first call).
+// mutation 1: BFS bits use the true local id without the 64-bit wrap (breaks n > 64).
+// mutation 2: the start node is not removed from the sphere.
+void oracle_refine_signatures(uint32_t G, const uint32_t* go, const uint32_t* ro, const uint32_t* ci, const uint8_t* nl,
+                              uint32_t view_size, uint64_t* sigs, uint64_t* reach, int mutation) {
+  const Graph g = make_graph(G, go, ro, ci, nl);
+  const int64_t N = static_cast<int64_t>(g.N());
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t v = 0; v < N; ++v) {
+    const u64 gid = g.graphID(static_cast<u32>(v));
+    const u32 prev = g.prev(static_cast<u32>(gid));
+    Bitset64 frontier, reachable;
+    Bitset64 old;
+    old.data = reach[v];
+    const u32 local = static_cast<u32>(v) - prev;
+    if (mutation == 1 && local >= 64) {
+      // the mistaken variant simply drops nodes it cannot represent
+      sigs[v] = 0;
+      continue;
+    }
+    frontier.set(static_cast<u8>(local));
+    reachable.set(static_cast<u8>(local));
+    for (u32 it = 0; it < view_size && !frontier.empty(); it++) {
+      Bitset64 next;
+      for (u32 idx = 0; idx < frontier.size(); idx++) {
+        const u32 u = frontier.getSetBit(static_cast<u8>(idx)) + prev;
+        for (u32 i = ro[u]; i < ro[u + 1]; ++i) {
+          const u32 nb = ci[i] - prev;
+          if (!reachable.get(static_cast<u8>(nb))) {
+            reachable.set(static_cast<u8>(nb));
+            next.set(static_cast<u8>(nb));
+          }
+        }
+      }
+      frontier = next;
+    }
+    if (mutation != 2) reachable.unset(static_cast<u8>(local));
+    reachable.difference(old);
+    u64 s = 0;
+    for (u32 idx = 0; idx < reachable.size(); idx++) {
+      const u32 u = reachable.getSetBit(static_cast<u8>(idx)) + prev;
+      sig_inc(s, nl[u]);
+    }
+    sigs[v] = s;
+    reachable.merge(old);
+    reach[v] = reachable.data;
+  }
+}
+
+// filterCandidates<Query>, isomorphism.hpp:22-62: d in C(q) iff the labels are equal or the
+// query label is the wildcard.  ORs into `words` (zeroed by the Candidates constructor,
+// candidates.hpp:165-179).  Rows are independent, so the loop runs over rows.
+void oracle_filter(uint64_t Q, const uint8_t* qlabels, uint64_t D, const uint8_t* dlabels, uint32_t* words) {
+  const u64 stride = (D + 31) / 32;
+  const Cands c{words, stride};
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t q = 0; q < static_cast<int64_t>(Q); ++q)
+    for (u64 d = 0; d < D; ++d) {
+      if (qlabels[q] != dlabels[d] && qlabels[q] != kWildcard) continue;
+      c.insert(static_cast<u32>(q), static_cast<u32>(d));
+    }
+}
+
+// refineCandidates<Query>, isomorphism.hpp:64-127: a set bit (q,d) survives iff
+// query count <= data count on all 16 labels (current-level signatures).
+// mutation 1: compare only labels 0..7.
+void oracle_refine_candidates(uint64_t Q, const uint64_t* qsig, uint64_t D, const uint64_t* dsig, uint32_t* words,
+                              int mutation) {
+  const u64 stride = (D + 31) / 32;
+  const Cands c{words, stride};
+  const int max_labels = mutation == 1 ? 8 : 16;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t q = 0; q < static_cast<int64_t>(Q); ++q) {
+    const u64 qs = qsig[q];
+    for (u64 d = 0; d < D; ++d) {
+      if (!c.contains(static_cast<u32>(q), static_cast<u32>(d))) continue;
+      const u64 ds = dsig[d];
+      bool keep = true;
+      for (int l = 0; l < max_labels && keep; l++) keep = keep && (sig_get(qs, static_cast<u8>(l)) <= sig_get(ds, static_cast<u8>(l)));
+      if (!keep) c.remove(static_cast<u32>(q), static_cast<u32>(d));
+    }
+  }
+}
+
+// Per-row popcounts, candidates.hpp:77-81 (the driver's candidate statistics, sigmo.cpp:201-214).
+void oracle_candidate_counts(uint64_t Q, uint64_t D, const uint32_t* words, uint32_t* counts) {
+  const u64 stride = (D + 31) / 32;
+  const Cands c{const_cast<u32*>(words), stride};
+#pragma omp parallel for
+  for (int64_t q = 0; q < static_cast<int64_t>(Q); ++q) counts[q] = c.count(static_cast<u32>(q));
+}
+
+// GMCR::generateGMCR, gmcr.hpp:34-128, kernel 1 + serial scan (offsets[Gd+1]).
+// Returns the total (query-graph, data-graph) pairs or -1 if a query graph has < 2 nodes.
+int64_t oracle_gmcr_offsets(uint32_t Gq, const uint32_t* qgo, uint32_t Gd, const uint32_t* dgo, uint64_t D,
+                            const uint32_t* words, uint32_t* offsets) {
+  for (u32 qg = 0; qg < Gq; ++qg)
+    if (qgo[qg + 1] - qgo[qg] < 2) return -1;
+  const Cands c{const_cast<u32*>(words), (D + 31) / 32};
+  std::memset(offsets, 0, sizeof(u32) * (Gd + 1));
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t dg = 0; dg < static_cast<int64_t>(Gd); ++dg) {
+    const u32 s = dgo[dg], e = dgo[dg + 1];
+    u32 cnt = 0;
+    for (u32 qg = 0; qg < Gq; ++qg) {  // kernel 1, gmcr.hpp:52-68
+      const u32 k = qgo[qg + 1] - qgo[qg], off = qgo[qg];
+      bool add = true;
+      for (u32 i = 0; i < k && add; i++) add = add && (c.count(off + i, s, e) > 0);
+      if (add) cnt++;
+    }
+    offsets[dg + 1] = cnt;
+  }
+  for (u64 i = 1; i <= Gd; i++) offsets[i] += offsets[i - 1];  // kernel 2, gmcr.hpp:73-75
+  return offsets[Gd];
+}
+
+// Kernel 3 (gmcr.hpp:94-115) with the within-segment order fixed to ascending query id.
+void oracle_gmcr_indices(uint32_t Gq, const uint32_t* qgo, uint32_t Gd, const uint32_t* dgo, uint64_t D,
+                         const uint32_t* words, const uint32_t* offsets, uint32_t* indices) {
+  const Cands c{const_cast<u32*>(words), (D + 31) / 32};
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t dg = 0; dg < static_cast<int64_t>(Gd); ++dg) {
+    const u32 s = dgo[dg], e = dgo[dg + 1];
+    u32 cur = offsets[dg];
+    for (u32 qg = 0; qg < Gq; ++qg) {
+      const u32 k = qgo[qg + 1] - qgo[qg], off = qgo[qg];
+      if (k <= 1) continue;
+      bool add = true;
+      for (u32 i = 0; i < k && add; i++) add = add && (c.count(off + i, s, e) > 0);
+      if (add) indices[cur++] = qg;
+    }
+  }
+}
+
+// joinCandidates, isomorphism.hpp:270-381 (DFS control flow :319-371, isValidMapping
+// :150-168).  Writes each pair's count (GMCR order) and returns the sum.
+// mutation 1: textbook backtracking: whenever a frame is popped (leaf or exhausted) the
+//             parent's current candidate is released; for molecules of <= 64 nodes this
+//             counts true induced embeddings.
+// mutation 2: mapping[] starts at global node 0 instead of the molecule's first node.
+// mutation 3: the induced test checks query edges only (non-edges not required).
+// mutation 4: visited is indexed by global id (no offset subtraction).
+int64_t oracle_join(uint32_t Gq, const uint32_t* qgo, const uint32_t* qro, const uint32_t* qci, uint32_t Gd,
+                    const uint32_t* dgo, const uint32_t* dro, const uint32_t* dci, uint64_t D, const uint32_t* words,
+                    const uint32_t* offsets, const uint32_t* indices, int find_first, int32_t* counts, int mutation) {
+  const Graph qgr = make_graph(Gq, qgo, qro, qci, nullptr);
+  const Graph dgr = make_graph(Gd, dgo, dro, dci, nullptr);
+  const Cands c{const_cast<u32*>(words), (D + 31) / 32};
+  int64_t total = 0;
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total) reduction(| : bad)
+  for (int64_t dg = 0; dg < static_cast<int64_t>(Gd); ++dg) {
+    const u32 s = dgo[dg], e = dgo[dg + 1];
+    for (u32 j = offsets[dg]; j < offsets[dg + 1]; ++j) {
+      const u32 qg = indices[j];
+      const u32 qoff = qgr.prev(qg);
+      const u32 k = qgr.nodes(qg);
+      if (k + 1 > static_cast<u32>(kMaxQueryNodes)) {
+        bad = 1;
+        counts[j] = -1;
+        continue;
+      }
+      struct Frame {
+        u32 depth;
+        u64 idx;
+      } stack[kMaxQueryNodes];
+      u32 mapping[kMaxQueryNodes];
+      for (auto& m : mapping) m = mutation == 2 ? 0u : s;
+      Bitset64 visited{mutation == 4 ? 0u : s};
+      u32 top = 0;
+      stack[top++] = {0, 0};
+      int64_t count = 0;
+      while (top > 0) {
+        const Frame f = stack[top - 1];
+        const u32 qn = f.depth;
+        if (f.depth == k) {  // :336-344
+          count++;
+          top--;
+          if (mutation == 1 && top > 0) visited.unset(static_cast<u8>(mapping[top - 1]));
+          if (find_first) break;
+          continue;
+        }
+        if (f.idx >= c.count(qn + qoff, s, e)) {  // :346-354
+          top--;
+          if (mutation != 1) {
+            visited.unset(static_cast<u8>(mapping[qn]));
+            if (top == 1) visited.clear();
+          } else if (top > 0) {
+            // textbook: release the candidate held by the frame we return to
+            visited.unset(static_cast<u8>(mapping[top - 1]));
+          }
+          continue;
+        }
+        const u32 cand = c.at(qn + qoff, static_cast<u32>(f.idx), s, e);  // :357
+        stack[top - 1].idx++;
+        if (visited.get(static_cast<u8>(cand))) continue;  // :363
+        bool valid = true;
+        if (f.depth != 0) {  // isValidMapping, :150-168
+          for (u32 i = 0; i < f.depth; i++) {
+            const bool qa = qgr.isNeighbor(i + qoff, f.depth + qoff);
+            const bool da = dgr.isNeighbor(mapping[i], cand);
+            if (mutation == 3 ? (qa && !da) : (qa != da)) {
+              valid = false;
+              break;
+            }
+          }
+        }
+        if (valid) {  // :366-370
+          mapping[f.depth] = cand;
+          visited.set(static_cast<u8>(cand));
+          stack[top++] = {f.depth + 1, 0};
+        }
+      }
+      if (count > INT32_MAX) bad = 1;
+      counts[j] = static_cast<int32_t>(std::min<int64_t>(count, INT32_MAX));
+      total += count;
+    }
+  }
+  return bad ? -1 : total;
+}
+
+}  // extern "C"

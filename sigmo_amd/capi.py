@@ -1,0 +1,540 @@
+"""ctypes binding of ``libsigmo_hip.so`` (the C ABI declared in ``include/sigmo_hip.h``).
+
+Thin by design: every call goes straight to the native library, whose entry points mirror
+the reference classes (paths relative to the SIGMo repository):
+
+=====================================  ==========================================================
+Python                                 reference
+=====================================  ==========================================================
+``HostGraphs.load_dat``                ``io::loadCSRGraphsFromFile`` (library/include/io.hpp:46-55)
+``Context``                            the ``sycl::queue`` (library/src/sigmo.cpp:19)
+``DeviceGraph``                        ``createDeviceCSRGraph`` (library/include/graph.hpp:255-314)
+``Signature``                          ``signature::Signature<>`` (library/include/signature.hpp:20-212)
+``Candidates``                         ``candidates::Candidates`` (library/include/candidates.hpp:14-204)
+``filter_candidates``                  ``isomorphism::filter::filterCandidates`` (isomorphism.hpp:22-62)
+``refine_candidates``                  ``isomorphism::filter::refineCandidates`` (isomorphism.hpp:64-127)
+``GMCR``                               ``isomorphism::mapping::GMCR`` (library/include/gmcr.hpp:18-135)
+``join_candidates``                    ``isomorphism::join::joinCandidates`` (isomorphism.hpp:270-381)
+=====================================  ==========================================================
+
+There is no CPU fallback: importing works without a GPU (so the host-side generator and
+.dat I/O stay usable), but every device entry point fails with :class:`SigmoError` if the
+library is missing or no HIP device is present.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("SIGMO_HIP_LIB", _PKG / "libsigmo_hip.so"))
+
+OK = 0
+ERR_INVALID, ERR_NOMEM, ERR_HIP, ERR_UNSUPPORTED, ERR_OVERFLOW, ERR_IO = -1, -2, -3, -4, -5, -6
+SCOPE_DATA, SCOPE_QUERY = 0, 1
+PHASE_SIG_DATA, PHASE_SIG_QUERY, PHASE_FILTER, PHASE_GMCR, PHASE_JOIN = range(5)
+PHASE_NAMES = ("sig_data", "sig_query", "filter", "gmcr", "join")
+SIZE_NORMAL, SIZE_LOGNORMAL = 0, 1
+
+# Every exported symbol of include/sigmo_hip.h, for the load check.
+EXPORTED = (
+    "sigmo_last_error", "sigmo_ctx_create", "sigmo_ctx_destroy", "sigmo_ctx_sync", "sigmo_phase_ms",
+    "sigmo_phase_reset", "sigmo_device_info", "sigmo_host_graphs_from_arrays", "sigmo_host_graphs_load_dat",
+    "sigmo_host_graphs_parse_dat", "sigmo_host_graphs_save_dat", "sigmo_host_graphs_info", "sigmo_host_graphs_arrays",
+    "sigmo_host_graphs_slice", "sigmo_host_graphs_concat", "sigmo_host_graphs_free", "sigmo_gen_molecules",
+    "sigmo_gen_queries", "sigmo_gen_arrivals", "sigmo_graph_upload", "sigmo_graph_free", "sigmo_graph_info",
+    "sigmo_graph_alloc_size", "sigmo_signatures_create", "sigmo_signatures_free", "sigmo_signatures_generate",
+    "sigmo_signatures_refine", "sigmo_signatures_download", "sigmo_signatures_alloc_size", "sigmo_candidates_create",
+    "sigmo_candidates_free", "sigmo_candidates_filter", "sigmo_candidates_refine", "sigmo_candidates_download",
+    "sigmo_candidates_counts", "sigmo_candidates_alloc_size", "sigmo_gmcr_create", "sigmo_gmcr_free",
+    "sigmo_gmcr_generate", "sigmo_gmcr_total", "sigmo_gmcr_download", "sigmo_join", "sigmo_pipeline_run",
+    "sigmo_pipeline_create", "sigmo_pipeline_execute", "sigmo_pipeline_pairs", "sigmo_pipeline_free",
+    "sigmo_kernel_stats",
+)
+KERNEL_NAMES = ("sig_generate", "sig_refine", "filter", "refine", "gmcr", "join")
+
+
+class SigmoError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class PipelineResult(C.Structure):
+    _fields_ = [
+        ("num_matches", C.c_uint64),
+        ("gmcr_pairs", C.c_uint64),
+        ("matched_pairs", C.c_uint64),
+        ("phase_ms", C.c_double * 5),
+        ("total_ms", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the native library (raises SigmoError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise SigmoError(ERR_INVALID, f"{LIB_PATH} is missing: run `python -m sigmo_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        vp, u32, u64, i32, dbl = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_double
+        pvp = C.POINTER(C.c_void_p)
+        sig = {
+            "sigmo_last_error": ([], C.c_char_p),
+            "sigmo_ctx_create": ([i32, pvp], i32),
+            "sigmo_ctx_destroy": ([vp], i32),
+            "sigmo_ctx_sync": ([vp], i32),
+            "sigmo_phase_ms": ([vp, i32, C.POINTER(dbl)], i32),
+            "sigmo_phase_reset": ([vp], i32),
+            "sigmo_device_info": ([vp, C.c_char_p, C.c_size_t, C.POINTER(u64)], i32),
+            "sigmo_host_graphs_from_arrays": ([u32, vp, vp, vp, vp, vp, pvp], i32),
+            "sigmo_host_graphs_load_dat": ([C.c_char_p, pvp], i32),
+            "sigmo_host_graphs_parse_dat": ([C.c_char_p, C.c_size_t, pvp], i32),
+            "sigmo_host_graphs_save_dat": ([vp, C.c_char_p], i32),
+            "sigmo_host_graphs_info": ([vp, C.POINTER(u32), C.POINTER(u64), C.POINTER(u64)], i32),
+            "sigmo_host_graphs_arrays": ([vp, pvp, pvp, pvp, pvp, pvp], i32),
+            "sigmo_host_graphs_slice": ([vp, u32, u32, pvp], i32),
+            "sigmo_host_graphs_concat": ([vp, vp, pvp], i32),
+            "sigmo_host_graphs_free": ([vp], i32),
+            "sigmo_gen_molecules": ([u64, u64, u64, i32, i32, pvp], i32),
+            "sigmo_gen_queries": ([u64, u32, u32, u32, i32, dbl, dbl, pvp], i32),
+            "sigmo_gen_arrivals": ([u64, u32, dbl, u32, u32, vp, vp], i32),
+            "sigmo_graph_upload": ([vp, vp, pvp], i32),
+            "sigmo_graph_free": ([vp], i32),
+            "sigmo_graph_info": ([vp, C.POINTER(u32), C.POINTER(u64), C.POINTER(u64)], i32),
+            "sigmo_graph_alloc_size": ([vp, C.POINTER(u64), C.POINTER(u64)], i32),
+            "sigmo_signatures_create": ([vp, u64, u64, pvp], i32),
+            "sigmo_signatures_free": ([vp], i32),
+            "sigmo_signatures_generate": ([vp, vp, vp, i32], i32),
+            "sigmo_signatures_refine": ([vp, vp, vp, i32, u32], i32),
+            "sigmo_signatures_download": ([vp, vp, i32, vp, u64], i32),
+            "sigmo_signatures_alloc_size": ([vp, i32, C.POINTER(u64)], i32),
+            "sigmo_candidates_create": ([vp, u64, u64, pvp], i32),
+            "sigmo_candidates_free": ([vp], i32),
+            "sigmo_candidates_filter": ([vp, vp, vp, vp, vp], i32),
+            "sigmo_candidates_refine": ([vp, vp, vp, vp, vp], i32),
+            "sigmo_candidates_download": ([vp, vp, vp, u64], i32),
+            "sigmo_candidates_counts": ([vp, vp, vp, u64], i32),
+            "sigmo_candidates_alloc_size": ([vp, C.POINTER(u64), C.POINTER(u64)], i32),
+            "sigmo_gmcr_create": ([vp, pvp], i32),
+            "sigmo_gmcr_free": ([vp], i32),
+            "sigmo_gmcr_generate": ([vp, vp, vp, vp, vp], i32),
+            "sigmo_gmcr_total": ([vp, C.POINTER(u64)], i32),
+            "sigmo_gmcr_download": ([vp, vp, vp, u64, vp, u64], i32),
+            "sigmo_join": ([vp, vp, vp, vp, vp, i32, C.POINTER(u64), vp], i32),
+            "sigmo_pipeline_run": ([vp, vp, vp, u32, i32, C.POINTER(PipelineResult)], i32),
+            "sigmo_pipeline_create": ([vp, vp, vp, pvp], i32),
+            "sigmo_pipeline_execute": ([vp, u32, i32, C.POINTER(PipelineResult)], i32),
+            "sigmo_pipeline_pairs": ([vp, vp, u64, vp, vp, u64], i32),
+            "sigmo_pipeline_free": ([vp], i32),
+            "sigmo_kernel_stats": ([vp, i32, C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl)], i32),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != OK:
+        raise SigmoError(status, lib().sigmo_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+# ---- host graphs ---------------------------------------------------------------------------------
+
+
+class HostGraphs:
+    """Host batched CSR (graph_offsets, row_offsets, column_indices, node_labels, edge_labels)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.sigmo_host_graphs_free(self._h)
+            self._h = None
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    @classmethod
+    def from_arrays(cls, graph_offsets, row_offsets, column_indices, node_labels, edge_labels=None) -> "HostGraphs":
+        go = np.ascontiguousarray(graph_offsets, dtype=np.uint32)
+        ro = np.ascontiguousarray(row_offsets, dtype=np.uint32)
+        ci = np.ascontiguousarray(column_indices, dtype=np.uint32)
+        nl = np.ascontiguousarray(node_labels, dtype=np.uint8)
+        el = np.zeros(len(ci), np.uint8) if edge_labels is None else np.ascontiguousarray(edge_labels, dtype=np.uint8)
+        h = C.c_void_p()
+        check(lib().sigmo_host_graphs_from_arrays(len(go) - 1, _ptr(go), _ptr(ro), _ptr(ci) if len(ci) else None,
+                                                  _ptr(nl) if len(nl) else None, _ptr(el) if len(el) else None, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_edge_lists(cls, graphs) -> "HostGraphs":
+        """graphs: iterable of (labels, edges[, edge_labels]) with local ids."""
+        lines = []
+        for g in graphs:
+            labels, edges = g[0], g[1]
+            el = g[2] if len(g) > 2 else [1] * len(edges)
+            toks = [f"n#{len(labels)}", "l#19"]
+            toks += [f"{i} {int(l)}" for i, l in enumerate(labels)]
+            toks.append(f"e#{len(edges)}")
+            toks += [f"{u} {v} {int(l)}" for (u, v), l in zip(edges, el)]
+            lines.append(" ".join(toks))
+        return cls.parse_dat("\n".join(lines) + "\n")
+
+    @classmethod
+    def load_dat(cls, path) -> "HostGraphs":
+        h = C.c_void_p()
+        check(lib().sigmo_host_graphs_load_dat(str(path).encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def parse_dat(cls, text: str) -> "HostGraphs":
+        b = text.encode()
+        h = C.c_void_p()
+        check(lib().sigmo_host_graphs_parse_dat(b, len(b), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def molecules(cls, seed: int, count: int, size_dist: int = SIZE_NORMAL, first: int = 0, threads: int = 0) -> "HostGraphs":
+        h = C.c_void_p()
+        check(lib().sigmo_gen_molecules(seed, first, count, size_dist, threads, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def queries(cls, seed: int, count: int, kmin: int, kmax: int, size_dist: int = SIZE_NORMAL,
+                perturb_frac: float = 0.0, ring_frac: float = -1.0) -> "HostGraphs":
+        h = C.c_void_p()
+        check(lib().sigmo_gen_queries(seed, count, kmin, kmax, size_dist, perturb_frac, ring_frac, C.byref(h)))
+        return cls(h)
+
+    def save_dat(self, path) -> None:
+        check(lib().sigmo_host_graphs_save_dat(self._h, str(path).encode()))
+
+    def info(self):
+        g, n, e = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        check(lib().sigmo_host_graphs_info(self._h, C.byref(g), C.byref(n), C.byref(e)))
+        return g.value, n.value, e.value
+
+    @property
+    def num_graphs(self) -> int:
+        return self.info()[0]
+
+    @property
+    def total_nodes(self) -> int:
+        return self.info()[1]
+
+    def arrays(self) -> dict:
+        """Copies of the five batched-CSR arrays as numpy arrays."""
+        G, N, E = self.info()
+        p = [C.c_void_p() for _ in range(5)]
+        check(lib().sigmo_host_graphs_arrays(self._h, *[C.byref(x) for x in p]))
+
+        def view(ptr, n, dt):
+            if n == 0:
+                return np.zeros(0, dt)
+            buf = (C.c_char * (n * np.dtype(dt).itemsize)).from_address(ptr.value)
+            return np.frombuffer(buf, dtype=dt).copy()
+
+        return {
+            "graph_offsets": view(p[0], G + 1, np.uint32),
+            "row_offsets": view(p[1], N + 1, np.uint32),
+            "column_indices": view(p[2], E, np.uint32),
+            "node_labels": view(p[3], N, np.uint8),
+            "edge_labels": view(p[4], E, np.uint8),
+        }
+
+    def slice(self, first: int, count: int) -> "HostGraphs":
+        h = C.c_void_p()
+        check(lib().sigmo_host_graphs_slice(self._h, first, count, C.byref(h)))
+        return HostGraphs(h)
+
+    def concat(self, other: "HostGraphs") -> "HostGraphs":
+        h = C.c_void_p()
+        check(lib().sigmo_host_graphs_concat(self._h, other._h, C.byref(h)))
+        return HostGraphs(h)
+
+
+def arrivals(seed: int, count: int, qps: float, qmin: int = 1, qmax: int = 50):
+    t = np.zeros(count, np.float64)
+    q = np.zeros(count, np.uint32)
+    check(lib().sigmo_gen_arrivals(seed, count, qps, qmin, qmax, _ptr(t), _ptr(q)))
+    return t, q
+
+
+# ---- device objects ---------------------------------------------------------------------------------
+
+
+class Context:
+    """One device + one HIP stream (the reference's sycl::queue)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().sigmo_ctx_create(device, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sigmo_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def sync(self):
+        check(lib().sigmo_ctx_sync(self._h))
+
+    def phase_ms(self) -> dict:
+        out = {}
+        for i, name in enumerate(PHASE_NAMES):
+            v = C.c_double()
+            check(lib().sigmo_phase_ms(self._h, i, C.byref(v)))
+            out[name] = v.value
+        return out
+
+    def reset_phases(self):
+        check(lib().sigmo_phase_reset(self._h))
+
+    def kernel_stats(self) -> dict:
+        """{kernel: (launches, total ms, total algorithmic HBM bytes)} since the last reset."""
+        out = {}
+        for i, name in enumerate(KERNEL_NAMES):
+            n, ms, by = C.c_uint64(), C.c_double(), C.c_double()
+            check(lib().sigmo_kernel_stats(self._h, i, C.byref(n), C.byref(ms), C.byref(by)))
+            out[name] = (n.value, ms.value, by.value)
+        return out
+
+    def device_info(self):
+        buf = C.create_string_buffer(256)
+        mem = C.c_uint64()
+        check(lib().sigmo_device_info(self._h, buf, 256, C.byref(mem)))
+        return buf.value.decode(), mem.value
+
+
+class DeviceGraph:
+    def __init__(self, ctx: Context, host: HostGraphs):
+        h = C.c_void_p()
+        check(lib().sigmo_graph_upload(ctx.handle, host.handle, C.byref(h)))
+        self._h, self.ctx = h, ctx
+        g, n, e = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        check(lib().sigmo_graph_info(h, C.byref(g), C.byref(n), C.byref(e)))
+        self.num_graphs, self.total_nodes, self.total_edges = g.value, n.value, e.value
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().sigmo_graph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.free()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def alloc_size(self):
+        a, b = C.c_uint64(), C.c_uint64()
+        check(lib().sigmo_graph_alloc_size(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+
+class Signature:
+    def __init__(self, ctx: Context, data_nodes: int, query_nodes: int):
+        h = C.c_void_p()
+        check(lib().sigmo_signatures_create(ctx.handle, data_nodes, query_nodes, C.byref(h)))
+        self._h, self.ctx = h, ctx
+        self.data_nodes, self.query_nodes = data_nodes, query_nodes
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().sigmo_signatures_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.free()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def generate_data(self, g: DeviceGraph):
+        check(lib().sigmo_signatures_generate(self.ctx.handle, self._h, g.handle, SCOPE_DATA))
+
+    def generate_query(self, g: DeviceGraph):
+        check(lib().sigmo_signatures_generate(self.ctx.handle, self._h, g.handle, SCOPE_QUERY))
+
+    def refine_data(self, g: DeviceGraph, view_size: int = 1):
+        check(lib().sigmo_signatures_refine(self.ctx.handle, self._h, g.handle, SCOPE_DATA, view_size))
+
+    def refine_query(self, g: DeviceGraph, view_size: int = 1):
+        check(lib().sigmo_signatures_refine(self.ctx.handle, self._h, g.handle, SCOPE_QUERY, view_size))
+
+    def download(self, scope: int) -> np.ndarray:
+        n = self.data_nodes if scope == SCOPE_DATA else self.query_nodes
+        out = np.zeros(n, np.uint64)
+        check(lib().sigmo_signatures_download(self.ctx.handle, self._h, scope, _ptr(out), n))
+        return out
+
+
+class Candidates:
+    def __init__(self, ctx: Context, source_nodes: int, target_nodes: int):
+        h = C.c_void_p()
+        check(lib().sigmo_candidates_create(ctx.handle, source_nodes, target_nodes, C.byref(h)))
+        self._h, self.ctx = h, ctx
+        self.source_nodes, self.target_nodes = source_nodes, target_nodes
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().sigmo_candidates_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.free()
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stride32(self) -> int:
+        return (self.target_nodes + 31) // 32
+
+    def download(self) -> np.ndarray:
+        """Reference layout: uint32 words, rows of ceil(target/32) words."""
+        out = np.zeros(self.source_nodes * self.stride32, np.uint32)
+        check(lib().sigmo_candidates_download(self.ctx.handle, self._h, _ptr(out), out.size))
+        return out
+
+    def counts(self) -> np.ndarray:
+        out = np.zeros(self.source_nodes, np.uint32)
+        check(lib().sigmo_candidates_counts(self.ctx.handle, self._h, _ptr(out), out.size))
+        return out
+
+
+def filter_candidates(ctx: Context, query: DeviceGraph, data: DeviceGraph, sig: Signature, cand: Candidates):
+    check(lib().sigmo_candidates_filter(ctx.handle, cand.handle, query.handle, data.handle, sig.handle))
+
+
+def refine_candidates(ctx: Context, query: DeviceGraph, data: DeviceGraph, sig: Signature, cand: Candidates):
+    check(lib().sigmo_candidates_refine(ctx.handle, cand.handle, query.handle, data.handle, sig.handle))
+
+
+class GMCR:
+    def __init__(self, ctx: Context):
+        h = C.c_void_p()
+        check(lib().sigmo_gmcr_create(ctx.handle, C.byref(h)))
+        self._h, self.ctx = h, ctx
+        self.num_data_graphs = 0
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().sigmo_gmcr_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.free()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def generate(self, query: DeviceGraph, data: DeviceGraph, cand: Candidates):
+        check(lib().sigmo_gmcr_generate(self.ctx.handle, self._h, query.handle, data.handle, cand.handle))
+        self.num_data_graphs = data.num_graphs
+
+    @property
+    def total(self) -> int:
+        v = C.c_uint64()
+        check(lib().sigmo_gmcr_total(self._h, C.byref(v)))
+        return v.value
+
+    def download(self):
+        off = np.zeros(self.num_data_graphs + 1, np.uint32)
+        idx = np.zeros(self.total, np.uint32)
+        check(lib().sigmo_gmcr_download(self.ctx.handle, self._h, _ptr(off), off.size,
+                                        _ptr(idx) if idx.size else None, idx.size))
+        return off, idx
+
+
+def join_candidates(ctx: Context, query: DeviceGraph, data: DeviceGraph, cand: Candidates, gmcr: GMCR,
+                    find_first: bool = True, per_pair: bool = True):
+    """Returns (num_matches, per-pair int32 counts in GMCR order or None)."""
+    total = C.c_uint64(0)
+    counts = np.zeros(gmcr.total, np.int32) if per_pair else None
+    check(lib().sigmo_join(ctx.handle, query.handle, data.handle, cand.handle, gmcr.handle, int(find_first),
+                           C.byref(total), _ptr(counts) if per_pair and counts.size else None))
+    return total.value, counts
+
+
+def pipeline_run(ctx: Context, query: DeviceGraph, data: DeviceGraph, refinement_steps: int, find_first: bool = True) -> dict:
+    r = PipelineResult()
+    check(lib().sigmo_pipeline_run(ctx.handle, query.handle, data.handle, refinement_steps, int(find_first), C.byref(r)))
+    return _result_dict(r)
+
+
+def _result_dict(r: PipelineResult) -> dict:
+    return {
+        "num_matches": r.num_matches,
+        "gmcr_pairs": r.gmcr_pairs,
+        "matched_pairs": r.matched_pairs,
+        "phase_ms": dict(zip(PHASE_NAMES, list(r.phase_ms))),
+        "total_ms": r.total_ms,
+    }
+
+
+class Pipeline:
+    """Plain per-phase pipeline with buffers allocated once (sigmo_pipeline_*)."""
+
+    def __init__(self, ctx: Context, query: DeviceGraph, data: DeviceGraph):
+        h = C.c_void_p()
+        check(lib().sigmo_pipeline_create(ctx.handle, query.handle, data.handle, C.byref(h)))
+        self._h, self.ctx, self.query, self.data = h, ctx, query, data
+        self.last = None
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().sigmo_pipeline_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.free()
+
+    def execute(self, refinement_steps: int, find_first: bool = True) -> dict:
+        r = PipelineResult()
+        check(lib().sigmo_pipeline_execute(self._h, refinement_steps, int(find_first), C.byref(r)))
+        self.last = _result_dict(r)
+        return self.last
+
+    def pairs(self):
+        """(data_graph_offsets, query_graph_indices, counts) of the last execute."""
+        P = self.last["gmcr_pairs"]
+        off = np.zeros(self.data.num_graphs + 1, np.uint32)
+        idx = np.zeros(P, np.uint32)
+        cnt = np.zeros(P, np.int32)
+        check(lib().sigmo_pipeline_pairs(self._h, _ptr(off), off.size, _ptr(idx) if P else None,
+                                         _ptr(cnt) if P else None, P))
+        return off, idx, cnt

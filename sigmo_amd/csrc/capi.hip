@@ -1,0 +1,878 @@
+// C ABI over the gfx950 kernels: context, device graphs, signatures, candidates, GMCR, join
+// and the plain per-phase pipeline.  Mirrors the reference's library/include classes
+// (paths relative to the SIGMo repository); every device buffer is owned by a handle and
+// every entry point reports failures as status codes plus sigmo_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sigmo_hip.h"
+#include "host_graphs.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+#define SIGMO_HIP_TRY(expr)                                                                          \
+  do {                                                                                               \
+    hipError_t _e = (expr);                                                                          \
+    if (_e != hipSuccess)                                                                            \
+      return sigmo_err::set(_e == hipErrorOutOfMemory ? SIGMO_ERR_NOMEM : SIGMO_ERR_HIP,             \
+                            std::string(#expr) + ": " + hipGetErrorString(_e));                      \
+  } while (0)
+
+#define SIGMO_TRY(expr)       \
+  do {                        \
+    int _s = (expr);          \
+    if (_s != SIGMO_OK) return _s; \
+  } while (0)
+
+template <typename T>
+int dev_alloc(T** p, uint64_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return sigmo_err::set(SIGMO_ERR_NOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
+  }
+  return SIGMO_OK;
+}
+
+template <typename T>
+void dev_free(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+}  // namespace
+
+// ---- objects -----------------------------------------------------------------------------------
+
+struct sigmo_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 256;
+  std::string name;
+  uint64_t global_mem = 0;
+  struct Timed {
+    hipEvent_t a, b;
+    int phase, kernel;
+    double bytes;
+  };
+  std::vector<Timed> pending;
+  double accum_ms[SIGMO_PHASE_COUNT] = {0, 0, 0, 0, 0};
+  uint64_t k_launches[SIGMO_KERNEL_COUNT] = {0, 0, 0, 0, 0, 0};
+  double k_ms[SIGMO_KERNEL_COUNT] = {0, 0, 0, 0, 0, 0};
+  double k_bytes[SIGMO_KERNEL_COUNT] = {0, 0, 0, 0, 0, 0};
+  std::vector<hipEvent_t> pool;
+  unsigned long long* stats = nullptr;       // join counters [4]
+  unsigned long long* stats_host = nullptr;  // pinned mirror
+
+  int event(hipEvent_t* e) {
+    if (!pool.empty()) {
+      *e = pool.back();
+      pool.pop_back();
+      return SIGMO_OK;
+    }
+    SIGMO_HIP_TRY(hipEventCreate(e));
+    return SIGMO_OK;
+  }
+  int collect() {
+    SIGMO_HIP_TRY(hipStreamSynchronize(stream));
+    for (auto& t : pending) {
+      float ms = 0;
+      SIGMO_HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
+      accum_ms[t.phase] += ms;
+      if (t.kernel >= 0) {
+        k_launches[t.kernel] += 1;
+        k_ms[t.kernel] += ms;
+        k_bytes[t.kernel] += t.bytes;
+      }
+      pool.push_back(t.a);
+      pool.push_back(t.b);
+    }
+    pending.clear();
+    return SIGMO_OK;
+  }
+};
+
+// Records hipEvents around the launches of one phase (utils.hpp:127-161 BatchedEvent).
+// `kernel` (SIGMO_KERNEL_*) and `bytes` (algorithmic HBM bytes of the launch) feed
+// sigmo_kernel_stats; kernel < 0 records phase time only.
+struct PhaseScope {
+  sigmo_ctx* c;
+  int phase, kernel;
+  double bytes;
+  hipEvent_t a = nullptr;
+  int status = SIGMO_OK;
+  PhaseScope(sigmo_ctx* ctx, int ph, int k = -1, double by = 0) : c(ctx), phase(ph), kernel(k), bytes(by) {
+    status = c->event(&a);
+    if (status == SIGMO_OK && hipEventRecord(a, c->stream) != hipSuccess) status = SIGMO_ERR_HIP;
+  }
+  int close() {
+    if (status != SIGMO_OK) return status;
+    hipEvent_t b;
+    SIGMO_TRY(c->event(&b));
+    SIGMO_HIP_TRY(hipEventRecord(b, c->stream));
+    c->pending.push_back({a, b, phase, kernel, bytes});
+    a = nullptr;
+    return SIGMO_OK;
+  }
+  ~PhaseScope() {
+    if (a) c->pool.push_back(a);
+  }
+};
+
+struct sigmo_graph {
+  sigmo_ctx* ctx = nullptr;
+  uint32_t G = 0;
+  uint64_t N = 0, E = 0;
+  uint32_t max_nodes = 0, min_nodes = 0;
+  std::vector<uint32_t> host_go;
+  uint32_t *go = nullptr, *ro = nullptr, *ci = nullptr, *node_graph = nullptr;
+  uint8_t *nl = nullptr, *el = nullptr;
+  uint64_t *adj = nullptr, *lplanes = nullptr, *lmask = nullptr;
+  sigmo_k::LabelSlots slots{};
+
+  sigmo_k::GraphDev dev() const {
+    sigmo_k::GraphDev g;
+    g.go = go;
+    g.ro = ro;
+    g.ci = ci;
+    g.nl = nl;
+    g.node_graph = node_graph;
+    g.adj = adj;
+    g.lplanes = lplanes;
+    g.lmask = lmask;
+    g.G = G;
+    g.N = N;
+    return g;
+  }
+  uint64_t csr_bytes() const { return N * 1 + (uint64_t(G) + 1) * 4 + (N + 1) * 4 + E * 4 + E * 1; }
+  uint64_t device_bytes() const {
+    return csr_bytes() + N * 4 + N * 16 + ((N + 63) / 64) * 64 + uint64_t(G) * 128;
+  }
+  ~sigmo_graph() {
+    dev_free(go);
+    dev_free(ro);
+    dev_free(ci);
+    dev_free(node_graph);
+    dev_free(nl);
+    dev_free(el);
+    dev_free(adj);
+    dev_free(lplanes);
+    dev_free(lmask);
+  }
+};
+
+struct sigmo_signatures {
+  sigmo_ctx* ctx = nullptr;
+  uint64_t D = 0, Q = 0;
+  uint64_t *dsig = nullptr, *qsig = nullptr, *dreach = nullptr, *qreach = nullptr, *dplanes = nullptr;
+  ~sigmo_signatures() {
+    dev_free(dsig);
+    dev_free(qsig);
+    dev_free(dreach);
+    dev_free(qreach);
+    dev_free(dplanes);
+  }
+};
+
+struct sigmo_candidates {
+  sigmo_ctx* ctx = nullptr;
+  uint64_t Q = 0, D = 0, W64 = 0;
+  uint64_t* bits = nullptr;
+  ~sigmo_candidates() { dev_free(bits); }
+};
+
+struct sigmo_gmcr {
+  sigmo_ctx* ctx = nullptr;
+  uint32_t Gq = 0, Gd = 0;
+  uint64_t P = 0;
+  bool built = false;
+  uint32_t *offsets = nullptr, *counts = nullptr, *indices = nullptr, *pair_graph = nullptr;
+  uint64_t *pair_bits = nullptr, *scan_tmp = nullptr, *total = nullptr;
+  uint64_t cap_indices = 0, cap_pair_graph = 0, cap_counts_g = 0, cap_offsets = 0, cap_pair_bits = 0, cap_scan = 0;
+  int32_t* pair_counts = nullptr;
+  uint64_t cap_counts = 0;
+  ~sigmo_gmcr() {
+    dev_free(offsets);
+    dev_free(counts);
+    dev_free(indices);
+    dev_free(pair_graph);
+    dev_free(pair_bits);
+    dev_free(scan_tmp);
+    dev_free(total);
+    dev_free(pair_counts);
+  }
+};
+
+namespace {
+
+int bind(sigmo_ctx* ctx) {
+  if (!ctx) return sigmo_err::set(SIGMO_ERR_INVALID, "null context");
+  SIGMO_HIP_TRY(hipSetDevice(ctx->device));
+  return SIGMO_OK;
+}
+
+int check_scope(int scope) {
+  if (scope != SIGMO_SCOPE_DATA && scope != SIGMO_SCOPE_QUERY) return sigmo_err::set(SIGMO_ERR_INVALID, "scope must be DATA or QUERY");
+  return SIGMO_OK;
+}
+
+template <typename T>
+int grow(T** p, uint64_t& cap, uint64_t need) {
+  if (*p && cap >= need) return SIGMO_OK;
+  dev_free(*p);
+  cap = 0;
+  SIGMO_TRY(dev_alloc(p, need));
+  cap = need;
+  return SIGMO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- context ---------------------------------------------------------------------------------
+
+int sigmo_ctx_create(int device, sigmo_ctx** out) {
+  if (!out) return sigmo_err::set(SIGMO_ERR_INVALID, "null out");
+  int n = 0;
+  SIGMO_HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return sigmo_err::set(SIGMO_ERR_INVALID, "device index out of range");
+  SIGMO_HIP_TRY(hipSetDevice(device));
+  auto* c = new sigmo_ctx();
+  c->device = device;
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDeviceProperties(&prop, device);
+  if (e == hipSuccess) {
+    c->num_cus = prop.multiProcessorCount;
+    c->name = prop.name;
+    c->global_mem = prop.totalGlobalMem;
+  }
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->stats), 4 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->stats_host), 4 * sizeof(unsigned long long), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    sigmo_ctx_destroy(c);
+    return sigmo_err::set(SIGMO_ERR_HIP, std::string("context setup: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return SIGMO_OK;
+}
+
+int sigmo_ctx_destroy(sigmo_ctx* ctx) {
+  if (!ctx) return SIGMO_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& t : ctx->pending) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  for (auto e : ctx->pool) (void)hipEventDestroy(e);
+  if (ctx->stats) (void)hipFree(ctx->stats);
+  if (ctx->stats_host) (void)hipHostFree(ctx->stats_host);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SIGMO_OK;
+}
+
+int sigmo_ctx_sync(sigmo_ctx* ctx) {
+  SIGMO_TRY(bind(ctx));
+  SIGMO_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return SIGMO_OK;
+}
+
+int sigmo_phase_ms(sigmo_ctx* ctx, int phase, double* ms) {
+  SIGMO_TRY(bind(ctx));
+  if (phase < 0 || phase >= SIGMO_PHASE_COUNT || !ms) return sigmo_err::set(SIGMO_ERR_INVALID, "bad phase");
+  SIGMO_TRY(ctx->collect());
+  *ms = ctx->accum_ms[phase];
+  return SIGMO_OK;
+}
+
+int sigmo_phase_reset(sigmo_ctx* ctx) {
+  SIGMO_TRY(bind(ctx));
+  SIGMO_TRY(ctx->collect());
+  for (double& a : ctx->accum_ms) a = 0;
+  for (int k = 0; k < SIGMO_KERNEL_COUNT; ++k) {
+    ctx->k_launches[k] = 0;
+    ctx->k_ms[k] = 0;
+    ctx->k_bytes[k] = 0;
+  }
+  return SIGMO_OK;
+}
+
+int sigmo_kernel_stats(sigmo_ctx* ctx, int kernel, uint64_t* launches, double* ms, double* bytes) {
+  SIGMO_TRY(bind(ctx));
+  if (kernel < 0 || kernel >= SIGMO_KERNEL_COUNT) return sigmo_err::set(SIGMO_ERR_INVALID, "bad kernel id");
+  SIGMO_TRY(ctx->collect());
+  if (launches) *launches = ctx->k_launches[kernel];
+  if (ms) *ms = ctx->k_ms[kernel];
+  if (bytes) *bytes = ctx->k_bytes[kernel];
+  return SIGMO_OK;
+}
+
+int sigmo_device_info(sigmo_ctx* ctx, char* name, size_t name_len, uint64_t* global_mem) {
+  if (!ctx) return sigmo_err::set(SIGMO_ERR_INVALID, "null context");
+  if (name && name_len) {
+    std::strncpy(name, ctx->name.c_str(), name_len - 1);
+    name[name_len - 1] = '\0';
+  }
+  if (global_mem) *global_mem = ctx->global_mem;
+  return SIGMO_OK;
+}
+
+// ---- graphs ----------------------------------------------------------------------------------
+
+int sigmo_graph_upload(sigmo_ctx* ctx, const sigmo_host_graphs* host, sigmo_graph** out) {
+  SIGMO_TRY(bind(ctx));
+  if (!host || !out) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  const uint32_t G = host->num_graphs();
+  const uint64_t N = host->total_nodes(), E = host->total_edges();
+  if (G == 0) return sigmo_err::set(SIGMO_ERR_INVALID, "empty graph batch");
+  if (N >= (1ull << 32) || E >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "batch exceeds u32 node/edge ids");
+  uint32_t mx = 0, mn = UINT32_MAX;
+  for (uint32_t g = 0; g < G; ++g) {
+    const uint32_t n = host->graph_offsets[g + 1] - host->graph_offsets[g];
+    mx = std::max(mx, n);
+    mn = std::min(mn, n);
+  }
+  if (mn == 0) return sigmo_err::set(SIGMO_ERR_INVALID, "graphs with zero nodes are not supported");
+  if (mx > SIGMO_MAX_DATA_GRAPH_NODES)
+    return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "graph with " + std::to_string(mx) + " nodes exceeds the 128-node limit");
+  auto* g = new sigmo_graph();
+  g->ctx = ctx;
+  g->G = G;
+  g->N = N;
+  g->E = E;
+  g->max_nodes = mx;
+  g->min_nodes = mn;
+  g->host_go = host->graph_offsets;
+  bool present[16] = {false};
+  for (uint8_t l : host->node_labels)
+    if (l < 16) present[l] = true;
+  g->slots.count = 0;
+  for (int l = 0; l < 16; ++l)
+    if (present[l]) g->slots.label[g->slots.count++] = static_cast<uint8_t>(l);
+  const uint64_t W64 = (N + 63) / 64;
+  int st = SIGMO_OK;
+  if (st == SIGMO_OK) st = dev_alloc(&g->go, uint64_t(G) + 1);
+  if (st == SIGMO_OK) st = dev_alloc(&g->ro, N + 1);
+  if (st == SIGMO_OK) st = dev_alloc(&g->ci, E);
+  if (st == SIGMO_OK) st = dev_alloc(&g->nl, N);
+  if (st == SIGMO_OK) st = dev_alloc(&g->el, E);
+  if (st == SIGMO_OK) st = dev_alloc(&g->node_graph, N);
+  if (st == SIGMO_OK) st = dev_alloc(&g->adj, 2 * N);
+  if (st == SIGMO_OK) st = dev_alloc(&g->lplanes, 8 * W64);
+  if (st == SIGMO_OK) st = dev_alloc(&g->lmask, 16 * uint64_t(G));
+  if (st != SIGMO_OK) {
+    delete g;
+    return st;
+  }
+  hipStream_t s = ctx->stream;
+  hipError_t e = hipMemcpyAsync(g->go, host->graph_offsets.data(), (uint64_t(G) + 1) * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(g->ro, host->row_offsets.data(), (N + 1) * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && E) e = hipMemcpyAsync(g->ci, host->column_indices.data(), E * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(g->nl, host->node_labels.data(), N, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && E) e = hipMemcpyAsync(g->el, host->edge_labels.data(), E, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = sigmo_k::derive_graph(s, g->go, G, g->ro, g->ci, g->nl, N, g->node_graph, g->adj, g->lplanes, g->lmask);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    delete g;
+    return sigmo_err::set(SIGMO_ERR_HIP, std::string("graph upload: ") + hipGetErrorString(e));
+  }
+  *out = g;
+  return SIGMO_OK;
+}
+
+int sigmo_graph_free(sigmo_graph* g) {
+  if (!g) return SIGMO_OK;
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  delete g;
+  return SIGMO_OK;
+}
+
+int sigmo_graph_info(const sigmo_graph* g, uint32_t* num_graphs, uint64_t* total_nodes, uint64_t* total_edges) {
+  if (!g) return sigmo_err::set(SIGMO_ERR_INVALID, "null graph");
+  if (num_graphs) *num_graphs = g->G;
+  if (total_nodes) *total_nodes = g->N;
+  if (total_edges) *total_edges = g->E;
+  return SIGMO_OK;
+}
+
+int sigmo_graph_alloc_size(const sigmo_graph* g, uint64_t* csr_bytes, uint64_t* device_bytes) {
+  if (!g) return sigmo_err::set(SIGMO_ERR_INVALID, "null graph");
+  if (csr_bytes) *csr_bytes = g->csr_bytes();
+  if (device_bytes) *device_bytes = g->device_bytes();
+  return SIGMO_OK;
+}
+
+// ---- signatures -------------------------------------------------------------------------------
+
+int sigmo_signatures_create(sigmo_ctx* ctx, uint64_t data_nodes, uint64_t query_nodes, sigmo_signatures** out) {
+  SIGMO_TRY(bind(ctx));
+  if (!out) return sigmo_err::set(SIGMO_ERR_INVALID, "null out");
+  auto* s = new sigmo_signatures();
+  s->ctx = ctx;
+  s->D = data_nodes;
+  s->Q = query_nodes;
+  const uint64_t W64 = (data_nodes + 63) / 64;
+  int st = dev_alloc(&s->dsig, data_nodes);
+  if (st == SIGMO_OK) st = dev_alloc(&s->qsig, query_nodes);
+  if (st == SIGMO_OK) st = dev_alloc(&s->dreach, data_nodes);
+  if (st == SIGMO_OK) st = dev_alloc(&s->qreach, query_nodes);
+  if (st == SIGMO_OK) st = dev_alloc(&s->dplanes, 64 * W64);
+  hipError_t e = hipSuccess;
+  // signatures filled with 0 (signature.hpp:166-167); reachables start empty (zero)
+  if (st == SIGMO_OK) e = sigmo_k::fill_u64(ctx->stream, s->dsig, data_nodes, 0);
+  if (st == SIGMO_OK && e == hipSuccess) e = sigmo_k::fill_u64(ctx->stream, s->qsig, query_nodes, 0);
+  if (st == SIGMO_OK && e == hipSuccess) e = sigmo_k::fill_u64(ctx->stream, s->dreach, data_nodes, 0);
+  if (st == SIGMO_OK && e == hipSuccess) e = sigmo_k::fill_u64(ctx->stream, s->qreach, query_nodes, 0);
+  if (st == SIGMO_OK && e == hipSuccess) e = sigmo_k::fill_u64(ctx->stream, s->dplanes, 64 * W64, 0);
+  if (st == SIGMO_OK && e != hipSuccess) st = sigmo_err::set(SIGMO_ERR_HIP, std::string("signature init: ") + hipGetErrorString(e));
+  if (st != SIGMO_OK) {
+    delete s;
+    return st;
+  }
+  *out = s;
+  return SIGMO_OK;
+}
+
+int sigmo_signatures_free(sigmo_signatures* s) {
+  if (!s) return SIGMO_OK;
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  delete s;
+  return SIGMO_OK;
+}
+
+static int sig_target(sigmo_signatures* s, const sigmo_graph* g, int scope, uint64_t** sigs, uint64_t** reach, uint64_t** planes) {
+  SIGMO_TRY(check_scope(scope));
+  if (!s || !g) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  const uint64_t n = scope == SIGMO_SCOPE_DATA ? s->D : s->Q;
+  if (g->N != n) return sigmo_err::set(SIGMO_ERR_INVALID, "graph node count does not match the signature allocation");
+  *sigs = scope == SIGMO_SCOPE_DATA ? s->dsig : s->qsig;
+  *reach = scope == SIGMO_SCOPE_DATA ? s->dreach : s->qreach;
+  *planes = scope == SIGMO_SCOPE_DATA ? s->dplanes : nullptr;
+  return SIGMO_OK;
+}
+
+// Algorithmic HBM bytes of one signature pass: CSR + labels read, 8-byte signature written.
+static double sig_bytes(const sigmo_graph* g) { return 4.0 * (g->N + 1) + 4.0 * g->E + g->N + 8.0 * g->N; }
+
+int sigmo_signatures_generate(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope) {
+  SIGMO_TRY(bind(ctx));
+  uint64_t *sigs, *reach, *planes;
+  SIGMO_TRY(sig_target(s, g, scope, &sigs, &reach, &planes));
+  PhaseScope ps(ctx, scope == SIGMO_SCOPE_DATA ? SIGMO_PHASE_SIG_DATA : SIGMO_PHASE_SIG_QUERY,
+                scope == SIGMO_SCOPE_DATA ? SIGMO_KERNEL_SIG_GENERATE : -1, sig_bytes(g));
+  SIGMO_HIP_TRY(sigmo_k::sig_generate(ctx->stream, g->dev(), sigs, planes));
+  return ps.close();
+}
+
+int sigmo_signatures_refine(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope, uint32_t view_size) {
+  SIGMO_TRY(bind(ctx));
+  uint64_t *sigs, *reach, *planes;
+  SIGMO_TRY(sig_target(s, g, scope, &sigs, &reach, &planes));
+  PhaseScope ps(ctx, scope == SIGMO_SCOPE_DATA ? SIGMO_PHASE_SIG_DATA : SIGMO_PHASE_SIG_QUERY,
+                scope == SIGMO_SCOPE_DATA ? SIGMO_KERNEL_SIG_REFINE : -1, sig_bytes(g));
+  SIGMO_HIP_TRY(sigmo_k::sig_refine(ctx->stream, g->dev(), view_size, sigs, reach, planes));
+  return ps.close();
+}
+
+int sigmo_signatures_download(sigmo_ctx* ctx, const sigmo_signatures* s, int scope, uint64_t* out, uint64_t count) {
+  SIGMO_TRY(bind(ctx));
+  SIGMO_TRY(check_scope(scope));
+  if (!s || (!out && count)) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  const uint64_t n = scope == SIGMO_SCOPE_DATA ? s->D : s->Q;
+  if (count > n) return sigmo_err::set(SIGMO_ERR_INVALID, "count exceeds the signature allocation");
+  if (!count) return SIGMO_OK;
+  SIGMO_HIP_TRY(hipMemcpyAsync(out, scope == SIGMO_SCOPE_DATA ? s->dsig : s->qsig, count * 8, hipMemcpyDeviceToHost, ctx->stream));
+  SIGMO_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return SIGMO_OK;
+}
+
+int sigmo_signatures_alloc_size(const sigmo_signatures* s, int scope, uint64_t* bytes) {
+  SIGMO_TRY(check_scope(scope));
+  if (!s || !bytes) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  // signature.hpp:181-198: signatures plus the 16-byte Bitset<uint64_t> reachables
+  *bytes = (scope == SIGMO_SCOPE_DATA ? s->D : s->Q) * (8 + 16);
+  return SIGMO_OK;
+}
+
+// ---- candidates ---------------------------------------------------------------------------------
+
+int sigmo_candidates_create(sigmo_ctx* ctx, uint64_t source_nodes, uint64_t target_nodes, sigmo_candidates** out) {
+  SIGMO_TRY(bind(ctx));
+  if (!out) return sigmo_err::set(SIGMO_ERR_INVALID, "null out");
+  auto* c = new sigmo_candidates();
+  c->ctx = ctx;
+  c->Q = source_nodes;
+  c->D = target_nodes;
+  c->W64 = (target_nodes + 63) / 64;
+  int st = dev_alloc(&c->bits, c->Q * c->W64);
+  if (st == SIGMO_OK) {
+    hipError_t e = sigmo_k::fill_u64(ctx->stream, c->bits, c->Q * c->W64, 0);  // candidates.hpp:169-178
+    if (e != hipSuccess) st = sigmo_err::set(SIGMO_ERR_HIP, hipGetErrorString(e));
+  }
+  if (st != SIGMO_OK) {
+    delete c;
+    return st;
+  }
+  *out = c;
+  return SIGMO_OK;
+}
+
+int sigmo_candidates_free(sigmo_candidates* c) {
+  if (!c) return SIGMO_OK;
+  (void)hipSetDevice(c->ctx->device);
+  (void)hipStreamSynchronize(c->ctx->stream);
+  delete c;
+  return SIGMO_OK;
+}
+
+static int check_cand(const sigmo_candidates* c, const sigmo_graph* q, const sigmo_graph* d) {
+  if (!c || !q || !d) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  if (c->Q != q->N || c->D != d->N) return sigmo_err::set(SIGMO_ERR_INVALID, "candidate shape does not match the graphs");
+  return SIGMO_OK;
+}
+
+int sigmo_candidates_filter(sigmo_ctx* ctx, sigmo_candidates* c, const sigmo_graph* query, const sigmo_graph* data,
+                            const sigmo_signatures* s) {
+  SIGMO_TRY(bind(ctx));
+  SIGMO_TRY(check_cand(c, query, data));
+  (void)s;  // signatures are captured but unused by the reference filter (isomorphism.hpp:39-44)
+  PhaseScope ps(ctx, SIGMO_PHASE_FILTER, SIGMO_KERNEL_FILTER, 8.0 * c->Q * c->W64 + c->D + c->Q);
+  SIGMO_HIP_TRY(sigmo_k::filter(ctx->stream, query->nl, c->Q, data->lplanes, c->D, c->bits, ctx->num_cus));
+  return ps.close();
+}
+
+int sigmo_candidates_refine(sigmo_ctx* ctx, sigmo_candidates* c, const sigmo_graph* query, const sigmo_graph* data,
+                            const sigmo_signatures* s) {
+  SIGMO_TRY(bind(ctx));
+  SIGMO_TRY(check_cand(c, query, data));
+  if (!s || s->D != c->D || s->Q != c->Q) return sigmo_err::set(SIGMO_ERR_INVALID, "signature shape does not match the candidates");
+  PhaseScope ps(ctx, SIGMO_PHASE_FILTER, SIGMO_KERNEL_REFINE, 16.0 * c->Q * c->W64 + 8.0 * c->D + 8.0 * c->Q);
+  SIGMO_HIP_TRY(sigmo_k::refine(ctx->stream, s->qsig, c->Q, s->dplanes, c->D, query->slots, c->bits, ctx->num_cus));
+  return ps.close();
+}
+
+int sigmo_candidates_download(sigmo_ctx* ctx, const sigmo_candidates* c, uint32_t* words, uint64_t n_words) {
+  SIGMO_TRY(bind(ctx));
+  if (!c || (!words && n_words)) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  const uint64_t S32 = (c->D + 31) / 32;
+  if (n_words != c->Q * S32) return sigmo_err::set(SIGMO_ERR_INVALID, "n_words must equal rows * ceil(target/32)");
+  if (!n_words) return SIGMO_OK;
+  // Row q of the reference image is the first 4*S32 bytes of device row q.
+  SIGMO_HIP_TRY(hipMemcpy2DAsync(words, S32 * 4, c->bits, c->W64 * 8, S32 * 4, c->Q, hipMemcpyDeviceToHost, ctx->stream));
+  SIGMO_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return SIGMO_OK;
+}
+
+int sigmo_candidates_counts(sigmo_ctx* ctx, const sigmo_candidates* c, uint32_t* counts, uint64_t n) {
+  SIGMO_TRY(bind(ctx));
+  if (!c || (!counts && n)) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  if (n != c->Q) return sigmo_err::set(SIGMO_ERR_INVALID, "n must equal the number of source rows");
+  if (!n) return SIGMO_OK;
+  uint32_t* d = nullptr;
+  SIGMO_TRY(dev_alloc(&d, n));
+  hipError_t e = sigmo_k::row_counts(ctx->stream, c->bits, c->Q, c->W64, d);
+  if (e == hipSuccess) e = hipMemcpyAsync(counts, d, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  dev_free(d);
+  if (e != hipSuccess) return sigmo_err::set(SIGMO_ERR_HIP, hipGetErrorString(e));
+  return SIGMO_OK;
+}
+
+int sigmo_candidates_alloc_size(const sigmo_candidates* c, uint64_t* ref_bytes, uint64_t* device_bytes) {
+  if (!c) return sigmo_err::set(SIGMO_ERR_INVALID, "null candidates");
+  if (ref_bytes) *ref_bytes = c->Q * ((c->D + 31) / 32) * 4;
+  if (device_bytes) *device_bytes = c->Q * c->W64 * 8;
+  return SIGMO_OK;
+}
+
+// ---- GMCR --------------------------------------------------------------------------------------
+
+int sigmo_gmcr_create(sigmo_ctx* ctx, sigmo_gmcr** out) {
+  SIGMO_TRY(bind(ctx));
+  if (!out) return sigmo_err::set(SIGMO_ERR_INVALID, "null out");
+  auto* m = new sigmo_gmcr();
+  m->ctx = ctx;
+  int st = dev_alloc(&m->total, 1);
+  if (st != SIGMO_OK) {
+    delete m;
+    return st;
+  }
+  *out = m;
+  return SIGMO_OK;
+}
+
+int sigmo_gmcr_free(sigmo_gmcr* m) {
+  if (!m) return SIGMO_OK;
+  (void)hipSetDevice(m->ctx->device);
+  (void)hipStreamSynchronize(m->ctx->stream);
+  delete m;
+  return SIGMO_OK;
+}
+
+static int check_queries(const sigmo_graph* q) {
+  for (uint32_t g = 0; g < q->G; ++g) {
+    const uint32_t k = q->host_go[g + 1] - q->host_go[g];
+    if (k < 2) return sigmo_err::set(SIGMO_ERR_INVALID, "query graph " + std::to_string(g) + " has fewer than 2 nodes");
+    if (k > SIGMO_MAX_QUERY_NODES)
+      return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "query graph " + std::to_string(g) + " exceeds 32 nodes");
+  }
+  return SIGMO_OK;
+}
+
+int sigmo_gmcr_generate(sigmo_ctx* ctx, sigmo_gmcr* m, const sigmo_graph* query, const sigmo_graph* data, const sigmo_candidates* c) {
+  SIGMO_TRY(bind(ctx));
+  if (!m) return sigmo_err::set(SIGMO_ERR_INVALID, "null gmcr");
+  SIGMO_TRY(check_cand(c, query, data));
+  SIGMO_TRY(check_queries(query));
+  const uint32_t Gq = query->G, Gd = data->G;
+  const uint64_t strips = (uint64_t(Gd) + 63) / 64;
+  SIGMO_TRY(grow(&m->pair_bits, m->cap_pair_bits, uint64_t(Gq) * strips));
+  SIGMO_TRY(grow(&m->counts, m->cap_counts_g, uint64_t(Gd) + 1));
+  SIGMO_TRY(grow(&m->offsets, m->cap_offsets, uint64_t(Gd) + 1));
+  SIGMO_TRY(grow(&m->scan_tmp, m->cap_scan, (uint64_t(Gd) + 1023) / 1024 + 1));
+  hipStream_t s = ctx->stream;
+  PhaseScope ps(ctx, SIGMO_PHASE_GMCR, SIGMO_KERNEL_GMCR, 8.0 * c->Q * c->W64 + 4.0 * (uint64_t(Gd) + 1));
+  SIGMO_HIP_TRY(sigmo_k::gmcr_mark(s, query->dev(), data->dev(), c->bits, c->W64, m->pair_bits));
+  SIGMO_HIP_TRY(sigmo_k::gmcr_count(s, m->pair_bits, Gq, Gd, m->counts));
+  SIGMO_HIP_TRY(sigmo_k::exclusive_scan(s, m->counts, Gd, m->offsets, m->scan_tmp, m->total));
+  // gmcr.hpp:79-84 reads the total back to size the index array
+  uint64_t P = 0;
+  SIGMO_HIP_TRY(hipMemcpyAsync(&P, m->total, 8, hipMemcpyDeviceToHost, s));
+  SIGMO_HIP_TRY(hipStreamSynchronize(s));
+  if (P >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "GMCR pair count exceeds u32");
+  SIGMO_TRY(grow(&m->indices, m->cap_indices, P));
+  SIGMO_TRY(grow(&m->pair_graph, m->cap_pair_graph, P));
+  SIGMO_HIP_TRY(sigmo_k::gmcr_scatter(s, m->pair_bits, Gq, Gd, m->offsets, m->indices, m->pair_graph));
+  SIGMO_TRY(ps.close());
+  m->Gq = Gq;
+  m->Gd = Gd;
+  m->P = P;
+  m->built = true;
+  return SIGMO_OK;
+}
+
+int sigmo_gmcr_total(const sigmo_gmcr* m, uint64_t* total_query_indices) {
+  if (!m || !total_query_indices) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  *total_query_indices = m->P;
+  return SIGMO_OK;
+}
+
+int sigmo_gmcr_download(sigmo_ctx* ctx, const sigmo_gmcr* m, uint32_t* data_graph_offsets, uint64_t n_offsets,
+                        uint32_t* query_graph_indices, uint64_t n_indices) {
+  SIGMO_TRY(bind(ctx));
+  if (!m || !m->built) return sigmo_err::set(SIGMO_ERR_INVALID, "GMCR not generated");
+  if (data_graph_offsets) {
+    if (n_offsets != uint64_t(m->Gd) + 1) return sigmo_err::set(SIGMO_ERR_INVALID, "n_offsets must be Gd+1");
+    SIGMO_HIP_TRY(hipMemcpyAsync(data_graph_offsets, m->offsets, n_offsets * 4, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  if (query_graph_indices) {
+    if (n_indices != m->P) return sigmo_err::set(SIGMO_ERR_INVALID, "n_indices must be the pair count");
+    if (n_indices)
+      SIGMO_HIP_TRY(hipMemcpyAsync(query_graph_indices, m->indices, n_indices * 4, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  SIGMO_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return SIGMO_OK;
+}
+
+// ---- join ----------------------------------------------------------------------------------------
+
+int sigmo_join(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data, const sigmo_candidates* c, const sigmo_gmcr* m,
+               int find_first, uint64_t* num_matches, int32_t* per_pair_counts) {
+  SIGMO_TRY(bind(ctx));
+  SIGMO_TRY(check_cand(c, query, data));
+  if (!m || !m->built) return sigmo_err::set(SIGMO_ERR_INVALID, "GMCR not generated");
+  if (m->Gq != query->G || m->Gd != data->G) return sigmo_err::set(SIGMO_ERR_INVALID, "GMCR built for other graphs");
+  SIGMO_TRY(check_queries(query));
+  auto* mm = const_cast<sigmo_gmcr*>(m);
+  SIGMO_TRY(grow(&mm->pair_counts, mm->cap_counts, m->P));
+  hipStream_t s = ctx->stream;
+  PhaseScope ps(ctx, SIGMO_PHASE_JOIN, SIGMO_KERNEL_JOIN, 0.0);
+  SIGMO_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(ctx->stats), 4, 0));
+  SIGMO_HIP_TRY(sigmo_k::join(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, m->P, find_first,
+                              mm->pair_counts, ctx->stats, ctx->num_cus));
+  SIGMO_TRY(ps.close());
+  SIGMO_HIP_TRY(hipMemcpyAsync(ctx->stats_host, ctx->stats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  if (per_pair_counts && m->P)
+    SIGMO_HIP_TRY(hipMemcpyAsync(per_pair_counts, mm->pair_counts, m->P * 4, hipMemcpyDeviceToHost, s));
+  SIGMO_HIP_TRY(hipStreamSynchronize(s));
+  if (ctx->stats_host[3]) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "a per-pair match count exceeded int32");
+  if (num_matches) *num_matches += ctx->stats_host[0];
+  return SIGMO_OK;
+}
+
+// ---- plain pipeline --------------------------------------------------------------------------------
+
+}  // extern "C"
+
+struct sigmo_pipeline {
+  sigmo_ctx* ctx = nullptr;
+  const sigmo_graph* q = nullptr;
+  const sigmo_graph* d = nullptr;
+  sigmo_signatures* sig = nullptr;
+  sigmo_candidates* cand = nullptr;
+  sigmo_gmcr* gm = nullptr;
+  bool has_counts = false;
+  ~sigmo_pipeline() {
+    sigmo_signatures_free(sig);
+    sigmo_candidates_free(cand);
+    sigmo_gmcr_free(gm);
+  }
+};
+
+extern "C" {
+
+int sigmo_pipeline_create(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data, sigmo_pipeline** out) {
+  SIGMO_TRY(bind(ctx));
+  if (!query || !data || !out) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  SIGMO_TRY(check_queries(query));
+  auto* p = new sigmo_pipeline();
+  p->ctx = ctx;
+  p->q = query;
+  p->d = data;
+  int st = sigmo_signatures_create(ctx, data->N, query->N, &p->sig);
+  if (st == SIGMO_OK) st = sigmo_candidates_create(ctx, query->N, data->N, &p->cand);
+  if (st == SIGMO_OK) st = sigmo_gmcr_create(ctx, &p->gm);
+  if (st == SIGMO_OK) st = sigmo_ctx_sync(ctx);
+  if (st != SIGMO_OK) {
+    delete p;
+    return st;
+  }
+  *out = p;
+  return SIGMO_OK;
+}
+
+int sigmo_pipeline_execute(sigmo_pipeline* p, uint32_t refinement_steps, int find_first, sigmo_pipeline_result* result) {
+  if (!p || !result) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  sigmo_ctx* ctx = p->ctx;
+  SIGMO_TRY(bind(ctx));
+  std::memset(result, 0, sizeof(*result));
+  SIGMO_TRY(ctx->collect());
+  double before[SIGMO_PHASE_COUNT];
+  std::memcpy(before, ctx->accum_ms, sizeof(before));
+  hipEvent_t t0, t1;
+  SIGMO_TRY(ctx->event(&t0));
+  SIGMO_TRY(ctx->event(&t1));
+  SIGMO_HIP_TRY(hipEventRecord(t0, ctx->stream));
+  // fresh Signature state (signature.hpp:166-167; reachables empty)
+  sigmo_signatures* sig = p->sig;
+  SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->dsig, sig->D, 0));
+  SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->qsig, sig->Q, 0));
+  SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->dreach, sig->D, 0));
+  SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->qreach, sig->Q, 0));
+  SIGMO_TRY(sigmo_signatures_generate(ctx, sig, p->d, SIGMO_SCOPE_DATA));
+  SIGMO_TRY(sigmo_signatures_generate(ctx, sig, p->q, SIGMO_SCOPE_QUERY));
+  SIGMO_TRY(sigmo_candidates_filter(ctx, p->cand, p->q, p->d, sig));
+  for (uint32_t r = 1; r <= refinement_steps; ++r) {
+    SIGMO_TRY(sigmo_signatures_refine(ctx, sig, p->d, SIGMO_SCOPE_DATA, r));
+    SIGMO_TRY(sigmo_signatures_refine(ctx, sig, p->q, SIGMO_SCOPE_QUERY, r));
+    SIGMO_TRY(sigmo_candidates_refine(ctx, p->cand, p->q, p->d, sig));
+  }
+  SIGMO_TRY(sigmo_gmcr_generate(ctx, p->gm, p->q, p->d, p->cand));
+  uint64_t matches = 0;
+  SIGMO_TRY(sigmo_join(ctx, p->q, p->d, p->cand, p->gm, find_first, &matches, nullptr));
+  SIGMO_HIP_TRY(hipEventRecord(t1, ctx->stream));
+  SIGMO_HIP_TRY(hipEventSynchronize(t1));
+  float total = 0;
+  SIGMO_HIP_TRY(hipEventElapsedTime(&total, t0, t1));
+  ctx->pool.push_back(t0);
+  ctx->pool.push_back(t1);
+  p->has_counts = true;
+  result->num_matches = matches;
+  result->gmcr_pairs = p->gm->P;
+  result->matched_pairs = ctx->stats_host[1];
+  result->total_ms = total;
+  SIGMO_TRY(ctx->collect());
+  for (int ph = 0; ph < SIGMO_PHASE_COUNT; ++ph) result->phase_ms[ph] = ctx->accum_ms[ph] - before[ph];
+  return SIGMO_OK;
+}
+
+int sigmo_pipeline_pairs(sigmo_pipeline* p, uint32_t* data_graph_offsets, uint64_t n_offsets, uint32_t* query_graph_indices,
+                         int32_t* counts, uint64_t n_pairs) {
+  if (!p || !p->has_counts) return sigmo_err::set(SIGMO_ERR_INVALID, "pipeline has not executed");
+  SIGMO_TRY(bind(p->ctx));
+  SIGMO_TRY(sigmo_gmcr_download(p->ctx, p->gm, data_graph_offsets, n_offsets, query_graph_indices, n_pairs));
+  if (counts) {
+    if (n_pairs != p->gm->P) return sigmo_err::set(SIGMO_ERR_INVALID, "n_pairs must be the pair count");
+    if (n_pairs) {
+      SIGMO_HIP_TRY(hipMemcpyAsync(counts, p->gm->pair_counts, n_pairs * 4, hipMemcpyDeviceToHost, p->ctx->stream));
+      SIGMO_HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    }
+  }
+  return SIGMO_OK;
+}
+
+int sigmo_pipeline_free(sigmo_pipeline* p) {
+  if (!p) return SIGMO_OK;
+  (void)hipSetDevice(p->ctx->device);
+  delete p;
+  return SIGMO_OK;
+}
+
+int sigmo_pipeline_run(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data, uint32_t refinement_steps, int find_first,
+                       sigmo_pipeline_result* result) {
+  SIGMO_TRY(bind(ctx));
+  if (!query || !data || !result) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  SIGMO_TRY(check_queries(query));
+  std::memset(result, 0, sizeof(*result));
+  // setup (outside the timed region, as sigmo.cpp:75-91)
+  sigmo_signatures* sig = nullptr;
+  sigmo_candidates* cand = nullptr;
+  sigmo_gmcr* gm = nullptr;
+  struct Guard {
+    sigmo_signatures*& s;
+    sigmo_candidates*& c;
+    sigmo_gmcr*& g;
+    ~Guard() {
+      sigmo_signatures_free(s);
+      sigmo_candidates_free(c);
+      sigmo_gmcr_free(g);
+    }
+  } guard{sig, cand, gm};
+  SIGMO_TRY(sigmo_signatures_create(ctx, data->N, query->N, &sig));
+  SIGMO_TRY(sigmo_candidates_create(ctx, query->N, data->N, &cand));
+  SIGMO_TRY(sigmo_gmcr_create(ctx, &gm));
+  SIGMO_TRY(sigmo_phase_reset(ctx));
+  hipEvent_t t0, t1;
+  SIGMO_TRY(ctx->event(&t0));
+  SIGMO_TRY(ctx->event(&t1));
+  SIGMO_HIP_TRY(hipEventRecord(t0, ctx->stream));
+  SIGMO_TRY(sigmo_signatures_generate(ctx, sig, data, SIGMO_SCOPE_DATA));
+  SIGMO_TRY(sigmo_signatures_generate(ctx, sig, query, SIGMO_SCOPE_QUERY));
+  SIGMO_TRY(sigmo_candidates_filter(ctx, cand, query, data, sig));
+  for (uint32_t r = 1; r <= refinement_steps; ++r) {
+    SIGMO_TRY(sigmo_signatures_refine(ctx, sig, data, SIGMO_SCOPE_DATA, r));
+    SIGMO_TRY(sigmo_signatures_refine(ctx, sig, query, SIGMO_SCOPE_QUERY, r));
+    SIGMO_TRY(sigmo_candidates_refine(ctx, cand, query, data, sig));
+  }
+  SIGMO_TRY(sigmo_gmcr_generate(ctx, gm, query, data, cand));
+  uint64_t matches = 0;
+  SIGMO_TRY(sigmo_join(ctx, query, data, cand, gm, find_first, &matches, nullptr));
+  SIGMO_HIP_TRY(hipEventRecord(t1, ctx->stream));
+  SIGMO_HIP_TRY(hipEventSynchronize(t1));
+  float total = 0;
+  SIGMO_HIP_TRY(hipEventElapsedTime(&total, t0, t1));
+  ctx->pool.push_back(t0);
+  ctx->pool.push_back(t1);
+  result->num_matches = matches;
+  result->gmcr_pairs = gm->P;
+  result->matched_pairs = ctx->stats_host[1];
+  result->total_ms = total;
+  for (int p = 0; p < SIGMO_PHASE_COUNT; ++p) SIGMO_TRY(sigmo_phase_ms(ctx, p, &result->phase_ms[p]));
+  return SIGMO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,690 @@
+// gfx950 kernels for SIGMo's hot path: signatures, candidate filter/refine, GMCR, join.
+//
+// Wave64 throughout.  The bitset kernels are streaming (one pass over 64-bit word strips,
+// coalesced across lanes) and bound by HBM; the join is a persistent lane-per-pair DFS with
+// wave-aggregated work stealing.  Reference semantics are cited per kernel (paths relative
+// to the SIGMo repository); the integer conventions for the reference's undefined
+// behaviour are: shift counts masked to width-1, Bitset<uint64_t> positions taken mod 64,
+// PowerGraph reachables starting at zero, per-row refine, ascending query ids inside GMCR
+// segments, fresh join state per pair with mapping[] = local node 0.
+#include "kernels.hpp"
+
+namespace sigmo_k {
+namespace {
+
+constexpr int kWave = 64;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+__device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & (kWave - 1)); }
+
+// Largest g with go[g] <= v: the upper-bound search of graph.hpp:100-111.
+__device__ __forceinline__ uint32_t graph_of(const uint32_t* go, uint32_t G, uint32_t v) {
+  uint32_t lo = 0, hi = G;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (v >= go[mid]) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo - 1;
+}
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 20) {
+  uint64_t g = (n + block - 1) / block;
+  if (g > cap) g = cap;
+  if (g == 0) g = 1;
+  return static_cast<unsigned>(g);
+}
+
+// ---- derived graph arrays ------------------------------------------------------------------
+
+// node_graph, 128-bit local adjacency, 8 label bit-planes per 64-node strip and the
+// per-graph 64-position label masks (lmask must be zero on entry).
+__global__ __launch_bounds__(256) void k_derive_nodes(const uint32_t* __restrict__ go, uint32_t G, const uint32_t* __restrict__ ro,
+                                                      const uint32_t* __restrict__ ci, const uint8_t* __restrict__ nl, uint64_t N,
+                                                      uint32_t* __restrict__ node_graph, uint64_t* __restrict__ adj,
+                                                      uint64_t* __restrict__ lplanes, unsigned long long* __restrict__ lmask) {
+  const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  const bool valid = v < N;
+  uint32_t lab = 0;
+  if (valid) {
+    const uint32_t g = graph_of(go, G, static_cast<uint32_t>(v));
+    node_graph[v] = g;
+    const uint32_t prev = go[g];
+    uint64_t lo = 0, hi = 0;
+    for (uint32_t i = ro[v]; i < ro[v + 1]; ++i) {
+      const uint32_t w = ci[i] - prev;  // < 128, checked at upload
+      if (w < 64) lo |= 1ull << w;
+      else hi |= 1ull << (w & 63);
+    }
+    adj[2 * v] = lo;
+    adj[2 * v + 1] = hi;
+    lab = nl[v];
+    const uint32_t local = static_cast<uint32_t>(v) - prev;
+    if (local < 64 && lab < 16) atomicOr(&lmask[static_cast<uint64_t>(g) * 16 + lab], 1ull << local);
+  }
+  uint64_t mine = 0;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const uint64_t m = __ballot((lab >> b) & 1u);
+    if (lane_id() == b) mine = m;
+  }
+  const uint64_t w = v >> 6;
+  if (lane_id() < 8 && (w << 6) < N) lplanes[w * 8 + lane_id()] = mine;
+}
+
+// ---- signatures --------------------------------------------------------------------------
+
+// SignatureDevice::incrementLabelCount(label, 1), signature.hpp:44-51: +1 below 15 only,
+// labels >= 16 ignored.
+__device__ __forceinline__ uint64_t sig_inc(uint64_t sig, uint32_t label) {
+  if (label < 16) {
+    const uint32_t sh = 4 * label;
+    const uint32_t c = static_cast<uint32_t>(sig >> sh) & 15u;
+    if (c < 15) sig = (sig & ~(15ull << sh)) | (static_cast<uint64_t>(c + 1) << sh);
+  }
+  return sig;
+}
+
+// Bit-slice the packed signatures of the block's 64-node strips: plane p (= 4*label+bit)
+// of strip w is ballot(bit p of sig) over the wave that holds the strip.  Staged through LDS
+// so each plane row is written in contiguous runs of (BLOCK/64) words.
+template <int BLOCK>
+__device__ __forceinline__ void write_planes(uint64_t sig, uint64_t W64, uint64_t* __restrict__ planes, uint64_t* tile) {
+  constexpr int WPB = BLOCK / kWave;
+  const int wave = threadIdx.x / kWave, lane = lane_id();
+  uint64_t mine = 0;
+#pragma unroll
+  for (int p = 0; p < 64; ++p) {
+    const uint64_t m = __ballot((sig >> p) & 1ull);
+    if (lane == p) mine = m;
+  }
+  tile[lane * WPB + wave] = mine;
+  __syncthreads();
+  const uint64_t w0 = static_cast<uint64_t>(blockIdx.x) * WPB;
+  const int p = threadIdx.x / WPB, j = threadIdx.x % WPB;
+  if (w0 + j < W64) planes[static_cast<uint64_t>(p) * W64 + w0 + j] = tile[p * WPB + j];
+}
+
+constexpr int kSigBlock = 512;
+
+// generateCSRSignatures, signature.hpp:125-152: one thread per node, neighbour labels
+// incremented onto the existing value.
+__global__ __launch_bounds__(kSigBlock) void k_sig_generate(const uint32_t* __restrict__ ro, const uint32_t* __restrict__ ci,
+                                                            const uint8_t* __restrict__ nl, uint64_t N, uint64_t* __restrict__ sigs,
+                                                            uint64_t* __restrict__ planes) {
+  __shared__ uint64_t tile[64 * (kSigBlock / kWave)];
+  const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  uint64_t s = 0;
+  if (v < N) {
+    s = sigs[v];
+    for (uint32_t i = ro[v]; i < ro[v + 1]; ++i) s = sig_inc(s, nl[ci[i]]);
+    sigs[v] = s;
+  }
+  if (planes) write_planes<kSigBlock>(s, (N + 63) / 64, planes, tile);
+}
+
+// refineCSRSignatures<PowerGraph>(view_size), signature.hpp:336-390.  The reference's
+// Bitset<uint64_t> holds local ids mod 64 and expands bit b as node go[g]+b, so the BFS
+// runs over the folded adjacency adj_lo|adj_hi of node go[g]+b.  The level signature is the
+// saturated label histogram of ball(view_size) minus the start bit minus the old
+// reachables; the old reachables then absorb it.
+__global__ __launch_bounds__(kSigBlock) void k_sig_refine(GraphDev g, uint32_t view_size, uint64_t* __restrict__ sigs,
+                                                          uint64_t* __restrict__ reach, uint64_t* __restrict__ planes) {
+  __shared__ uint64_t tile[64 * (kSigBlock / kWave)];
+  const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  uint64_t s = 0;
+  if (v < g.N) {
+    const uint32_t gid = g.node_graph[v];
+    const uint32_t prev = g.go[gid];
+    const uint64_t self = 1ull << ((static_cast<uint32_t>(v) - prev) & 63u);
+    uint64_t ball = self, frontier = self;
+    for (uint32_t it = 0; it < view_size && frontier; ++it) {
+      uint64_t next = 0, f = frontier;
+      while (f) {
+        const uint32_t b = __builtin_ctzll(f);
+        f &= f - 1;
+        const uint64_t u = static_cast<uint64_t>(prev) + b;
+        next |= g.adj[2 * u] | g.adj[2 * u + 1];
+      }
+      next &= ~ball;
+      ball |= next;
+      frontier = next;
+    }
+    const uint64_t old = reach[v];
+    const uint64_t r = ball & ~self & ~old;
+    const uint64_t* lm = g.lmask + static_cast<uint64_t>(gid) * 16;
+#pragma unroll
+    for (int l = 0; l < 16; ++l) {
+      uint32_t c = __popcll(r & lm[l]);
+      c = c < 15 ? c : 15;
+      s |= static_cast<uint64_t>(c) << (4 * l);
+    }
+    sigs[v] = s;
+    reach[v] = r | old;
+  }
+  if (planes) write_planes<kSigBlock>(s, (g.N + 63) / 64, planes, tile);
+}
+
+// ---- candidate filter / refine -----------------------------------------------------------
+
+constexpr int kBitsBlock = 256;
+
+inline unsigned rows_per_block(uint64_t Q, uint64_t col_blocks, int num_cus) {
+  const uint64_t target_blocks = static_cast<uint64_t>(num_cus > 0 ? num_cus : 256) * 16;
+  uint64_t chunks = (target_blocks + col_blocks - 1) / col_blocks;
+  if (chunks < 1) chunks = 1;
+  uint64_t rpb = (Q + chunks - 1) / chunks;
+  if (rpb < 32) rpb = 32;
+  if (rpb > Q) rpb = Q;
+  return static_cast<unsigned>(rpb > 0 ? rpb : 1);
+}
+
+// filterCandidates<Query>, isomorphism.hpp:22-62: bit (q,d) <=> label(q) == label(d) or
+// label(q) == 255.  Thread = one 64-bit word of a row; the strip's 8 label bit-planes stay in
+// registers across the block's rows; every word is written once (no atomics).
+__global__ __launch_bounds__(kBitsBlock) void k_filter(const uint8_t* __restrict__ qlabels, uint64_t Q, const uint64_t* __restrict__ lplanes,
+                                                       uint64_t D, uint64_t* __restrict__ bits, unsigned rpb) {
+  const uint64_t W64 = (D + 63) / 64;
+  const uint64_t w = blockIdx.x * static_cast<uint64_t>(kBitsBlock) + threadIdx.x;
+  if (w >= W64) return;
+  uint64_t P[8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) P[b] = lplanes[w * 8 + b];
+  const uint64_t valid = (w == W64 - 1 && (D & 63)) ? (~0ull >> (64 - (D & 63))) : ~0ull;
+  const uint64_t q0 = static_cast<uint64_t>(blockIdx.y) * rpb;
+  const uint64_t q1 = q0 + rpb < Q ? q0 + rpb : Q;
+  for (uint64_t q = q0; q < q1; ++q) {
+    const uint32_t ql = qlabels[q];
+    uint64_t m = valid;
+    if (ql != 255u) {
+#pragma unroll
+      for (int b = 0; b < 8; ++b) m &= ((ql >> b) & 1u) ? P[b] : ~P[b];
+    }
+    bits[q * W64 + w] = m;
+  }
+}
+
+// x >= c for 64 four-bit values held as bit planes x[0..3] and a wave-uniform c.
+__device__ __forceinline__ uint64_t ge4(const uint64_t x[4], uint32_t c) {
+  uint64_t gt = 0, eq = ~0ull;
+#pragma unroll
+  for (int b = 3; b >= 0; --b) {
+    if ((c >> b) & 1u) {
+      eq &= x[b];
+    } else {
+      gt |= eq & x[b];
+      eq &= ~x[b];
+    }
+  }
+  return gt | eq;
+}
+
+// refineCandidates<Query>, isomorphism.hpp:64-127: a set bit (q,d) survives iff for every
+// label l < 16, count_q(l) <= count_d(l) at the current level.  Only labels that occur as
+// query node labels can have nonzero query counts; their data planes (4 per label) sit in
+// registers, and each query row costs one bit-sliced >= per nonzero query count.
+template <int U>
+__global__ __launch_bounds__(kBitsBlock) void k_refine(const uint64_t* __restrict__ qsig, uint64_t Q, const uint64_t* __restrict__ planes,
+                                                       uint64_t D, LabelSlots slots, uint64_t* __restrict__ bits, unsigned rpb) {
+  const uint64_t W64 = (D + 63) / 64;
+  const uint64_t w = blockIdx.x * static_cast<uint64_t>(kBitsBlock) + threadIdx.x;
+  if (w >= W64) return;
+  uint64_t P[U][4];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) P[u][b] = planes[(static_cast<uint64_t>(slots.label[u]) * 4 + b) * W64 + w];
+  const uint64_t q0 = static_cast<uint64_t>(blockIdx.y) * rpb;
+  const uint64_t q1 = q0 + rpb < Q ? q0 + rpb : Q;
+  for (uint64_t q = q0; q < q1; ++q) {
+    const uint64_t old = bits[q * W64 + w];
+    const uint64_t qs = qsig[q];
+    uint64_t keep = ~0ull;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t c = static_cast<uint32_t>(qs >> (4 * slots.label[u])) & 15u;
+      if (c) keep &= ge4(P[u], c);
+    }
+    const uint64_t nw = old & keep;
+    if (nw != old) bits[q * W64 + w] = nw;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_row_counts(const uint64_t* __restrict__ bits, uint64_t W64, uint32_t* __restrict__ counts) {
+  __shared__ uint32_t part[256 / kWave];
+  const uint64_t q = blockIdx.x;
+  uint32_t c = 0;
+  for (uint64_t w = threadIdx.x; w < W64; w += blockDim.x) c += __popcll(bits[q * W64 + w]);
+  for (int off = kWave / 2; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if (lane_id() == 0) part[threadIdx.x / kWave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int i = 0; i < 256 / kWave; ++i) t += part[i];
+    counts[q] = t;
+  }
+}
+
+// ---- GMCR ----------------------------------------------------------------------------------
+
+// Any set bit of a bitset row inside the global node range [s, e) (e > s).
+__device__ __forceinline__ bool row_any(const uint64_t* __restrict__ row, uint32_t s, uint32_t e) {
+  const uint32_t ws = s >> 6, we = (e - 1) >> 6;
+  uint64_t any = 0;
+  for (uint32_t w = ws; w <= we; ++w) {
+    uint64_t x = row[w];
+    if (w == ws) x &= ~0ull << (s & 63);
+    if (w == we) x &= ~0ull >> (63 - ((e - 1) & 63));
+    any |= x;
+  }
+  return any != 0;
+}
+
+// gmcr.hpp:47-68 / :94-115 predicate for (query graph = blockIdx.y, data graph = lane):
+// every query node has a candidate inside the molecule.  One ballot word per 64 molecules.
+__global__ __launch_bounds__(256) void k_gmcr_mark(GraphDev q, GraphDev d, const uint64_t* __restrict__ bits, uint64_t W64,
+                                                   uint64_t* __restrict__ pair_bits) {
+  const uint32_t qg = blockIdx.y;
+  const uint64_t dg = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  const uint64_t strips = (static_cast<uint64_t>(d.G) + 63) / 64;
+  bool alive = dg < d.G;
+  uint32_t s = 0, e = 0;
+  if (alive) {
+    s = d.go[dg];
+    e = d.go[dg + 1];
+  }
+  const uint32_t qoff = q.go[qg], k = q.go[qg + 1] - qoff;
+  for (uint32_t i = 0; i < k; ++i) {
+    if (__ballot(alive) == 0) break;
+    if (alive) alive = row_any(bits + static_cast<uint64_t>(qoff + i) * W64, s, e);
+  }
+  const uint64_t res = __ballot(alive);
+  if (lane_id() == 0 && (dg >> 6) < strips) pair_bits[static_cast<uint64_t>(qg) * strips + (dg >> 6)] = res;
+}
+
+__global__ __launch_bounds__(256) void k_gmcr_count(const uint64_t* __restrict__ pair_bits, uint32_t Gq, uint32_t Gd,
+                                                    uint32_t* __restrict__ counts) {
+  const uint64_t dg = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (dg >= Gd) return;
+  const uint64_t strips = (static_cast<uint64_t>(Gd) + 63) / 64;
+  const uint64_t strip = dg >> 6;
+  const uint32_t bit = dg & 63;
+  uint32_t c = 0;
+  for (uint32_t qg = 0; qg < Gq; ++qg) c += (pair_bits[qg * strips + strip] >> bit) & 1ull;
+  counts[dg] = c;
+}
+
+// Kernel 3 of gmcr.hpp (:94-115) with a deterministic order: each molecule's segment lists its
+// query graphs in ascending id.  Also records the molecule of each pair for the join.
+__global__ __launch_bounds__(256) void k_gmcr_scatter(const uint64_t* __restrict__ pair_bits, uint32_t Gq, uint32_t Gd,
+                                                      const uint32_t* __restrict__ offsets, uint32_t* __restrict__ indices,
+                                                      uint32_t* __restrict__ pair_graph) {
+  const uint64_t dg = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (dg >= Gd) return;
+  const uint64_t strips = (static_cast<uint64_t>(Gd) + 63) / 64;
+  const uint64_t strip = dg >> 6;
+  const uint32_t bit = dg & 63;
+  uint32_t pos = offsets[dg];
+  for (uint32_t qg = 0; qg < Gq; ++qg)
+    if ((pair_bits[qg * strips + strip] >> bit) & 1ull) {
+      indices[pos] = qg;
+      pair_graph[pos] = static_cast<uint32_t>(dg);
+      ++pos;
+    }
+}
+
+// ---- exclusive scan (u32 counts, u64 carries) -------------------------------------------------
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ uint64_t wave_inclusive(uint64_t x) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint64_t y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  return x;
+}
+
+// Exclusive scan of one value per thread across the block; returns the block total.
+__device__ __forceinline__ uint64_t block_exclusive(uint64_t x, uint64_t& excl, uint64_t* wave_tot) {
+  const int wave = threadIdx.x / kWave, lane = lane_id();
+  const uint64_t inc = wave_inclusive(x);
+  if (lane == kWave - 1) wave_tot[wave] = inc;
+  __syncthreads();
+  uint64_t base = 0, total = 0;
+  for (int i = 0; i < kScanBlock / kWave; ++i) {
+    if (i < wave) base += wave_tot[i];
+    total += wave_tot[i];
+  }
+  excl = base + inc - x;
+  __syncthreads();
+  return total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t* __restrict__ in, uint64_t n, uint64_t* __restrict__ block_sums) {
+  __shared__ uint64_t wave_tot[kScanBlock / kWave];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+  uint64_t x = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < n) x += in[base + i];
+  uint64_t excl;
+  const uint64_t total = block_exclusive(x, excl, wave_tot);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+// Single block: exclusive scan of the block sums in place, total to total_dev.
+__global__ __launch_bounds__(kScanBlock) void k_scan_top(uint64_t* __restrict__ block_sums, uint64_t nb, uint64_t* __restrict__ total_dev) {
+  __shared__ uint64_t wave_tot[kScanBlock / kWave];
+  uint64_t carry = 0;
+  for (uint64_t base = 0; base < nb; base += kScanBlock) {
+    const uint64_t i = base + threadIdx.x;
+    const uint64_t x = i < nb ? block_sums[i] : 0;
+    uint64_t excl;
+    const uint64_t total = block_exclusive(x, excl, wave_tot);
+    if (i < nb) block_sums[i] = carry + excl;
+    carry += total;
+  }
+  if (threadIdx.x == 0) *total_dev = carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_final(const uint32_t* __restrict__ in, uint64_t n, const uint64_t* __restrict__ block_sums,
+                                                           const uint64_t* __restrict__ total_dev, uint32_t* __restrict__ out) {
+  __shared__ uint64_t wave_tot[kScanBlock / kWave];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint64_t x = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    v[i] = base + i < n ? in[base + i] : 0u;
+    x += v[i];
+  }
+  uint64_t excl;
+  block_exclusive(x, excl, wave_tot);
+  uint64_t run = block_sums[blockIdx.x] + excl;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    if (base + i < n) out[base + i] = static_cast<uint32_t>(run);
+    run += v[i];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = static_cast<uint32_t>(*total_dev);
+}
+
+// ---- join ------------------------------------------------------------------------------------
+
+constexpr int kJoinBlock = 256;
+constexpr int kMaxDepth = 32;
+
+// Next candidate of a query node at local position >= pos inside the molecule [s, e):
+// ascending order of candidates.hpp:132-162, over at most three 64-bit words (n <= 128).
+__device__ __forceinline__ uint32_t next_candidate(const uint64_t* __restrict__ row, uint32_t s, uint32_t e, uint32_t pos) {
+  const uint32_t g = s + pos;
+  if (g >= e) return kNone;
+  uint32_t w = g >> 6;
+  const uint32_t last = (e - 1) >> 6;
+  uint64_t word = row[w] & (~0ull << (g & 63));
+  while (true) {
+    if (w == last) word &= ~0ull >> (63 - ((e - 1) & 63));
+    if (word) return (w << 6) + static_cast<uint32_t>(__builtin_ctzll(word)) - s;
+    if (w == last) return kNone;
+    ++w;
+    word = row[w];
+  }
+}
+
+// joinCandidates, isomorphism.hpp:270-381.  Each lane runs one (query, molecule) pair's DFS
+// in the reference's order and bookkeeping; finished lanes take the next pair from a global
+// cursor (one atomic per wave per refill).  The reference frame stack collapses to
+// mapping[] (LDS, one byte per depth, local ids): a frame below the top resumes at
+// mapping[depth] + 1, because its last tried candidate is the one it accepted.
+//   * leaf (depth == k): count, pop; find-first ends the pair (:336-344);
+//   * exhausted frame: unset visited bit of mapping[depth] (possibly stale), pop, clear all
+//     of visited when only the root frame remains (:346-354);
+//   * candidate c: skip if visited bit (c mod 64) is set (:363); accept if depth == 0 or
+//     for all i < depth, qAdj(i, depth) == dAdj(mapping[i], c) (:366, :150-168).
+template <bool FIND_FIRST>
+__global__ __launch_bounds__(kJoinBlock) void k_join(GraphDev q, GraphDev d, const uint64_t* __restrict__ bits, uint64_t W64,
+                                                     const uint32_t* __restrict__ indices, const uint32_t* __restrict__ pair_graph,
+                                                     uint64_t P, int32_t* __restrict__ counts, unsigned long long* __restrict__ stats) {
+  __shared__ uint8_t mapping[kMaxDepth][kJoinBlock];
+  const int tid = threadIdx.x, lane = lane_id();
+  bool active = false;
+  bool exhausted = false;  // wave-uniform
+  uint64_t j = 0;
+  uint32_t s = 0, e = 0, qoff = 0, k = 0, depth = 0, pos = 0;
+  uint64_t visited = 0;
+  uint64_t count = 0;
+  uint64_t sum = 0, matched = 0;
+  unsigned long long overflow = 0;
+  while (true) {
+    const uint64_t need = __ballot(!active);
+    if (need && !exhausted) {
+      const int leader = __ffsll(static_cast<long long>(need)) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(&stats[2], static_cast<unsigned long long>(__popcll(need)));
+      base = __shfl(base, leader);
+      if (!active) {
+        const uint64_t jj = base + __popcll(need & ((1ull << lane) - 1));
+        if (jj < P) {
+          j = jj;
+          const uint32_t dg = pair_graph[j], qg = indices[j];
+          s = d.go[dg];
+          e = d.go[dg + 1];
+          qoff = q.go[qg];
+          k = q.go[qg + 1] - qoff;
+          for (uint32_t i = 0; i < k; ++i) mapping[i][tid] = 0;
+          depth = 0;
+          pos = 0;
+          visited = 0;
+          count = 0;
+          active = true;
+        }
+      }
+      if (base + __popcll(need) >= P) exhausted = true;
+    }
+    if (__ballot(active) == 0) break;
+    if (!active) continue;
+    bool done = false;
+    if (depth == k) {
+      ++count;
+      if (FIND_FIRST) {
+        done = true;
+      } else {
+        depth = k - 1;
+        pos = static_cast<uint32_t>(mapping[depth][tid]) + 1;
+      }
+    } else {
+      const uint32_t c = next_candidate(bits + static_cast<uint64_t>(qoff + depth) * W64, s, e, pos);
+      if (c == kNone) {
+        visited &= ~(1ull << (mapping[depth][tid] & 63));
+        if (depth == 0) {
+          done = true;
+        } else {
+          if (depth == 1) visited = 0;
+          --depth;
+          pos = static_cast<uint32_t>(mapping[depth][tid]) + 1;
+        }
+      } else {
+        pos = c + 1;
+        if (!((visited >> (c & 63)) & 1ull)) {
+          bool ok = true;
+          if (depth > 0) {
+            const uint64_t node = static_cast<uint64_t>(s) + c;
+            const uint64_t a0 = d.adj[2 * node], a1 = d.adj[2 * node + 1];
+            const uint32_t qa = static_cast<uint32_t>(q.adj[2 * static_cast<uint64_t>(qoff + depth)]);
+            for (uint32_t i = 0; i < depth && ok; ++i) {
+              const uint32_t m = mapping[i][tid];
+              const uint32_t dbit = static_cast<uint32_t>(((m < 64 ? a0 : a1) >> (m & 63)) & 1ull);
+              ok = dbit == ((qa >> i) & 1u);
+            }
+          }
+          if (ok) {
+            mapping[depth][tid] = static_cast<uint8_t>(c);
+            visited |= 1ull << (c & 63);
+            ++depth;
+            pos = 0;
+          }
+        }
+      }
+    }
+    if (done) {
+      if (count > 0x7FFFFFFFull) overflow = 1;
+      counts[j] = static_cast<int32_t>(count > 0x7FFFFFFFull ? 0x7FFFFFFFull : count);
+      sum += count;
+      matched += count > 0;
+      active = false;
+    }
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    sum += __shfl_xor(sum, off);
+    matched += __shfl_xor(matched, off);
+    overflow |= __shfl_xor(overflow, off);
+  }
+  if (lane == 0) {
+    if (sum) atomicAdd(&stats[0], static_cast<unsigned long long>(sum));
+    if (matched) atomicAdd(&stats[1], static_cast<unsigned long long>(matched));
+    if (overflow) atomicOr(&stats[3], 1ull);
+  }
+}
+
+}  // namespace
+
+// ---- launchers -------------------------------------------------------------------------------
+
+hipError_t fill_u64(hipStream_t s, uint64_t* p, uint64_t n, uint64_t v) {
+  if (!n) return hipSuccess;
+  k_fill_u64<<<grid_for(n, 256, 16384), 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+hipError_t fill_u32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v) {
+  if (!n) return hipSuccess;
+  k_fill_u32<<<grid_for(n, 256, 16384), 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t derive_graph(hipStream_t s, const uint32_t* go, uint32_t G, const uint32_t* ro, const uint32_t* ci, const uint8_t* nl,
+                        uint64_t N, uint32_t* node_graph, uint64_t* adj, uint64_t* lplanes, uint64_t* lmask) {
+  hipError_t err = fill_u64(s, lmask, static_cast<uint64_t>(G) * 16, 0);
+  if (err != hipSuccess || !N) return err;
+  const unsigned blocks = static_cast<unsigned>((N + 255) / 256);
+  k_derive_nodes<<<blocks, 256, 0, s>>>(go, G, ro, ci, nl, N, node_graph, adj, lplanes,
+                                        reinterpret_cast<unsigned long long*>(lmask));
+  return hipGetLastError();
+}
+
+hipError_t sig_generate(hipStream_t s, const GraphDev& g, uint64_t* sigs, uint64_t* planes) {
+  if (!g.N) return hipSuccess;
+  const unsigned blocks = static_cast<unsigned>((g.N + kSigBlock - 1) / kSigBlock);
+  k_sig_generate<<<blocks, kSigBlock, 0, s>>>(g.ro, g.ci, g.nl, g.N, sigs, planes);
+  return hipGetLastError();
+}
+
+hipError_t sig_refine(hipStream_t s, const GraphDev& g, uint32_t view_size, uint64_t* sigs, uint64_t* reach, uint64_t* planes) {
+  if (!g.N) return hipSuccess;
+  const unsigned blocks = static_cast<unsigned>((g.N + kSigBlock - 1) / kSigBlock);
+  k_sig_refine<<<blocks, kSigBlock, 0, s>>>(g, view_size, sigs, reach, planes);
+  return hipGetLastError();
+}
+
+hipError_t filter(hipStream_t s, const uint8_t* qlabels, uint64_t Q, const uint64_t* lplanes, uint64_t D, uint64_t* bits, int num_cus) {
+  const uint64_t W64 = (D + 63) / 64;
+  if (!Q || !W64) return hipSuccess;
+  const uint64_t cols = (W64 + kBitsBlock - 1) / kBitsBlock;
+  const unsigned rpb = rows_per_block(Q, cols, num_cus);
+  const dim3 grid(static_cast<unsigned>(cols), static_cast<unsigned>((Q + rpb - 1) / rpb));
+  k_filter<<<grid, kBitsBlock, 0, s>>>(qlabels, Q, lplanes, D, bits, rpb);
+  return hipGetLastError();
+}
+
+template <int U>
+static hipError_t launch_refine(hipStream_t s, dim3 grid, const uint64_t* qsig, uint64_t Q, const uint64_t* planes, uint64_t D,
+                                const LabelSlots& slots, uint64_t* bits, unsigned rpb) {
+  k_refine<U><<<grid, kBitsBlock, 0, s>>>(qsig, Q, planes, D, slots, bits, rpb);
+  return hipGetLastError();
+}
+
+hipError_t refine(hipStream_t s, const uint64_t* qsig, uint64_t Q, const uint64_t* planes, uint64_t D, const LabelSlots& slots,
+                  uint64_t* bits, int num_cus) {
+  const uint64_t W64 = (D + 63) / 64;
+  if (!Q || !W64 || slots.count == 0) return hipSuccess;
+  const uint64_t cols = (W64 + kBitsBlock - 1) / kBitsBlock;
+  const unsigned rpb = rows_per_block(Q, cols, num_cus);
+  const dim3 grid(static_cast<unsigned>(cols), static_cast<unsigned>((Q + rpb - 1) / rpb));
+  switch (slots.count) {
+#define SIGMO_REFINE_CASE(u) \
+  case u: return launch_refine<u>(s, grid, qsig, Q, planes, D, slots, bits, rpb);
+    SIGMO_REFINE_CASE(1) SIGMO_REFINE_CASE(2) SIGMO_REFINE_CASE(3) SIGMO_REFINE_CASE(4)
+    SIGMO_REFINE_CASE(5) SIGMO_REFINE_CASE(6) SIGMO_REFINE_CASE(7) SIGMO_REFINE_CASE(8)
+    SIGMO_REFINE_CASE(9) SIGMO_REFINE_CASE(10) SIGMO_REFINE_CASE(11) SIGMO_REFINE_CASE(12)
+    SIGMO_REFINE_CASE(13) SIGMO_REFINE_CASE(14) SIGMO_REFINE_CASE(15) SIGMO_REFINE_CASE(16)
+#undef SIGMO_REFINE_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t row_counts(hipStream_t s, const uint64_t* bits, uint64_t Q, uint64_t W64, uint32_t* counts) {
+  if (!Q) return hipSuccess;
+  k_row_counts<<<static_cast<unsigned>(Q), 256, 0, s>>>(bits, W64, counts);
+  return hipGetLastError();
+}
+
+hipError_t gmcr_mark(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, uint64_t* pair_bits) {
+  if (!q.G || !d.G) return hipSuccess;
+  const dim3 grid(static_cast<unsigned>((d.G + 255) / 256), q.G);
+  k_gmcr_mark<<<grid, 256, 0, s>>>(q, d, bits, W64, pair_bits);
+  return hipGetLastError();
+}
+
+hipError_t gmcr_count(hipStream_t s, const uint64_t* pair_bits, uint32_t Gq, uint32_t Gd, uint32_t* counts) {
+  if (!Gd) return hipSuccess;
+  k_gmcr_count<<<(Gd + 255) / 256, 256, 0, s>>>(pair_bits, Gq, Gd, counts);
+  return hipGetLastError();
+}
+
+hipError_t exclusive_scan(hipStream_t s, const uint32_t* counts, uint64_t n, uint32_t* offsets, uint64_t* block_tmp, uint64_t* total_dev) {
+  const uint64_t nb = (n + kScanTile - 1) / kScanTile;
+  if (nb == 0) {
+    k_fill_u32<<<1, 64, 0, s>>>(offsets, 1, 0);
+    k_fill_u64<<<1, 64, 0, s>>>(total_dev, 1, 0);
+    return hipGetLastError();
+  }
+  k_scan_reduce<<<static_cast<unsigned>(nb), kScanBlock, 0, s>>>(counts, n, block_tmp);
+  k_scan_top<<<1, kScanBlock, 0, s>>>(block_tmp, nb, total_dev);
+  k_scan_final<<<static_cast<unsigned>(nb), kScanBlock, 0, s>>>(counts, n, block_tmp, total_dev, offsets);
+  return hipGetLastError();
+}
+
+hipError_t gmcr_scatter(hipStream_t s, const uint64_t* pair_bits, uint32_t Gq, uint32_t Gd, const uint32_t* offsets, uint32_t* indices,
+                        uint32_t* pair_graph) {
+  if (!Gd) return hipSuccess;
+  k_gmcr_scatter<<<(Gd + 255) / 256, 256, 0, s>>>(pair_bits, Gq, Gd, offsets, indices, pair_graph);
+  return hipGetLastError();
+}
+
+hipError_t join(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
+                const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts, unsigned long long* stats, int num_cus) {
+  if (!P) return hipSuccess;
+  uint64_t blocks = (P + kJoinBlock - 1) / kJoinBlock;
+  const uint64_t cap = static_cast<uint64_t>(num_cus > 0 ? num_cus : 256) * 8;
+  if (blocks > cap) blocks = cap;
+  if (find_first)
+    k_join<true><<<static_cast<unsigned>(blocks), kJoinBlock, 0, s>>>(q, d, bits, W64, indices, pair_graph, P, counts, stats);
+  else
+    k_join<false><<<static_cast<unsigned>(blocks), kJoinBlock, 0, s>>>(q, d, bits, W64, indices, pair_graph, P, counts, stats);
+  return hipGetLastError();
+}
+
+}  // namespace sigmo_k

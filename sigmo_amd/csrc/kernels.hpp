@@ -1,0 +1,81 @@
+// Launch interface of the gfx950 kernels (sigmo_amd/csrc/kernels.hip).
+//
+// Device data layout (all arrays in HBM, 64-bit words little-endian):
+//   candidates   bits[q * W64 + w], W64 = ceil(D/64); bit j of word w <=> data node 64w+j is
+//                a candidate of query node q.  Byte-identical to the reference's uint32
+//                rows (candidates.hpp:18-36) whenever ceil(D/32) is even.
+//   adjacency    adj[2v], adj[2v+1]: bits of v's neighbours by local id (graph-relative,
+//                < 128), low and high half.
+//   label planes lplanes[w * 8 + b]: bit j = bit b of node_labels[64w + j].
+//   label masks  lmask[g * 16 + l]: bit b = (node_labels[go[g] + b] == l), b < min(n, 64),
+//                the 64-position view the reference's Bitset<uint64_t> BFS reads labels
+//                through (signature.hpp:378-383).
+//   sig planes   planes[p * W64 + w] (p = 4*label + bit): bit-sliced packed signatures.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sigmo_k {
+
+struct GraphDev {
+  const uint32_t* go;  // graph_offsets [G+1]
+  const uint32_t* ro;  // row_offsets   [N+1]
+  const uint32_t* ci;  // column_indices[E]
+  const uint8_t* nl;   // node_labels   [N]
+  const uint32_t* node_graph;  // [N]
+  const uint64_t* adj;         // [2N]
+  const uint64_t* lplanes;     // [8 * ceil(N/64)]
+  const uint64_t* lmask;       // [16 G]
+  uint32_t G;
+  uint64_t N;
+};
+
+// Labels < 16 that may carry a nonzero signature count, compacted into slots.
+struct LabelSlots {
+  uint8_t label[16];
+  int count;
+};
+
+hipError_t fill_u64(hipStream_t s, uint64_t* p, uint64_t n, uint64_t v);
+hipError_t fill_u32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v);
+
+// Derived per-node / per-graph arrays of an uploaded batch.
+hipError_t derive_graph(hipStream_t s, const uint32_t* go, uint32_t G, const uint32_t* ro, const uint32_t* ci, const uint8_t* nl,
+                        uint64_t N, uint32_t* node_graph, uint64_t* adj, uint64_t* lplanes, uint64_t* lmask);
+
+// generateCSRSignatures: increments the neighbour-label histogram onto sigs.
+hipError_t sig_generate(hipStream_t s, const GraphDev& g, uint64_t* sigs, uint64_t* planes);
+// refineCSRSignatures<PowerGraph>(view_size).
+hipError_t sig_refine(hipStream_t s, const GraphDev& g, uint32_t view_size, uint64_t* sigs, uint64_t* reach, uint64_t* planes);
+
+// filterCandidates<Query>: overwrites every word of the Q x W64 bitset.
+hipError_t filter(hipStream_t s, const uint8_t* qlabels, uint64_t Q, const uint64_t* lplanes, uint64_t D, uint64_t* bits,
+                  int num_cus);
+// refineCandidates<Query> against bit-sliced data planes.
+hipError_t refine(hipStream_t s, const uint64_t* qsig, uint64_t Q, const uint64_t* planes, uint64_t D, const LabelSlots& slots,
+                  uint64_t* bits, int num_cus);
+// Per-row popcounts.
+hipError_t row_counts(hipStream_t s, const uint64_t* bits, uint64_t Q, uint64_t W64, uint32_t* counts);
+// Copy into the reference layout (uint32 words, stride ceil(D/32)).
+hipError_t to_reference_layout(hipStream_t s, const uint64_t* bits, uint64_t Q, uint64_t D, uint32_t* out);
+
+// GMCR.  pair_bits[qg * ceil(Gd/64) + strip]; counts[Gd]; offsets[Gd+1]; indices[P].
+hipError_t gmcr_mark(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64,
+                     uint64_t* pair_bits);
+hipError_t gmcr_count(hipStream_t s, const uint64_t* pair_bits, uint32_t Gq, uint32_t Gd, uint32_t* counts);
+// Exclusive scan of counts[0..n) into offsets[0..n]; offsets[n] = total; block_tmp holds
+// ceil(n/1024)+1 uint64 words.
+hipError_t exclusive_scan(hipStream_t s, const uint32_t* counts, uint64_t n, uint32_t* offsets, uint64_t* block_tmp,
+                          uint64_t* total_dev);
+hipError_t gmcr_scatter(hipStream_t s, const uint64_t* pair_bits, uint32_t Gq, uint32_t Gd, const uint32_t* offsets,
+                        uint32_t* indices, uint32_t* pair_graph);
+
+// Join.  stats[0] += sum of counts, stats[1] += pairs with count > 0, stats[2] = work cursor
+// (zeroed by the caller), stats[3] |= 1 on int32 overflow.
+hipError_t join(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64,
+                const uint32_t* indices, const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts,
+                unsigned long long* stats, int num_cus);
+
+}  // namespace sigmo_k
