@@ -1,0 +1,429 @@
+// sigmo.hpp -- C++ API with the reference's namespaces, class and function names, over the
+// C ABI of sigmo_hip.h (gfx950 kernels).  A reference driver such as library/src/sigmo.cpp
+// compiles against it after replacing `sycl::queue` with `sigmo::Queue` (paths relative to
+// the SIGMo repository).  Errors surface as std::runtime_error, as in the reference.
+//
+// Reference headers mirrored:
+//   types.hpp:14-28            sigmo::types
+//   device.hpp:14-21           sigmo::CandidatesDomain, sigmo::device::deviceOptions
+//   utils.hpp:127-161          sigmo::utils::BatchedEvent
+//   graph.hpp:37-112,255-432   sigmo::CSRGraph, DeviceBatchedCSRGraph, createDeviceCSRGraph, ...
+//   io.hpp:37-55               sigmo::io::loadCSRGraphsFromFile / FromLines
+//   signature.hpp:17-212       sigmo::signature::Signature<PowerGraph, 4>
+//   candidates.hpp:14-204      sigmo::candidates::Candidates
+//   isomorphism.hpp:22-127     sigmo::isomorphism::filter::{filterCandidates, refineCandidates}
+//   gmcr.hpp:18-135            sigmo::isomorphism::mapping::GMCR
+//   isomorphism.hpp:270-381    sigmo::isomorphism::join::joinCandidates
+#pragma once
+
+#include <chrono>
+#include <cstdint>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sigmo_hip.h"
+
+namespace sigmo {
+
+namespace types {
+using row_offset_t = uint32_t;
+using col_index_t = uint32_t;
+using label_t = uint8_t;
+using node_t = uint32_t;
+using candidates_t = uint32_t;
+constexpr label_t WILDCARD_NODE = 255;
+constexpr node_t NULL_NODE = 0xFFFFFFFFu;
+}  // namespace types
+
+enum class CandidatesDomain { Query, Data };
+
+namespace detail {
+inline void check(int status) {
+  if (status != SIGMO_OK) throw std::runtime_error(sigmo_last_error());
+}
+}  // namespace detail
+
+// The device + stream every reference class holds a sycl::queue for.
+class Queue {
+ public:
+  explicit Queue(int device = 0) { detail::check(sigmo_ctx_create(device, &ctx_)); }
+  ~Queue() { sigmo_ctx_destroy(ctx_); }
+  Queue(const Queue&) = delete;
+  Queue& operator=(const Queue&) = delete;
+  sigmo_ctx* get() const { return ctx_; }
+  void wait() { detail::check(sigmo_ctx_sync(ctx_)); }
+  void wait_and_throw() { wait(); }
+  std::string deviceName() const {
+    char buf[256];
+    detail::check(sigmo_device_info(ctx_, buf, sizeof(buf), nullptr));
+    return buf;
+  }
+  uint64_t globalMemSize() const {
+    uint64_t m = 0;
+    detail::check(sigmo_device_info(ctx_, nullptr, 0, &m));
+    return m;
+  }
+
+ private:
+  sigmo_ctx* ctx_ = nullptr;
+};
+
+namespace device {
+// Accepted for CLI compatibility (device.hpp:17-21); gfx950 launch shapes are chosen by the
+// library.
+struct DeviceOptions {
+  size_t join_work_group_size = 128;
+  size_t filter_work_group_size = 512;
+};
+inline DeviceOptions deviceOptions;
+}  // namespace device
+
+namespace utils {
+// The kernels launched by one API call; getProfilingInfo sums their device durations.
+class BatchedEvent {
+ public:
+  BatchedEvent() = default;
+  BatchedEvent(sigmo_ctx* ctx, uint64_t from, uint64_t to) : ctx_(ctx), from_(from), to_(to) {}
+  void wait() {
+    if (ctx_) detail::check(sigmo_ctx_sync(ctx_));
+  }
+  size_t numEvents() const { return static_cast<size_t>(to_ - from_); }
+  std::chrono::duration<double> getProfilingInfo() {
+    double ms = 0;
+    if (ctx_) detail::check(sigmo_event_elapsed(ctx_, from_, to_, &ms));
+    return std::chrono::duration<double>(ms / 1e3);
+  }
+
+ private:
+  sigmo_ctx* ctx_ = nullptr;
+  uint64_t from_ = 0, to_ = 0;
+};
+
+// Records the launch range of one call.
+class EventScope {
+ public:
+  explicit EventScope(sigmo_ctx* ctx) : ctx_(ctx) { detail::check(sigmo_event_mark(ctx_, &from_)); }
+  BatchedEvent done() {
+    uint64_t to = 0;
+    detail::check(sigmo_event_mark(ctx_, &to));
+    return BatchedEvent(ctx_, from_, to);
+  }
+
+ private:
+  sigmo_ctx* ctx_;
+  uint64_t from_ = 0;
+};
+}  // namespace utils
+
+// Host graph (graph.hpp:37-60).
+class CSRGraph {
+ public:
+  CSRGraph(std::vector<types::row_offset_t> row_offsets, std::vector<types::col_index_t> column_indices,
+           std::vector<types::label_t> node_labels, std::vector<types::label_t> edge_labels, size_t num_nodes)
+      : row_offsets_(std::move(row_offsets)),
+        column_indices_(std::move(column_indices)),
+        node_labels_(std::move(node_labels)),
+        edge_labels_(std::move(edge_labels)),
+        num_nodes_(num_nodes) {}
+  const types::row_offset_t* getRowOffsets() const { return row_offsets_.data(); }
+  const types::col_index_t* getColumnIndices() const { return column_indices_.data(); }
+  const types::label_t* getNodeLabels() const { return node_labels_.data(); }
+  const types::label_t* getEdgeLabels() const { return edge_labels_.data(); }
+  size_t getNumNodes() const { return num_nodes_; }
+
+ private:
+  std::vector<types::row_offset_t> row_offsets_;
+  std::vector<types::col_index_t> column_indices_;
+  std::vector<types::label_t> node_labels_;
+  std::vector<types::label_t> edge_labels_;
+  size_t num_nodes_;
+};
+
+// Device batched CSR (graph.hpp:62-70); the arrays live behind `handle`.
+struct DeviceBatchedCSRGraph {
+  sigmo_graph* handle = nullptr;
+  uint32_t num_graphs = 0;
+  size_t total_nodes = 0;
+  size_t total_edges = 0;
+};
+
+namespace detail {
+// Owning wrapper of a host batch.
+struct HostBatch {
+  sigmo_host_graphs* h = nullptr;
+  ~HostBatch() { sigmo_host_graphs_free(h); }
+};
+
+inline std::vector<CSRGraph> split(const sigmo_host_graphs* b) {
+  uint32_t G = 0;
+  uint64_t N = 0, E = 0;
+  check(sigmo_host_graphs_info(b, &G, &N, &E));
+  const uint32_t *go, *ro, *ci;
+  const uint8_t *nl, *el;
+  check(sigmo_host_graphs_arrays(b, &go, &ro, &ci, &nl, &el));
+  std::vector<CSRGraph> out;
+  out.reserve(G);
+  for (uint32_t g = 0; g < G; ++g) {
+    const uint32_t lo = go[g], hi = go[g + 1], n = hi - lo;
+    const uint32_t e0 = ro[lo], e1 = ro[hi];
+    std::vector<uint32_t> rows(n + 1), cols(e1 - e0);
+    for (uint32_t i = 0; i <= n; ++i) rows[i] = ro[lo + i] - e0;
+    for (uint32_t e = e0; e < e1; ++e) cols[e - e0] = ci[e] - lo;
+    out.emplace_back(std::move(rows), std::move(cols), std::vector<uint8_t>(nl + lo, nl + hi),
+                     std::vector<uint8_t>(el + e0, el + e1), n);
+  }
+  return out;
+}
+}  // namespace detail
+
+namespace io {
+inline std::vector<CSRGraph> loadCSRGraphsFromFile(const std::string& filename) {
+  detail::HostBatch b;
+  detail::check(sigmo_host_graphs_load_dat(filename.c_str(), &b.h));
+  return detail::split(b.h);
+}
+inline std::vector<CSRGraph> loadCSRGraphsFromLines(const std::vector<std::string>& lines) {
+  std::string text;
+  for (const auto& l : lines) text += l + "\n";
+  detail::HostBatch b;
+  detail::check(sigmo_host_graphs_parse_dat(text.data(), text.size(), &b.h));
+  return detail::split(b.h);
+}
+}  // namespace io
+
+// createDeviceCSRGraph (graph.hpp:255-314): concatenate with global ids, upload.
+inline DeviceBatchedCSRGraph createDeviceCSRGraph(Queue& queue, std::vector<CSRGraph>& graphs) {
+  std::vector<uint32_t> go{0}, ro{0}, ci;
+  std::vector<uint8_t> nl, el;
+  for (const auto& g : graphs) {
+    const uint32_t base = go.back(), cbase = static_cast<uint32_t>(ci.size());
+    const size_t n = g.getNumNodes();
+    const uint32_t m = g.getRowOffsets()[n];
+    for (size_t i = 1; i <= n; ++i) ro.push_back(g.getRowOffsets()[i] + cbase);
+    for (uint32_t e = 0; e < m; ++e) {
+      ci.push_back(g.getColumnIndices()[e] + base);
+      el.push_back(g.getEdgeLabels()[e]);
+    }
+    nl.insert(nl.end(), g.getNodeLabels(), g.getNodeLabels() + n);
+    go.push_back(base + static_cast<uint32_t>(n));
+  }
+  detail::HostBatch b;
+  detail::check(sigmo_host_graphs_from_arrays(static_cast<uint32_t>(graphs.size()), go.data(), ro.data(), ci.data(), nl.data(),
+                                              el.data(), &b.h));
+  DeviceBatchedCSRGraph d;
+  detail::check(sigmo_graph_upload(queue.get(), b.h, &d.handle));
+  uint64_t N = 0, E = 0;
+  detail::check(sigmo_graph_info(d.handle, &d.num_graphs, &N, &E));
+  d.total_nodes = N;
+  d.total_edges = E;
+  return d;
+}
+
+inline void destroyDeviceCSRGraph(DeviceBatchedCSRGraph& g, Queue&) {
+  sigmo_graph_free(g.handle);
+  g.handle = nullptr;
+}
+
+// Bytes of the reference CSR arrays (graph.hpp:324-328).
+inline size_t getDeviceGraphAllocSize(const DeviceBatchedCSRGraph& g) {
+  uint64_t csr = 0;
+  detail::check(sigmo_graph_alloc_size(g.handle, &csr, nullptr));
+  return csr;
+}
+
+namespace signature {
+enum class Algorithm { ViewBased, PowerGraph };
+enum class SignatureScope { Data, Query };
+
+template <Algorithm A = Algorithm::PowerGraph, size_t Bits = 4>
+class Signature {
+  static_assert(A == Algorithm::PowerGraph && Bits == 4, "the driver path uses Signature<PowerGraph, 4>");
+
+ public:
+  Signature(Queue& queue, size_t data_nodes, size_t query_nodes) : queue_(queue) {
+    detail::check(sigmo_signatures_create(queue.get(), data_nodes, query_nodes, &s_));
+    data_nodes_ = data_nodes;
+    query_nodes_ = query_nodes;
+  }
+  ~Signature() { sigmo_signatures_free(s_); }
+  Signature(const Signature&) = delete;
+
+  utils::BatchedEvent generateDataSignatures(DeviceBatchedCSRGraph& g) { return gen(g, SIGMO_SCOPE_DATA); }
+  utils::BatchedEvent generateQuerySignatures(DeviceBatchedCSRGraph& g) { return gen(g, SIGMO_SCOPE_QUERY); }
+  utils::BatchedEvent refineDataSignatures(DeviceBatchedCSRGraph& g, size_t view_size = 1) { return ref(g, SIGMO_SCOPE_DATA, view_size); }
+  utils::BatchedEvent refineQuerySignatures(DeviceBatchedCSRGraph& g, size_t view_size = 1) { return ref(g, SIGMO_SCOPE_QUERY, view_size); }
+
+  std::vector<uint64_t> getDataSignatures() const { return download(SIGMO_SCOPE_DATA, data_nodes_); }
+  std::vector<uint64_t> getQuerySignatures() const { return download(SIGMO_SCOPE_QUERY, query_nodes_); }
+  size_t getMaxLabels() const { return 64 / Bits; }
+  size_t getDataSignatureAllocationSize() const { return alloc(SIGMO_SCOPE_DATA); }
+  size_t getQuerySignatureAllocationSize() const { return alloc(SIGMO_SCOPE_QUERY); }
+  sigmo_signatures* handle() const { return s_; }
+
+ private:
+  utils::BatchedEvent gen(DeviceBatchedCSRGraph& g, int scope) {
+    utils::EventScope ev(queue_.get());
+    detail::check(sigmo_signatures_generate(queue_.get(), s_, g.handle, scope));
+    return ev.done();
+  }
+  utils::BatchedEvent ref(DeviceBatchedCSRGraph& g, int scope, size_t view) {
+    utils::EventScope ev(queue_.get());
+    detail::check(sigmo_signatures_refine(queue_.get(), s_, g.handle, scope, static_cast<uint32_t>(view)));
+    return ev.done();
+  }
+  std::vector<uint64_t> download(int scope, size_t n) const {
+    std::vector<uint64_t> out(n);
+    detail::check(sigmo_signatures_download(queue_.get(), s_, scope, out.data(), n));
+    return out;
+  }
+  size_t alloc(int scope) const {
+    uint64_t b = 0;
+    detail::check(sigmo_signatures_alloc_size(s_, scope, &b));
+    return b;
+  }
+  Queue& queue_;
+  sigmo_signatures* s_ = nullptr;
+  size_t data_nodes_ = 0, query_nodes_ = 0;
+};
+}  // namespace signature
+
+namespace candidates {
+class Candidates {
+ public:
+  // Host copy in the reference layout (candidates.hpp:16-28,77-104).
+  struct CandidatesDevice {
+    std::vector<types::candidates_t> words;
+    types::candidates_t* candidates = nullptr;
+    size_t source_nodes = 0, target_nodes = 0, single_node_size = 0;
+    static constexpr types::candidates_t num_bits = 32;
+    uint32_t getCandidatesCount(types::node_t s) const {
+      uint32_t c = 0;
+      for (size_t i = 0; i < single_node_size; ++i) c += static_cast<uint32_t>(__builtin_popcount(candidates[s * single_node_size + i]));
+      return c;
+    }
+    bool contains(types::node_t s, types::node_t c) const {
+      return (candidates[s * single_node_size + c / 32] >> (c % 32)) & 1u;
+    }
+  };
+
+  Candidates(Queue& queue, size_t source_nodes, size_t target_nodes) : queue_(queue) {
+    detail::check(sigmo_candidates_create(queue.get(), source_nodes, target_nodes, &c_));
+    source_ = source_nodes;
+    target_ = target_nodes;
+  }
+  ~Candidates() { sigmo_candidates_free(c_); }
+  Candidates(const Candidates&) = delete;
+
+  CandidatesDevice getHostCandidates() const {
+    CandidatesDevice h;
+    h.source_nodes = source_;
+    h.target_nodes = target_;
+    h.single_node_size = (target_ + 31) / 32;
+    h.words.resize(source_ * h.single_node_size);
+    detail::check(sigmo_candidates_download(queue_.get(), c_, h.words.data(), h.words.size()));
+    h.candidates = h.words.data();
+    return h;
+  }
+  std::vector<uint32_t> getCandidatesCounts() const {
+    std::vector<uint32_t> out(source_);
+    detail::check(sigmo_candidates_counts(queue_.get(), c_, out.data(), out.size()));
+    return out;
+  }
+  size_t getCandidatesCount(types::node_t source_node) const { return getCandidatesCounts().at(source_node); }
+  size_t getAllocationSize() const {
+    uint64_t ref = 0;
+    detail::check(sigmo_candidates_alloc_size(c_, &ref, nullptr));
+    return ref;
+  }
+  sigmo_candidates* handle() const { return c_; }
+
+ private:
+  Queue& queue_;
+  sigmo_candidates* c_ = nullptr;
+  size_t source_ = 0, target_ = 0;
+};
+}  // namespace candidates
+
+namespace isomorphism {
+namespace filter {
+template <CandidatesDomain D = CandidatesDomain::Query>
+utils::BatchedEvent filterCandidates(Queue& queue, DeviceBatchedCSRGraph& query_graph, DeviceBatchedCSRGraph& data_graph,
+                                     signature::Signature<>& signatures, candidates::Candidates& candidates) {
+  static_assert(D == CandidatesDomain::Query, "the driver path filters in the Query domain");
+  utils::EventScope ev(queue.get());
+  detail::check(sigmo_candidates_filter(queue.get(), candidates.handle(), query_graph.handle, data_graph.handle, signatures.handle()));
+  return ev.done();
+}
+
+template <CandidatesDomain D = CandidatesDomain::Query>
+utils::BatchedEvent refineCandidates(Queue& queue, DeviceBatchedCSRGraph& query_graph, DeviceBatchedCSRGraph& data_graph,
+                                     signature::Signature<>& signatures, candidates::Candidates& candidates) {
+  static_assert(D == CandidatesDomain::Query, "the driver path refines in the Query domain");
+  utils::EventScope ev(queue.get());
+  detail::check(sigmo_candidates_refine(queue.get(), candidates.handle(), query_graph.handle, data_graph.handle, signatures.handle()));
+  return ev.done();
+}
+}  // namespace filter
+
+namespace mapping {
+class GMCR {
+ public:
+  struct GMCRHost {
+    std::vector<uint32_t> data_graph_offsets;
+    std::vector<uint32_t> query_graph_indices;
+    size_t total_query_indices = 0;
+  };
+  explicit GMCR(Queue& queue) : queue_(queue) { detail::check(sigmo_gmcr_create(queue.get(), &m_)); }
+  ~GMCR() { sigmo_gmcr_free(m_); }
+  GMCR(const GMCR&) = delete;
+
+  utils::BatchedEvent generateGMCR(DeviceBatchedCSRGraph& query_graphs, DeviceBatchedCSRGraph& data_graphs,
+                                   candidates::Candidates& candidates) {
+    utils::EventScope ev(queue_.get());
+    detail::check(sigmo_gmcr_generate(queue_.get(), m_, query_graphs.handle, data_graphs.handle, candidates.handle()));
+    num_data_graphs_ = data_graphs.num_graphs;
+    return ev.done();
+  }
+  size_t getTotalQueryIndices() const {
+    uint64_t p = 0;
+    detail::check(sigmo_gmcr_total(m_, &p));
+    return p;
+  }
+  GMCRHost getGMCR() const {
+    GMCRHost h;
+    h.total_query_indices = getTotalQueryIndices();
+    h.data_graph_offsets.resize(num_data_graphs_ + 1);
+    h.query_graph_indices.resize(h.total_query_indices);
+    detail::check(sigmo_gmcr_download(queue_.get(), m_, h.data_graph_offsets.data(), h.data_graph_offsets.size(),
+                                      h.query_graph_indices.data(), h.query_graph_indices.size()));
+    return h;
+  }
+  sigmo_gmcr* handle() const { return m_; }
+
+ private:
+  Queue& queue_;
+  sigmo_gmcr* m_ = nullptr;
+  uint32_t num_data_graphs_ = 0;
+};
+}  // namespace mapping
+
+namespace join {
+// Adds the number of matches to *num_matches (isomorphism.hpp:374-375 accumulate).
+inline utils::BatchedEvent joinCandidates(Queue& queue, DeviceBatchedCSRGraph& query_graphs, DeviceBatchedCSRGraph& data_graphs,
+                                          candidates::Candidates& candidates, mapping::GMCR& gmcr, size_t* num_matches,
+                                          bool find_first = true, std::vector<int32_t>* per_pair_counts = nullptr) {
+  utils::EventScope ev(queue.get());
+  uint64_t total = 0;
+  if (per_pair_counts) per_pair_counts->assign(gmcr.getTotalQueryIndices(), 0);
+  detail::check(sigmo_join(queue.get(), query_graphs.handle, data_graphs.handle, candidates.handle(), gmcr.handle(), find_first ? 1 : 0,
+                           &total, per_pair_counts ? per_pair_counts->data() : nullptr));
+  if (num_matches) *num_matches += total;
+  return ev.done();
+}
+}  // namespace join
+}  // namespace isomorphism
+}  // namespace sigmo

@@ -76,6 +76,11 @@ int sigmo_ctx_sync(sigmo_ctx* ctx);
  * the analogue of utils.hpp:149-157 BatchedEvent::getProfilingInfo). */
 int sigmo_phase_ms(sigmo_ctx* ctx, int phase, double* ms);
 int sigmo_phase_reset(sigmo_ctx* ctx);
+/* Launch marks for per-call timing (utils.hpp:127-161 BatchedEvent): every timed kernel
+ * launch on the context gets a sequence number; sigmo_event_mark returns the next one and
+ * sigmo_event_elapsed sums the kernel milliseconds of launches [from, to) (synchronises). */
+int sigmo_event_mark(sigmo_ctx* ctx, uint64_t* mark);
+int sigmo_event_elapsed(sigmo_ctx* ctx, uint64_t from, uint64_t to, double* ms);
 /* Device name and global memory size (sigmo.cpp:20-21). */
 int sigmo_device_info(sigmo_ctx* ctx, char* name, size_t name_len, uint64_t* global_mem);
 
@@ -179,6 +184,7 @@ typedef struct sigmo_pipeline_result {
   uint64_t matched_pairs;  /* pairs with count > 0 (== num_matches in find-first mode)    */
   double phase_ms[SIGMO_PHASE_COUNT];
   double total_ms;         /* device time from first signature kernel to join completion */
+  uint64_t dfs_steps;      /* join DFS frame steps (accepts + backtracks + leaves)         */
 } sigmo_pipeline_result;
 /* Runs gen(D), gen(Q), filter, R x {refsig(D,r), refsig(Q,r), refine}, GMCR, join on
  * device-resident graphs, with one launch per phase; allocates and frees its buffers. */

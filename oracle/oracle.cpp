@@ -343,7 +343,8 @@ void oracle_gmcr_indices(uint32_t Gq, const uint32_t* qgo, uint32_t Gd, const ui
 //             counts true induced embeddings.
 // mutation 2: mapping[] starts at global node 0 instead of the molecule's first node.
 // mutation 3: the induced test checks query edges only (non-edges not required).
-// mutation 4: visited is indexed by global id (no offset subtraction).
+// (Indexing visited by global id instead of id - offset is not a mistake: it rotates every
+// position by the same constant, so the mod-64 classes and all results are unchanged.)
 int64_t oracle_join(uint32_t Gq, const uint32_t* qgo, const uint32_t* qro, const uint32_t* qci, uint32_t Gd,
                     const uint32_t* dgo, const uint32_t* dro, const uint32_t* dci, uint64_t D, const uint32_t* words,
                     const uint32_t* offsets, const uint32_t* indices, int find_first, int32_t* counts, int mutation) {
@@ -370,7 +371,7 @@ int64_t oracle_join(uint32_t Gq, const uint32_t* qgo, const uint32_t* qro, const
       } stack[kMaxQueryNodes];
       u32 mapping[kMaxQueryNodes];
       for (auto& m : mapping) m = mutation == 2 ? 0u : s;
-      Bitset64 visited{mutation == 4 ? 0u : s};
+      Bitset64 visited{s};
       u32 top = 0;
       stack[top++] = {0, 0};
       int64_t count = 0;

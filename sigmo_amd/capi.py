@@ -52,7 +52,7 @@ EXPORTED = (
     "sigmo_candidates_counts", "sigmo_candidates_alloc_size", "sigmo_gmcr_create", "sigmo_gmcr_free",
     "sigmo_gmcr_generate", "sigmo_gmcr_total", "sigmo_gmcr_download", "sigmo_join", "sigmo_pipeline_run",
     "sigmo_pipeline_create", "sigmo_pipeline_execute", "sigmo_pipeline_pairs", "sigmo_pipeline_free",
-    "sigmo_kernel_stats",
+    "sigmo_kernel_stats", "sigmo_event_mark", "sigmo_event_elapsed",
 )
 KERNEL_NAMES = ("sig_generate", "sig_refine", "filter", "refine", "gmcr", "join")
 
@@ -70,6 +70,7 @@ class PipelineResult(C.Structure):
         ("matched_pairs", C.c_uint64),
         ("phase_ms", C.c_double * 5),
         ("total_ms", C.c_double),
+        ("dfs_steps", C.c_uint64),
     ]
 
 
@@ -134,6 +135,8 @@ def lib() -> C.CDLL:
             "sigmo_pipeline_pairs": ([vp, vp, u64, vp, vp, u64], i32),
             "sigmo_pipeline_free": ([vp], i32),
             "sigmo_kernel_stats": ([vp, i32, C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl)], i32),
+            "sigmo_event_mark": ([vp, C.POINTER(u64)], i32),
+            "sigmo_event_elapsed": ([vp, u64, u64, C.POINTER(dbl)], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -502,6 +505,7 @@ def _result_dict(r: PipelineResult) -> dict:
         "matched_pairs": r.matched_pairs,
         "phase_ms": dict(zip(PHASE_NAMES, list(r.phase_ms))),
         "total_ms": r.total_ms,
+        "dfs_steps": r.dfs_steps,
     }
 
 

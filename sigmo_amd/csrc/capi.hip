@@ -63,14 +63,18 @@ struct sigmo_ctx {
     hipEvent_t a, b;
     int phase, kernel;
     double bytes;
+    uint64_t seq;
   };
+  uint64_t launch_seq = 0;          // sequence number of the next timed launch
+  uint64_t launch_base = 0;         // sequence number of launch_ms[0]
+  std::vector<double> launch_ms;    // per-launch kernel ms, indexed by seq - launch_base
   std::vector<Timed> pending;
   double accum_ms[SIGMO_PHASE_COUNT] = {0, 0, 0, 0, 0};
   uint64_t k_launches[SIGMO_KERNEL_COUNT] = {0, 0, 0, 0, 0, 0};
   double k_ms[SIGMO_KERNEL_COUNT] = {0, 0, 0, 0, 0, 0};
   double k_bytes[SIGMO_KERNEL_COUNT] = {0, 0, 0, 0, 0, 0};
   std::vector<hipEvent_t> pool;
-  unsigned long long* stats = nullptr;       // join counters [4]
+  unsigned long long* stats = nullptr;       // join counters [8]: matches, matched pairs, cursor, overflow, steps
   unsigned long long* stats_host = nullptr;  // pinned mirror
 
   int event(hipEvent_t* e) {
@@ -88,6 +92,11 @@ struct sigmo_ctx {
       float ms = 0;
       SIGMO_HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
       accum_ms[t.phase] += ms;
+      if (t.seq >= launch_base) {
+        const uint64_t i = t.seq - launch_base;
+        if (launch_ms.size() <= i) launch_ms.resize(i + 1, 0.0);
+        launch_ms[i] = ms;
+      }
       if (t.kernel >= 0) {
         k_launches[t.kernel] += 1;
         k_ms[t.kernel] += ms;
@@ -97,6 +106,12 @@ struct sigmo_ctx {
       pool.push_back(t.b);
     }
     pending.clear();
+    constexpr size_t kKeep = 1u << 20;  // bounded history for long-running contexts
+    if (launch_ms.size() > 2 * kKeep) {
+      const size_t drop = launch_ms.size() - kKeep;
+      launch_ms.erase(launch_ms.begin(), launch_ms.begin() + drop);
+      launch_base += drop;
+    }
     return SIGMO_OK;
   }
 };
@@ -119,7 +134,7 @@ struct PhaseScope {
     hipEvent_t b;
     SIGMO_TRY(c->event(&b));
     SIGMO_HIP_TRY(hipEventRecord(b, c->stream));
-    c->pending.push_back({a, b, phase, kernel, bytes});
+    c->pending.push_back({a, b, phase, kernel, bytes, c->launch_seq++});
     a = nullptr;
     return SIGMO_OK;
   }
@@ -257,8 +272,8 @@ int sigmo_ctx_create(int device, sigmo_ctx** out) {
     c->global_mem = prop.totalGlobalMem;
   }
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->stats), 4 * sizeof(unsigned long long));
-  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->stats_host), 4 * sizeof(unsigned long long), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->stats), 8 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->stats_host), 8 * sizeof(unsigned long long), hipHostMallocDefault);
   if (e != hipSuccess) {
     sigmo_ctx_destroy(c);
     return sigmo_err::set(SIGMO_ERR_HIP, std::string("context setup: ") + hipGetErrorString(e));
@@ -316,6 +331,23 @@ int sigmo_kernel_stats(sigmo_ctx* ctx, int kernel, uint64_t* launches, double* m
   if (launches) *launches = ctx->k_launches[kernel];
   if (ms) *ms = ctx->k_ms[kernel];
   if (bytes) *bytes = ctx->k_bytes[kernel];
+  return SIGMO_OK;
+}
+
+int sigmo_event_mark(sigmo_ctx* ctx, uint64_t* mark) {
+  if (!ctx || !mark) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  *mark = ctx->launch_seq;
+  return SIGMO_OK;
+}
+
+int sigmo_event_elapsed(sigmo_ctx* ctx, uint64_t from, uint64_t to, double* ms) {
+  SIGMO_TRY(bind(ctx));
+  if (!ms || to < from || to > ctx->launch_seq) return sigmo_err::set(SIGMO_ERR_INVALID, "bad launch range");
+  SIGMO_TRY(ctx->collect());
+  if (from < ctx->launch_base) return sigmo_err::set(SIGMO_ERR_INVALID, "launch range no longer retained");
+  double t = 0;
+  for (uint64_t i = from; i < to; ++i) t += ctx->launch_ms[i - ctx->launch_base];
+  *ms = t;
   return SIGMO_OK;
 }
 
@@ -701,11 +733,11 @@ int sigmo_join(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data
   SIGMO_TRY(grow(&mm->pair_counts, mm->cap_counts, m->P));
   hipStream_t s = ctx->stream;
   PhaseScope ps(ctx, SIGMO_PHASE_JOIN, SIGMO_KERNEL_JOIN, 0.0);
-  SIGMO_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(ctx->stats), 4, 0));
+  SIGMO_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(ctx->stats), 8, 0));
   SIGMO_HIP_TRY(sigmo_k::join(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, m->P, find_first,
                               mm->pair_counts, ctx->stats, ctx->num_cus));
   SIGMO_TRY(ps.close());
-  SIGMO_HIP_TRY(hipMemcpyAsync(ctx->stats_host, ctx->stats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  SIGMO_HIP_TRY(hipMemcpyAsync(ctx->stats_host, ctx->stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   if (per_pair_counts && m->P)
     SIGMO_HIP_TRY(hipMemcpyAsync(per_pair_counts, mm->pair_counts, m->P * 4, hipMemcpyDeviceToHost, s));
   SIGMO_HIP_TRY(hipStreamSynchronize(s));
@@ -794,6 +826,7 @@ int sigmo_pipeline_execute(sigmo_pipeline* p, uint32_t refinement_steps, int fin
   result->num_matches = matches;
   result->gmcr_pairs = p->gm->P;
   result->matched_pairs = ctx->stats_host[1];
+  result->dfs_steps = ctx->stats_host[4];
   result->total_ms = total;
   SIGMO_TRY(ctx->collect());
   for (int ph = 0; ph < SIGMO_PHASE_COUNT; ++ph) result->phase_ms[ph] = ctx->accum_ms[ph] - before[ph];
@@ -870,6 +903,7 @@ int sigmo_pipeline_run(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_gra
   result->num_matches = matches;
   result->gmcr_pairs = gm->P;
   result->matched_pairs = ctx->stats_host[1];
+  result->dfs_steps = ctx->stats_host[4];
   result->total_ms = total;
   for (int p = 0; p < SIGMO_PHASE_COUNT; ++p) SIGMO_TRY(sigmo_phase_ms(ctx, p, &result->phase_ms[p]));
   return SIGMO_OK;

@@ -560,6 +560,156 @@ __global__ __launch_bounds__(kJoinBlock) void k_join(GraphDev q, GraphDev d, con
   }
 }
 
+// ---- join, one wavefront per pair ------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t rl32(uint32_t v, uint32_t lane) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(lane)));
+}
+__device__ __forceinline__ uint64_t rl64(uint64_t v, uint32_t lane) {
+  return static_cast<uint64_t>(rl32(static_cast<uint32_t>(v), lane)) | (static_cast<uint64_t>(rl32(static_cast<uint32_t>(v >> 32), lane)) << 32);
+}
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  return static_cast<uint64_t>(static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)))) |
+         (static_cast<uint64_t>(static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v >> 32)))) << 32);
+}
+__device__ __forceinline__ uint64_t bits_from(uint32_t pos) { return pos >= 64 ? 0ull : (~0ull << pos); }
+
+// joinCandidates, isomorphism.hpp:270-381, with one wavefront per (query, molecule) pair.
+// Lane p holds molecule node p's local adjacency (and node p+64's when n > 64); lane d holds
+// query node d's candidate mask inside the molecule, its query adjacency, mapping[d] and, per
+// molecule position, the set of depths currently mapped onto it.  One DFS frame step costs a
+// few ballots and no memory access:
+//   A = positions mapped at depths i < d with qAdj(i,d), N = same with !qAdj(i,d);
+//   valid(c) <=> (adj(c) & A) == A and (adj(c) & N) == 0, which is the per-i test of
+//   isValidMapping (:150-168) also when mapping[] repeats a node;
+//   the accepted candidate is the first c >= pos with cand(c), !visited(c mod 64), valid(c),
+//   exactly the one the reference's candidate-by-candidate loop (:356-370) accepts.
+// The reference's bookkeeping is kept literally (uniform scalar state): visited is a 64-bit
+// word indexed by local id mod 64; an exhausted frame unsets mapping[depth] (possibly stale),
+// pops, and clears visited when only the root remains; a frame resumes at mapping[depth]+1.
+// mapping[] starts at local node 0 for every pair.
+// Pairs are taken `chunk` at a time from a global cursor (one atomic per chunk): consecutive
+// pairs share their molecule (GMCR order), whose adjacency registers are then reused.
+template <bool FIND_FIRST>
+__global__ __launch_bounds__(256) void k_join_wave(GraphDev q, GraphDev d, const uint64_t* __restrict__ bits, uint64_t W64,
+                                                   const uint32_t* __restrict__ indices, const uint32_t* __restrict__ pair_graph, uint64_t P,
+                                                   uint32_t chunk, int32_t* __restrict__ counts, unsigned long long* __restrict__ stats) {
+  const uint32_t lane = static_cast<uint32_t>(lane_id());
+  uint64_t sum = 0, matched = 0, steps = 0, overflow = 0;
+  uint64_t next = 0, end = 0;
+  uint32_t cur_dg = 0xFFFFFFFFu;
+  uint64_t alo = 0, ahi = 0, blo = 0, bhi = 0;  // molecule adjacency: position `lane` and `lane + 64`
+  while (true) {
+    if (next >= end) {
+      unsigned long long jj = 0;
+      if (lane == 0) jj = atomicAdd(&stats[2], static_cast<unsigned long long>(chunk));
+      next = uniform64(__shfl(jj, 0));
+      end = next + chunk < P ? next + chunk : P;
+      if (next >= P) break;
+    }
+    const uint64_t j = next++;
+    const uint32_t dg = pair_graph[j], qg = indices[j];
+    const uint32_t s = d.go[dg], e = d.go[dg + 1], n = e - s;
+    const uint32_t qoff = q.go[qg], k = q.go[qg + 1] - qoff;
+    const bool wide = n > 64;
+    if (dg != cur_dg) {
+      cur_dg = dg;
+      alo = ahi = blo = bhi = 0;
+      if (lane < n) {
+        alo = d.adj[2 * (static_cast<uint64_t>(s) + lane)];
+        ahi = d.adj[2 * (static_cast<uint64_t>(s) + lane) + 1];
+      }
+      if (lane + 64 < n) {
+        blo = d.adj[2 * (static_cast<uint64_t>(s) + lane + 64)];
+        bhi = d.adj[2 * (static_cast<uint64_t>(s) + lane + 64) + 1];
+      }
+    }
+    // query node `lane`: candidates inside [s, e) as local positions 0..127, adjacency
+    uint64_t clo = 0, chi = 0;
+    uint32_t qa = 0;
+    if (lane < k) {
+      const uint64_t* row = bits + static_cast<uint64_t>(qoff + lane) * W64;
+      const uint32_t w0 = s >> 6, sh = s & 63, last = (e - 1) >> 6;
+      const uint64_t x0 = row[w0];
+      const uint64_t x1 = w0 + 1 <= last ? row[w0 + 1] : 0;
+      const uint64_t x2 = w0 + 2 <= last ? row[w0 + 2] : 0;
+      clo = sh ? (x0 >> sh) | (x1 << (64 - sh)) : x0;
+      chi = sh ? (x1 >> sh) | (x2 << (64 - sh)) : x1;
+      if (n < 64) clo &= (1ull << n) - 1;
+      if (n <= 64) chi = 0;
+      else if (n < 128) chi &= (1ull << (n - 64)) - 1;
+      qa = static_cast<uint32_t>(q.adj[2 * static_cast<uint64_t>(qoff + lane)]);
+    }
+    // mapping[lane] and, per position, the depths mapped onto it (all start at local node 0)
+    uint32_t map = 0;
+    const uint32_t all_depths = k >= 32 ? 0xFFFFFFFFu : ((1u << k) - 1);
+    uint32_t dmlo = lane == 0 ? all_depths : 0u, dmhi = 0;
+
+    uint32_t depth = 0, pos = 0;
+    uint64_t visited = 0, count = 0;
+    while (true) {
+      ++steps;
+      if (depth == k) {  // leaf (:336-344)
+        ++count;
+        if (FIND_FIRST) break;
+        depth = k - 1;
+        pos = rl32(map, depth) + 1;
+        continue;
+      }
+      const uint64_t cand_lo = rl64(clo, depth);
+      const uint64_t cand_hi = wide ? rl64(chi, depth) : 0ull;
+      const uint32_t qad = rl32(qa, depth);
+      const uint32_t low = (1u << depth) - 1;  // depth < 32
+      const uint32_t need = qad & low, forbid = ~qad & low;
+      const uint64_t Alo = __ballot((dmlo & need) != 0), Nlo = __ballot((dmlo & forbid) != 0);
+      const uint64_t Ahi = wide ? __ballot((dmhi & need) != 0) : 0ull;
+      const uint64_t Nhi = wide ? __ballot((dmhi & forbid) != 0) : 0ull;
+      const uint64_t vlo = __ballot((alo & Alo) == Alo && (ahi & Ahi) == Ahi && !(alo & Nlo) && !(ahi & Nhi));
+      uint64_t elig_lo = cand_lo & vlo & ~visited & bits_from(pos);
+      uint64_t elig_hi = 0;
+      if (wide && !elig_lo) {
+        const uint64_t vhi = __ballot((blo & Alo) == Alo && (bhi & Ahi) == Ahi && !(blo & Nlo) && !(bhi & Nhi));
+        elig_hi = cand_hi & vhi & ~visited & (pos <= 64 ? ~0ull : bits_from(pos - 64));
+      }
+      if (elig_lo || elig_hi) {  // accept (:366-370)
+        const uint32_t c = elig_lo ? static_cast<uint32_t>(__builtin_ctzll(elig_lo)) : 64u + static_cast<uint32_t>(__builtin_ctzll(elig_hi));
+        const uint32_t old = rl32(map, depth);
+        const uint32_t bd = 1u << depth;
+        if (lane == (old & 63)) {
+          if (old < 64) dmlo &= ~bd;
+          else dmhi &= ~bd;
+        }
+        if (lane == (c & 63)) {
+          if (c < 64) dmlo |= bd;
+          else dmhi |= bd;
+        }
+        if (lane == depth) map = c;
+        visited |= 1ull << (c & 63);
+        ++depth;
+        pos = 0;
+      } else {  // exhausted frame (:346-354)
+        visited &= ~(1ull << (rl32(map, depth) & 63));
+        if (depth == 0) break;
+        if (depth == 1) visited = 0;
+        --depth;
+        pos = rl32(map, depth) + 1;
+      }
+    }
+    if (lane == 0) {
+      if (count > 0x7FFFFFFFull) overflow = 1;
+      counts[j] = static_cast<int32_t>(count > 0x7FFFFFFFull ? 0x7FFFFFFFull : count);
+    }
+    sum += count;
+    matched += count > 0;
+  }
+  if (lane == 0) {
+    if (sum) atomicAdd(&stats[0], static_cast<unsigned long long>(sum));
+    if (matched) atomicAdd(&stats[1], static_cast<unsigned long long>(matched));
+    if (overflow) atomicOr(&stats[3], 1ull);
+    atomicAdd(&stats[4], static_cast<unsigned long long>(steps));
+  }
+}
+
 }  // namespace
 
 // ---- launchers -------------------------------------------------------------------------------
@@ -677,13 +827,17 @@ hipError_t gmcr_scatter(hipStream_t s, const uint64_t* pair_bits, uint32_t Gq, u
 hipError_t join(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
                 const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts, unsigned long long* stats, int num_cus) {
   if (!P) return hipSuccess;
-  uint64_t blocks = (P + kJoinBlock - 1) / kJoinBlock;
+  // one wavefront per pair, persistent: 8 waves per SIMD; chunks give every wave ~8 grabs
+  uint64_t blocks = (P + 3) / 4;
   const uint64_t cap = static_cast<uint64_t>(num_cus > 0 ? num_cus : 256) * 8;
   if (blocks > cap) blocks = cap;
+  uint64_t chunk = P / (blocks * 4 * 8);
+  chunk = chunk < 1 ? 1 : (chunk > 64 ? 64 : chunk);
+  const uint32_t ch = static_cast<uint32_t>(chunk);
   if (find_first)
-    k_join<true><<<static_cast<unsigned>(blocks), kJoinBlock, 0, s>>>(q, d, bits, W64, indices, pair_graph, P, counts, stats);
+    k_join_wave<true><<<static_cast<unsigned>(blocks), 256, 0, s>>>(q, d, bits, W64, indices, pair_graph, P, ch, counts, stats);
   else
-    k_join<false><<<static_cast<unsigned>(blocks), kJoinBlock, 0, s>>>(q, d, bits, W64, indices, pair_graph, P, counts, stats);
+    k_join_wave<false><<<static_cast<unsigned>(blocks), 256, 0, s>>>(q, d, bits, W64, indices, pair_graph, P, ch, counts, stats);
   return hipGetLastError();
 }
 
