@@ -185,16 +185,31 @@ struct sigmo_graph {
   }
 };
 
+// Per-scope refinement history.  The reachables always equal ball(M) minus the start
+// position for the largest view size M refined so far (balls only grow), so when the last
+// BFS layer is kept (`frontier` valid) a call with view_size M+1 is one BFS step.
+struct RefineState {
+  uint32_t max_view = 0;
+  bool frontier_valid = false;
+  bool tainted = false;  // refined with more than one graph: always the full BFS
+  const sigmo_graph* graph = nullptr;
+  void reset() { *this = RefineState(); }
+};
+
 struct sigmo_signatures {
   sigmo_ctx* ctx = nullptr;
   uint64_t D = 0, Q = 0;
   uint64_t *dsig = nullptr, *qsig = nullptr, *dreach = nullptr, *qreach = nullptr, *dplanes = nullptr;
+  uint64_t *dfront = nullptr, *qfront = nullptr;
+  RefineState dstate, qstate;
   ~sigmo_signatures() {
     dev_free(dsig);
     dev_free(qsig);
     dev_free(dreach);
     dev_free(qreach);
     dev_free(dplanes);
+    dev_free(dfront);
+    dev_free(qfront);
   }
 };
 
@@ -462,6 +477,8 @@ int sigmo_signatures_create(sigmo_ctx* ctx, uint64_t data_nodes, uint64_t query_
   if (st == SIGMO_OK) st = dev_alloc(&s->dreach, data_nodes);
   if (st == SIGMO_OK) st = dev_alloc(&s->qreach, query_nodes);
   if (st == SIGMO_OK) st = dev_alloc(&s->dplanes, 64 * W64);
+  if (st == SIGMO_OK) st = dev_alloc(&s->dfront, data_nodes);
+  if (st == SIGMO_OK) st = dev_alloc(&s->qfront, query_nodes);
   hipError_t e = hipSuccess;
   // signatures filled with 0 (signature.hpp:166-167); reachables start empty (zero)
   if (st == SIGMO_OK) e = sigmo_k::fill_u64(ctx->stream, s->dsig, data_nodes, 0);
@@ -514,9 +531,26 @@ int sigmo_signatures_refine(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_gra
   SIGMO_TRY(bind(ctx));
   uint64_t *sigs, *reach, *planes;
   SIGMO_TRY(sig_target(s, g, scope, &sigs, &reach, &planes));
+  RefineState& st = scope == SIGMO_SCOPE_DATA ? s->dstate : s->qstate;
+  uint64_t* front = scope == SIGMO_SCOPE_DATA ? s->dfront : s->qfront;
+  if (st.graph && st.graph != g) {
+    st.tainted = true;
+    st.frontier_valid = false;
+  }
+  st.graph = g;
   PhaseScope ps(ctx, scope == SIGMO_SCOPE_DATA ? SIGMO_PHASE_SIG_DATA : SIGMO_PHASE_SIG_QUERY,
                 scope == SIGMO_SCOPE_DATA ? SIGMO_KERNEL_SIG_REFINE : -1, sig_bytes(g));
-  SIGMO_HIP_TRY(sigmo_k::sig_refine(ctx->stream, g->dev(), view_size, sigs, reach, planes));
+  if (!st.tainted && st.frontier_valid && view_size == st.max_view + 1) {
+    SIGMO_HIP_TRY(sigmo_k::sig_refine_next(ctx->stream, g->dev(), sigs, reach, front, planes));
+    st.max_view = view_size;
+  } else {
+    const bool keep = !st.tainted && view_size > st.max_view;
+    SIGMO_HIP_TRY(sigmo_k::sig_refine(ctx->stream, g->dev(), view_size, sigs, reach, keep ? front : nullptr, planes));
+    if (view_size > st.max_view) {
+      st.max_view = view_size;
+      st.frontier_valid = keep;
+    }
+  }
   return ps.close();
 }
 
@@ -805,6 +839,8 @@ int sigmo_pipeline_execute(sigmo_pipeline* p, uint32_t refinement_steps, int fin
   SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->qsig, sig->Q, 0));
   SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->dreach, sig->D, 0));
   SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->qreach, sig->Q, 0));
+  sig->dstate.reset();
+  sig->qstate.reset();
   SIGMO_TRY(sigmo_signatures_generate(ctx, sig, p->d, SIGMO_SCOPE_DATA));
   SIGMO_TRY(sigmo_signatures_generate(ctx, sig, p->q, SIGMO_SCOPE_QUERY));
   SIGMO_TRY(sigmo_candidates_filter(ctx, p->cand, p->q, p->d, sig));

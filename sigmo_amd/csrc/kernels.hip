@@ -1,8 +1,8 @@
 // gfx950 kernels for SIGMo's hot path: signatures, candidate filter/refine, GMCR, join.
 //
 // Wave64 throughout.  The bitset kernels are streaming (one pass over 64-bit word strips,
-// coalesced across lanes) and bound by HBM; the join is a persistent lane-per-pair DFS with
-// wave-aggregated work stealing.  Reference semantics are cited per kernel (paths relative
+// coalesced across lanes) and bound by HBM; the join is a persistent one-wavefront-per-pair
+// DFS whose state is distributed over the lanes' registers.  Reference semantics are cited per kernel (paths relative
 // to the SIGMo repository); the integer conventions for the reference's undefined
 // behaviour are: shift counts masked to width-1, Bitset<uint64_t> positions taken mod 64,
 // PowerGraph reachables starting at zero, per-row refine, ascending query ids inside GMCR
@@ -13,7 +13,6 @@ namespace sigmo_k {
 namespace {
 
 constexpr int kWave = 64;
-constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 __device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & (kWave - 1)); }
 
@@ -116,6 +115,19 @@ __device__ __forceinline__ void write_planes(uint64_t sig, uint64_t W64, uint64_
 
 constexpr int kSigBlock = 512;
 
+// Saturated 16-label histogram of a position set, labels read through the graph's 64-position
+// label masks (the +1-below-15 rule of signature.hpp:44-51 applied count times).
+__device__ __forceinline__ uint64_t label_histogram(uint64_t r, const uint64_t* __restrict__ lm) {
+  uint64_t s = 0;
+#pragma unroll
+  for (int l = 0; l < 16; ++l) {
+    uint32_t c = __popcll(r & lm[l]);
+    c = c < 15 ? c : 15;
+    s |= static_cast<uint64_t>(c) << (4 * l);
+  }
+  return s;
+}
+
 // generateCSRSignatures, signature.hpp:125-152: one thread per node, neighbour labels
 // incremented onto the existing value.
 __global__ __launch_bounds__(kSigBlock) void k_sig_generate(const uint32_t* __restrict__ ro, const uint32_t* __restrict__ ci,
@@ -137,8 +149,11 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_generate(const uint32_t* __re
 // runs over the folded adjacency adj_lo|adj_hi of node go[g]+b.  The level signature is the
 // saturated label histogram of ball(view_size) minus the start bit minus the old
 // reachables; the old reachables then absorb it.
+// With `frontier` non-null the last BFS layer (the sphere at view_size) is stored for
+// k_sig_refine_next.
 __global__ __launch_bounds__(kSigBlock) void k_sig_refine(GraphDev g, uint32_t view_size, uint64_t* __restrict__ sigs,
-                                                          uint64_t* __restrict__ reach, uint64_t* __restrict__ planes) {
+                                                          uint64_t* __restrict__ reach, uint64_t* __restrict__ frontier_out,
+                                                          uint64_t* __restrict__ planes) {
   __shared__ uint64_t tile[64 * (kSigBlock / kWave)];
   const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   uint64_t s = 0;
@@ -161,15 +176,39 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_refine(GraphDev g, uint32_t v
     }
     const uint64_t old = reach[v];
     const uint64_t r = ball & ~self & ~old;
-    const uint64_t* lm = g.lmask + static_cast<uint64_t>(gid) * 16;
-#pragma unroll
-    for (int l = 0; l < 16; ++l) {
-      uint32_t c = __popcll(r & lm[l]);
-      c = c < 15 ? c : 15;
-      s |= static_cast<uint64_t>(c) << (4 * l);
-    }
+    s = label_histogram(r, g.lmask + static_cast<uint64_t>(gid) * 16);
     sigs[v] = s;
     reach[v] = r | old;
+    if (frontier_out) frontier_out[v] = frontier;
+  }
+  if (planes) write_planes<kSigBlock>(s, (g.N + 63) / 64, planes, tile);
+}
+
+// One more level after a call at view_size M, when reach = ball(M) minus the start position
+// and frontier = the sphere at M (the state every call sequence 1, 2, ..., M leaves): the
+// reference's ball(M+1) minus old reachables is exactly the next BFS layer.
+__global__ __launch_bounds__(kSigBlock) void k_sig_refine_next(GraphDev g, uint64_t* __restrict__ sigs, uint64_t* __restrict__ reach,
+                                                               uint64_t* __restrict__ frontier, uint64_t* __restrict__ planes) {
+  __shared__ uint64_t tile[64 * (kSigBlock / kWave)];
+  const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  uint64_t s = 0;
+  if (v < g.N) {
+    const uint32_t gid = g.node_graph[v];
+    const uint32_t prev = g.go[gid];
+    const uint64_t self = 1ull << ((static_cast<uint32_t>(v) - prev) & 63u);
+    const uint64_t old = reach[v];
+    uint64_t next = 0, f = frontier[v];
+    while (f) {
+      const uint32_t b = __builtin_ctzll(f);
+      f &= f - 1;
+      const uint64_t u = static_cast<uint64_t>(prev) + b;
+      next |= g.adj[2 * u] | g.adj[2 * u + 1];
+    }
+    next &= ~(old | self);
+    s = label_histogram(next, g.lmask + static_cast<uint64_t>(gid) * 16);
+    sigs[v] = s;
+    reach[v] = old | next;
+    frontier[v] = next;
   }
   if (planes) write_planes<kSigBlock>(s, (g.N + 63) / 64, planes, tile);
 }
@@ -178,12 +217,15 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_refine(GraphDev g, uint32_t v
 
 constexpr int kBitsBlock = 256;
 
-inline unsigned rows_per_block(uint64_t Q, uint64_t col_blocks, int num_cus) {
-  const uint64_t target_blocks = static_cast<uint64_t>(num_cus > 0 ? num_cus : 256) * 16;
+// Rows per block for the word-strip kernels.  Each thread loads its strip's planes once per
+// block, so a block should cover at least `min_rows` rows (planes then cost <= ~10% of the
+// row traffic); beyond that, enough row chunks to give every CU ~4 blocks of 4 waves.
+inline unsigned rows_per_block(uint64_t Q, uint64_t col_blocks, int num_cus, uint64_t min_rows, uint64_t blocks_per_cu = 4) {
+  const uint64_t target_blocks = static_cast<uint64_t>(num_cus > 0 ? num_cus : 256) * blocks_per_cu;
   uint64_t chunks = (target_blocks + col_blocks - 1) / col_blocks;
   if (chunks < 1) chunks = 1;
   uint64_t rpb = (Q + chunks - 1) / chunks;
-  if (rpb < 32) rpb = 32;
+  if (rpb < min_rows) rpb = min_rows;
   if (rpb > Q) rpb = Q;
   return static_cast<unsigned>(rpb > 0 ? rpb : 1);
 }
@@ -245,16 +287,30 @@ __global__ __launch_bounds__(kBitsBlock) void k_refine(const uint64_t* __restric
     for (int b = 0; b < 4; ++b) P[u][b] = planes[(static_cast<uint64_t>(slots.label[u]) * 4 + b) * W64 + w];
   const uint64_t q0 = static_cast<uint64_t>(blockIdx.y) * rpb;
   const uint64_t q1 = q0 + rpb < Q ? q0 + rpb : Q;
-  for (uint64_t q = q0; q < q1; ++q) {
-    const uint64_t old = bits[q * W64 + w];
-    const uint64_t qs = qsig[q];
+  auto keep_of = [&](uint64_t qs) {
     uint64_t keep = ~0ull;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t c = static_cast<uint32_t>(qs >> (4 * slots.label[u])) & 15u;
       if (c) keep &= ge4(P[u], c);
     }
-    const uint64_t nw = old & keep;
+    return keep;
+  };
+  constexpr int kRows = 4;  // loads in flight per thread
+  uint64_t q = q0;
+  for (; q + kRows <= q1; q += kRows) {
+    uint64_t old[kRows];
+#pragma unroll
+    for (int i = 0; i < kRows; ++i) old[i] = bits[(q + i) * W64 + w];
+#pragma unroll
+    for (int i = 0; i < kRows; ++i) {
+      const uint64_t nw = old[i] & keep_of(qsig[q + i]);
+      if (nw != old[i]) bits[(q + i) * W64 + w] = nw;
+    }
+  }
+  for (; q < q1; ++q) {
+    const uint64_t old = bits[q * W64 + w];
+    const uint64_t nw = old & keep_of(qsig[q]);
     if (nw != old) bits[q * W64 + w] = nw;
   }
 }
@@ -423,143 +479,6 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_final(const uint32_t* __res
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = static_cast<uint32_t>(*total_dev);
 }
 
-// ---- join ------------------------------------------------------------------------------------
-
-constexpr int kJoinBlock = 256;
-constexpr int kMaxDepth = 32;
-
-// Next candidate of a query node at local position >= pos inside the molecule [s, e):
-// ascending order of candidates.hpp:132-162, over at most three 64-bit words (n <= 128).
-__device__ __forceinline__ uint32_t next_candidate(const uint64_t* __restrict__ row, uint32_t s, uint32_t e, uint32_t pos) {
-  const uint32_t g = s + pos;
-  if (g >= e) return kNone;
-  uint32_t w = g >> 6;
-  const uint32_t last = (e - 1) >> 6;
-  uint64_t word = row[w] & (~0ull << (g & 63));
-  while (true) {
-    if (w == last) word &= ~0ull >> (63 - ((e - 1) & 63));
-    if (word) return (w << 6) + static_cast<uint32_t>(__builtin_ctzll(word)) - s;
-    if (w == last) return kNone;
-    ++w;
-    word = row[w];
-  }
-}
-
-// joinCandidates, isomorphism.hpp:270-381.  Each lane runs one (query, molecule) pair's DFS
-// in the reference's order and bookkeeping; finished lanes take the next pair from a global
-// cursor (one atomic per wave per refill).  The reference frame stack collapses to
-// mapping[] (LDS, one byte per depth, local ids): a frame below the top resumes at
-// mapping[depth] + 1, because its last tried candidate is the one it accepted.
-//   * leaf (depth == k): count, pop; find-first ends the pair (:336-344);
-//   * exhausted frame: unset visited bit of mapping[depth] (possibly stale), pop, clear all
-//     of visited when only the root frame remains (:346-354);
-//   * candidate c: skip if visited bit (c mod 64) is set (:363); accept if depth == 0 or
-//     for all i < depth, qAdj(i, depth) == dAdj(mapping[i], c) (:366, :150-168).
-template <bool FIND_FIRST>
-__global__ __launch_bounds__(kJoinBlock) void k_join(GraphDev q, GraphDev d, const uint64_t* __restrict__ bits, uint64_t W64,
-                                                     const uint32_t* __restrict__ indices, const uint32_t* __restrict__ pair_graph,
-                                                     uint64_t P, int32_t* __restrict__ counts, unsigned long long* __restrict__ stats) {
-  __shared__ uint8_t mapping[kMaxDepth][kJoinBlock];
-  const int tid = threadIdx.x, lane = lane_id();
-  bool active = false;
-  bool exhausted = false;  // wave-uniform
-  uint64_t j = 0;
-  uint32_t s = 0, e = 0, qoff = 0, k = 0, depth = 0, pos = 0;
-  uint64_t visited = 0;
-  uint64_t count = 0;
-  uint64_t sum = 0, matched = 0;
-  unsigned long long overflow = 0;
-  while (true) {
-    const uint64_t need = __ballot(!active);
-    if (need && !exhausted) {
-      const int leader = __ffsll(static_cast<long long>(need)) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(&stats[2], static_cast<unsigned long long>(__popcll(need)));
-      base = __shfl(base, leader);
-      if (!active) {
-        const uint64_t jj = base + __popcll(need & ((1ull << lane) - 1));
-        if (jj < P) {
-          j = jj;
-          const uint32_t dg = pair_graph[j], qg = indices[j];
-          s = d.go[dg];
-          e = d.go[dg + 1];
-          qoff = q.go[qg];
-          k = q.go[qg + 1] - qoff;
-          for (uint32_t i = 0; i < k; ++i) mapping[i][tid] = 0;
-          depth = 0;
-          pos = 0;
-          visited = 0;
-          count = 0;
-          active = true;
-        }
-      }
-      if (base + __popcll(need) >= P) exhausted = true;
-    }
-    if (__ballot(active) == 0) break;
-    if (!active) continue;
-    bool done = false;
-    if (depth == k) {
-      ++count;
-      if (FIND_FIRST) {
-        done = true;
-      } else {
-        depth = k - 1;
-        pos = static_cast<uint32_t>(mapping[depth][tid]) + 1;
-      }
-    } else {
-      const uint32_t c = next_candidate(bits + static_cast<uint64_t>(qoff + depth) * W64, s, e, pos);
-      if (c == kNone) {
-        visited &= ~(1ull << (mapping[depth][tid] & 63));
-        if (depth == 0) {
-          done = true;
-        } else {
-          if (depth == 1) visited = 0;
-          --depth;
-          pos = static_cast<uint32_t>(mapping[depth][tid]) + 1;
-        }
-      } else {
-        pos = c + 1;
-        if (!((visited >> (c & 63)) & 1ull)) {
-          bool ok = true;
-          if (depth > 0) {
-            const uint64_t node = static_cast<uint64_t>(s) + c;
-            const uint64_t a0 = d.adj[2 * node], a1 = d.adj[2 * node + 1];
-            const uint32_t qa = static_cast<uint32_t>(q.adj[2 * static_cast<uint64_t>(qoff + depth)]);
-            for (uint32_t i = 0; i < depth && ok; ++i) {
-              const uint32_t m = mapping[i][tid];
-              const uint32_t dbit = static_cast<uint32_t>(((m < 64 ? a0 : a1) >> (m & 63)) & 1ull);
-              ok = dbit == ((qa >> i) & 1u);
-            }
-          }
-          if (ok) {
-            mapping[depth][tid] = static_cast<uint8_t>(c);
-            visited |= 1ull << (c & 63);
-            ++depth;
-            pos = 0;
-          }
-        }
-      }
-    }
-    if (done) {
-      if (count > 0x7FFFFFFFull) overflow = 1;
-      counts[j] = static_cast<int32_t>(count > 0x7FFFFFFFull ? 0x7FFFFFFFull : count);
-      sum += count;
-      matched += count > 0;
-      active = false;
-    }
-  }
-  for (int off = kWave / 2; off > 0; off >>= 1) {
-    sum += __shfl_xor(sum, off);
-    matched += __shfl_xor(matched, off);
-    overflow |= __shfl_xor(overflow, off);
-  }
-  if (lane == 0) {
-    if (sum) atomicAdd(&stats[0], static_cast<unsigned long long>(sum));
-    if (matched) atomicAdd(&stats[1], static_cast<unsigned long long>(matched));
-    if (overflow) atomicOr(&stats[3], 1ull);
-  }
-}
-
 // ---- join, one wavefront per pair ------------------------------------------------------------------
 
 __device__ __forceinline__ uint32_t rl32(uint32_t v, uint32_t lane) {
@@ -574,12 +493,163 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
 }
 __device__ __forceinline__ uint64_t bits_from(uint32_t pos) { return pos >= 64 ? 0ull : (~0ull << pos); }
 
+// Per-pair operands of the wavefront join: molecule range, query range, the lane-distributed
+// candidate masks / query adjacency (lane d = query node d) and molecule adjacency (lane p =
+// node p, and node p+64 for molecules of more than 64 nodes).
+struct JoinPair {
+  uint32_t dg, s, e, n, qoff, k;
+  uint64_t clo, chi;  // candidates of query node `lane` at molecule positions 0-63 / 64-127
+  uint32_t qa;        // adjacency of query node `lane` by query-local id
+  uint64_t alo, ahi, blo, bhi;
+};
+
+__device__ __forceinline__ void join_load(const GraphDev& q, const GraphDev& d, const uint64_t* __restrict__ bits, uint64_t W64,
+                                          const uint32_t* __restrict__ indices, const uint32_t* __restrict__ pair_graph, uint64_t j,
+                                          const JoinPair* prev, JoinPair& o) {
+  const uint32_t lane = static_cast<uint32_t>(lane_id());
+  const uint32_t dg = pair_graph[j], qg = indices[j];
+  o.dg = dg;
+  o.s = d.go[dg];
+  o.e = d.go[dg + 1];
+  o.n = o.e - o.s;
+  o.qoff = q.go[qg];
+  o.k = q.go[qg + 1] - o.qoff;
+  const uint32_t s = o.s, e = o.e, n = o.n, k = o.k;
+  if (prev && prev->dg == dg) {
+    o.alo = prev->alo;
+    o.ahi = prev->ahi;
+    o.blo = prev->blo;
+    o.bhi = prev->bhi;
+  } else {
+    o.alo = o.ahi = o.blo = o.bhi = 0;
+    if (lane < n) {
+      o.alo = d.adj[2 * (static_cast<uint64_t>(s) + lane)];
+      o.ahi = d.adj[2 * (static_cast<uint64_t>(s) + lane) + 1];
+    }
+    if (lane + 64 < n) {
+      o.blo = d.adj[2 * (static_cast<uint64_t>(s) + lane + 64)];
+      o.bhi = d.adj[2 * (static_cast<uint64_t>(s) + lane + 64) + 1];
+    }
+  }
+  o.clo = o.chi = 0;
+  o.qa = 0;
+  if (lane < k) {
+    const uint64_t* row = bits + static_cast<uint64_t>(o.qoff + lane) * W64;
+    const uint32_t w0 = s >> 6, last = (e - 1) >> 6;
+    o.clo = row[w0];
+    o.chi = w0 + 1 <= last ? row[w0 + 1] : 0;
+    o.qa = static_cast<uint32_t>(q.adj[2 * static_cast<uint64_t>(o.qoff + lane)]);
+    // a third word (molecules straddling three words) is read in join_finish
+  }
+}
+
+// Turn the raw words of join_load into molecule-local candidate masks (funnel shift by s mod 64).
+__device__ __forceinline__ void join_finish(const uint64_t* __restrict__ bits, uint64_t W64, JoinPair& o) {
+  const uint32_t lane = static_cast<uint32_t>(lane_id());
+  if (lane < o.k) {
+    const uint32_t sh = o.s & 63, w0 = o.s >> 6, last = (o.e - 1) >> 6;
+    uint64_t x0 = o.clo, x1 = o.chi, x2 = 0;
+    if (w0 + 2 <= last) x2 = bits[static_cast<uint64_t>(o.qoff + lane) * W64 + w0 + 2];
+    uint64_t lo = sh ? (x0 >> sh) | (x1 << (64 - sh)) : x0;
+    uint64_t hi = sh ? (x1 >> sh) | (x2 << (64 - sh)) : x1;
+    if (o.n < 64) lo &= (1ull << o.n) - 1;
+    if (o.n <= 64) hi = 0;
+    else if (o.n < 128) hi &= (1ull << (o.n - 64)) - 1;
+    o.clo = lo;
+    o.chi = hi;
+  }
+}
+
+// The reference DFS of one pair on the lane-distributed operands (see k_join_wave).  WIDE
+// handles molecules of 65..128 nodes (positions 64..127 in the `hi` words).
+__device__ __forceinline__ uint32_t uniform32(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+}
+
+template <bool FIND_FIRST, bool WIDE>
+__device__ __forceinline__ uint64_t join_dfs(const JoinPair& o, uint64_t& steps) {
+  const uint32_t lane = static_cast<uint32_t>(lane_id());
+  const uint32_t k = uniform32(o.k);
+  uint32_t map = 0;             // lane d: mapping[d] (local id; every entry starts at node 0)
+  uint64_t mslo = 0, mshi = 0;  // lane d: positions mapped at depths < d on the current path
+  uint32_t depth = 0, pos = 0;
+  uint64_t visited = 0, count = 0;
+  while (true) {
+    ++steps;
+    if (depth == k) {  // leaf (:336-344)
+      ++count;
+      if (FIND_FIRST) break;
+      depth = k - 1;
+      pos = rl32(map, depth) + 1;
+      continue;
+    }
+    const uint64_t cl = rl64(o.clo, depth) & ~visited & bits_from(pos);
+    const uint64_t ch = WIDE ? rl64(o.chi, depth) & ~visited & (pos <= 64 ? ~0ull : bits_from(pos - 64)) : 0ull;
+    uint64_t el = 0, eh = 0, Ml = 0, Mh = 0;
+    if (cl | ch) {
+      if (depth == 0) {
+        el = cl;
+        eh = ch;
+      } else {
+        // A: positions that must be adjacent (mapped query neighbours), N: must not be
+        const uint32_t qad = rl32(o.qa, depth), low = (1u << depth) - 1;
+        uint64_t Al = 0, Ah = 0, Nl, Nh;
+        for (uint32_t t = qad & low; t; t &= t - 1) {
+          const uint32_t m = rl32(map, static_cast<uint32_t>(__builtin_ctz(t)));
+          if (!WIDE || m < 64) Al |= 1ull << (m & 63);
+          else Ah |= 1ull << (m & 63);
+        }
+        Ml = rl64(mslo, depth);
+        Mh = WIDE ? rl64(mshi, depth) : 0ull;
+        if (static_cast<uint32_t>(__popcll(Ml) + __popcll(Mh)) == depth) {  // injective: N = mapped \ A
+          Nl = Ml & ~Al;
+          Nh = Mh & ~Ah;
+        } else {  // a stale unset let a node be mapped twice: collect N exactly
+          Nl = Nh = 0;
+          for (uint32_t t = ~qad & low; t; t &= t - 1) {
+            const uint32_t m = rl32(map, static_cast<uint32_t>(__builtin_ctz(t)));
+            if (!WIDE || m < 64) Nl |= 1ull << (m & 63);
+            else Nh |= 1ull << (m & 63);
+          }
+        }
+        if (cl) {
+          const bool ok = (o.alo & Al) == Al && !(o.alo & Nl) && (!WIDE || ((o.ahi & Ah) == Ah && !(o.ahi & Nh)));
+          el = cl & __ballot(ok);
+        }
+        if (WIDE && !el && ch) {
+          const bool ok = (o.blo & Al) == Al && !(o.blo & Nl) && (o.bhi & Ah) == Ah && !(o.bhi & Nh);
+          eh = ch & __ballot(ok);
+        }
+      }
+    }
+    if (el | eh) {  // accept (:366-370)
+      const uint32_t c = el ? static_cast<uint32_t>(__builtin_ctzll(el)) : 64u + static_cast<uint32_t>(__builtin_ctzll(eh));
+      if (lane == depth) map = c;
+      if (lane == depth + 1) {
+        mslo = Ml | (c < 64 ? 1ull << c : 0ull);
+        if (WIDE) mshi = Mh | (c >= 64 ? 1ull << (c & 63) : 0ull);
+      }
+      visited |= 1ull << (c & 63);
+      ++depth;
+      pos = 0;
+    } else {  // exhausted frame (:346-354)
+      visited &= ~(1ull << (rl32(map, depth) & 63));
+      if (depth == 0) break;
+      if (depth == 1) visited = 0;
+      --depth;
+      pos = rl32(map, depth) + 1;
+    }
+  }
+  return count;
+}
+
 // joinCandidates, isomorphism.hpp:270-381, with one wavefront per (query, molecule) pair.
 // Lane p holds molecule node p's local adjacency (and node p+64's when n > 64); lane d holds
-// query node d's candidate mask inside the molecule, its query adjacency, mapping[d] and, per
-// molecule position, the set of depths currently mapped onto it.  One DFS frame step costs a
-// few ballots and no memory access:
-//   A = positions mapped at depths i < d with qAdj(i,d), N = same with !qAdj(i,d);
+// query node d's candidate mask inside the molecule, its query adjacency, mapping[d] and the
+// set of positions mapped at depths < d.  One DFS frame step costs one or two ballots and no
+// memory access:
+//   A = positions mapped at depths i < d with qAdj(i,d) (scalar loop over d's query
+//   neighbours), N = positions mapped at depths i < d without it;
 //   valid(c) <=> (adj(c) & A) == A and (adj(c) & N) == 0, which is the per-i test of
 //   isValidMapping (:150-168) also when mapping[] repeats a node;
 //   the accepted candidate is the first c >= pos with cand(c), !visited(c mod 64), valid(c),
@@ -587,9 +657,9 @@ __device__ __forceinline__ uint64_t bits_from(uint32_t pos) { return pos >= 64 ?
 // The reference's bookkeeping is kept literally (uniform scalar state): visited is a 64-bit
 // word indexed by local id mod 64; an exhausted frame unsets mapping[depth] (possibly stale),
 // pops, and clears visited when only the root remains; a frame resumes at mapping[depth]+1.
-// mapping[] starts at local node 0 for every pair.
-// Pairs are taken `chunk` at a time from a global cursor (one atomic per chunk): consecutive
-// pairs share their molecule (GMCR order), whose adjacency registers are then reused.
+// Pairs are taken `chunk` at a time from a global cursor (one atomic per chunk).  The next
+// pair's operands are loaded while the current pair's DFS runs; consecutive pairs share their
+// molecule (GMCR order), whose adjacency registers are then reused.
 template <bool FIND_FIRST>
 __global__ __launch_bounds__(256) void k_join_wave(GraphDev q, GraphDev d, const uint64_t* __restrict__ bits, uint64_t W64,
                                                    const uint32_t* __restrict__ indices, const uint32_t* __restrict__ pair_graph, uint64_t P,
@@ -597,110 +667,33 @@ __global__ __launch_bounds__(256) void k_join_wave(GraphDev q, GraphDev d, const
   const uint32_t lane = static_cast<uint32_t>(lane_id());
   uint64_t sum = 0, matched = 0, steps = 0, overflow = 0;
   uint64_t next = 0, end = 0;
-  uint32_t cur_dg = 0xFFFFFFFFu;
-  uint64_t alo = 0, ahi = 0, blo = 0, bhi = 0;  // molecule adjacency: position `lane` and `lane + 64`
-  while (true) {
-    if (next >= end) {
-      unsigned long long jj = 0;
-      if (lane == 0) jj = atomicAdd(&stats[2], static_cast<unsigned long long>(chunk));
-      next = uniform64(__shfl(jj, 0));
-      end = next + chunk < P ? next + chunk : P;
-      if (next >= P) break;
-    }
-    const uint64_t j = next++;
-    const uint32_t dg = pair_graph[j], qg = indices[j];
-    const uint32_t s = d.go[dg], e = d.go[dg + 1], n = e - s;
-    const uint32_t qoff = q.go[qg], k = q.go[qg + 1] - qoff;
-    const bool wide = n > 64;
-    if (dg != cur_dg) {
-      cur_dg = dg;
-      alo = ahi = blo = bhi = 0;
-      if (lane < n) {
-        alo = d.adj[2 * (static_cast<uint64_t>(s) + lane)];
-        ahi = d.adj[2 * (static_cast<uint64_t>(s) + lane) + 1];
-      }
-      if (lane + 64 < n) {
-        blo = d.adj[2 * (static_cast<uint64_t>(s) + lane + 64)];
-        bhi = d.adj[2 * (static_cast<uint64_t>(s) + lane + 64) + 1];
-      }
-    }
-    // query node `lane`: candidates inside [s, e) as local positions 0..127, adjacency
-    uint64_t clo = 0, chi = 0;
-    uint32_t qa = 0;
-    if (lane < k) {
-      const uint64_t* row = bits + static_cast<uint64_t>(qoff + lane) * W64;
-      const uint32_t w0 = s >> 6, sh = s & 63, last = (e - 1) >> 6;
-      const uint64_t x0 = row[w0];
-      const uint64_t x1 = w0 + 1 <= last ? row[w0 + 1] : 0;
-      const uint64_t x2 = w0 + 2 <= last ? row[w0 + 2] : 0;
-      clo = sh ? (x0 >> sh) | (x1 << (64 - sh)) : x0;
-      chi = sh ? (x1 >> sh) | (x2 << (64 - sh)) : x1;
-      if (n < 64) clo &= (1ull << n) - 1;
-      if (n <= 64) chi = 0;
-      else if (n < 128) chi &= (1ull << (n - 64)) - 1;
-      qa = static_cast<uint32_t>(q.adj[2 * static_cast<uint64_t>(qoff + lane)]);
-    }
-    // mapping[lane] and, per position, the depths mapped onto it (all start at local node 0)
-    uint32_t map = 0;
-    const uint32_t all_depths = k >= 32 ? 0xFFFFFFFFu : ((1u << k) - 1);
-    uint32_t dmlo = lane == 0 ? all_depths : 0u, dmhi = 0;
-
-    uint32_t depth = 0, pos = 0;
-    uint64_t visited = 0, count = 0;
+  auto grab = [&]() {
+    unsigned long long jj = 0;
+    if (lane == 0) jj = atomicAdd(&stats[2], static_cast<unsigned long long>(chunk));
+    next = uniform64(__shfl(jj, 0));
+    end = next + chunk < P ? next + chunk : P;
+  };
+  grab();
+  if (next < P) {
+    JoinPair cur, nxt;
+    join_load(q, d, bits, W64, indices, pair_graph, next, nullptr, cur);
+    join_finish(bits, W64, cur);
     while (true) {
-      ++steps;
-      if (depth == k) {  // leaf (:336-344)
-        ++count;
-        if (FIND_FIRST) break;
-        depth = k - 1;
-        pos = rl32(map, depth) + 1;
-        continue;
+      const uint64_t j = next++;
+      if (next >= end) grab();
+      const bool more = next < P;
+      if (more) join_load(q, d, bits, W64, indices, pair_graph, next, &cur, nxt);
+      const uint64_t count = cur.n > 64 ? join_dfs<FIND_FIRST, true>(cur, steps) : join_dfs<FIND_FIRST, false>(cur, steps);
+      if (lane == 0) {
+        if (count > 0x7FFFFFFFull) overflow = 1;
+        counts[j] = static_cast<int32_t>(count > 0x7FFFFFFFull ? 0x7FFFFFFFull : count);
       }
-      const uint64_t cand_lo = rl64(clo, depth);
-      const uint64_t cand_hi = wide ? rl64(chi, depth) : 0ull;
-      const uint32_t qad = rl32(qa, depth);
-      const uint32_t low = (1u << depth) - 1;  // depth < 32
-      const uint32_t need = qad & low, forbid = ~qad & low;
-      const uint64_t Alo = __ballot((dmlo & need) != 0), Nlo = __ballot((dmlo & forbid) != 0);
-      const uint64_t Ahi = wide ? __ballot((dmhi & need) != 0) : 0ull;
-      const uint64_t Nhi = wide ? __ballot((dmhi & forbid) != 0) : 0ull;
-      const uint64_t vlo = __ballot((alo & Alo) == Alo && (ahi & Ahi) == Ahi && !(alo & Nlo) && !(ahi & Nhi));
-      uint64_t elig_lo = cand_lo & vlo & ~visited & bits_from(pos);
-      uint64_t elig_hi = 0;
-      if (wide && !elig_lo) {
-        const uint64_t vhi = __ballot((blo & Alo) == Alo && (bhi & Ahi) == Ahi && !(blo & Nlo) && !(bhi & Nhi));
-        elig_hi = cand_hi & vhi & ~visited & (pos <= 64 ? ~0ull : bits_from(pos - 64));
-      }
-      if (elig_lo || elig_hi) {  // accept (:366-370)
-        const uint32_t c = elig_lo ? static_cast<uint32_t>(__builtin_ctzll(elig_lo)) : 64u + static_cast<uint32_t>(__builtin_ctzll(elig_hi));
-        const uint32_t old = rl32(map, depth);
-        const uint32_t bd = 1u << depth;
-        if (lane == (old & 63)) {
-          if (old < 64) dmlo &= ~bd;
-          else dmhi &= ~bd;
-        }
-        if (lane == (c & 63)) {
-          if (c < 64) dmlo |= bd;
-          else dmhi |= bd;
-        }
-        if (lane == depth) map = c;
-        visited |= 1ull << (c & 63);
-        ++depth;
-        pos = 0;
-      } else {  // exhausted frame (:346-354)
-        visited &= ~(1ull << (rl32(map, depth) & 63));
-        if (depth == 0) break;
-        if (depth == 1) visited = 0;
-        --depth;
-        pos = rl32(map, depth) + 1;
-      }
+      sum += count;
+      matched += count > 0;
+      if (!more) break;
+      join_finish(bits, W64, nxt);
+      cur = nxt;
     }
-    if (lane == 0) {
-      if (count > 0x7FFFFFFFull) overflow = 1;
-      counts[j] = static_cast<int32_t>(count > 0x7FFFFFFFull ? 0x7FFFFFFFull : count);
-    }
-    sum += count;
-    matched += count > 0;
   }
   if (lane == 0) {
     if (sum) atomicAdd(&stats[0], static_cast<unsigned long long>(sum));
@@ -742,10 +735,18 @@ hipError_t sig_generate(hipStream_t s, const GraphDev& g, uint64_t* sigs, uint64
   return hipGetLastError();
 }
 
-hipError_t sig_refine(hipStream_t s, const GraphDev& g, uint32_t view_size, uint64_t* sigs, uint64_t* reach, uint64_t* planes) {
+hipError_t sig_refine(hipStream_t s, const GraphDev& g, uint32_t view_size, uint64_t* sigs, uint64_t* reach, uint64_t* frontier_out,
+                      uint64_t* planes) {
   if (!g.N) return hipSuccess;
   const unsigned blocks = static_cast<unsigned>((g.N + kSigBlock - 1) / kSigBlock);
-  k_sig_refine<<<blocks, kSigBlock, 0, s>>>(g, view_size, sigs, reach, planes);
+  k_sig_refine<<<blocks, kSigBlock, 0, s>>>(g, view_size, sigs, reach, frontier_out, planes);
+  return hipGetLastError();
+}
+
+hipError_t sig_refine_next(hipStream_t s, const GraphDev& g, uint64_t* sigs, uint64_t* reach, uint64_t* frontier, uint64_t* planes) {
+  if (!g.N) return hipSuccess;
+  const unsigned blocks = static_cast<unsigned>((g.N + kSigBlock - 1) / kSigBlock);
+  k_sig_refine_next<<<blocks, kSigBlock, 0, s>>>(g, sigs, reach, frontier, planes);
   return hipGetLastError();
 }
 
@@ -753,7 +754,7 @@ hipError_t filter(hipStream_t s, const uint8_t* qlabels, uint64_t Q, const uint6
   const uint64_t W64 = (D + 63) / 64;
   if (!Q || !W64) return hipSuccess;
   const uint64_t cols = (W64 + kBitsBlock - 1) / kBitsBlock;
-  const unsigned rpb = rows_per_block(Q, cols, num_cus);
+  const unsigned rpb = rows_per_block(Q, cols, num_cus, 32, 16);  // store-only: favour parallelism
   const dim3 grid(static_cast<unsigned>(cols), static_cast<unsigned>((Q + rpb - 1) / rpb));
   k_filter<<<grid, kBitsBlock, 0, s>>>(qlabels, Q, lplanes, D, bits, rpb);
   return hipGetLastError();
@@ -771,7 +772,8 @@ hipError_t refine(hipStream_t s, const uint64_t* qsig, uint64_t Q, const uint64_
   const uint64_t W64 = (D + 63) / 64;
   if (!Q || !W64 || slots.count == 0) return hipSuccess;
   const uint64_t cols = (W64 + kBitsBlock - 1) / kBitsBlock;
-  const unsigned rpb = rows_per_block(Q, cols, num_cus);
+  // 32 B of planes per label and word against 16 B of row traffic per row
+  const unsigned rpb = rows_per_block(Q, cols, num_cus, 16 * static_cast<uint64_t>(slots.count));
   const dim3 grid(static_cast<unsigned>(cols), static_cast<unsigned>((Q + rpb - 1) / rpb));
   switch (slots.count) {
 #define SIGMO_REFINE_CASE(u) \

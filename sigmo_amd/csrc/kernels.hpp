@@ -47,8 +47,11 @@ hipError_t derive_graph(hipStream_t s, const uint32_t* go, uint32_t G, const uin
 
 // generateCSRSignatures: increments the neighbour-label histogram onto sigs.
 hipError_t sig_generate(hipStream_t s, const GraphDev& g, uint64_t* sigs, uint64_t* planes);
-// refineCSRSignatures<PowerGraph>(view_size).
-hipError_t sig_refine(hipStream_t s, const GraphDev& g, uint32_t view_size, uint64_t* sigs, uint64_t* reach, uint64_t* planes);
+// refineCSRSignatures<PowerGraph>(view_size); frontier_out (nullable) receives the last BFS layer.
+hipError_t sig_refine(hipStream_t s, const GraphDev& g, uint32_t view_size, uint64_t* sigs, uint64_t* reach, uint64_t* frontier_out,
+                      uint64_t* planes);
+// The same for view_size = M+1 when reach/frontier hold the state left by view_size = M.
+hipError_t sig_refine_next(hipStream_t s, const GraphDev& g, uint64_t* sigs, uint64_t* reach, uint64_t* frontier, uint64_t* planes);
 
 // filterCandidates<Query>: overwrites every word of the Q x W64 bitset.
 hipError_t filter(hipStream_t s, const uint8_t* qlabels, uint64_t Q, const uint64_t* lplanes, uint64_t D, uint64_t* bits,

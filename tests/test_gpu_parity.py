@@ -147,3 +147,48 @@ def test_config1_full_against_golden(ctx):
     assert digest(ca) == gold["counts_all_digest"]
     # size-independent property: find-first marks exactly the pairs find-all counts
     np.testing.assert_array_equal(cf, (ca > 0).astype(np.int32))
+
+
+@pytest.mark.parametrize("views", [(1, 2, 3, 4, 5, 6), (1, 2, 4, 5, 5, 6, 2, 7), (3, 4, 1, 5), (2, 2, 3)])
+def test_refine_view_sequences(ctx, views):
+    """Sequential views take the one-layer path, jumps and repeats the full BFS; both must
+    equal the reference restatement call by call (molecules up to 128 nodes)."""
+    dh = capi.HostGraphs.molecules(8, 400, capi.SIZE_LOGNORMAL)
+    da = dh.arrays()
+    dd = capi.DeviceGraph(ctx, dh)
+    sig = capi.Signature(ctx, dd.total_nodes, 1)
+    osig, oreach = np.zeros(dd.total_nodes, np.uint64), np.zeros(dd.total_nodes, np.uint64)
+    for view in views:
+        sig.refine_data(dd, view)
+        O.refine_signatures(da, view, osig, oreach)
+        np.testing.assert_array_equal(sig.download(capi.SCOPE_DATA), osig, err_msg=f"views {views} at {view}")
+
+
+def test_refine_with_two_graphs_of_equal_size(ctx):
+    """Reachables are per node id, not per graph: refining graph B after graph A combines
+    them exactly as the reference does (the one-layer shortcut must not be taken)."""
+    a = capi.HostGraphs.molecules(9, 200)
+    n = a.total_nodes
+    # graph B: same node count, every molecule's edges reversed in id order
+    arr = a.arrays()
+    go = arr["graph_offsets"]
+    graphs = []
+    for g in range(len(go) - 1):
+        lo, hi = int(go[g]), int(go[g + 1])
+        m = hi - lo
+        edges = set()
+        for v in range(lo, hi):
+            for w in arr["column_indices"][arr["row_offsets"][v]:arr["row_offsets"][v + 1]]:
+                u1, u2 = m - 1 - (v - lo), m - 1 - (int(w) - lo)
+                edges.add((min(u1, u2), max(u1, u2)))
+        graphs.append((arr["node_labels"][lo:hi].tolist(), sorted(edges)))
+    b = capi.HostGraphs.from_edge_lists(graphs)
+    assert b.total_nodes == n
+    da, db = a.arrays(), b.arrays()
+    ga, gb = capi.DeviceGraph(ctx, a), capi.DeviceGraph(ctx, b)
+    sig = capi.Signature(ctx, n, 1)
+    osig, oreach = np.zeros(n, np.uint64), np.zeros(n, np.uint64)
+    for graph, arrays, view in ((ga, da, 1), (ga, da, 2), (gb, db, 3), (gb, db, 4), (ga, da, 5)):
+        sig.refine_data(graph, view)
+        O.refine_signatures(arrays, view, osig, oreach)
+        np.testing.assert_array_equal(sig.download(capi.SCOPE_DATA), osig, err_msg=f"view {view}")
