@@ -26,15 +26,35 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 METRIC = "matched (query,molecule) pairs/sec on 1 MI355X; serving p50/p99 latency"
-HBM_KERNELS = ("refine", "filter", "gmcr", "sig_refine", "sig_generate")
+HBM_KERNELS = ("refine", "filter", "gmcr", "sig_refine", "sig_generate", "pair_filter", "db_prepare")
+
+
+# name: (seed, molecules, size law, queries, kmin, kmax, perturbed fraction, ring fraction)
+CONFIGS = {
+    1: (1, 100_000, "normal", 100, 4, 20, 0.2, -1.0),
+    2: (2, 1_000_000, "lognormal", 1000, 4, 32, 0.0, 0.3),
+}
 
 
 def workload(args):
     from sigmo_amd import capi
 
-    qh = capi.HostGraphs.queries(1, args.queries, 4, 20, capi.SIZE_NORMAL, perturb_frac=0.2)
-    dh = capi.HostGraphs.molecules(1, args.molecules, capi.SIZE_NORMAL)
+    seed, _, law, _, kmin, kmax, perturb, ring = CONFIGS[args.config]
+    dist = capi.SIZE_NORMAL if law == "normal" else capi.SIZE_LOGNORMAL
+    qh = capi.HostGraphs.queries(seed, args.queries, kmin, kmax, dist, perturb_frac=perturb, ring_frac=ring)
+    dh = capi.HostGraphs.molecules(seed, args.molecules, dist)
     return qh, dh
+
+
+def describe(args) -> str:
+    seed, _, law, _, kmin, kmax, perturb, ring = CONFIGS[args.config]
+    sizes = "Normal(24,8) sizes" if law == "normal" else "LogNormal(ln 24,0.45) sizes"
+    extra = f" {int(perturb * 100)}% perturbed" if perturb > 0 else ""
+    extra += f" {int(ring * 100)}% with a ring" if ring >= 0 else ""
+    mode = "find-first" if not args.find_all else "find-all"
+    form = "plain per-phase form" if args.form == "plain" else "batched pair form"
+    return (f"config {args.config} ({form}): {args.molecules} molecules seed {seed} {sizes}, "
+            f"{args.queries} queries k~U[{kmin},{kmax}]{extra}, R={args.refinement_steps}, {mode}")
 
 
 def cpu_baseline(qh, dh, args) -> dict:
@@ -46,15 +66,15 @@ def cpu_baseline(qh, dh, args) -> dict:
     sample = dh.slice(0, min(args.cpu_molecules, dh.num_graphs))
     qa, sa = qh.arrays(), sample.arrays()
     t0 = time.perf_counter()
-    r = O.pipeline(qa, sa, args.refinement_steps, modes=[True])
+    r = O.pipeline(qa, sa, args.refinement_steps, modes=[not args.find_all])
     dt = time.perf_counter() - t0
     return {
-        "value": r["total_first"] / dt,
+        "value": (r["total_first"] if not args.find_all else int((r["counts_all"] > 0).sum())) / dt,
         "unit": "pairs/s",
         "cores": O.threads(),
         "kind": "port",
         "sample": f"first {sample.num_graphs} of the {dh.num_graphs} molecules x all {qh.num_graphs} queries, "
-                  f"R={args.refinement_steps}, find-first; {r['total_first']} matched pairs in {dt:.2f} s",
+                  f"R={args.refinement_steps}; {dt:.2f} s",
     }
 
 
@@ -63,12 +83,21 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--molecules", type=int, default=100_000)
-    ap.add_argument("--queries", type=int, default=100)
+    ap.add_argument("--config", type=int, default=1, choices=sorted(CONFIGS))
+    ap.add_argument("--molecules", type=int, default=None, help="override the config's molecule count")
+    ap.add_argument("--queries", type=int, default=None, help="override the config's query count")
+    ap.add_argument("--find-all", action="store_true", help="count every embedding instead of find-first")
     ap.add_argument("--refinement-steps", type=int, default=6)
     ap.add_argument("--cpu-molecules", type=int, default=20_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--form", choices=("plain", "batched"), default="plain",
+                    help="plain: per-phase kernels over the Q x D bitset; batched: per-pair evaluation from "
+                         "per-molecule level planes (recomputed inside every step)")
     args = ap.parse_args()
+    if args.molecules is None:
+        args.molecules = CONFIGS[args.config][1]
+    if args.queries is None:
+        args.queries = CONFIGS[args.config][3]
 
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs SIGMo's hot path on a single MI355X "
@@ -81,27 +110,40 @@ def main() -> int:
     qh, dh = workload(args)
     ctx = capi.Context(0)
     qd, dd = capi.DeviceGraph(ctx, qh), capi.DeviceGraph(ctx, dh)
-    pipe = capi.Pipeline(ctx, qd, dd)
+    find_first = not args.find_all
+    if args.form == "plain":
+        pipe = capi.Pipeline(ctx, qd, dd)
+
+        def step():
+            return pipe.execute(args.refinement_steps, find_first)
+    else:
+        db = capi.Database(ctx, dd, args.refinement_steps)
+        batch = capi.Batch(ctx, db)
+
+        def step():
+            db.refresh()  # the data side is recomputed every step, as the plain form does
+            return batch.run(qd, find_first)
     print(f"[bench] uploaded: Q={qd.total_nodes} query nodes, D={dd.total_nodes} data nodes; warmup {args.warmup}",
           file=sys.stderr, flush=True)
     for _ in range(args.warmup):
-        pipe.execute(args.refinement_steps, True)
+        step()
     ctx.sync()
     ctx.reset_phases()
     results = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        results.append(pipe.execute(args.refinement_steps, True))
+        results.append(step())
     ctx.sync()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
-    matched = results[-1]["num_matches"]
-    assert all(r["num_matches"] == matched for r in results), "non-deterministic match count"
+    matched = results[-1]["matched_pairs"]
+    assert all(r["matched_pairs"] == matched for r in results), "non-deterministic match count"
     value = matched / (ms_per_step / 1000.0)
     print(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step, {matched} matched pairs", file=sys.stderr, flush=True)
 
     kstats = ctx.kernel_stats()
-    phase_ms = {k: round(sum(r["phase_ms"][k] for r in results) / len(results), 4) for k in capi.PHASE_NAMES}
+    phase_ms = ctx.phase_ms()
+    phase_ms = {k: round(phase_ms[k] / args.steps, 4) for k in capi.PHASE_NAMES}
     kernels = {}
     for name, (n, ms, by) in kstats.items():
         if n:
@@ -148,8 +190,7 @@ def main() -> int:
         "dtype": "u64",
         "data": "synthetic",
         "config": {
-            "workload": f"config 1 (plain per-phase form): {args.molecules} molecules seed 1 Normal(24,8) sizes, "
-                        f"{args.queries} queries k~U[4,20] 20% perturbed, R={args.refinement_steps}, find-first",
+            "workload": describe(args),
             "molecules": args.molecules,
             "queries": args.queries,
             "query_nodes": qd.total_nodes,
@@ -157,7 +198,9 @@ def main() -> int:
             "refinement_steps": args.refinement_steps,
             "gmcr_pairs": results[-1]["gmcr_pairs"],
             "matched_pairs": matched,
-            "form": "plain per-phase",
+            "embeddings": results[-1]["num_matches"] if args.find_all else None,
+            "dfs_steps": results[-1]["dfs_steps"],
+            "form": args.form,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

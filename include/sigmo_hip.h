@@ -208,10 +208,42 @@ int sigmo_pipeline_free(sigmo_pipeline* p);
 #define SIGMO_KERNEL_REFINE 3
 #define SIGMO_KERNEL_GMCR 4
 #define SIGMO_KERNEL_JOIN 5
-#define SIGMO_KERNEL_COUNT 6
+#define SIGMO_KERNEL_PAIR_FILTER 6 /* batched form: fused filter/refine/GMCR predicate */
+#define SIGMO_KERNEL_DB_PREPARE 7  /* batched form: per-molecule level planes            */
+#define SIGMO_KERNEL_COUNT 8
 /* Launch count, summed milliseconds and summed algorithmic HBM bytes of one kernel family
  * since the last sigmo_phase_reset. */
 int sigmo_kernel_stats(sigmo_ctx* ctx, int kernel, uint64_t* launches, double* ms, double* bytes);
+
+/* ---- batched form (no reference counterpart: same results as the plain pipeline) ------- */
+/* The data side is prepared once per device graph and stays resident: every molecule's
+ * PowerGraph level signatures for levels 1..R as bit-planes over its node positions.  A
+ * batch run then evaluates each (query graph, molecule) pair directly -- the label filter
+ * and the R refine levels per query node inside the molecule, stopping at the first query
+ * node without candidates (the GMCR predicate) -- keeps the surviving pairs' candidate masks,
+ * orders them as GMCR does and joins them.  No Q x D bitset is materialised.  GMCR arrays and
+ * per-pair counts equal those of sigmo_pipeline_run with the same R. */
+typedef struct sigmo_db sigmo_db;
+typedef struct sigmo_batch sigmo_batch;
+typedef struct sigmo_batch_result {
+  uint64_t num_matches;   /* sum of per-pair counts                      */
+  uint64_t matched_pairs; /* pairs with count > 0                         */
+  uint64_t gmcr_pairs;    /* surviving (query graph, molecule) pairs      */
+  uint64_t dfs_steps;     /* join DFS frame steps                         */
+} sigmo_batch_result;
+int sigmo_db_prepare(sigmo_ctx* ctx, const sigmo_graph* data, uint32_t refinement_steps, sigmo_db** out);
+/* Recompute the level planes from the data graph's current device contents (same result as a
+ * fresh sigmo_db_prepare; no allocation). */
+int sigmo_db_refresh(sigmo_db* db);
+int sigmo_db_free(sigmo_db* db);
+int sigmo_db_alloc_size(const sigmo_db* db, uint64_t* bytes);
+int sigmo_batch_create(sigmo_ctx* ctx, const sigmo_db* db, sigmo_batch** out);
+int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, sigmo_batch_result* result);
+/* GMCR arrays (data_graph_offsets[Gd+1], query_graph_indices[P]) and per-pair counts of the
+ * last run; any pointer may be null. */
+int sigmo_batch_pairs(sigmo_batch* b, uint32_t* data_graph_offsets, uint64_t n_offsets, uint32_t* query_graph_indices,
+                      int32_t* counts, uint64_t n_pairs);
+int sigmo_batch_free(sigmo_batch* b);
 
 #ifdef __cplusplus
 }

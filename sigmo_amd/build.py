@@ -39,7 +39,7 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wextra", "-march=x86-64-v2"]
 
-HIP_SOURCES = ["kernels.hip", "capi.hip"]
+HIP_SOURCES = ["kernels.hip", "capi.hip", "batched.hip"]
 HOST_SOURCES = ["host_graphs.cpp", "generator.cpp"]
 
 

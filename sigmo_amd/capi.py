@@ -52,9 +52,10 @@ EXPORTED = (
     "sigmo_candidates_counts", "sigmo_candidates_alloc_size", "sigmo_gmcr_create", "sigmo_gmcr_free",
     "sigmo_gmcr_generate", "sigmo_gmcr_total", "sigmo_gmcr_download", "sigmo_join", "sigmo_pipeline_run",
     "sigmo_pipeline_create", "sigmo_pipeline_execute", "sigmo_pipeline_pairs", "sigmo_pipeline_free",
-    "sigmo_kernel_stats", "sigmo_event_mark", "sigmo_event_elapsed",
+    "sigmo_kernel_stats", "sigmo_event_mark", "sigmo_event_elapsed", "sigmo_db_prepare", "sigmo_db_free",
+    "sigmo_db_alloc_size", "sigmo_db_refresh", "sigmo_batch_create", "sigmo_batch_run", "sigmo_batch_pairs", "sigmo_batch_free",
 )
-KERNEL_NAMES = ("sig_generate", "sig_refine", "filter", "refine", "gmcr", "join")
+KERNEL_NAMES = ("sig_generate", "sig_refine", "filter", "refine", "gmcr", "join", "pair_filter", "db_prepare")
 
 
 class SigmoError(RuntimeError):
@@ -70,6 +71,15 @@ class PipelineResult(C.Structure):
         ("matched_pairs", C.c_uint64),
         ("phase_ms", C.c_double * 5),
         ("total_ms", C.c_double),
+        ("dfs_steps", C.c_uint64),
+    ]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [
+        ("num_matches", C.c_uint64),
+        ("matched_pairs", C.c_uint64),
+        ("gmcr_pairs", C.c_uint64),
         ("dfs_steps", C.c_uint64),
     ]
 
@@ -137,6 +147,14 @@ def lib() -> C.CDLL:
             "sigmo_kernel_stats": ([vp, i32, C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl)], i32),
             "sigmo_event_mark": ([vp, C.POINTER(u64)], i32),
             "sigmo_event_elapsed": ([vp, u64, u64, C.POINTER(dbl)], i32),
+            "sigmo_db_prepare": ([vp, vp, u32, pvp], i32),
+            "sigmo_db_free": ([vp], i32),
+            "sigmo_db_refresh": ([vp], i32),
+            "sigmo_db_alloc_size": ([vp, C.POINTER(u64)], i32),
+            "sigmo_batch_create": ([vp, vp, pvp], i32),
+            "sigmo_batch_run": ([vp, vp, i32, C.POINTER(BatchResult)], i32),
+            "sigmo_batch_pairs": ([vp, vp, u64, vp, vp, u64], i32),
+            "sigmo_batch_free": ([vp], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -541,4 +559,70 @@ class Pipeline:
         cnt = np.zeros(P, np.int32)
         check(lib().sigmo_pipeline_pairs(self._h, _ptr(off), off.size, _ptr(idx) if P else None,
                                          _ptr(cnt) if P else None, P))
+        return off, idx, cnt
+
+
+class Database:
+    """Resident, query-independent data side of the batched form (sigmo_db_*)."""
+
+    def __init__(self, ctx: Context, data: DeviceGraph, refinement_steps: int):
+        h = C.c_void_p()
+        check(lib().sigmo_db_prepare(ctx.handle, data.handle, refinement_steps, C.byref(h)))
+        self._h, self.ctx, self.data, self.refinement_steps = h, ctx, data, refinement_steps
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().sigmo_db_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.free()
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def device_bytes(self) -> int:
+        v = C.c_uint64()
+        check(lib().sigmo_db_alloc_size(self._h, C.byref(v)))
+        return v.value
+
+    def refresh(self) -> None:
+        check(lib().sigmo_db_refresh(self._h))
+
+
+class Batch:
+    """Batched-form runs of query graphs against a Database (sigmo_batch_*)."""
+
+    def __init__(self, ctx: Context, db: Database):
+        h = C.c_void_p()
+        check(lib().sigmo_batch_create(ctx.handle, db.handle, C.byref(h)))
+        self._h, self.ctx, self.db = h, ctx, db
+        self.last = None
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().sigmo_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.free()
+
+    def run(self, query: DeviceGraph, find_first: bool = True) -> dict:
+        r = BatchResult()
+        check(lib().sigmo_batch_run(self._h, query.handle, int(find_first), C.byref(r)))
+        self.last = {"num_matches": r.num_matches, "matched_pairs": r.matched_pairs, "gmcr_pairs": r.gmcr_pairs,
+                     "dfs_steps": r.dfs_steps}
+        return self.last
+
+    def pairs(self):
+        """(data_graph_offsets, query_graph_indices, counts) of the last run, GMCR order."""
+        P = self.last["gmcr_pairs"]
+        off = np.zeros(self.db.data.num_graphs + 1, np.uint32)
+        idx = np.zeros(P, np.uint32)
+        cnt = np.zeros(P, np.int32)
+        check(lib().sigmo_batch_pairs(self._h, _ptr(off), off.size, _ptr(idx) if P else None, _ptr(cnt) if P else None, P))
         return off, idx, cnt

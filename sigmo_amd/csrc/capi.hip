@@ -70,9 +70,9 @@ struct sigmo_ctx {
   std::vector<double> launch_ms;    // per-launch kernel ms, indexed by seq - launch_base
   std::vector<Timed> pending;
   double accum_ms[SIGMO_PHASE_COUNT] = {0, 0, 0, 0, 0};
-  uint64_t k_launches[SIGMO_KERNEL_COUNT] = {0, 0, 0, 0, 0, 0};
-  double k_ms[SIGMO_KERNEL_COUNT] = {0, 0, 0, 0, 0, 0};
-  double k_bytes[SIGMO_KERNEL_COUNT] = {0, 0, 0, 0, 0, 0};
+  uint64_t k_launches[SIGMO_KERNEL_COUNT] = {};
+  double k_ms[SIGMO_KERNEL_COUNT] = {};
+  double k_bytes[SIGMO_KERNEL_COUNT] = {};
   std::vector<hipEvent_t> pool;
   unsigned long long* stats = nullptr;       // join counters [8]: matches, matched pairs, cursor, overflow, steps
   unsigned long long* stats_host = nullptr;  // pinned mirror
@@ -266,6 +266,46 @@ int grow(T** p, uint64_t& cap, uint64_t need) {
 }
 
 }  // namespace
+
+// Accessors used by the batched form (batched.hip).
+namespace sigmo_internal {
+int ctx_info(sigmo_ctx* ctx, int* device, hipStream_t* stream, int* num_cus, unsigned long long** stats,
+             unsigned long long** stats_host) {
+  SIGMO_TRY(bind(ctx));
+  *device = ctx->device;
+  *stream = ctx->stream;
+  *num_cus = ctx->num_cus;
+  *stats = ctx->stats;
+  *stats_host = ctx->stats_host;
+  return SIGMO_OK;
+}
+// begin: start a timed region for (phase, kernel); end: close the region of `token`.
+int ctx_timed(sigmo_ctx* ctx, int phase, int kernel, double bytes, bool begin, void** token) {
+  if (begin) {
+    auto* ps = new PhaseScope(ctx, phase, kernel, bytes);
+    if (ps->status != SIGMO_OK) {
+      const int st = ps->status;
+      delete ps;
+      return sigmo_err::set(st, "event record failed");
+    }
+    *token = ps;
+    return SIGMO_OK;
+  }
+  auto* ps = static_cast<PhaseScope*>(*token);
+  *token = nullptr;
+  if (!ps) return SIGMO_OK;
+  const int st = ps->close();
+  delete ps;
+  return st;
+}
+// Bytes known only once a kernel has finished (data-dependent outputs) join its accumulator.
+void ctx_add_bytes(sigmo_ctx* ctx, int kernel, double bytes) {
+  if (kernel >= 0 && kernel < SIGMO_KERNEL_COUNT) ctx->k_bytes[kernel] += bytes;
+}
+const sigmo_k::GraphDev graph_dev(const sigmo_graph* g) { return g->dev(); }
+const std::vector<uint32_t>& graph_host_offsets(const sigmo_graph* g) { return g->host_go; }
+sigmo_ctx* graph_ctx(const sigmo_graph* g) { return g->ctx; }
+}  // namespace sigmo_internal
 
 extern "C" {
 

@@ -1,0 +1,57 @@
+// Device helpers shared by the plain-form and batched-form kernels (wave64, gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sigmo_k {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & (kWave - 1)); }
+
+__device__ __forceinline__ uint32_t rl32(uint32_t v, uint32_t lane) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(lane)));
+}
+__device__ __forceinline__ uint64_t rl64(uint64_t v, uint32_t lane) {
+  return static_cast<uint64_t>(rl32(static_cast<uint32_t>(v), lane)) | (static_cast<uint64_t>(rl32(static_cast<uint32_t>(v >> 32), lane)) << 32);
+}
+__device__ __forceinline__ uint32_t uniform32(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+}
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  return static_cast<uint64_t>(uniform32(static_cast<uint32_t>(v))) | (static_cast<uint64_t>(uniform32(static_cast<uint32_t>(v >> 32))) << 32);
+}
+// Bits at positions >= pos of a 64-bit word (0 when pos >= 64).
+__device__ __forceinline__ uint64_t bits_from(uint32_t pos) { return pos >= 64 ? 0ull : (~0ull << pos); }
+
+// x >= c for 64 four-bit values held as bit planes x[0..3] and a wave-uniform c.
+__device__ __forceinline__ uint64_t ge4(const uint64_t x[4], uint32_t c) {
+  uint64_t gt = 0, eq = ~0ull;
+#pragma unroll
+  for (int b = 3; b >= 0; --b) {
+    if ((c >> b) & 1u) {
+      eq &= x[b];
+    } else {
+      gt |= eq & x[b];
+      eq &= ~x[b];
+    }
+  }
+  return gt | eq;
+}
+
+// The same with a per-lane c (no branches).
+__device__ __forceinline__ uint64_t ge4_lane(uint64_t x0, uint64_t x1, uint64_t x2, uint64_t x3, uint32_t c) {
+  const uint64_t x[4] = {x0, x1, x2, x3};
+  uint64_t gt = 0, eq = ~0ull;
+#pragma unroll
+  for (int b = 3; b >= 0; --b) {
+    const uint64_t m = 0ull - static_cast<uint64_t>((c >> b) & 1u);  // all ones if bit b of c is set
+    gt |= eq & x[b] & ~m;
+    eq &= ~(x[b] ^ m);
+  }
+  return gt | eq;
+}
+
+}  // namespace sigmo_k

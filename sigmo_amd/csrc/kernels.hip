@@ -8,13 +8,12 @@
 // PowerGraph reachables starting at zero, per-row refine, ascending query ids inside GMCR
 // segments, fresh join state per pair with mapping[] = local node 0.
 #include "kernels.hpp"
+#include "device_common.hpp"
+#include "join_dfs.hpp"
 
 namespace sigmo_k {
 namespace {
 
-constexpr int kWave = 64;
-
-__device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & (kWave - 1)); }
 
 // Largest g with go[g] <= v: the upper-bound search of graph.hpp:100-111.
 __device__ __forceinline__ uint32_t graph_of(const uint32_t* go, uint32_t G, uint32_t v) {
@@ -255,21 +254,6 @@ __global__ __launch_bounds__(kBitsBlock) void k_filter(const uint8_t* __restrict
   }
 }
 
-// x >= c for 64 four-bit values held as bit planes x[0..3] and a wave-uniform c.
-__device__ __forceinline__ uint64_t ge4(const uint64_t x[4], uint32_t c) {
-  uint64_t gt = 0, eq = ~0ull;
-#pragma unroll
-  for (int b = 3; b >= 0; --b) {
-    if ((c >> b) & 1u) {
-      eq &= x[b];
-    } else {
-      gt |= eq & x[b];
-      eq &= ~x[b];
-    }
-  }
-  return gt | eq;
-}
-
 // refineCandidates<Query>, isomorphism.hpp:64-127: a set bit (q,d) survives iff for every
 // label l < 16, count_q(l) <= count_d(l) at the current level.  Only labels that occur as
 // query node labels can have nonzero query counts; their data planes (4 per label) sit in
@@ -477,170 +461,6 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_final(const uint32_t* __res
     run += v[i];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = static_cast<uint32_t>(*total_dev);
-}
-
-// ---- join, one wavefront per pair ------------------------------------------------------------------
-
-__device__ __forceinline__ uint32_t rl32(uint32_t v, uint32_t lane) {
-  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(lane)));
-}
-__device__ __forceinline__ uint64_t rl64(uint64_t v, uint32_t lane) {
-  return static_cast<uint64_t>(rl32(static_cast<uint32_t>(v), lane)) | (static_cast<uint64_t>(rl32(static_cast<uint32_t>(v >> 32), lane)) << 32);
-}
-__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
-  return static_cast<uint64_t>(static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)))) |
-         (static_cast<uint64_t>(static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v >> 32)))) << 32);
-}
-__device__ __forceinline__ uint64_t bits_from(uint32_t pos) { return pos >= 64 ? 0ull : (~0ull << pos); }
-
-// Per-pair operands of the wavefront join: molecule range, query range, the lane-distributed
-// candidate masks / query adjacency (lane d = query node d) and molecule adjacency (lane p =
-// node p, and node p+64 for molecules of more than 64 nodes).
-struct JoinPair {
-  uint32_t dg, s, e, n, qoff, k;
-  uint64_t clo, chi;  // candidates of query node `lane` at molecule positions 0-63 / 64-127
-  uint32_t qa;        // adjacency of query node `lane` by query-local id
-  uint64_t alo, ahi, blo, bhi;
-};
-
-__device__ __forceinline__ void join_load(const GraphDev& q, const GraphDev& d, const uint64_t* __restrict__ bits, uint64_t W64,
-                                          const uint32_t* __restrict__ indices, const uint32_t* __restrict__ pair_graph, uint64_t j,
-                                          const JoinPair* prev, JoinPair& o) {
-  const uint32_t lane = static_cast<uint32_t>(lane_id());
-  const uint32_t dg = pair_graph[j], qg = indices[j];
-  o.dg = dg;
-  o.s = d.go[dg];
-  o.e = d.go[dg + 1];
-  o.n = o.e - o.s;
-  o.qoff = q.go[qg];
-  o.k = q.go[qg + 1] - o.qoff;
-  const uint32_t s = o.s, e = o.e, n = o.n, k = o.k;
-  if (prev && prev->dg == dg) {
-    o.alo = prev->alo;
-    o.ahi = prev->ahi;
-    o.blo = prev->blo;
-    o.bhi = prev->bhi;
-  } else {
-    o.alo = o.ahi = o.blo = o.bhi = 0;
-    if (lane < n) {
-      o.alo = d.adj[2 * (static_cast<uint64_t>(s) + lane)];
-      o.ahi = d.adj[2 * (static_cast<uint64_t>(s) + lane) + 1];
-    }
-    if (lane + 64 < n) {
-      o.blo = d.adj[2 * (static_cast<uint64_t>(s) + lane + 64)];
-      o.bhi = d.adj[2 * (static_cast<uint64_t>(s) + lane + 64) + 1];
-    }
-  }
-  o.clo = o.chi = 0;
-  o.qa = 0;
-  if (lane < k) {
-    const uint64_t* row = bits + static_cast<uint64_t>(o.qoff + lane) * W64;
-    const uint32_t w0 = s >> 6, last = (e - 1) >> 6;
-    o.clo = row[w0];
-    o.chi = w0 + 1 <= last ? row[w0 + 1] : 0;
-    o.qa = static_cast<uint32_t>(q.adj[2 * static_cast<uint64_t>(o.qoff + lane)]);
-    // a third word (molecules straddling three words) is read in join_finish
-  }
-}
-
-// Turn the raw words of join_load into molecule-local candidate masks (funnel shift by s mod 64).
-__device__ __forceinline__ void join_finish(const uint64_t* __restrict__ bits, uint64_t W64, JoinPair& o) {
-  const uint32_t lane = static_cast<uint32_t>(lane_id());
-  if (lane < o.k) {
-    const uint32_t sh = o.s & 63, w0 = o.s >> 6, last = (o.e - 1) >> 6;
-    uint64_t x0 = o.clo, x1 = o.chi, x2 = 0;
-    if (w0 + 2 <= last) x2 = bits[static_cast<uint64_t>(o.qoff + lane) * W64 + w0 + 2];
-    uint64_t lo = sh ? (x0 >> sh) | (x1 << (64 - sh)) : x0;
-    uint64_t hi = sh ? (x1 >> sh) | (x2 << (64 - sh)) : x1;
-    if (o.n < 64) lo &= (1ull << o.n) - 1;
-    if (o.n <= 64) hi = 0;
-    else if (o.n < 128) hi &= (1ull << (o.n - 64)) - 1;
-    o.clo = lo;
-    o.chi = hi;
-  }
-}
-
-// The reference DFS of one pair on the lane-distributed operands (see k_join_wave).  WIDE
-// handles molecules of 65..128 nodes (positions 64..127 in the `hi` words).
-__device__ __forceinline__ uint32_t uniform32(uint32_t v) {
-  return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
-}
-
-template <bool FIND_FIRST, bool WIDE>
-__device__ __forceinline__ uint64_t join_dfs(const JoinPair& o, uint64_t& steps) {
-  const uint32_t lane = static_cast<uint32_t>(lane_id());
-  const uint32_t k = uniform32(o.k);
-  uint32_t map = 0;             // lane d: mapping[d] (local id; every entry starts at node 0)
-  uint64_t mslo = 0, mshi = 0;  // lane d: positions mapped at depths < d on the current path
-  uint32_t depth = 0, pos = 0;
-  uint64_t visited = 0, count = 0;
-  while (true) {
-    ++steps;
-    if (depth == k) {  // leaf (:336-344)
-      ++count;
-      if (FIND_FIRST) break;
-      depth = k - 1;
-      pos = rl32(map, depth) + 1;
-      continue;
-    }
-    const uint64_t cl = rl64(o.clo, depth) & ~visited & bits_from(pos);
-    const uint64_t ch = WIDE ? rl64(o.chi, depth) & ~visited & (pos <= 64 ? ~0ull : bits_from(pos - 64)) : 0ull;
-    uint64_t el = 0, eh = 0, Ml = 0, Mh = 0;
-    if (cl | ch) {
-      if (depth == 0) {
-        el = cl;
-        eh = ch;
-      } else {
-        // A: positions that must be adjacent (mapped query neighbours), N: must not be
-        const uint32_t qad = rl32(o.qa, depth), low = (1u << depth) - 1;
-        uint64_t Al = 0, Ah = 0, Nl, Nh;
-        for (uint32_t t = qad & low; t; t &= t - 1) {
-          const uint32_t m = rl32(map, static_cast<uint32_t>(__builtin_ctz(t)));
-          if (!WIDE || m < 64) Al |= 1ull << (m & 63);
-          else Ah |= 1ull << (m & 63);
-        }
-        Ml = rl64(mslo, depth);
-        Mh = WIDE ? rl64(mshi, depth) : 0ull;
-        if (static_cast<uint32_t>(__popcll(Ml) + __popcll(Mh)) == depth) {  // injective: N = mapped \ A
-          Nl = Ml & ~Al;
-          Nh = Mh & ~Ah;
-        } else {  // a stale unset let a node be mapped twice: collect N exactly
-          Nl = Nh = 0;
-          for (uint32_t t = ~qad & low; t; t &= t - 1) {
-            const uint32_t m = rl32(map, static_cast<uint32_t>(__builtin_ctz(t)));
-            if (!WIDE || m < 64) Nl |= 1ull << (m & 63);
-            else Nh |= 1ull << (m & 63);
-          }
-        }
-        if (cl) {
-          const bool ok = (o.alo & Al) == Al && !(o.alo & Nl) && (!WIDE || ((o.ahi & Ah) == Ah && !(o.ahi & Nh)));
-          el = cl & __ballot(ok);
-        }
-        if (WIDE && !el && ch) {
-          const bool ok = (o.blo & Al) == Al && !(o.blo & Nl) && (o.bhi & Ah) == Ah && !(o.bhi & Nh);
-          eh = ch & __ballot(ok);
-        }
-      }
-    }
-    if (el | eh) {  // accept (:366-370)
-      const uint32_t c = el ? static_cast<uint32_t>(__builtin_ctzll(el)) : 64u + static_cast<uint32_t>(__builtin_ctzll(eh));
-      if (lane == depth) map = c;
-      if (lane == depth + 1) {
-        mslo = Ml | (c < 64 ? 1ull << c : 0ull);
-        if (WIDE) mshi = Mh | (c >= 64 ? 1ull << (c & 63) : 0ull);
-      }
-      visited |= 1ull << (c & 63);
-      ++depth;
-      pos = 0;
-    } else {  // exhausted frame (:346-354)
-      visited &= ~(1ull << (rl32(map, depth) & 63));
-      if (depth == 0) break;
-      if (depth == 1) visited = 0;
-      --depth;
-      pos = rl32(map, depth) + 1;
-    }
-  }
-  return count;
 }
 
 // joinCandidates, isomorphism.hpp:270-381, with one wavefront per (query, molecule) pair.

@@ -1,0 +1,154 @@
+"""GPU parity of the batched form (sigmo_db_* / sigmo_batch_*) against the CPU oracle.
+
+The batched form never materialises the candidate bitset: it evaluates each (query graph,
+molecule) pair from level signatures prepared once per molecule.  Its GMCR arrays and per-pair
+join counts must equal the oracle's restatement of the reference driver
+(library/src/sigmo.cpp:102-159) bit for bit, in both join modes.
+"""
+from __future__ import annotations
+
+import json
+
+import numpy as np
+import pytest
+
+from sigmo_amd import capi
+from oracle import oracle as O
+from tests._util import GOLDEN, digest
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    if name == "cfg0":
+        return (capi.HostGraphs.queries(0, 20, 4, 12, capi.SIZE_NORMAL), capi.HostGraphs.molecules(0, 2000), 6)
+    if name == "cfg1_slice":
+        return (capi.HostGraphs.queries(1, 100, 4, 20, capi.SIZE_NORMAL, perturb_frac=0.2),
+                capi.HostGraphs.molecules(1, 5000), 6)
+    if name == "lognormal_aliasing":  # molecules of 65..128 nodes use the second position word
+        return (capi.HostGraphs.queries(2, 40, 4, 32, capi.SIZE_LOGNORMAL, ring_frac=0.3),
+                capi.HostGraphs.molecules(2, 3000, capi.SIZE_LOGNORMAL), 6)
+    if name == "reference_fixtures":
+        return (capi.HostGraphs.load_dat(GOLDEN / "test_query.dat"), capi.HostGraphs.load_dat(GOLDEN / "test_data.dat"), 3)
+    raise KeyError(name)
+
+
+def _batched(ctx, qh, dh, R, modes=(True, False), db=None, batch=None):
+    qd, dd = capi.DeviceGraph(ctx, qh), capi.DeviceGraph(ctx, dh)
+    db = db or capi.Database(ctx, dd, R)
+    batch = batch or capi.Batch(ctx, db)
+    out = {}
+    for mode in modes:
+        r = batch.run(qd, find_first=mode)
+        off, idx, cnt = batch.pairs()
+        key = "first" if mode else "all"
+        out["offsets"], out["indices"] = off, idx
+        out[f"counts_{key}"], out[f"total_{key}"] = cnt, r["num_matches"]
+        out[f"result_{key}"] = r
+    return out
+
+
+def _assert_equal(g, o, modes=(True, False)):
+    np.testing.assert_array_equal(g["offsets"], o["offsets"], err_msg="GMCR offsets")
+    np.testing.assert_array_equal(g["indices"], o["indices"], err_msg="GMCR indices")
+    for mode in modes:
+        key = "first" if mode else "all"
+        np.testing.assert_array_equal(g[f"counts_{key}"], o[f"counts_{key}"], err_msg=f"join counts ({key})")
+        assert g[f"total_{key}"] == o[f"total_{key}"]
+        r = g[f"result_{key}"]
+        assert r["gmcr_pairs"] == len(o["indices"])
+        assert r["matched_pairs"] == int((o[f"counts_{key}"] > 0).sum())
+
+
+@pytest.mark.parametrize("name", ["cfg0", "cfg1_slice", "lognormal_aliasing", "reference_fixtures"])
+def test_batched_parity(ctx, name):
+    qh, dh, R = _case(name)
+    print(f"batched case {name}: queries {qh.info()} data {dh.info()} R={R}", flush=True)
+    g = _batched(ctx, qh, dh, R)
+    o = O.pipeline(qh.arrays(), dh.arrays(), R, modes=[True, False])
+    _assert_equal(g, o)
+
+
+@pytest.mark.parametrize("R", [1, 2, 8])
+def test_batched_refinement_depths(ctx, R):
+    qh, dh = capi.HostGraphs.queries(3, 30, 4, 16, capi.SIZE_NORMAL, perturb_frac=0.1), capi.HostGraphs.molecules(3, 1500)
+    print(f"batched R={R}", flush=True)
+    _assert_equal(_batched(ctx, qh, dh, R), O.pipeline(qh.arrays(), dh.arrays(), R, modes=[True, False]))
+
+
+def test_batched_wildcards_overflow_and_replay(ctx):
+    """Wildcard queries keep nearly every pair: the record and arena buffers overflow their first
+    sizing and the run is replayed with larger ones; labels above 15 use the slow label path."""
+    qs = [([255, 255], [(0, 1)])] * 40 + [([255, 8, 255], [(0, 1), (1, 2)])] * 20 + [([20, 8], [(0, 1)])] * 4
+    mols = [([8, 0, 7, 8, 8, 9, 8], [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (5, 6), (6, 0)]),
+            ([20, 8, 8], [(0, 1), (1, 2)]), ([3], [])]
+    extra = capi.HostGraphs.molecules(4, 1500)
+    qh = capi.HostGraphs.from_edge_lists(qs)
+    dh = capi.HostGraphs.from_edge_lists(mols).concat(extra)
+    print("batched wildcard case", flush=True)
+    g = _batched(ctx, qh, dh, 4)
+    o = O.pipeline(qh.arrays(), dh.arrays(), 4, modes=[True, False])
+    _assert_equal(g, o)
+    assert len(o["indices"]) > 65536  # beyond the initial record capacity
+
+
+def test_batched_reuse_across_query_sets(ctx):
+    """One database and one batch object serve successive query sets, larger then smaller."""
+    dh = capi.HostGraphs.molecules(6, 2500)
+    dd = capi.DeviceGraph(ctx, dh)
+    db = capi.Database(ctx, dd, 6)
+    batch = capi.Batch(ctx, db)
+    for seed, count in ((6, 60), (7, 9), (8, 25)):
+        qh = capi.HostGraphs.queries(seed, count, 4, 20, capi.SIZE_NORMAL, perturb_frac=0.2)
+        print(f"batched reuse seed {seed}", flush=True)
+        g = _batched(ctx, qh, dh, 6, db=db, batch=batch)
+        _assert_equal(g, O.pipeline(qh.arrays(), dh.arrays(), 6, modes=[True, False]))
+    assert db.device_bytes > 0
+
+
+def test_batched_matches_plain_pipeline(ctx):
+    qh, dh, R = _case("cfg1_slice")
+    qd, dd = capi.DeviceGraph(ctx, qh), capi.DeviceGraph(ctx, dh)
+    pipe = capi.Pipeline(ctx, qd, dd)
+    batch = capi.Batch(ctx, capi.Database(ctx, dd, R))
+    for mode in (True, False):
+        p = pipe.execute(R, mode)
+        b = batch.run(qd, find_first=mode)
+        assert (p["num_matches"], p["gmcr_pairs"], p["matched_pairs"]) == (b["num_matches"], b["gmcr_pairs"], b["matched_pairs"])
+        assert p["dfs_steps"] == b["dfs_steps"]
+        po, pi, pc = pipe.pairs()
+        bo, bi, bc = batch.pairs()
+        np.testing.assert_array_equal(po, bo)
+        np.testing.assert_array_equal(pi, bi)
+        np.testing.assert_array_equal(pc, bc)
+
+
+def test_batched_config1_full_against_golden(ctx):
+    """Config 1 at full size (100,000 molecules) against the oracle's committed digests."""
+    gold = json.loads((GOLDEN / "config1.json").read_text())
+    qh = capi.HostGraphs.queries(1, 100, 4, 20, capi.SIZE_NORMAL, perturb_frac=0.2)
+    dh = capi.HostGraphs.molecules(1, 100_000)
+    qd, dd = capi.DeviceGraph(ctx, qh), capi.DeviceGraph(ctx, dh)
+    batch = capi.Batch(ctx, capi.Database(ctx, dd, 6))
+    first = batch.run(qd, find_first=True)
+    off, idx, cf = batch.pairs()
+    assert digest(off, idx) == gold["gmcr_digest"]
+    assert first["num_matches"] == gold["matches_first"]
+    assert digest(cf) == gold["counts_first_digest"]
+    allr = batch.run(qd, find_first=False)
+    _, _, ca = batch.pairs()
+    assert allr["num_matches"] == gold["matches_all"]
+    assert digest(ca) == gold["counts_all_digest"]
+
+
+def test_batched_rejects_unsupported_inputs(ctx):
+    dh = capi.HostGraphs.molecules(9, 20)
+    dd = capi.DeviceGraph(ctx, dh)
+    with pytest.raises(capi.SigmoError):
+        capi.Database(ctx, dd, 0)
+    with pytest.raises(capi.SigmoError):
+        capi.Database(ctx, dd, 9)
+    batch = capi.Batch(ctx, capi.Database(ctx, dd, 2))
+    single = capi.DeviceGraph(ctx, capi.HostGraphs.from_edge_lists([([8], [])]))
+    with pytest.raises(capi.SigmoError):
+        batch.run(single)
