@@ -57,13 +57,27 @@ def describe(args) -> str:
             f"{args.queries} queries k~U[{kmin},{kmax}]{extra}, R={args.refinement_steps}, {mode}")
 
 
+CPU_SAMPLE_PAIRS = 10_000_000  # ~3 s of oracle work on 16 cores at config 1 rates; whole workload for config 1
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(qh, dh, args) -> dict:
     """Oracle (CPU restatement of the reference kernels) on the first molecules of the workload."""
     from oracle import oracle as O
 
     threads = int(os.environ.get("SIGMO_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
     O.set_threads(threads)
-    sample = dh.slice(0, min(args.cpu_molecules, dh.num_graphs))
+    count = args.cpu_molecules or max(1, CPU_SAMPLE_PAIRS // max(1, qh.num_graphs))
+    sample = dh.slice(0, min(count, dh.num_graphs))
     qa, sa = qh.arrays(), sample.arrays()
     t0 = time.perf_counter()
     r = O.pipeline(qa, sa, args.refinement_steps, modes=[not args.find_all])
@@ -74,7 +88,8 @@ def cpu_baseline(qh, dh, args) -> dict:
         "cores": O.threads(),
         "kind": "port",
         "sample": f"first {sample.num_graphs} of the {dh.num_graphs} molecules x all {qh.num_graphs} queries, "
-                  f"R={args.refinement_steps}; {dt:.2f} s",
+                  f"R={args.refinement_steps}, {'find-first' if not args.find_all else 'find-all'}; {dt:.2f} s on "
+                  f"{O.threads()} OpenMP threads ({_cpu_model()}, {os.cpu_count()} logical CPUs visible)",
     }
 
 
@@ -88,7 +103,11 @@ def main() -> int:
     ap.add_argument("--queries", type=int, default=None, help="override the config's query count")
     ap.add_argument("--find-all", action="store_true", help="count every embedding instead of find-first")
     ap.add_argument("--refinement-steps", type=int, default=6)
-    ap.add_argument("--cpu-molecules", type=int, default=20_000)
+    ap.add_argument("--cpu-molecules", type=int, default=0,
+                    help="molecules in the CPU baseline sample (default: about 1e7 query x molecule pairs)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 3 plain device copies of MIB MiB (the reference kernel for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--form", choices=("plain", "batched"), default="plain",
                     help="plain: per-phase kernels over the Q x D bitset; batched: per-pair evaluation from "
@@ -98,6 +117,19 @@ def main() -> int:
         args.molecules = CONFIGS[args.config][1]
     if args.queries is None:
         args.queries = CONFIGS[args.config][3]
+
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy * 1024 * 1024 // 4
+        a = torch.ones(n, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n * 4, "copies": 3}))
+        return 0
 
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs SIGMo's hot path on a single MI355X "
@@ -155,8 +187,8 @@ def main() -> int:
     n, ms, by = hbm[dom]
     achieved = by / n / (ms / n / 1e3) / 1e9
     traffic = None
-    tfile = ROOT / "profiles" / f"{dom}_traffic.json"
-    if tfile.exists():
+    tfile = ROOT / "profiles" / f"traffic_{dom}_config{args.config}_{args.form}.json"
+    if tfile.exists() and args.molecules == CONFIGS[args.config][1] and args.queries == CONFIGS[args.config][3]:
         traffic = json.loads(tfile.read_text()).get("traffic_bytes_per_launch")
     roofline = {
         "bound": "hbm",
@@ -173,7 +205,7 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        print(f"[bench] CPU baseline on {args.cpu_molecules} molecules", file=sys.stderr, flush=True)
+        print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(qh, dh, args)
 
     line = {

@@ -30,12 +30,26 @@
 namespace sigmo_k {
 namespace {
 
-constexpr uint32_t kMaxLevels = 8;     // requirement encoding keeps 3 bits of level
-constexpr uint32_t kReqStride = kMaxLevels * 16;
+constexpr uint32_t kMaxLevels = 8;        // token encoding keeps 3 bits of level
 constexpr uint32_t kNoWide = 0xFFFFFFFFu;
+constexpr uint16_t kEndToken = 0xFFFF;    // closes a query node's token segment
+constexpr uint32_t kArenaChunk = 1024;    // arena words a wavefront claims at a time
+constexpr uint32_t kRecordChunk = 256;    // pair records a wavefront claims at a time
 
-// Level-r signature of every node's sphere, one BFS layer per level (the reachables of the
-// reference equal the BFS ball of the previous level when levels are refined in order).
+// Tokens of one query node: [label] [requirement]* [end], requirement = r | l << 3 | c << 7
+// (level r < 8, label l < 16, count 1 <= c <= 15).  Segments are 4-token aligned with one spare
+// word, so the reader's one-word lookahead stays inside the segment.
+__host__ __device__ constexpr uint32_t token_segment(uint32_t R) { return ((2 + 16 * R + 3) & ~3u) + 4; }
+
+// Orders this wavefront's LDS accesses across lanes (the compiler may not move them across).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Level-r sphere of every node, one BFS layer per call (the reachables of the reference are
+// the BFS ball of the previous level when levels are refined in order).  Positions are taken
+// mod 64: bit b of the frontier expands node prev + b, as in signature.hpp:336-390.
 __device__ __forceinline__ uint64_t next_layer(const GraphDev& g, uint32_t prev, uint64_t& ball, uint64_t& frontier) {
   uint64_t next = 0, f = frontier;
   while (f) {
@@ -50,6 +64,8 @@ __device__ __forceinline__ uint64_t next_layer(const GraphDev& g, uint32_t prev,
   return next;
 }
 
+// 16 four-bit label counts of the nodes in r (labels >= 16 are not counted; counts saturate
+// at 15), signature.hpp:44-51.
 __device__ __forceinline__ uint64_t histogram16(uint64_t r, const uint64_t* __restrict__ lm) {
   uint64_t s = 0;
 #pragma unroll
@@ -61,13 +77,54 @@ __device__ __forceinline__ uint64_t histogram16(uint64_t r, const uint64_t* __re
   return s;
 }
 
+// Nibble-wise a <= b for all 16 nibbles (even and odd nibbles in 8-bit fields, borrow-free).
+__device__ __forceinline__ bool nib_le_all(uint64_t a, uint64_t b) {
+  constexpr uint64_t L = 0x0F0F0F0F0F0F0F0Full, H = 0x1010101010101010ull;
+  const uint64_t te = ((b & L) | H) - (a & L);
+  const uint64_t to = (((b >> 4) & L) | H) - ((a >> 4) & L);
+  return (te & to & H) == H;
+}
+
+// Nibble-wise max.
+__device__ __forceinline__ uint64_t nib_max(uint64_t a, uint64_t b) {
+  constexpr uint64_t L = 0x0F0F0F0F0F0F0F0Full, H = 0x1010101010101010ull;
+  const uint64_t ae = a & L, be = b & L, ao = (a >> 4) & L, bo = (b >> 4) & L;
+  const uint64_t me = ((((ae | H) - be) & H) >> 4) * 0xFull;  // 0x0F in the fields where a >= b
+  const uint64_t mo = ((((ao | H) - bo) & H) >> 4) * 0xFull;
+  return ((ae & me) | (be & ~me)) | (((ao & mo) | (bo & ~mo)) << 4);
+}
+
+__device__ __forceinline__ uint64_t wave_nib_max(uint64_t x) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) x = nib_max(x, static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(x), o)));
+  return x;
+}
+
+// 64 x 64 bit-matrix transpose across the wavefront: lane p holds row p on entry, lane i
+// holds column i (bit p = bit i of lane p's row) on exit.  Six butterfly stages, each
+// exchanging the off-diagonal j x j blocks between lanes p and p ^ j.
+__device__ __forceinline__ uint64_t transpose64(uint64_t x) {
+  const uint32_t lane = static_cast<uint32_t>(lane_id());
+  constexpr uint64_t keep[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                                0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+  for (int t = 0; t < 6; ++t) {
+    const uint32_t j = 32u >> t;
+    const uint64_t L = keep[t];
+    const uint64_t y = static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(x), static_cast<int>(j)));
+    x = (lane & j) ? ((x & ~L) | ((y & ~L) >> j)) : ((x & L) | ((y & L) << j));
+  }
+  return x;
+}
+
 // One wavefront per molecule: lane p is position p (and p+64 for molecules of 65..128 nodes).
 // planes[(m*R + r)*64 + i]: bit p = bit i of the level-(r+1) signature of position p, i.e.
 // plane 4*label + bit; planes_hi the same for positions 64..127; lab_hi[w*16+l]: positions
-// 64..127 with label l.  Positions 0..63 use the graph's label masks.
+// 64..127 with label l (positions 0..63 use the graph's label masks); dmax[m*R + r]: the
+// nibble-wise max of the level-(r+1) signatures over the molecule's positions.
 __global__ __launch_bounds__(256) void k_db_prepare(GraphDev g, uint32_t R, const uint32_t* __restrict__ wide_idx,
                                                     uint64_t* __restrict__ planes, uint64_t* __restrict__ planes_hi,
-                                                    uint64_t* __restrict__ lab_hi) {
+                                                    uint64_t* __restrict__ lab_hi, uint64_t* __restrict__ dmax) {
   const uint32_t lane = static_cast<uint32_t>(lane_id());
   const uint64_t m = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (m >= g.G) return;  // wave-uniform
@@ -82,14 +139,11 @@ __global__ __launch_bounds__(256) void k_db_prepare(GraphDev g, uint32_t R, cons
     for (uint32_t r = 0; r < R; ++r) {
       uint64_t sig = 0;
       if (valid) sig = histogram16(next_layer(g, s, ball, frontier), lm);
-      uint64_t mine = 0;
-#pragma unroll
-      for (int i = 0; i < 64; ++i) {
-        const uint64_t b = __ballot((sig >> i) & 1ull);
-        if (lane == static_cast<uint32_t>(i)) mine = b;
-      }
-      if (h == 0) planes[(m * R + r) * 64 + lane] = mine;
-      else planes_hi[(static_cast<uint64_t>(wi) * R + r) * 64 + lane] = mine;
+      const uint64_t col = transpose64(sig);
+      if (h == 0) planes[(m * R + r) * 64 + lane] = col;
+      else planes_hi[(static_cast<uint64_t>(wi) * R + r) * 64 + lane] = col;
+      const uint64_t mx = wave_nib_max(sig);
+      if (lane == 0) dmax[m * R + r] = h == 0 ? mx : nib_max(dmax[m * R + r], mx);
     }
     if (h == 1) {
       const uint32_t lab = valid ? g.nl[s + pos] : 0xFFu;
@@ -104,87 +158,97 @@ __global__ __launch_bounds__(256) void k_db_prepare(GraphDev g, uint32_t R, cons
   }
 }
 
-// Query side: all R level signatures of every query node (query graphs have <= 32 nodes, so
-// positions never alias), turned into requirement lists: (level, label, count) with count > 0,
-// encoded as (r) | (l << 3) | (c << 7), levels ascending.
-__global__ __launch_bounds__(256) void k_query_requirements(GraphDev q, uint32_t R, uint16_t* __restrict__ reqs,
-                                                            uint8_t* __restrict__ nreq) {
-  const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-  if (v >= q.N) return;
-  const uint32_t gid = q.node_graph[v];
-  const uint32_t prev = q.go[gid];
-  uint64_t self = 1ull << ((static_cast<uint32_t>(v) - prev) & 63u), ball = self, frontier = self;
-  uint32_t cnt = 0;
+// One wavefront per query graph, lane i = node i (query graphs have <= 32 nodes, so
+// positions never alias): writes each node's token segment, the graph's label set (labels
+// < 16) and, per level, the nibble-wise max of its nodes' signatures.
+__global__ __launch_bounds__(256) void k_query_prepare(GraphDev q, uint32_t R, uint16_t* __restrict__ tokens,
+                                                       uint32_t* __restrict__ qlab, uint64_t* __restrict__ qmax) {
+  const uint32_t lane = static_cast<uint32_t>(lane_id());
+  const uint64_t g = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (g >= q.G) return;  // wave-uniform
+  const uint32_t prev = q.go[g], k = q.go[g + 1] - prev;
+  const bool valid = lane < k;
+  const uint32_t seg = token_segment(R);
+  uint16_t* t = tokens + (static_cast<uint64_t>(prev) + lane) * seg;
+  const uint32_t label = valid ? q.nl[prev + lane] : 0u;
+  if (valid) t[0] = static_cast<uint16_t>(label);
+  uint64_t self = 1ull << (lane & 63), ball = self, frontier = self;
+  uint32_t cnt = 1;
   for (uint32_t r = 0; r < R; ++r) {
-    const uint64_t sig = histogram16(next_layer(q, prev, ball, frontier), q.lmask + static_cast<uint64_t>(gid) * 16);
-    for (uint32_t l = 0; l < 16; ++l) {
-      const uint32_t c = static_cast<uint32_t>(sig >> (4 * l)) & 15u;
-      if (c) reqs[v * kReqStride + cnt++] = static_cast<uint16_t>(r | (l << 3) | (c << 7));
+    uint64_t sig = 0;
+    if (valid) {
+      sig = histogram16(next_layer(q, prev, ball, frontier), q.lmask + g * 16);
+      for (uint32_t l = 0; l < 16; ++l) {
+        const uint32_t c = static_cast<uint32_t>(sig >> (4 * l)) & 15u;
+        if (c) t[cnt++] = static_cast<uint16_t>(r | (l << 3) | (c << 7));
+      }
     }
+    const uint64_t mx = wave_nib_max(sig);
+    if (lane == 0) qmax[g * R + r] = mx;
   }
-  nreq[v] = static_cast<uint8_t>(cnt);
+  if (valid) t[cnt] = kEndToken;
+  uint32_t bits = valid && label < 16 ? 1u << label : 0u;
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) bits |= static_cast<uint32_t>(__shfl_xor(static_cast<int>(bits), o));
+  if (lane == 0) qlab[g] = bits;
 }
 
 struct PairArgs {
   GraphDev q, d;
-  uint32_t R;
+  uint32_t R, has_wide, mol_chunk;
   const uint64_t* planes;
   const uint64_t* planes_hi;
   const uint64_t* lab_hi;
   const uint32_t* wide_idx;
-  const uint16_t* reqs;
-  const uint8_t* nreq;
-  uint64_t* scratch;      // [wave][lane][32][2]
-  uint64_t* arena;        // survivor masks: k words (2k for molecules of > 64 nodes)
+  const uint64_t* dmax;
+  const uint64_t* tok64;  // token segments as 64-bit words
+  const uint32_t* qlab;
+  const uint64_t* qmax;
+  uint64_t* scratch;      // per wavefront: masks [32][64] (low words), then [32][64] (high words)
+  uint32_t* slot;         // per wavefront: arena offset of each surviving query graph [Gq]
+  uint64_t* arena;        // survivor masks: k words (2k, interleaved, for molecules of > 64 nodes)
   uint64_t arena_cap;
-  uint32_t* rec_qg;       // records, per molecule a contiguous block in query-graph order
+  uint32_t* rec_qg;       // records: per molecule a contiguous block in query-graph order
   uint32_t* rec_off;
   uint64_t rec_cap;
   uint32_t* mol_block;    // [Gd] first record of the molecule
   uint32_t* mol_count;    // [Gd]
-  unsigned long long* counters;  // [0] molecule cursor, [1] arena top, [2] record top, [3] overflow flag
-  uint32_t mol_chunk;
+  unsigned long long* counters;  // [0] molecule cursor, [1] arena top, [2] record top, [3] overflow flags
 };
 
-// Candidate mask of query node q at molecule m restricted to the label test (filter).
-__device__ __forceinline__ void label_mask(const PairArgs& a, uint64_t m, uint32_t s, uint32_t n, uint32_t wi, uint32_t ql,
-                                           uint64_t& lo, uint64_t& hi) {
-  const uint64_t all_lo = n >= 64 ? ~0ull : ((1ull << n) - 1);
-  const uint64_t all_hi = n <= 64 ? 0ull : (n >= 128 ? ~0ull : ((1ull << (n - 64)) - 1));
-  if (ql == 255u) {
-    lo = all_lo;
-    hi = all_hi;
-  } else if (ql < 16u) {
-    lo = a.d.lmask[m * 16 + ql];
-    hi = n > 64 ? a.lab_hi[static_cast<uint64_t>(wi) * 16 + ql] : 0ull;
-  } else {
-    lo = hi = 0;
-    for (uint32_t p = 0; p < n; ++p)
-      if (a.d.nl[s + p] == ql) {
-        if (p < 64) lo |= 1ull << p;
-        else hi |= 1ull << (p - 64);
-      }
-  }
+__host__ __device__ inline uint64_t pair_lds_words(uint32_t R, uint32_t has_wide, uint32_t Gq) {
+  return static_cast<uint64_t>(R) * 64 * (has_wide ? 2 : 1) + 32 + (Gq + 63) / 64 + 32;
 }
 
-// Persistent pair filter.  A wavefront takes molecules from a cursor; its lanes take query
-// graphs of that molecule in ascending order, each lane running one pair as a small state
-// machine (start node / apply one requirement / finish node) so that lanes whose pair died
-// take the next query graph at once.  Per molecule, surviving query graphs are collected in
-// LDS (bitmask + arena offset) and emitted as one block of records in query-graph order.
+// Persistent pair filter.  A wavefront takes molecules from a cursor and stages the molecule's
+// level planes and label masks in LDS.  Query graphs pass two necessary tests first, 64 at a
+// time: label set (labels < 16) contained in the molecule's, and per level the graph's
+// nibble-wise signature max <= the molecule's (a node fits nowhere if it exceeds the max).
+// Passing graphs are handed to idle lanes; a lane runs its pair as a small state machine over
+// the query's token segments (label test, one requirement per step, node end), so lanes whose
+// pair died take the next graph at once, and the molecule-wide early exit is the GMCR predicate
+// of gmcr.hpp:56-61.  Survivors' masks go to the arena, per molecule one block of records in
+// query-graph order.  Arena and record space is claimed per wavefront in chunks.
 __global__ __launch_bounds__(256) void k_pair_filter(PairArgs a) {
-  extern __shared__ uint8_t smem[];
+  extern __shared__ uint64_t lds[];
   const uint32_t lane = static_cast<uint32_t>(lane_id());
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t Gq = a.q.G;
   const uint32_t words = (Gq + 63) / 64;
-  // per-wave LDS: survivor bits [words] u64, arena offsets [Gq] u32
-  const size_t per_wave = static_cast<size_t>(words) * 8 + static_cast<size_t>(Gq) * 4;
-  uint64_t* surv = reinterpret_cast<uint64_t*>(smem + wave * ((per_wave + 15) & ~size_t(15)));
-  uint32_t* slot = reinterpret_cast<uint32_t*>(surv + words);
-  const uint64_t gwave = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + wave;
-  uint64_t* scratch = a.scratch + (gwave * 64 + lane) * 64;
   const uint32_t R = a.R;
+  uint64_t* lp = lds + wave * pair_lds_words(R, a.has_wide, Gq);
+  uint64_t* lph = lp + R * 64;
+  uint64_t* llm = lph + (a.has_wide ? R * 64 : 0);
+  uint64_t* llh = llm + 16;
+  uint64_t* surv = llh + 16;
+  uint32_t* xfer = reinterpret_cast<uint32_t*>(surv + words);
+  const uint64_t gwave = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + wave;
+  uint64_t* sc_lo = a.scratch + gwave * (a.has_wide ? 2 * 32 * 64 : 32 * 64);
+  uint64_t* sc_hi = sc_lo + 32 * 64;
+  uint32_t* slot = a.slot + gwave * Gq;
+  const uint32_t seg = token_segment(R);
+  const uint64_t below = (1ull << lane) - 1;
+  uint64_t ck_next = 0, ck_end = 0, rk_next = 0, rk_end = 0;  // claimed arena / record chunks
 
   uint64_t mnext = 0, mend = 0;
   while (true) {
@@ -199,80 +263,185 @@ __global__ __launch_bounds__(256) void k_pair_filter(PairArgs a) {
     const uint32_t s = a.d.go[m], n = a.d.go[m + 1] - s;
     const bool wide = n > 64;
     const uint32_t wi = a.wide_idx[m];
+    {
+      const uint64_t* src = a.planes + m * R * 64;
+      for (uint32_t t = lane; t < R * 64; t += kWave) lp[t] = src[t];
+      if (wide) {
+        const uint64_t* srch = a.planes_hi + static_cast<uint64_t>(wi) * R * 64;
+        for (uint32_t t = lane; t < R * 64; t += kWave) lph[t] = srch[t];
+      }
+    }
+    uint64_t lmv = 0, lhv = 0;
+    if (lane < 16) {
+      lmv = a.d.lmask[m * 16 + lane];
+      lhv = wide ? a.lab_hi[static_cast<uint64_t>(wi) * 16 + lane] : 0ull;
+      llm[lane] = lmv;
+      llh[lane] = lhv;
+    }
+    const uint32_t mlab = static_cast<uint32_t>(__ballot((lmv | lhv) != 0)) & 0xFFFFu;
     for (uint32_t w = lane; w < words; w += kWave) surv[w] = 0;
-    __builtin_amdgcn_wave_barrier();
+    const uint64_t* mmax = a.dmax + m * R;
+    const uint64_t all_lo = n >= 64 ? ~0ull : ((1ull << n) - 1);
+    const uint64_t all_hi = n <= 64 ? 0ull : (n >= 128 ? ~0ull : ((1ull << (n - 64)) - 1));
+    wave_sync();
 
     bool have = false, fresh = false;
-    uint32_t qg = 0, i = 0, k = 0, qoff = 0, ri = 0, nr = 0;
-    uint64_t lo = 0, hi = 0;
-    uint32_t next_qg = 0;  // wave-uniform
+    uint32_t qg = 0, i = 0, k = 0, tp = 0;
+    uint64_t nodebase = 0, tw = 0, tn = 0, pf0 = 0, pf1 = 0, lo = 0, hi = 0;
+    uint32_t scan = 0, pend_base = 0;  // wave-uniform
+    uint64_t pend = 0;                 // wave-uniform: graphs pend_base + j that passed the tests
     while (true) {
-      const uint64_t need = __ballot(!have);
-      if (need && next_qg < Gq) {
-        const uint32_t rank = __popcll(need & ((1ull << lane) - 1));
-        if (!have && next_qg + rank < Gq) {
-          qg = next_qg + rank;
-          qoff = a.q.go[qg];
+      // hand passing query graphs to idle lanes
+      uint64_t need = __ballot(!have);
+      while (need) {
+        if (!pend) {
+          if (scan >= Gq) break;
+          const uint32_t g = scan + lane;
+          bool ok = g < Gq;
+          if (ok) ok = (a.qlab[g] & ~mlab) == 0u;
+          for (uint32_t r = 0; r < R && ok; ++r) ok = nib_le_all(a.qmax[static_cast<uint64_t>(g) * R + r], mmax[r]);
+          pend = __ballot(ok);
+          pend_base = scan;
+          scan += kWave;
+          continue;
+        }
+        const uint32_t cnt = min(__popcll(pend), __popcll(need));
+        const bool give = ((pend >> lane) & 1ull) && static_cast<uint32_t>(__popcll(pend & below)) < cnt;
+        const uint32_t rn = __popcll(need & below);
+        const bool take = ((need >> lane) & 1ull) && rn < cnt;
+        if (give) xfer[__popcll(pend & below)] = pend_base + lane;
+        wave_sync();
+        if (take) {
+          qg = xfer[rn];
+          const uint32_t qoff = a.q.go[qg];
           k = a.q.go[qg + 1] - qoff;
           i = 0;
+          nodebase = static_cast<uint64_t>(qoff) * seg;
+          tw = a.tok64[nodebase / 4];
+          tn = a.tok64[nodebase / 4 + 1];
+          if (k > 1) {
+            pf0 = a.tok64[(nodebase + seg) / 4];
+            pf1 = a.tok64[(nodebase + seg) / 4 + 1];
+          }
           have = true;
           fresh = true;
         }
-        next_qg = next_qg + __popcll(need) < Gq ? next_qg + __popcll(need) : Gq;
+        wave_sync();
+        pend &= ~__ballot(give);
+        need &= ~__ballot(take);
       }
       if (__ballot(have) == 0) break;
+      bool done = false;
       if (have) {
-        const uint32_t q = qoff + i;
-        if (fresh) {  // start query node i of the pair: the filter's label test
-          label_mask(a, m, s, n, wi, a.q.nl[q], lo, hi);
-          ri = 0;
-          nr = a.nreq[q];
+        if (fresh) {  // node start: the filter's label test (isomorphism.hpp:49)
+          const uint32_t ql = static_cast<uint32_t>(tw & 0xFFFFu);
+          if (ql == 255u) {
+            lo = all_lo;
+            hi = all_hi;
+          } else if (ql < 16u) {
+            lo = llm[ql];
+            hi = llh[ql];
+          } else {
+            lo = hi = 0;
+            for (uint32_t p = 0; p < n; ++p)
+              if (a.d.nl[s + p] == ql) {
+                if (p < 64) lo |= 1ull << p;
+                else hi |= 1ull << (p - 64);
+              }
+          }
+          tp = 1;
           fresh = false;
           if (!(lo | hi)) have = false;
-        } else if (ri < nr) {  // one refine requirement: count_q(l) <= count_p(l) at level r
-          const uint32_t req = a.reqs[static_cast<uint64_t>(q) * kReqStride + ri];
-          const uint32_t r = req & 7u, l = (req >> 3) & 15u, c = (req >> 7) & 15u;
-          const uint64_t* pl = a.planes + (m * R + r) * 64 + l * 4;
-          lo &= ge4_lane(pl[0], pl[1], pl[2], pl[3], c);
-          if (wide) {
-            const uint64_t* ph = a.planes_hi + (static_cast<uint64_t>(wi) * R + r) * 64 + l * 4;
-            hi &= ge4_lane(ph[0], ph[1], ph[2], ph[3], c);
-          }
-          ++ri;
-          if (!(lo | hi)) have = false;
-        } else {  // node done: keep its mask
-          scratch[2 * i] = lo;
-          scratch[2 * i + 1] = hi;
-          ++i;
-          if (i < k) {
-            fresh = true;
-          } else {  // every query node has a candidate: the pair survives GMCR
-            const uint32_t stride = wide ? 2 : 1;
-            const unsigned long long off = atomicAdd(&a.counters[1], static_cast<unsigned long long>(k * stride));
-            if (off + k * stride <= a.arena_cap) {
-              for (uint32_t t = 0; t < k; ++t) {
-                a.arena[off + t * stride] = scratch[2 * t];
-                if (wide) a.arena[off + t * stride + 1] = scratch[2 * t + 1];
-              }
+        } else {
+          const uint32_t tok = static_cast<uint32_t>(tw >> (16 * (tp & 3))) & 0xFFFFu;
+          if (tok == kEndToken) {  // node done: keep its mask
+            sc_lo[i * 64 + lane] = lo;
+            if (wide) sc_hi[i * 64 + lane] = hi;
+            ++i;
+            if (i == k) {  // every query node has a candidate: the pair survives GMCR
+              done = true;
+              have = false;
             } else {
-              atomicOr(&a.counters[3], 1ull);
+              nodebase += seg;
+              tw = pf0;
+              tn = pf1;
+              tp = 0;
+              fresh = true;
+              if (i + 1 < k) {
+                pf0 = a.tok64[(nodebase + seg) / 4];
+                pf1 = a.tok64[(nodebase + seg) / 4 + 1];
+              }
             }
-            slot[qg] = static_cast<uint32_t>(off);
-            atomicOr(reinterpret_cast<unsigned long long*>(&surv[qg >> 6]), 1ull << (qg & 63));
-            have = false;
+          } else {  // requirement: count_q(l) <= count_p(l) at level r (isomorphism.hpp:103-110)
+            const uint32_t r = tok & 7u, l = (tok >> 3) & 15u, c = (tok >> 7) & 15u;
+            const uint64_t* pl = lp + r * 64 + l * 4;
+            lo &= ge4_lane(pl[0], pl[1], pl[2], pl[3], c);
+            if (wide) {
+              const uint64_t* ph = lph + r * 64 + l * 4;
+              hi &= ge4_lane(ph[0], ph[1], ph[2], ph[3], c);
+            }
+            if (!(lo | hi)) have = false;
+            ++tp;
+            if ((tp & 3) == 0) {
+              tw = tn;
+              tn = a.tok64[(nodebase + tp) / 4 + 1];
+            }
           }
         }
       }
+      // survivors of this step claim arena space from the wavefront's chunk
+      if (__ballot(done)) {
+        const uint32_t stride = wide ? 2 : 1;
+        const uint32_t want = done ? k * stride : 0u;
+        uint32_t incl = want;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+          const uint32_t y = __shfl_up(incl, o);
+          if (lane >= static_cast<uint32_t>(o)) incl += y;
+        }
+        const uint32_t total = uniform32(__shfl(incl, kWave - 1));
+        if (ck_next + total > ck_end) {
+          const uint64_t grab = total > kArenaChunk ? total : kArenaChunk;
+          unsigned long long b = 0;
+          if (lane == 0) b = atomicAdd(&a.counters[1], static_cast<unsigned long long>(grab));
+          ck_next = uniform64(__shfl(b, 0));
+          ck_end = ck_next + grab;
+        }
+        const uint64_t off = ck_next + incl - want;
+        ck_next += total;
+        if (done) {
+          if (off + want <= a.arena_cap) {
+            for (uint32_t t = 0; t < k; ++t) {
+              a.arena[off + t * stride] = sc_lo[t * 64 + lane];
+              if (wide) a.arena[off + t * stride + 1] = sc_hi[t * 64 + lane];
+            }
+          } else {
+            atomicOr(&a.counters[3], 1ull);
+          }
+          slot[qg] = static_cast<uint32_t>(off);
+          atomicOr(reinterpret_cast<unsigned long long*>(&surv[qg >> 6]), 1ull << (qg & 63));
+        }
+      }
     }
-    __builtin_amdgcn_wave_barrier();
+    wave_sync();
     // emit this molecule's survivors as one block of records, ascending query graph
     uint32_t mine = 0;
     for (uint32_t w = lane; w < words; w += kWave) mine += __popcll(surv[w]);
     uint32_t total = mine;
     for (int o = kWave / 2; o > 0; o >>= 1) total += __shfl_xor(total, o);
-    unsigned long long blk = 0;
-    if (lane == 0 && total) blk = atomicAdd(&a.counters[2], static_cast<unsigned long long>(total));
-    blk = uniform64(__shfl(blk, 0));
+    total = uniform32(total);
+    uint64_t blk = 0;
+    if (total) {
+      if (rk_next + total > rk_end) {
+        const uint64_t grab = total > kRecordChunk ? total : kRecordChunk;
+        unsigned long long b = 0;
+        if (lane == 0) b = atomicAdd(&a.counters[2], static_cast<unsigned long long>(grab));
+        rk_next = uniform64(__shfl(b, 0));
+        rk_end = rk_next + grab;
+      }
+      blk = rk_next;
+      rk_next += total;
+    }
     if (lane == 0) {
       a.mol_block[m] = static_cast<uint32_t>(blk);
       a.mol_count[m] = total;
@@ -285,7 +454,8 @@ __global__ __launch_bounds__(256) void k_pair_filter(PairArgs a) {
       for (uint32_t w0 = 0; w0 < words; w0 += kWave) {
         const uint32_t w = w0 + lane;
         const uint64_t x = w < words ? surv[w] : 0ull;
-        uint32_t c = __popcll(x), incl = c;
+        const uint32_t c = __popcll(x);
+        uint32_t incl = c;
         for (int o = 1; o < kWave; o <<= 1) {
           const uint32_t y = __shfl_up(incl, o);
           if (lane >= static_cast<uint32_t>(o)) incl += y;
@@ -300,7 +470,7 @@ __global__ __launch_bounds__(256) void k_pair_filter(PairArgs a) {
         base += __shfl(incl, kWave - 1);
       }
     }
-    __builtin_amdgcn_wave_barrier();
+    wave_sync();
   }
 }
 
@@ -407,15 +577,15 @@ __global__ __launch_bounds__(256) void k_join_arena(GraphDev q, GraphDev d, cons
 }  // namespace
 
 hipError_t db_prepare(hipStream_t s, const GraphDev& g, uint32_t R, const uint32_t* wide_idx, uint64_t* planes, uint64_t* planes_hi,
-                      uint64_t* lab_hi) {
+                      uint64_t* lab_hi, uint64_t* dmax) {
   if (!g.G) return hipSuccess;
-  k_db_prepare<<<(g.G + 3) / 4, 256, 0, s>>>(g, R, wide_idx, planes, planes_hi, lab_hi);
+  k_db_prepare<<<(g.G + 3) / 4, 256, 0, s>>>(g, R, wide_idx, planes, planes_hi, lab_hi, dmax);
   return hipGetLastError();
 }
 
-hipError_t query_requirements(hipStream_t s, const GraphDev& q, uint32_t R, uint16_t* reqs, uint8_t* nreq) {
-  if (!q.N) return hipSuccess;
-  k_query_requirements<<<static_cast<unsigned>((q.N + 255) / 256), 256, 0, s>>>(q, R, reqs, nreq);
+hipError_t query_prepare(hipStream_t s, const GraphDev& q, uint32_t R, uint16_t* tokens, uint32_t* qlab, uint64_t* qmax) {
+  if (!q.G) return hipSuccess;
+  k_query_prepare<<<(q.G + 3) / 4, 256, 0, s>>>(q, R, tokens, qlab, qmax);
   return hipGetLastError();
 }
 
@@ -475,6 +645,7 @@ void ctx_add_bytes(sigmo_ctx* ctx, int kernel, double bytes);
 const sigmo_k::GraphDev graph_dev(const sigmo_graph* g);
 const std::vector<uint32_t>& graph_host_offsets(const sigmo_graph* g);
 sigmo_ctx* graph_ctx(const sigmo_graph* g);
+uint64_t graph_edges(const sigmo_graph* g);
 }  // namespace sigmo_internal
 
 using namespace sigmo_internal;
@@ -485,13 +656,14 @@ struct sigmo_db {
   uint32_t R = 0;
   uint32_t num_wide = 0;
   uint32_t* wide_idx = nullptr;
-  uint64_t *planes = nullptr, *planes_hi = nullptr, *lab_hi = nullptr;
+  uint64_t *planes = nullptr, *planes_hi = nullptr, *lab_hi = nullptr, *dmax = nullptr;
   uint64_t bytes = 0;
   ~sigmo_db() {
     bfree(wide_idx);
     bfree(planes);
     bfree(planes_hi);
     bfree(lab_hi);
+    bfree(dmax);
   }
 };
 
@@ -499,12 +671,15 @@ struct sigmo_batch {
   sigmo_ctx* ctx = nullptr;
   const sigmo_db* db = nullptr;
   // query side
-  uint16_t* reqs = nullptr;
-  uint8_t* nreq = nullptr;
-  uint64_t cap_reqs = 0, cap_nreq = 0;
+  uint16_t* tokens = nullptr;
+  uint32_t* qlab = nullptr;
+  uint64_t* qmax = nullptr;
+  uint64_t cap_tokens = 0, cap_qlab = 0, cap_qmax = 0;
   // pair filter
   uint64_t* scratch = nullptr;
   uint64_t cap_scratch = 0;
+  uint32_t* slot = nullptr;
+  uint64_t cap_slot = 0;
   uint64_t* arena = nullptr;
   uint64_t cap_arena = 0;
   uint32_t *rec_qg = nullptr, *rec_off = nullptr;
@@ -524,9 +699,11 @@ struct sigmo_batch {
   uint32_t Gq = 0;
   bool ran = false;
   ~sigmo_batch() {
-    bfree(reqs);
-    bfree(nreq);
+    bfree(tokens);
+    bfree(qlab);
+    bfree(qmax);
     bfree(scratch);
+    bfree(slot);
     bfree(arena);
     bfree(rec_qg);
     bfree(rec_off);
@@ -569,11 +746,12 @@ int sigmo_db_prepare(sigmo_ctx* ctx, const sigmo_graph* data, uint32_t refinemen
   if (stt == SIGMO_OK) stt = balloc(&db->planes, uint64_t(G) * R * 64);
   if (stt == SIGMO_OK) stt = balloc(&db->planes_hi, uint64_t(nw) * R * 64);
   if (stt == SIGMO_OK) stt = balloc(&db->lab_hi, uint64_t(nw) * 16);
+  if (stt == SIGMO_OK) stt = balloc(&db->dmax, uint64_t(G) * R);
   if (stt != SIGMO_OK) {
     delete db;
     return stt;
   }
-  db->bytes = G * 4ull + (uint64_t(G) + nw) * R * 512 + nw * 128ull;
+  db->bytes = G * 4ull + (uint64_t(G) + nw) * R * 512 + nw * 128ull + uint64_t(G) * R * 8;
   hipError_t e = hipMemcpyAsync(db->wide_idx, widx.data(), G * 4ull, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
@@ -597,11 +775,12 @@ int sigmo_db_refresh(sigmo_db* db) {
   unsigned long long *st, *sh;
   BATCH_TRY(ctx_info(db->ctx, &device, &s, &cus, &st, &sh));
   const sigmo_k::GraphDev g = graph_dev(db->data);
-  // algorithmic bytes: CSR adjacency words and label masks read, planes written
-  const double bytes = double(g.N) * 16.0 + double(g.G) * 128.0 + double(db->bytes);
+  // algorithmic bytes: the CSR and labels read once, R level signatures per data node written:
+  // 4(D+1) + 4E + D + 8 R D
+  const double bytes = 4.0 * (double(g.N) + 1.0) + 4.0 * double(graph_edges(db->data)) + double(g.N) + 8.0 * db->R * double(g.N);
   void* tok = nullptr;
   BATCH_TRY(ctx_timed(db->ctx, SIGMO_PHASE_SIG_DATA, SIGMO_KERNEL_DB_PREPARE, bytes, true, &tok));
-  BATCH_HIP_TRY(sigmo_k::db_prepare(s, g, db->R, db->wide_idx, db->planes, db->planes_hi, db->lab_hi));
+  BATCH_HIP_TRY(sigmo_k::db_prepare(s, g, db->R, db->wide_idx, db->planes, db->planes_hi, db->lab_hi, db->dmax));
   BATCH_TRY(ctx_timed(db->ctx, 0, 0, 0.0, false, &tok));
   return SIGMO_OK;
 }
@@ -669,64 +848,82 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
     if (k > SIGMO_MAX_QUERY_NODES) return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "query graph exceeds 32 nodes");
     ksum += k;
   }
-  const size_t lds_per_wave = ((static_cast<size_t>((Gq + 63) / 64) * 8 + static_cast<size_t>(Gq) * 4) + 15) & ~size_t(15);
-  if (lds_per_wave * 4 > 64 * 1024)
-    return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "the batched form takes at most 3968 query graphs per run");
-  // buffers
-  const uint64_t waves = static_cast<uint64_t>(cus) * 8 * 4;  // persistent pair-filter grid: 8 blocks of 4 waves per CU
-  BATCH_TRY(bgrow(&b->reqs, b->cap_reqs, qd.N * sigmo_k::kReqStride));
-  BATCH_TRY(bgrow(&b->nreq, b->cap_nreq, qd.N));
-  BATCH_TRY(bgrow(&b->scratch, b->cap_scratch, waves * 64 * 64));
+  const uint32_t R = db->R;
+  const uint32_t has_wide = db->num_wide ? 1u : 0u;
+  const size_t lds_block = sigmo_k::pair_lds_words(R, has_wide, Gq) * 8 * 4;
+  if (lds_block > 64 * 1024) return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "too many query graphs for one batched run");
+  // persistent pair-filter grid: as many 4-wave blocks as registers and LDS keep resident
+  int resident = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, sigmo_k::k_pair_filter, 256, lds_block) != hipSuccess) {
+    (void)hipGetLastError();
+    resident = 4;
+  }
+  const uint64_t per_cu = std::max<uint64_t>(1, std::min<uint64_t>(8, static_cast<uint64_t>(resident)));
+  const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>(uint64_t(cus) * per_cu, (uint64_t(Gd) + 3) / 4));
+  const uint64_t waves = blocks * 4;
+  const uint64_t seg = sigmo_k::token_segment(R);
+  BATCH_TRY(bgrow(&b->tokens, b->cap_tokens, qd.N * seg));
+  BATCH_TRY(bgrow(&b->qlab, b->cap_qlab, Gq));
+  BATCH_TRY(bgrow(&b->qmax, b->cap_qmax, uint64_t(Gq) * R));
+  BATCH_TRY(bgrow(&b->scratch, b->cap_scratch, waves * 32 * 64 * (has_wide ? 2 : 1)));
+  BATCH_TRY(bgrow(&b->slot, b->cap_slot, waves * Gq));
   BATCH_TRY(bgrow(&b->mol_block, b->cap_mb, Gd));
   BATCH_TRY(bgrow(&b->mol_count, b->cap_mc, uint64_t(Gd) + 1));
   BATCH_TRY(bgrow(&b->offsets, b->cap_off, uint64_t(Gd) + 1));
   BATCH_TRY(bgrow(&b->scan_tmp, b->cap_scan, (uint64_t(Gd) + 1023) / 1024 + 1));
-  // record / arena capacity: start from 1/8 of the pair space (grown and replayed on overflow)
-  const double mean_k = Gq ? double(ksum) / Gq : 0.0;
-  uint64_t want_rec = std::max<uint64_t>(uint64_t(Gq) * Gd / 8, 1 << 16);
-  if (b->cap_rec > want_rec) want_rec = b->cap_rec;
   {
     void* tok = nullptr;
     BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_SIG_QUERY, -1, 0.0, true, &tok));
-    BATCH_HIP_TRY(sigmo_k::query_requirements(s, qd, db->R, b->reqs, b->nreq));
+    BATCH_HIP_TRY(sigmo_k::query_prepare(s, qd, R, b->tokens, b->qlab, b->qmax));
     BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
   }
+  // record / arena capacity: start from 1/8 of the pair space (at most 2^26 pairs) plus every
+  // wavefront's open chunk.  On overflow the counters still hold the exact demand, so one replay
+  // with that size suffices; capacities are kept for later runs.
+  const double mean_k = Gq ? double(ksum) / Gq : 0.0;
+  const uint64_t slack_rec = waves * sigmo_k::kRecordChunk, slack_arena = waves * sigmo_k::kArenaChunk;
+  const uint64_t guess = std::min<uint64_t>(std::max<uint64_t>(uint64_t(Gq) * Gd / 8, 1 << 16), 1ull << 26);
+  uint64_t want_rec = guess + slack_rec;
+  uint64_t want_arena = static_cast<uint64_t>(double(guess) * (mean_k + 1.0) * (has_wide ? 2.0 : 1.0)) + slack_arena;
+  want_rec = std::max(want_rec, std::min(b->cap_rec, b->cap_rec2));
+  want_arena = std::max(want_arena, b->cap_arena);
   for (int attempt = 0; attempt < 6; ++attempt) {
-    const uint64_t want_arena = static_cast<uint64_t>(want_rec * (mean_k + 1.0) * 1.1) + 64;
     BATCH_TRY(bgrow(&b->rec_qg, b->cap_rec, want_rec));
     BATCH_TRY(bgrow(&b->rec_off, b->cap_rec2, want_rec));
     BATCH_TRY(bgrow(&b->arena, b->cap_arena, want_arena));
     if (b->cap_arena >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "candidate arena exceeds 2^32 words");
 
-    // algorithmic reads: each molecule's level planes (every plane a query node may test), label
-    // masks and offsets, and the query requirement lists; writes are added once they are known
-    const double reads = double(Gd) * db->R * 512.0 + double(db->num_wide) * db->R * 512.0 + double(Gd) * (128.0 + 8.0) +
-                         double(qd.N) * (1.0 + 32.0 * db->R);
+    // algorithmic reads: the data side once per run -- a label byte and R level signatures per
+    // data node, (1 + 8R) D -- plus the query side; writes are added once they are known.
+    const double reads = (1.0 + 8.0 * R) * double(dd.N) + double(qd.N) * (1.0 + 2.0 * 16.0 * R) + double(Gq) * (4.0 + 8.0 * R);
     void* tok = nullptr;
     BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_FILTER, SIGMO_KERNEL_PAIR_FILTER, reads, true, &tok));
     BATCH_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(b->counters), 8, 0));
     sigmo_k::PairArgs a;
     a.q = qd;
     a.d = dd;
-    a.R = db->R;
+    a.R = R;
+    a.has_wide = has_wide;
+    a.mol_chunk = 4;
     a.planes = db->planes;
     a.planes_hi = db->planes_hi;
     a.lab_hi = db->lab_hi;
     a.wide_idx = db->wide_idx;
-    a.reqs = b->reqs;
-    a.nreq = b->nreq;
+    a.dmax = db->dmax;
+    a.tok64 = reinterpret_cast<const uint64_t*>(b->tokens);
+    a.qlab = b->qlab;
+    a.qmax = b->qmax;
     a.scratch = b->scratch;
+    a.slot = b->slot;
     a.arena = b->arena;
     a.arena_cap = b->cap_arena;
     a.rec_qg = b->rec_qg;
     a.rec_off = b->rec_off;
-    a.rec_cap = b->cap_rec < b->cap_rec2 ? b->cap_rec : b->cap_rec2;
+    a.rec_cap = std::min(b->cap_rec, b->cap_rec2);
     a.mol_block = b->mol_block;
     a.mol_count = b->mol_count;
     a.counters = b->counters;
-    a.mol_chunk = 4;
-    const uint64_t blocks = std::min<uint64_t>(waves / 4, (uint64_t(Gd) + 3) / 4);
-    hipLaunchKernelGGL(sigmo_k::k_pair_filter, dim3(static_cast<unsigned>(blocks)), dim3(256), lds_per_wave * 4, s, a);
+    hipLaunchKernelGGL(sigmo_k::k_pair_filter, dim3(static_cast<unsigned>(blocks)), dim3(256), lds_block, s, a);
     BATCH_HIP_TRY(hipGetLastError());
     BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
     unsigned long long cnt[4];
@@ -736,7 +933,9 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
     ctx_add_bytes(b->ctx, SIGMO_KERNEL_PAIR_FILTER, double(cnt[1]) * 8.0 + double(cnt[2]) * 8.0 + double(Gd) * 8.0);
     if (!cnt[3]) break;
     if (attempt == 5) return sigmo_err::set(SIGMO_ERR_NOMEM, "pair buffers overflowed repeatedly");
-    want_rec = std::max<uint64_t>(want_rec * 2, cnt[2] + 1);  // replay with larger buffers
+    // replay with larger buffers
+    if (cnt[3] & 2ull) want_rec = std::max<uint64_t>(want_rec * 2, cnt[2] + slack_rec);
+    if (cnt[3] & 1ull) want_arena = std::max<uint64_t>(want_arena * 2, cnt[1] + slack_arena);
   }
   // canonical GMCR: scan the per-molecule survivor counts, scatter the blocks
   void* tok = nullptr;

@@ -305,6 +305,7 @@ void ctx_add_bytes(sigmo_ctx* ctx, int kernel, double bytes) {
 const sigmo_k::GraphDev graph_dev(const sigmo_graph* g) { return g->dev(); }
 const std::vector<uint32_t>& graph_host_offsets(const sigmo_graph* g) { return g->host_go; }
 sigmo_ctx* graph_ctx(const sigmo_graph* g) { return g->ctx; }
+uint64_t graph_edges(const sigmo_graph* g) { return g->E; }
 }  // namespace sigmo_internal
 
 extern "C" {

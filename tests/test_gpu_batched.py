@@ -152,3 +152,19 @@ def test_batched_rejects_unsupported_inputs(ctx):
     single = capi.DeviceGraph(ctx, capi.HostGraphs.from_edge_lists([([8], [])]))
     with pytest.raises(capi.SigmoError):
         batch.run(single)
+
+
+def test_batched_node_with_every_label_at_one_level(ctx):
+    """A query node whose level-1 sphere holds all 16 labels has 16 requirements at R=1: its token
+    segment is full, the reader's lookahead ends exactly at the spare word."""
+    star = (list(range(16)) + [3], [(16, i) for i in range(16)])
+    path = ([3, 7, 3], [(0, 1), (1, 2)])
+    mol_star = (list(range(16)) + [3, 9], [(16, i) for i in range(16)] + [(16, 17)])
+    qh = capi.HostGraphs.from_edge_lists([path, star])  # the star is the last query graph
+    dh = capi.HostGraphs.from_edge_lists([mol_star, path, mol_star]).concat(capi.HostGraphs.molecules(10, 200))
+    for R in (1, 2):
+        print(f"batched full segment R={R}", flush=True)
+        g = _batched(ctx, qh, dh, R)
+        o = O.pipeline(qh.arrays(), dh.arrays(), R, modes=[True, False])
+        _assert_equal(g, o)
+        assert o["counts_all"].sum() > 0
