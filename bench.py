@@ -232,6 +232,8 @@ def main() -> int:
             "matched_pairs": matched,
             "embeddings": results[-1]["num_matches"] if args.find_all else None,
             "dfs_steps": results[-1]["dfs_steps"],
+            **({k: results[-1][k] for k in ("filter_wave_steps", "filter_lane_steps", "filter_scans")}
+               if args.form == "batched" else {}),
             "form": args.form,
         },
         "roofline": roofline,

@@ -230,6 +230,9 @@ typedef struct sigmo_batch_result {
   uint64_t matched_pairs; /* pairs with count > 0                         */
   uint64_t gmcr_pairs;    /* surviving (query graph, molecule) pairs      */
   uint64_t dfs_steps;     /* join DFS frame steps                         */
+  uint64_t filter_wave_steps; /* pair evaluation: wavefront steps        */
+  uint64_t filter_lane_steps; /* pair evaluation: active lanes summed    */
+  uint64_t filter_scans;      /* pair evaluation: 64-pair pre-test scans */
 } sigmo_batch_result;
 int sigmo_db_prepare(sigmo_ctx* ctx, const sigmo_graph* data, uint32_t refinement_steps, sigmo_db** out);
 /* Recompute the level planes from the data graph's current device contents (same result as a

@@ -81,6 +81,9 @@ class BatchResult(C.Structure):
         ("matched_pairs", C.c_uint64),
         ("gmcr_pairs", C.c_uint64),
         ("dfs_steps", C.c_uint64),
+        ("filter_wave_steps", C.c_uint64),
+        ("filter_lane_steps", C.c_uint64),
+        ("filter_scans", C.c_uint64),
     ]
 
 
@@ -615,7 +618,8 @@ class Batch:
         r = BatchResult()
         check(lib().sigmo_batch_run(self._h, query.handle, int(find_first), C.byref(r)))
         self.last = {"num_matches": r.num_matches, "matched_pairs": r.matched_pairs, "gmcr_pairs": r.gmcr_pairs,
-                     "dfs_steps": r.dfs_steps}
+                     "dfs_steps": r.dfs_steps, "filter_wave_steps": r.filter_wave_steps,
+                     "filter_lane_steps": r.filter_lane_steps, "filter_scans": r.filter_scans}
         return self.last
 
     def pairs(self):

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/sigmo_hip.h"
@@ -33,8 +34,6 @@ namespace {
 constexpr uint32_t kMaxLevels = 8;        // token encoding keeps 3 bits of level
 constexpr uint32_t kNoWide = 0xFFFFFFFFu;
 constexpr uint16_t kEndToken = 0xFFFF;    // closes a query node's token segment
-constexpr uint32_t kArenaChunk = 1024;    // arena words a wavefront claims at a time
-constexpr uint32_t kRecordChunk = 256;    // pair records a wavefront claims at a time
 
 // Tokens of one query node: [label] [requirement]* [end], requirement = r | l << 3 | c << 7
 // (level r < 8, label l < 16, count 1 <= c <= 15).  Segments are 4-token aligned with one spare
@@ -117,33 +116,67 @@ __device__ __forceinline__ uint64_t transpose64(uint64_t x) {
   return x;
 }
 
+// Per-molecule facts the pair pre-test reads, stored in width-class order.
+struct MolInfo {
+  uint32_t m, s, n, x, wi, mlab;  // molecule, first node, nodes, plane index, wide index, labels < 16 present
+};
+
+// Per-query-graph facts the pair pre-test reads.
+struct QueryInfo {
+  uint32_t qoff, k, qlab, pad;  // first node, nodes, labels < 16 present
+  uint64_t first;               // first token word of the graph's first node
+};
+
 // One wavefront per molecule: lane p is position p (and p+64 for molecules of 65..128 nodes).
-// planes[(m*R + r)*64 + i]: bit p = bit i of the level-(r+1) signature of position p, i.e.
-// plane 4*label + bit; planes_hi the same for positions 64..127; lab_hi[w*16+l]: positions
-// 64..127 with label l (positions 0..63 use the graph's label masks); dmax[m*R + r]: the
+// Level planes, bit p = bit i of the level-(r+1) signature of position p for plane i =
+// 4*label + bit, go to one of two arrays by the molecule's width class: planes32[(x*R + r)*64
+// + i] (u32) for molecules of <= 32 nodes, planes64[(x*R + r)*64 + i] (u64) otherwise, x =
+// pidx[m]; planes_hi holds positions 64..127 and lab_hi[w*16+l] the positions 64..127 with
+// label l (w = wide_idx[m]; positions 0..63 use the graph's label masks).  Wavefront c handles
+// molecule m = cls_list[c] (width-class order) and writes minfo[c] and dmax[c*R + r], the
 // nibble-wise max of the level-(r+1) signatures over the molecule's positions.
-__global__ __launch_bounds__(256) void k_db_prepare(GraphDev g, uint32_t R, const uint32_t* __restrict__ wide_idx,
-                                                    uint64_t* __restrict__ planes, uint64_t* __restrict__ planes_hi,
-                                                    uint64_t* __restrict__ lab_hi, uint64_t* __restrict__ dmax) {
+__global__ __launch_bounds__(256) void k_db_prepare(GraphDev g, uint32_t R, const uint32_t* __restrict__ cls_list,
+                                                    const uint32_t* __restrict__ pidx, const uint32_t* __restrict__ wide_idx,
+                                                    uint32_t* __restrict__ planes32, uint64_t* __restrict__ planes64,
+                                                    uint64_t* __restrict__ planes_hi, uint64_t* __restrict__ lab_hi,
+                                                    uint64_t* __restrict__ dmax, MolInfo* __restrict__ minfo) {
+  __shared__ uint64_t fold_all[4][kWave];
   const uint32_t lane = static_cast<uint32_t>(lane_id());
-  const uint64_t m = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
-  if (m >= g.G) return;  // wave-uniform
+  const uint64_t c = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (c >= g.G) return;  // wave-uniform
+  const uint64_t m = cls_list[c];
   const uint32_t s = g.go[m], n = g.go[m + 1] - s;
   const uint64_t* lm = g.lmask + m * 16;
   const uint32_t halves = n > 64 ? 2 : 1;
+  const uint64_t x = pidx[m];
   const uint32_t wi = wide_idx[m];
+  // Only positions 0..63 are ever expanded (frontier bit b expands node s + b), each by its
+  // neighbours folded mod 64: stage those 64 words in LDS once.
+  uint64_t* fold = fold_all[threadIdx.x / kWave];
+  fold[lane] = lane < n ? (g.adj[2 * (static_cast<uint64_t>(s) + lane)] | g.adj[2 * (static_cast<uint64_t>(s) + lane) + 1]) : 0ull;
+  wave_sync();
+  uint32_t labels = 0;
   for (uint32_t h = 0; h < halves; ++h) {
     const uint32_t pos = lane + 64 * h;
     const bool valid = pos < n;
+    if (valid && g.nl[s + pos] < 16) labels |= 1u << g.nl[s + pos];
     uint64_t self = 1ull << (pos & 63), ball = self, frontier = self;
     for (uint32_t r = 0; r < R; ++r) {
       uint64_t sig = 0;
-      if (valid) sig = histogram16(next_layer(g, s, ball, frontier), lm);
+      if (valid) {
+        uint64_t next = 0;
+        for (uint64_t f = frontier; f; f &= f - 1) next |= fold[__builtin_ctzll(f)];
+        next &= ~ball;
+        ball |= next;
+        frontier = next;
+        sig = histogram16(next, lm);
+      }
       const uint64_t col = transpose64(sig);
-      if (h == 0) planes[(m * R + r) * 64 + lane] = col;
-      else planes_hi[(static_cast<uint64_t>(wi) * R + r) * 64 + lane] = col;
+      if (h == 1) planes_hi[(static_cast<uint64_t>(wi) * R + r) * 64 + lane] = col;
+      else if (n <= 32) planes32[(x * R + r) * 64 + lane] = static_cast<uint32_t>(col);
+      else planes64[(x * R + r) * 64 + lane] = col;
       const uint64_t mx = wave_nib_max(sig);
-      if (lane == 0) dmax[m * R + r] = h == 0 ? mx : nib_max(dmax[m * R + r], mx);
+      if (lane == 0) dmax[c * R + r] = h == 0 ? mx : nib_max(dmax[c * R + r], mx);
     }
     if (h == 1) {
       const uint32_t lab = valid ? g.nl[s + pos] : 0xFFu;
@@ -156,13 +189,25 @@ __global__ __launch_bounds__(256) void k_db_prepare(GraphDev g, uint32_t R, cons
       if (lane < 16) lab_hi[static_cast<uint64_t>(wi) * 16 + lane] = mine;
     }
   }
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) labels |= static_cast<uint32_t>(__shfl_xor(static_cast<int>(labels), o));
+  if (lane == 0) {
+    MolInfo info;
+    info.m = static_cast<uint32_t>(m);
+    info.s = s;
+    info.n = n;
+    info.x = static_cast<uint32_t>(x);
+    info.wi = wi;
+    info.mlab = labels;
+    minfo[c] = info;
+  }
 }
 
 // One wavefront per query graph, lane i = node i (query graphs have <= 32 nodes, so
 // positions never alias): writes each node's token segment, the graph's label set (labels
 // < 16) and, per level, the nibble-wise max of its nodes' signatures.
 __global__ __launch_bounds__(256) void k_query_prepare(GraphDev q, uint32_t R, uint16_t* __restrict__ tokens,
-                                                       uint32_t* __restrict__ qlab, uint64_t* __restrict__ qmax) {
+                                                       QueryInfo* __restrict__ qinfo, uint64_t* __restrict__ qmax) {
   const uint32_t lane = static_cast<uint32_t>(lane_id());
   const uint64_t g = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (g >= q.G) return;  // wave-uniform
@@ -172,6 +217,7 @@ __global__ __launch_bounds__(256) void k_query_prepare(GraphDev q, uint32_t R, u
   uint16_t* t = tokens + (static_cast<uint64_t>(prev) + lane) * seg;
   const uint32_t label = valid ? q.nl[prev + lane] : 0u;
   if (valid) t[0] = static_cast<uint16_t>(label);
+  uint64_t first = label;  // lane 0: the first four tokens of node 0
   uint64_t self = 1ull << (lane & 63), ball = self, frontier = self;
   uint32_t cnt = 1;
   for (uint32_t r = 0; r < R; ++r) {
@@ -180,313 +226,394 @@ __global__ __launch_bounds__(256) void k_query_prepare(GraphDev q, uint32_t R, u
       sig = histogram16(next_layer(q, prev, ball, frontier), q.lmask + g * 16);
       for (uint32_t l = 0; l < 16; ++l) {
         const uint32_t c = static_cast<uint32_t>(sig >> (4 * l)) & 15u;
-        if (c) t[cnt++] = static_cast<uint16_t>(r | (l << 3) | (c << 7));
+        if (c) {
+          const uint32_t tk = r | (l << 3) | (c << 7);
+          if (cnt < 4) first |= static_cast<uint64_t>(tk) << (16 * cnt);
+          t[cnt++] = static_cast<uint16_t>(tk);
+        }
       }
     }
     const uint64_t mx = wave_nib_max(sig);
     if (lane == 0) qmax[g * R + r] = mx;
   }
   if (valid) t[cnt] = kEndToken;
+  if (cnt < 4) first |= static_cast<uint64_t>(kEndToken) << (16 * cnt);
   uint32_t bits = valid && label < 16 ? 1u << label : 0u;
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) bits |= static_cast<uint32_t>(__shfl_xor(static_cast<int>(bits), o));
-  if (lane == 0) qlab[g] = bits;
+  if (lane == 0) {
+    QueryInfo info;
+    info.qoff = prev;
+    info.k = k;
+    info.qlab = bits;
+    info.pad = 0;
+    info.first = first;
+    qinfo[g] = info;
+  }
 }
 
-struct PairArgs {
-  GraphDev q, d;
-  uint32_t R, has_wide, mol_chunk;
-  const uint64_t* planes;
-  const uint64_t* planes_hi;
-  const uint64_t* lab_hi;
-  const uint32_t* wide_idx;
-  const uint64_t* dmax;
-  const uint64_t* tok64;  // token segments as 64-bit words
-  const uint32_t* qlab;
-  const uint64_t* qmax;
-  uint64_t* scratch;      // per wavefront: masks [32][64] (low words), then [32][64] (high words)
-  uint32_t* slot;         // per wavefront: arena offset of each surviving query graph [Gq]
-  uint64_t* arena;        // survivor masks: k words (2k, interleaved, for molecules of > 64 nodes)
-  uint64_t arena_cap;
-  uint32_t* rec_qg;       // records: per molecule a contiguous block in query-graph order
-  uint32_t* rec_off;
-  uint64_t rec_cap;
-  uint32_t* mol_block;    // [Gd] first record of the molecule
-  uint32_t* mol_count;    // [Gd]
-  unsigned long long* counters;  // [0] molecule cursor, [1] arena top, [2] record top, [3] overflow flags
+// A pre-tested (query graph, molecule) pair waiting for a lane.
+struct PairEntry {
+  uint32_t slot, qg, qoff, k;
+  uint64_t tw;  // first token word of the pair's first query node
 };
 
-__host__ __device__ inline uint64_t pair_lds_words(uint32_t R, uint32_t has_wide, uint32_t Gq) {
-  return static_cast<uint64_t>(R) * 64 * (has_wide ? 2 : 1) + 32 + (Gq + 63) / 64 + 32;
+struct SurviveArgs {
+  GraphDev q, d;
+  uint32_t R, Gq, W;       // W: survival-bitmask words per molecule
+  uint32_t count;          // molecules of the width class
+  uint32_t claim;          // molecules a wavefront claims at a time
+  const MolInfo* minfo;    // the width class's molecules, ascending
+  const uint64_t* dmax;    // the width class's [molecule][R] signature maxima
+  const uint32_t* planes32;
+  const uint64_t* planes64;
+  const uint64_t* planes_hi;
+  const uint64_t* lab_hi;
+  const uint64_t* tok64;   // token segments as 64-bit words
+  const QueryInfo* qinfo;
+  const uint64_t* qmax;
+  uint64_t* arena;         // survivor masks: k words (2k, interleaved, for molecules of > 64 nodes)
+  uint64_t arena_cap;
+  unsigned long long* surv;      // [Gd][W] bit qg of row m: the pair survives GMCR
+  uint32_t* pair_off;            // [Gd * Gq] arena offset of each surviving pair
+  unsigned long long* counters;  // [0] molecule cursor, [1] arena top, [3] overflow, [4..6] statistics
+};
+
+constexpr uint32_t kSurviveWaves = 4;     // wavefronts per block
+constexpr uint32_t kLaneChunk = 64;       // arena words a lane owns at a time (>= the largest pair, 2 x 32)
+constexpr uint32_t kWaveChunk = 64 * kLaneChunk;
+
+// Molecule slots per wavefront and LDS words (u64) per slot, by width class.
+__host__ __device__ constexpr uint32_t ring_slots(int nw) { return nw == 1 ? 4u : (nw == 2 ? 3u : 2u); }
+__host__ __device__ constexpr uint32_t slot_words(uint32_t R, int nw) {
+  return nw == 1 ? R * 32 + 8 : (nw == 2 ? R * 64 + 16 : 2 * R * 64 + 32);
+}
+__host__ __device__ constexpr uint32_t plist_words() { return kWave * sizeof(PairEntry) / 8; }
+__host__ __device__ constexpr uint32_t survive_lds_bytes(uint32_t R, int nw) {
+  return kSurviveWaves * (ring_slots(nw) * slot_words(R, nw) + plist_words()) * 8;
 }
 
-// Persistent pair filter.  A wavefront takes molecules from a cursor and stages the molecule's
-// level planes and label masks in LDS.  Query graphs pass two necessary tests first, 64 at a
-// time: label set (labels < 16) contained in the molecule's, and per level the graph's
-// nibble-wise signature max <= the molecule's (a node fits nowhere if it exceeds the max).
-// Passing graphs are handed to idle lanes; a lane runs its pair as a small state machine over
-// the query's token segments (label test, one requirement per step, node end), so lanes whose
-// pair died take the next graph at once, and the molecule-wide early exit is the GMCR predicate
-// of gmcr.hpp:56-61.  Survivors' masks go to the arena, per molecule one block of records in
-// query-graph order.  Arena and record space is claimed per wavefront in chunks.
-__global__ __launch_bounds__(256) void k_pair_filter(PairArgs a) {
+// Candidate masks of one width class: NW = 1 (<= 32 positions, u32), 2 (<= 64, u64) or
+// 4 (<= 128, two u64 words).
+template <int NW>
+struct Mask {
+  using T = typename std::conditional<NW == 1, uint32_t, uint64_t>::type;
+  T lo = 0;
+  uint64_t hi = 0;  // NW == 4 only
+  __device__ __forceinline__ bool any() const { return NW == 4 ? ((static_cast<uint64_t>(lo) | hi) != 0) : (lo != 0); }
+};
+
+// Persistent pair evaluation of one width class.  Each wavefront keeps a ring of molecule slots
+// in LDS -- the molecule's level planes and label masks, staged with coalesced loads when the
+// wavefront claims it -- and pre-tests the open molecule against 64 query graphs at a time with
+// two exact necessary tests: the query graph's label set (labels < 16) is contained in the
+// molecule's, and at every level the graph's nibble-wise signature max is <= the molecule's (a
+// node fits nowhere if it exceeds the max).  Passing pairs wait in an LDS list and go to idle
+// lanes, which may hold pairs of any slot; a lane runs its pair over the query's token
+// segments, one token word per step: a node's label token sets its mask to the filter's label
+// test (isomorphism.hpp:49), each requirement token ANDs one bit-sliced refine test
+// (isomorphism.hpp:103-110) read from the slot, and the end token stores the node's mask.  The
+// pair stops at the first node without candidates, which is the GMCR predicate
+// (gmcr.hpp:56-61).  A slot is reused once its molecule is fully pre-tested and none of its
+// pairs is pending.  Each lane owns an arena sub-chunk: node masks go straight to the pair's
+// final place, and a pair that dies leaves its space to the lane's next pair.
+template <int NW>
+__global__ __launch_bounds__(256) void k_pair_survive(SurviveArgs a) {
+  using T = typename Mask<NW>::T;
+  constexpr uint32_t stride = NW == 4 ? 2u : 1u;
+  constexpr uint32_t S = ring_slots(NW);
   extern __shared__ uint64_t lds[];
   const uint32_t lane = static_cast<uint32_t>(lane_id());
   const uint32_t wave = threadIdx.x / kWave;
-  const uint32_t Gq = a.q.G;
-  const uint32_t words = (Gq + 63) / 64;
-  const uint32_t R = a.R;
-  uint64_t* lp = lds + wave * pair_lds_words(R, a.has_wide, Gq);
-  uint64_t* lph = lp + R * 64;
-  uint64_t* llm = lph + (a.has_wide ? R * 64 : 0);
-  uint64_t* llh = llm + 16;
-  uint64_t* surv = llh + 16;
-  uint32_t* xfer = reinterpret_cast<uint32_t*>(surv + words);
-  const uint64_t gwave = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + wave;
-  uint64_t* sc_lo = a.scratch + gwave * (a.has_wide ? 2 * 32 * 64 : 32 * 64);
-  uint64_t* sc_hi = sc_lo + 32 * 64;
-  uint32_t* slot = a.slot + gwave * Gq;
+  const uint32_t R = a.R, Gq = a.Gq;
+  const uint32_t sw = slot_words(R, NW);
+  uint64_t* ring = lds + static_cast<uint64_t>(wave) * (S * sw + plist_words());
+  PairEntry* plist = reinterpret_cast<PairEntry*>(ring + S * sw);
   const uint32_t seg = token_segment(R);
   const uint64_t below = (1ull << lane) - 1;
-  uint64_t ck_next = 0, ck_end = 0, rk_next = 0, rk_end = 0;  // claimed arena / record chunks
 
-  uint64_t mnext = 0, mend = 0;
+  // wave-uniform state
+  uint64_t wc_next = 0, wc_end = 0;           // claimed arena chunk
+  uint32_t lhead = 0, lcount = 0;             // pending list entries [lhead, lcount)
+  uint32_t pending[S];                        // passing pairs of the slot's molecule not yet finished
+  MolInfo smol[S];                            // the slot's molecule
+  uint32_t open = S, scan_qg = 0;             // slot being pre-tested (S: none) and its next query graph
+  uint64_t omax[kMaxLevels];                  // the open molecule's signature maxima
+  uint32_t omlab = 0;                         // the open molecule's labels < 16
+  bool exhausted = false;
+  uint64_t mc_next = 0, mc_end = 0;           // claimed molecules [mc_next, mc_end)
+#pragma unroll
+  for (uint32_t sl = 0; sl < S; ++sl) pending[sl] = 0;
+  uint64_t wave_steps = 0, lane_steps = 0, scans = 0;  // statistics
+
+  // per-lane pair state
+  bool have = false;
+  uint32_t sl_of = 0, qg = 0, k = 0, i = 0, tp = 0;
+  uint64_t nodebase = 0, tw = 0;
+  uint64_t lc_next = 0, lc_end = 0, base = 0;  // the lane's arena sub-chunk; the pair's first word
+  Mask<NW> mask;
   while (true) {
-    if (mnext >= mend) {
-      unsigned long long b = 0;
-      if (lane == 0) b = atomicAdd(&a.counters[0], static_cast<unsigned long long>(a.mol_chunk));
-      mnext = uniform64(__shfl(b, 0));
-      mend = mnext + a.mol_chunk < a.d.G ? mnext + a.mol_chunk : a.d.G;
-      if (mnext >= a.d.G) break;
-    }
-    const uint64_t m = mnext++;
-    const uint32_t s = a.d.go[m], n = a.d.go[m + 1] - s;
-    const bool wide = n > 64;
-    const uint32_t wi = a.wide_idx[m];
-    {
-      const uint64_t* src = a.planes + m * R * 64;
-      for (uint32_t t = lane; t < R * 64; t += kWave) lp[t] = src[t];
-      if (wide) {
-        const uint64_t* srch = a.planes_hi + static_cast<uint64_t>(wi) * R * 64;
-        for (uint32_t t = lane; t < R * 64; t += kWave) lph[t] = srch[t];
-      }
-    }
-    uint64_t lmv = 0, lhv = 0;
-    if (lane < 16) {
-      lmv = a.d.lmask[m * 16 + lane];
-      lhv = wide ? a.lab_hi[static_cast<uint64_t>(wi) * 16 + lane] : 0ull;
-      llm[lane] = lmv;
-      llh[lane] = lhv;
-    }
-    const uint32_t mlab = static_cast<uint32_t>(__ballot((lmv | lhv) != 0)) & 0xFFFFu;
-    for (uint32_t w = lane; w < words; w += kWave) surv[w] = 0;
-    const uint64_t* mmax = a.dmax + m * R;
-    const uint64_t all_lo = n >= 64 ? ~0ull : ((1ull << n) - 1);
-    const uint64_t all_hi = n <= 64 ? 0ull : (n >= 128 ? ~0ull : ((1ull << (n - 64)) - 1));
-    wave_sync();
-
-    bool have = false, fresh = false;
-    uint32_t qg = 0, i = 0, k = 0, tp = 0;
-    uint64_t nodebase = 0, tw = 0, tn = 0, pf0 = 0, pf1 = 0, lo = 0, hi = 0;
-    uint32_t scan = 0, pend_base = 0;  // wave-uniform
-    uint64_t pend = 0;                 // wave-uniform: graphs pend_base + j that passed the tests
-    while (true) {
-      // hand passing query graphs to idle lanes
-      uint64_t need = __ballot(!have);
-      while (need) {
-        if (!pend) {
-          if (scan >= Gq) break;
-          const uint32_t g = scan + lane;
-          bool ok = g < Gq;
-          if (ok) ok = (a.qlab[g] & ~mlab) == 0u;
-          for (uint32_t r = 0; r < R && ok; ++r) ok = nib_le_all(a.qmax[static_cast<uint64_t>(g) * R + r], mmax[r]);
-          pend = __ballot(ok);
-          pend_base = scan;
-          scan += kWave;
-          continue;
-        }
-        const uint32_t cnt = min(__popcll(pend), __popcll(need));
-        const bool give = ((pend >> lane) & 1ull) && static_cast<uint32_t>(__popcll(pend & below)) < cnt;
+    uint64_t need = __ballot(!have);
+    while (need) {
+      if (lhead < lcount) {  // hand waiting pairs to idle lanes
+        const uint32_t cnt = min(lcount - lhead, static_cast<uint32_t>(__popcll(need)));
         const uint32_t rn = __popcll(need & below);
         const bool take = ((need >> lane) & 1ull) && rn < cnt;
-        if (give) xfer[__popcll(pend & below)] = pend_base + lane;
-        wave_sync();
         if (take) {
-          qg = xfer[rn];
-          const uint32_t qoff = a.q.go[qg];
-          k = a.q.go[qg + 1] - qoff;
+          const PairEntry& e = plist[lhead + rn];
+          sl_of = e.slot;
+          qg = e.qg;
+          k = e.k;
+          tw = e.tw;
+          nodebase = static_cast<uint64_t>(e.qoff) * seg;
           i = 0;
-          nodebase = static_cast<uint64_t>(qoff) * seg;
-          tw = a.tok64[nodebase / 4];
-          tn = a.tok64[nodebase / 4 + 1];
-          if (k > 1) {
-            pf0 = a.tok64[(nodebase + seg) / 4];
-            pf1 = a.tok64[(nodebase + seg) / 4 + 1];
-          }
+          tp = 0;
           have = true;
-          fresh = true;
         }
         wave_sync();
-        pend &= ~__ballot(give);
+        lhead += cnt;
         need &= ~__ballot(take);
-      }
-      if (__ballot(have) == 0) break;
-      bool done = false;
-      if (have) {
-        if (fresh) {  // node start: the filter's label test (isomorphism.hpp:49)
-          const uint32_t ql = static_cast<uint32_t>(tw & 0xFFFFu);
-          if (ql == 255u) {
-            lo = all_lo;
-            hi = all_hi;
-          } else if (ql < 16u) {
-            lo = llm[ql];
-            hi = llh[ql];
-          } else {
-            lo = hi = 0;
-            for (uint32_t p = 0; p < n; ++p)
-              if (a.d.nl[s + p] == ql) {
-                if (p < 64) lo |= 1ull << p;
-                else hi |= 1ull << (p - 64);
-              }
+        // lanes whose sub-chunk cannot hold the new pair get a fresh one from the wavefront's chunk
+        const bool fresh_chunk = take && lc_next + k * stride > lc_end;
+        const uint64_t fm = __ballot(fresh_chunk);
+        if (fm) {
+          const uint32_t nf = __popcll(fm);
+          if (wc_next + static_cast<uint64_t>(nf) * kLaneChunk > wc_end) {
+            unsigned long long b = 0;
+            if (lane == 0) {
+              b = atomicAdd(&a.counters[1], static_cast<unsigned long long>(kWaveChunk));
+              if (b + kWaveChunk > a.arena_cap) atomicOr(&a.counters[3], 1ull);
+            }
+            wc_next = uniform64(__shfl(b, 0));
+            wc_end = wc_next + kWaveChunk;
           }
-          tp = 1;
-          fresh = false;
-          if (!(lo | hi)) have = false;
-        } else {
-          const uint32_t tok = static_cast<uint32_t>(tw >> (16 * (tp & 3))) & 0xFFFFu;
-          if (tok == kEndToken) {  // node done: keep its mask
-            sc_lo[i * 64 + lane] = lo;
-            if (wide) sc_hi[i * 64 + lane] = hi;
-            ++i;
-            if (i == k) {  // every query node has a candidate: the pair survives GMCR
-              done = true;
-              have = false;
-            } else {
-              nodebase += seg;
-              tw = pf0;
-              tn = pf1;
-              tp = 0;
-              fresh = true;
-              if (i + 1 < k) {
-                pf0 = a.tok64[(nodebase + seg) / 4];
-                pf1 = a.tok64[(nodebase + seg) / 4 + 1];
-              }
-            }
-          } else {  // requirement: count_q(l) <= count_p(l) at level r (isomorphism.hpp:103-110)
-            const uint32_t r = tok & 7u, l = (tok >> 3) & 15u, c = (tok >> 7) & 15u;
-            const uint64_t* pl = lp + r * 64 + l * 4;
-            lo &= ge4_lane(pl[0], pl[1], pl[2], pl[3], c);
-            if (wide) {
-              const uint64_t* ph = lph + r * 64 + l * 4;
-              hi &= ge4_lane(ph[0], ph[1], ph[2], ph[3], c);
-            }
-            if (!(lo | hi)) have = false;
-            ++tp;
-            if ((tp & 3) == 0) {
-              tw = tn;
-              tn = a.tok64[(nodebase + tp) / 4 + 1];
-            }
+          if (fresh_chunk) {
+            lc_next = wc_next + static_cast<uint64_t>(__popcll(fm & below)) * kLaneChunk;
+            lc_end = lc_next + kLaneChunk;
           }
+          wc_next += static_cast<uint64_t>(nf) * kLaneChunk;
         }
+        if (take) base = lc_next;
+        continue;
       }
-      // survivors of this step claim arena space from the wavefront's chunk
-      if (__ballot(done)) {
-        const uint32_t stride = wide ? 2 : 1;
-        const uint32_t want = done ? k * stride : 0u;
-        uint32_t incl = want;
+      if (open < S && scan_qg < Gq) {  // pre-test the open molecule against the next 64 query graphs
+        const uint32_t g = scan_qg + lane;
+        bool ok = g < Gq;
+        // every load issued before any use: clamped indices, results masked afterwards
+        const uint32_t gs = ok ? g : Gq - 1;
+        const QueryInfo qi = a.qinfo[gs];
+        uint64_t qm[kMaxLevels];
 #pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-          const uint32_t y = __shfl_up(incl, o);
-          if (lane >= static_cast<uint32_t>(o)) incl += y;
+        for (uint32_t r = 0; r < kMaxLevels; ++r) qm[r] = a.qmax[static_cast<uint64_t>(gs) * R + (r < R ? r : R - 1)];
+        bool fits = (qi.qlab & ~omlab) == 0u;
+#pragma unroll
+        for (uint32_t r = 0; r < kMaxLevels; ++r) fits = fits & nib_le_all(qm[r], omax[r]);
+        ok = ok && fits;
+        const uint64_t okm = __ballot(ok);
+        ++scans;
+        if (ok) {
+          PairEntry& e = plist[__popcll(okm & below)];
+          e.slot = open;
+          e.qg = g;
+          e.qoff = qi.qoff;
+          e.k = qi.k;
+          e.tw = qi.first;
         }
-        const uint32_t total = uniform32(__shfl(incl, kWave - 1));
-        if (ck_next + total > ck_end) {
-          const uint64_t grab = total > kArenaChunk ? total : kArenaChunk;
-          unsigned long long b = 0;
-          if (lane == 0) b = atomicAdd(&a.counters[1], static_cast<unsigned long long>(grab));
-          ck_next = uniform64(__shfl(b, 0));
-          ck_end = ck_next + grab;
-        }
-        const uint64_t off = ck_next + incl - want;
-        ck_next += total;
-        if (done) {
-          if (off + want <= a.arena_cap) {
-            for (uint32_t t = 0; t < k; ++t) {
-              a.arena[off + t * stride] = sc_lo[t * 64 + lane];
-              if (wide) a.arena[off + t * stride + 1] = sc_hi[t * 64 + lane];
-            }
-          } else {
-            atomicOr(&a.counters[3], 1ull);
-          }
-          slot[qg] = static_cast<uint32_t>(off);
-          atomicOr(reinterpret_cast<unsigned long long*>(&surv[qg >> 6]), 1ull << (qg & 63));
-        }
+        wave_sync();
+        lhead = 0;
+        lcount = __popcll(okm);
+#pragma unroll
+        for (uint32_t sl = 0; sl < S; ++sl)
+          if (sl == open) pending[sl] += lcount;
+        scan_qg += kWave;
+        continue;
       }
-    }
-    wave_sync();
-    // emit this molecule's survivors as one block of records, ascending query graph
-    uint32_t mine = 0;
-    for (uint32_t w = lane; w < words; w += kWave) mine += __popcll(surv[w]);
-    uint32_t total = mine;
-    for (int o = kWave / 2; o > 0; o >>= 1) total += __shfl_xor(total, o);
-    total = uniform32(total);
-    uint64_t blk = 0;
-    if (total) {
-      if (rk_next + total > rk_end) {
-        const uint64_t grab = total > kRecordChunk ? total : kRecordChunk;
+      // the open molecule is fully pre-tested: open the next one in a free slot
+      if (exhausted) break;
+      uint32_t free_slot = S;
+#pragma unroll
+      for (uint32_t sl = 0; sl < S; ++sl)
+        if (free_slot == S && pending[sl] == 0) free_slot = sl;
+      if (free_slot == S) break;  // every slot still has pairs in flight
+      if (mc_next >= mc_end) {  // claim the next few molecules (one atomic per claim)
         unsigned long long b = 0;
-        if (lane == 0) b = atomicAdd(&a.counters[2], static_cast<unsigned long long>(grab));
-        rk_next = uniform64(__shfl(b, 0));
-        rk_end = rk_next + grab;
+        if (lane == 0) b = atomicAdd(&a.counters[0], static_cast<unsigned long long>(a.claim));
+        mc_next = uniform64(__shfl(b, 0));
+        mc_end = mc_next + a.claim < a.count ? mc_next + a.claim : a.count;
       }
-      blk = rk_next;
-      rk_next += total;
-    }
-    if (lane == 0) {
-      a.mol_block[m] = static_cast<uint32_t>(blk);
-      a.mol_count[m] = total;
-    }
-    if (total && blk + total > a.rec_cap) {
-      if (lane == 0) atomicOr(&a.counters[3], 2ull);
-    } else if (total) {
-      // lane-strided words; exclusive prefix of the popcounts in word order
-      uint64_t base = 0;
-      for (uint32_t w0 = 0; w0 < words; w0 += kWave) {
-        const uint32_t w = w0 + lane;
-        const uint64_t x = w < words ? surv[w] : 0ull;
-        const uint32_t c = __popcll(x);
-        uint32_t incl = c;
-        for (int o = 1; o < kWave; o <<= 1) {
-          const uint32_t y = __shfl_up(incl, o);
-          if (lane >= static_cast<uint32_t>(o)) incl += y;
-        }
-        uint64_t pos = blk + base + incl - c;
-        for (uint64_t t = x; t; t &= t - 1) {
-          const uint32_t g = w * 64 + __builtin_ctzll(t);
-          a.rec_qg[pos] = g;
-          a.rec_off[pos] = slot[g];
-          ++pos;
-        }
-        base += __shfl(incl, kWave - 1);
+      if (mc_next >= a.count) {
+        exhausted = true;
+        open = S;
+        break;
       }
+      const uint64_t c = mc_next++;
+      const MolInfo mi = a.minfo[c];
+#pragma unroll
+      for (uint32_t r = 0; r < kMaxLevels; ++r) omax[r] = a.dmax[c * R + (r < R ? r : R - 1)];
+      uint64_t* slot = ring + free_slot * sw;
+      if (NW == 1) {
+        const uint64_t* src = reinterpret_cast<const uint64_t*>(a.planes32 + static_cast<uint64_t>(mi.x) * R * 64);
+        for (uint32_t t = lane; t < R * 32; t += kWave) slot[t] = src[t];
+        if (lane < 16) reinterpret_cast<uint32_t*>(slot + R * 32)[lane] = static_cast<uint32_t>(a.d.lmask[static_cast<uint64_t>(mi.m) * 16 + lane]);
+      } else {
+        const uint64_t* src = a.planes64 + static_cast<uint64_t>(mi.x) * R * 64;
+        for (uint32_t t = lane; t < R * 64; t += kWave) slot[t] = src[t];
+        const uint32_t lm_at = NW == 4 ? 2 * R * 64 : R * 64;
+        if (lane < 16) slot[lm_at + lane] = a.d.lmask[static_cast<uint64_t>(mi.m) * 16 + lane];
+        if (NW == 4) {
+          const uint64_t* srch = a.planes_hi + static_cast<uint64_t>(mi.wi) * R * 64;
+          for (uint32_t t = lane; t < R * 64; t += kWave) slot[R * 64 + t] = srch[t];
+          if (lane < 16) slot[lm_at + 16 + lane] = a.lab_hi[static_cast<uint64_t>(mi.wi) * 16 + lane];
+        }
+      }
+      wave_sync();
+#pragma unroll
+      for (uint32_t sl = 0; sl < S; ++sl)
+        if (sl == free_slot) smol[sl] = mi;
+      open = free_slot;
+      omlab = mi.mlab;
+      scan_qg = 0;
     }
-    wave_sync();
+    const uint64_t active = __ballot(have);
+    if (active == 0) break;
+    ++wave_steps;
+    lane_steps += __popcll(active);
+    bool finished = false;
+    if (have) {
+      // one token word per step: its label / requirement reads (LDS) and the next word's load are
+      // all in flight together; requirements of a node commute, so their order within the word is free
+      const uint64_t* slot = ring + sl_of * sw;
+      uint32_t tok[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) tok[j] = static_cast<uint32_t>(tw >> (16 * j)) & 0xFFFFu;
+      const uint32_t e = tok[0] == kEndToken ? 0u : tok[1] == kEndToken ? 1u : tok[2] == kEndToken ? 2u : tok[3] == kEndToken ? 3u : 4u;
+      const bool last = e < 4 && i + 1 == k;  // the pair's last node ends in this word
+      uint64_t tw_next = 0;
+      if (!last) tw_next = a.tok64[(e < 4 ? nodebase + seg : nodebase + tp + 4) / 4];
+      // all four tokens' slot reads issued together (clamped addresses), masked afterwards
+      const uint32_t* slot32 = reinterpret_cast<const uint32_t*>(slot);
+      uint64_t pv[4][4], ph[4][4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t r = min(tok[j] & 7u, R - 1), l = (tok[j] >> 3) & 15u;
+        if (NW == 1) {
+          const uint4 v = *reinterpret_cast<const uint4*>(slot32 + r * 64 + l * 4);
+          pv[j][0] = v.x;
+          pv[j][1] = v.y;
+          pv[j][2] = v.z;
+          pv[j][3] = v.w;
+        } else {
+#pragma unroll
+          for (int b = 0; b < 4; ++b) pv[j][b] = slot[r * 64 + l * 4 + b];
+          if (NW == 4) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) ph[j][b] = slot[R * 64 + r * 64 + l * 4 + b];
+          }
+        }
+      }
+      const uint32_t ql = tok[0];
+      const uint32_t lm_at = NW == 1 ? 2 * R * 32 : (NW == 4 ? 2 * R * 64 : R * 64);  // u32 index for NW == 1
+      uint64_t lab_lo = NW == 1 ? slot32[lm_at + (ql & 15u)] : slot[lm_at + (ql & 15u)];
+      const uint64_t lab_hi_w = NW == 4 ? slot[lm_at + 16 + (ql & 15u)] : 0ull;
+      if (tp == 0) {  // the node's label token: the filter's label test (isomorphism.hpp:49)
+        if (ql < 16u) {
+          mask.lo = static_cast<T>(lab_lo);
+          if (NW == 4) mask.hi = lab_hi_w;
+        } else {
+          MolInfo mi = {};
+#pragma unroll
+          for (uint32_t sl = 0; sl < S; ++sl)
+            if (sl == sl_of) mi = smol[sl];
+          if (ql == 255u) {
+            mask.lo = static_cast<T>(mi.n >= 64 ? ~0ull : ((1ull << mi.n) - 1));
+            if (NW == 4) mask.hi = mi.n >= 128 ? ~0ull : ((1ull << (mi.n - 64)) - 1);
+          } else {
+            uint64_t l0 = 0, l1 = 0;
+            for (uint32_t p = 0; p < mi.n; ++p)
+              if (a.d.nl[mi.s + p] == ql) {
+                if (p < 64) l0 |= 1ull << p;
+                else l1 |= 1ull << (p - 64);
+              }
+            mask.lo = static_cast<T>(l0);
+            if (NW == 4) mask.hi = l1;
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {  // count_q(l) <= count_p(l) at level r (isomorphism.hpp:103-110)
+        const bool req = static_cast<uint32_t>(j) < e && (tp != 0 || j != 0);
+        const uint32_t c = req ? (tok[j] >> 7) & 15u : 0u;  // c = 0: always true
+        mask.lo &= static_cast<T>(ge4_bits<T>(static_cast<T>(pv[j][0]), static_cast<T>(pv[j][1]), static_cast<T>(pv[j][2]),
+                                              static_cast<T>(pv[j][3]), c));
+        if (NW == 4) mask.hi &= ge4_bits<uint64_t>(ph[j][0], ph[j][1], ph[j][2], ph[j][3], c);
+      }
+      if (!mask.any()) {
+        have = false;
+        finished = true;
+      } else if (e < 4) {  // node done: its mask goes to the pair's arena block
+        const uint64_t at = base + static_cast<uint64_t>(i) * stride;
+        if (at + stride <= a.arena_cap) {
+          a.arena[at] = static_cast<uint64_t>(mask.lo);
+          if (NW == 4) a.arena[at + 1] = mask.hi;
+        }
+        ++i;
+        if (i == k) {  // every query node has a candidate: the pair survives GMCR
+          uint32_t m = 0;
+#pragma unroll
+          for (uint32_t sl = 0; sl < S; ++sl)
+            if (sl == sl_of) m = smol[sl].m;
+          a.pair_off[static_cast<uint64_t>(m) * Gq + qg] = static_cast<uint32_t>(base);
+          atomicOr(&a.surv[static_cast<uint64_t>(m) * a.W + (qg >> 6)], 1ull << (qg & 63));
+          lc_next = base + static_cast<uint64_t>(k) * stride;
+          have = false;
+          finished = true;
+        } else {
+          nodebase += seg;
+          tp = 0;
+        }
+      } else {
+        tp += 4;
+      }
+      tw = tw_next;
+    }
+    // pairs finished this step release their slot's count
+#pragma unroll
+    for (uint32_t sl = 0; sl < S; ++sl) pending[sl] -= __popcll(__ballot(finished && sl_of == sl));
+  }
+  if (lane == 0) {
+    atomicAdd(&a.counters[4], static_cast<unsigned long long>(wave_steps));
+    atomicAdd(&a.counters[5], static_cast<unsigned long long>(lane_steps));
+    atomicAdd(&a.counters[6], static_cast<unsigned long long>(scans));
   }
 }
 
-// Canonical GMCR arrays from the per-molecule record blocks.
-__global__ __launch_bounds__(256) void k_pairs_scatter(uint32_t Gd, const uint32_t* __restrict__ offsets,
-                                                       const uint32_t* __restrict__ mol_block, const uint32_t* __restrict__ rec_qg,
-                                                       const uint32_t* __restrict__ rec_off, uint32_t* __restrict__ indices,
-                                                       uint32_t* __restrict__ pair_graph, uint32_t* __restrict__ mask_off) {
+// Survivors per molecule (popcount of its row of the survival bitmask).
+__global__ __launch_bounds__(256) void k_surv_count(uint32_t Gd, uint32_t W, const unsigned long long* __restrict__ surv,
+                                                    uint32_t* __restrict__ counts) {
   const uint64_t m = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   if (m >= Gd) return;
-  const uint32_t o = offsets[m], c = offsets[m + 1] - o, b = mol_block[m];
-  for (uint32_t i = 0; i < c; ++i) {
-    indices[o + i] = rec_qg[b + i];
-    pair_graph[o + i] = static_cast<uint32_t>(m);
-    mask_off[o + i] = rec_off[b + i];
-  }
+  uint32_t c = 0;
+  for (uint32_t w = 0; w < W; ++w) c += __popcll(surv[m * W + w]);
+  counts[m] = c;
+}
+
+// Canonical GMCR arrays (gmcr.hpp:73-115 order: molecules ascending, query graphs ascending
+// within a molecule) from the survival bitmask, with each pair's arena offset.
+__global__ __launch_bounds__(256) void k_surv_scatter(uint32_t Gd, uint32_t Gq, uint32_t W, const unsigned long long* __restrict__ surv,
+                                                      const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ pair_off,
+                                                      uint32_t* __restrict__ indices, uint32_t* __restrict__ pair_graph,
+                                                      uint32_t* __restrict__ mask_off) {
+  const uint64_t m = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (m >= Gd) return;
+  uint32_t o = offsets[m];
+  for (uint32_t w = 0; w < W; ++w)
+    for (uint64_t x = surv[m * W + w]; x; x &= x - 1) {
+      const uint32_t g = w * 64 + __builtin_ctzll(x);
+      indices[o] = g;
+      pair_graph[o] = static_cast<uint32_t>(m);
+      mask_off[o] = pair_off[m * Gq + g];
+      ++o;
+    }
 }
 
 // Join over the canonical pairs with candidate masks from the arena (the molecule-local masks
@@ -576,16 +703,16 @@ __global__ __launch_bounds__(256) void k_join_arena(GraphDev q, GraphDev d, cons
 
 }  // namespace
 
-hipError_t db_prepare(hipStream_t s, const GraphDev& g, uint32_t R, const uint32_t* wide_idx, uint64_t* planes, uint64_t* planes_hi,
-                      uint64_t* lab_hi, uint64_t* dmax) {
+hipError_t db_prepare(hipStream_t s, const GraphDev& g, uint32_t R, const uint32_t* cls_list, const uint32_t* pidx, const uint32_t* wide_idx,
+                      uint32_t* planes32, uint64_t* planes64, uint64_t* planes_hi, uint64_t* lab_hi, uint64_t* dmax, MolInfo* minfo) {
   if (!g.G) return hipSuccess;
-  k_db_prepare<<<(g.G + 3) / 4, 256, 0, s>>>(g, R, wide_idx, planes, planes_hi, lab_hi, dmax);
+  k_db_prepare<<<(g.G + 3) / 4, 256, 0, s>>>(g, R, cls_list, pidx, wide_idx, planes32, planes64, planes_hi, lab_hi, dmax, minfo);
   return hipGetLastError();
 }
 
-hipError_t query_prepare(hipStream_t s, const GraphDev& q, uint32_t R, uint16_t* tokens, uint32_t* qlab, uint64_t* qmax) {
+hipError_t query_prepare(hipStream_t s, const GraphDev& q, uint32_t R, uint16_t* tokens, QueryInfo* qinfo, uint64_t* qmax) {
   if (!q.G) return hipSuccess;
-  k_query_prepare<<<(q.G + 3) / 4, 256, 0, s>>>(q, R, tokens, qlab, qmax);
+  k_query_prepare<<<(q.G + 3) / 4, 256, 0, s>>>(q, R, tokens, qinfo, qmax);
   return hipGetLastError();
 }
 
@@ -655,15 +782,24 @@ struct sigmo_db {
   const sigmo_graph* data = nullptr;
   uint32_t R = 0;
   uint32_t num_wide = 0;
+  uint32_t cls_count[3] = {0, 0, 0};  // molecules of <= 32, 33..64 and 65..128 nodes
+  uint32_t* cls_list = nullptr;       // the three classes' molecules, ascending, one after another
+  uint32_t* pidx = nullptr;           // index of the molecule's planes in planes32 or planes64
   uint32_t* wide_idx = nullptr;
-  uint64_t *planes = nullptr, *planes_hi = nullptr, *lab_hi = nullptr, *dmax = nullptr;
+  uint32_t* planes32 = nullptr;
+  uint64_t *planes64 = nullptr, *planes_hi = nullptr, *lab_hi = nullptr, *dmax = nullptr;  // dmax in width-class order
+  sigmo_k::MolInfo* minfo = nullptr;                                                       // width-class order
   uint64_t bytes = 0;
   ~sigmo_db() {
+    bfree(cls_list);
+    bfree(pidx);
     bfree(wide_idx);
-    bfree(planes);
+    bfree(planes32);
+    bfree(planes64);
     bfree(planes_hi);
     bfree(lab_hi);
     bfree(dmax);
+    bfree(minfo);
   }
 };
 
@@ -672,20 +808,18 @@ struct sigmo_batch {
   const sigmo_db* db = nullptr;
   // query side
   uint16_t* tokens = nullptr;
-  uint32_t* qlab = nullptr;
+  sigmo_k::QueryInfo* qinfo = nullptr;
   uint64_t* qmax = nullptr;
-  uint64_t cap_tokens = 0, cap_qlab = 0, cap_qmax = 0;
-  // pair filter
-  uint64_t* scratch = nullptr;
-  uint64_t cap_scratch = 0;
-  uint32_t* slot = nullptr;
-  uint64_t cap_slot = 0;
+  uint64_t cap_tokens = 0, cap_qinfo = 0, cap_qmax = 0;
+  // pair evaluation
   uint64_t* arena = nullptr;
   uint64_t cap_arena = 0;
-  uint32_t *rec_qg = nullptr, *rec_off = nullptr;
-  uint64_t cap_rec = 0, cap_rec2 = 0;
-  uint32_t *mol_block = nullptr, *mol_count = nullptr;
-  uint64_t cap_mb = 0, cap_mc = 0;
+  unsigned long long* surv = nullptr;
+  uint64_t cap_surv = 0;
+  uint32_t* pair_off = nullptr;
+  uint64_t cap_pair_off = 0;
+  uint32_t* mol_count = nullptr;
+  uint64_t cap_mc = 0;
   unsigned long long* counters = nullptr;
   // canonical GMCR and join
   uint32_t *offsets = nullptr, *indices = nullptr, *pair_graph = nullptr, *mask_off = nullptr;
@@ -700,14 +834,11 @@ struct sigmo_batch {
   bool ran = false;
   ~sigmo_batch() {
     bfree(tokens);
-    bfree(qlab);
+    bfree(qinfo);
     bfree(qmax);
-    bfree(scratch);
-    bfree(slot);
     bfree(arena);
-    bfree(rec_qg);
-    bfree(rec_off);
-    bfree(mol_block);
+    bfree(surv);
+    bfree(pair_off);
     bfree(mol_count);
     bfree(counters);
     bfree(offsets);
@@ -732,27 +863,43 @@ int sigmo_db_prepare(sigmo_ctx* ctx, const sigmo_graph* data, uint32_t refinemen
     return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "the batched form supports 1..8 refinement levels");
   const auto& go = graph_host_offsets(data);
   const uint32_t G = static_cast<uint32_t>(go.size() - 1);
-  std::vector<uint32_t> widx(G, sigmo_k::kNoWide);
-  uint32_t nw = 0;
-  for (uint32_t g = 0; g < G; ++g)
-    if (go[g + 1] - go[g] > 64) widx[g] = nw++;
+  std::vector<uint32_t> widx(G, sigmo_k::kNoWide), pidx(G, 0), lists[3];
+  uint32_t nw = 0, n32 = 0, n64 = 0;
+  for (uint32_t g = 0; g < G; ++g) {
+    const uint32_t n = go[g + 1] - go[g];
+    const int c = n <= 32 ? 0 : (n <= 64 ? 1 : 2);
+    lists[c].push_back(g);
+    pidx[g] = c == 0 ? n32++ : n64++;
+    if (c == 2) widx[g] = nw++;
+  }
+  std::vector<uint32_t> cls;
+  cls.reserve(G);
+  for (int c = 0; c < 3; ++c) cls.insert(cls.end(), lists[c].begin(), lists[c].end());
   auto* db = new sigmo_db();
   db->ctx = ctx;
   db->data = data;
   db->R = refinement_steps;
   db->num_wide = nw;
+  for (int c = 0; c < 3; ++c) db->cls_count[c] = static_cast<uint32_t>(lists[c].size());
   const uint64_t R = refinement_steps;
-  int stt = balloc(&db->wide_idx, G);
-  if (stt == SIGMO_OK) stt = balloc(&db->planes, uint64_t(G) * R * 64);
+  int stt = balloc(&db->cls_list, G);
+  if (stt == SIGMO_OK) stt = balloc(&db->pidx, G);
+  if (stt == SIGMO_OK) stt = balloc(&db->wide_idx, G);
+  if (stt == SIGMO_OK) stt = balloc(&db->planes32, uint64_t(n32) * R * 64);
+  if (stt == SIGMO_OK) stt = balloc(&db->planes64, uint64_t(n64) * R * 64);
   if (stt == SIGMO_OK) stt = balloc(&db->planes_hi, uint64_t(nw) * R * 64);
   if (stt == SIGMO_OK) stt = balloc(&db->lab_hi, uint64_t(nw) * 16);
   if (stt == SIGMO_OK) stt = balloc(&db->dmax, uint64_t(G) * R);
+  if (stt == SIGMO_OK) stt = balloc(&db->minfo, G);
   if (stt != SIGMO_OK) {
     delete db;
     return stt;
   }
-  db->bytes = G * 4ull + (uint64_t(G) + nw) * R * 512 + nw * 128ull + uint64_t(G) * R * 8;
+  db->bytes = G * (12ull + sizeof(sigmo_k::MolInfo)) + (uint64_t(n32) * 256 + (uint64_t(n64) + nw) * 512) * R + nw * 128ull +
+              uint64_t(G) * R * 8;
   hipError_t e = hipMemcpyAsync(db->wide_idx, widx.data(), G * 4ull, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(db->pidx, pidx.data(), G * 4ull, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(db->cls_list, cls.data(), G * 4ull, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     delete db;
@@ -780,7 +927,8 @@ int sigmo_db_refresh(sigmo_db* db) {
   const double bytes = 4.0 * (double(g.N) + 1.0) + 4.0 * double(graph_edges(db->data)) + double(g.N) + 8.0 * db->R * double(g.N);
   void* tok = nullptr;
   BATCH_TRY(ctx_timed(db->ctx, SIGMO_PHASE_SIG_DATA, SIGMO_KERNEL_DB_PREPARE, bytes, true, &tok));
-  BATCH_HIP_TRY(sigmo_k::db_prepare(s, g, db->R, db->wide_idx, db->planes, db->planes_hi, db->lab_hi, db->dmax));
+  BATCH_HIP_TRY(sigmo_k::db_prepare(s, g, db->R, db->cls_list, db->pidx, db->wide_idx, db->planes32, db->planes64, db->planes_hi, db->lab_hi,
+                                    db->dmax, db->minfo));
   BATCH_TRY(ctx_timed(db->ctx, 0, 0, 0.0, false, &tok));
   return SIGMO_OK;
 }
@@ -850,46 +998,52 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
   }
   const uint32_t R = db->R;
   const uint32_t has_wide = db->num_wide ? 1u : 0u;
-  const size_t lds_block = sigmo_k::pair_lds_words(R, has_wide, Gq) * 8 * 4;
-  if (lds_block > 64 * 1024) return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "too many query graphs for one batched run");
-  // persistent pair-filter grid: as many 4-wave blocks as registers and LDS keep resident
-  int resident = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, sigmo_k::k_pair_filter, 256, lds_block) != hipSuccess) {
-    (void)hipGetLastError();
-    resident = 4;
+  const uint32_t W = (Gq + 63) / 64;
+  const uint64_t pairs = uint64_t(Gq) * Gd;
+  // persistent grid: as many 4-wave blocks as stay resident
+  // one persistent launch per width class, as many 4-wave blocks as registers and LDS keep resident
+  const void* kern[3] = {reinterpret_cast<const void*>(sigmo_k::k_pair_survive<1>), reinterpret_cast<const void*>(sigmo_k::k_pair_survive<2>),
+                         reinterpret_cast<const void*>(sigmo_k::k_pair_survive<4>)};
+  const uint32_t lds_bytes[3] = {sigmo_k::survive_lds_bytes(R, 1), sigmo_k::survive_lds_bytes(R, 2), sigmo_k::survive_lds_bytes(R, 4)};
+  uint64_t cls_blocks[3] = {0, 0, 0}, waves_total = 0;
+  for (int c = 0; c < 3; ++c) {
+    if (!db->cls_count[c]) continue;
+    if (lds_bytes[c] > 64 * 1024) return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "molecule slots exceed 64 KB of LDS per block");
+    int resident = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern[c], 256, lds_bytes[c]) != hipSuccess) {
+      (void)hipGetLastError();
+      resident = 2;
+    }
+    const uint64_t per_cu = std::max<uint64_t>(1, std::min<uint64_t>(8, static_cast<uint64_t>(resident)));
+    cls_blocks[c] = std::max<uint64_t>(1, std::min<uint64_t>(uint64_t(cus) * per_cu, (uint64_t(db->cls_count[c]) + 3) / 4));
+    waves_total += cls_blocks[c] * sigmo_k::kSurviveWaves;
   }
-  const uint64_t per_cu = std::max<uint64_t>(1, std::min<uint64_t>(8, static_cast<uint64_t>(resident)));
-  const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>(uint64_t(cus) * per_cu, (uint64_t(Gd) + 3) / 4));
-  const uint64_t waves = blocks * 4;
   const uint64_t seg = sigmo_k::token_segment(R);
+  if (qd.N * seg >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "query batch too large for one batched run");
   BATCH_TRY(bgrow(&b->tokens, b->cap_tokens, qd.N * seg));
-  BATCH_TRY(bgrow(&b->qlab, b->cap_qlab, Gq));
+  BATCH_TRY(bgrow(&b->qinfo, b->cap_qinfo, Gq));
   BATCH_TRY(bgrow(&b->qmax, b->cap_qmax, uint64_t(Gq) * R));
-  BATCH_TRY(bgrow(&b->scratch, b->cap_scratch, waves * 32 * 64 * (has_wide ? 2 : 1)));
-  BATCH_TRY(bgrow(&b->slot, b->cap_slot, waves * Gq));
-  BATCH_TRY(bgrow(&b->mol_block, b->cap_mb, Gd));
+  BATCH_TRY(bgrow(&b->surv, b->cap_surv, uint64_t(Gd) * W));
+  BATCH_TRY(bgrow(&b->pair_off, b->cap_pair_off, pairs));
   BATCH_TRY(bgrow(&b->mol_count, b->cap_mc, uint64_t(Gd) + 1));
   BATCH_TRY(bgrow(&b->offsets, b->cap_off, uint64_t(Gd) + 1));
   BATCH_TRY(bgrow(&b->scan_tmp, b->cap_scan, (uint64_t(Gd) + 1023) / 1024 + 1));
   {
     void* tok = nullptr;
     BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_SIG_QUERY, -1, 0.0, true, &tok));
-    BATCH_HIP_TRY(sigmo_k::query_prepare(s, qd, R, b->tokens, b->qlab, b->qmax));
+    BATCH_HIP_TRY(sigmo_k::query_prepare(s, qd, R, b->tokens, b->qinfo, b->qmax));
     BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
   }
-  // record / arena capacity: start from 1/8 of the pair space (at most 2^26 pairs) plus every
-  // wavefront's open chunk.  On overflow the counters still hold the exact demand, so one replay
-  // with that size suffices; capacities are kept for later runs.
+  // arena capacity: start from 1/8 of the pairs (at most 2^26) plus every wavefront's open chunk.
+  // On overflow the counter still holds the exact demand, so one replay with that size
+  // suffices; the capacity is kept for later runs.
   const double mean_k = Gq ? double(ksum) / Gq : 0.0;
-  const uint64_t slack_rec = waves * sigmo_k::kRecordChunk, slack_arena = waves * sigmo_k::kArenaChunk;
-  const uint64_t guess = std::min<uint64_t>(std::max<uint64_t>(uint64_t(Gq) * Gd / 8, 1 << 16), 1ull << 26);
-  uint64_t want_rec = guess + slack_rec;
+  // every wavefront may leave its chunk's tail and each lane its sub-chunk's tail unused
+  const uint64_t slack_arena = waves_total * 2 * sigmo_k::kWaveChunk;
+  const uint64_t guess = std::min<uint64_t>(std::max<uint64_t>(pairs / 8, 1 << 16), 1ull << 26);
   uint64_t want_arena = static_cast<uint64_t>(double(guess) * (mean_k + 1.0) * (has_wide ? 2.0 : 1.0)) + slack_arena;
-  want_rec = std::max(want_rec, std::min(b->cap_rec, b->cap_rec2));
   want_arena = std::max(want_arena, b->cap_arena);
-  for (int attempt = 0; attempt < 6; ++attempt) {
-    BATCH_TRY(bgrow(&b->rec_qg, b->cap_rec, want_rec));
-    BATCH_TRY(bgrow(&b->rec_off, b->cap_rec2, want_rec));
+  for (int attempt = 0; attempt < 3; ++attempt) {
     BATCH_TRY(bgrow(&b->arena, b->cap_arena, want_arena));
     if (b->cap_arena >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "candidate arena exceeds 2^32 words");
 
@@ -899,47 +1053,61 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
     void* tok = nullptr;
     BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_FILTER, SIGMO_KERNEL_PAIR_FILTER, reads, true, &tok));
     BATCH_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(b->counters), 8, 0));
-    sigmo_k::PairArgs a;
+    BATCH_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(b->surv), uint64_t(Gd) * W, 0));
+    sigmo_k::SurviveArgs a;
     a.q = qd;
     a.d = dd;
     a.R = R;
-    a.has_wide = has_wide;
-    a.mol_chunk = 4;
-    a.planes = db->planes;
+    a.Gq = Gq;
+    a.W = W;
+    a.planes32 = db->planes32;
+    a.planes64 = db->planes64;
     a.planes_hi = db->planes_hi;
     a.lab_hi = db->lab_hi;
-    a.wide_idx = db->wide_idx;
-    a.dmax = db->dmax;
     a.tok64 = reinterpret_cast<const uint64_t*>(b->tokens);
-    a.qlab = b->qlab;
+    a.qinfo = b->qinfo;
     a.qmax = b->qmax;
-    a.scratch = b->scratch;
-    a.slot = b->slot;
     a.arena = b->arena;
     a.arena_cap = b->cap_arena;
-    a.rec_qg = b->rec_qg;
-    a.rec_off = b->rec_off;
-    a.rec_cap = std::min(b->cap_rec, b->cap_rec2);
-    a.mol_block = b->mol_block;
-    a.mol_count = b->mol_count;
+    a.surv = b->surv;
+    a.pair_off = b->pair_off;
     a.counters = b->counters;
-    hipLaunchKernelGGL(sigmo_k::k_pair_filter, dim3(static_cast<unsigned>(blocks)), dim3(256), lds_block, s, a);
-    BATCH_HIP_TRY(hipGetLastError());
+    uint64_t first = 0;
+    for (int c = 0; c < 3; ++c) {
+      a.minfo = db->minfo + first;
+      a.dmax = db->dmax + first * R;
+      a.count = db->cls_count[c];
+      first += db->cls_count[c];
+      if (!a.count || !Gq) continue;
+      a.claim = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(16, a.count / (cls_blocks[c] * sigmo_k::kSurviveWaves * 4))));
+      // each launch starts its own molecule cursor
+      BATCH_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(b->counters), 1, 0));
+      const dim3 grid(static_cast<unsigned>(cls_blocks[c]));
+      if (c == 0) hipLaunchKernelGGL(sigmo_k::k_pair_survive<1>, grid, dim3(256), lds_bytes[0], s, a);
+      else if (c == 1) hipLaunchKernelGGL(sigmo_k::k_pair_survive<2>, grid, dim3(256), lds_bytes[1], s, a);
+      else hipLaunchKernelGGL(sigmo_k::k_pair_survive<4>, grid, dim3(256), lds_bytes[2], s, a);
+      BATCH_HIP_TRY(hipGetLastError());
+    }
     BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
-    unsigned long long cnt[4];
+    unsigned long long cnt[8];
     BATCH_HIP_TRY(hipMemcpyAsync(cnt, b->counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    res->filter_wave_steps = cnt[4];
+    res->filter_lane_steps = cnt[5];
+    res->filter_scans = cnt[6];
     BATCH_HIP_TRY(hipStreamSynchronize(s));
-    // writes: candidate masks, (query graph, arena offset) records, per-molecule block and count
-    ctx_add_bytes(b->ctx, SIGMO_KERNEL_PAIR_FILTER, double(cnt[1]) * 8.0 + double(cnt[2]) * 8.0 + double(Gd) * 8.0);
+    // writes: candidate masks, the survival bits and arena offsets
+    ctx_add_bytes(b->ctx, SIGMO_KERNEL_PAIR_FILTER, double(cnt[1]) * 8.0 + double(Gd) * W * 8.0);
     if (!cnt[3]) break;
-    if (attempt == 5) return sigmo_err::set(SIGMO_ERR_NOMEM, "pair buffers overflowed repeatedly");
-    // replay with larger buffers
-    if (cnt[3] & 2ull) want_rec = std::max<uint64_t>(want_rec * 2, cnt[2] + slack_rec);
-    if (cnt[3] & 1ull) want_arena = std::max<uint64_t>(want_arena * 2, cnt[1] + slack_arena);
+    if (attempt == 2) return sigmo_err::set(SIGMO_ERR_NOMEM, "candidate arena overflowed repeatedly");
+    want_arena = std::max<uint64_t>(want_arena * 2, cnt[1] + slack_arena);  // replay with the exact demand
   }
-  // canonical GMCR: scan the per-molecule survivor counts, scatter the blocks
+  // canonical GMCR: count survivors per molecule, scan, scatter in (molecule, query graph) order
   void* tok = nullptr;
   BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_GMCR, -1, 0.0, true, &tok));
+  if (Gd) {
+    hipLaunchKernelGGL(sigmo_k::k_surv_count, dim3((Gd + 255) / 256), dim3(256), 0, s, Gd, W, b->surv, b->mol_count);
+    BATCH_HIP_TRY(hipGetLastError());
+  }
   BATCH_HIP_TRY(sigmo_k::exclusive_scan(s, b->mol_count, Gd, b->offsets, b->scan_tmp, b->total));
   uint64_t P = 0;
   BATCH_HIP_TRY(hipMemcpyAsync(&P, b->total, 8, hipMemcpyDeviceToHost, s));
@@ -950,8 +1118,8 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
   BATCH_TRY(bgrow(&b->mask_off, b->cap_mo, P));
   BATCH_TRY(bgrow(&b->counts, b->cap_counts, P));
   if (Gd) {
-    hipLaunchKernelGGL(sigmo_k::k_pairs_scatter, dim3((Gd + 255) / 256), dim3(256), 0, s, Gd, b->offsets, b->mol_block, b->rec_qg,
-                       b->rec_off, b->indices, b->pair_graph, b->mask_off);
+    hipLaunchKernelGGL(sigmo_k::k_surv_scatter, dim3((Gd + 255) / 256), dim3(256), 0, s, Gd, Gq, W, b->surv, b->offsets, b->pair_off,
+                       b->indices, b->pair_graph, b->mask_off);
     BATCH_HIP_TRY(hipGetLastError());
   }
   BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));

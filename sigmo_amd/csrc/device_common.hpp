@@ -54,4 +54,17 @@ __device__ __forceinline__ uint64_t ge4_lane(uint64_t x0, uint64_t x1, uint64_t 
   return gt | eq;
 }
 
+// x >= c for four-bit values held as bit planes x0..x3 and a per-lane c, from the lowest bit
+// up: ge = c_b ? (x_b & ge) : (x_b | ge), starting from x_0 >= c_0.  Each stage is one
+// bit-select of an AND and an OR.
+template <typename T>
+__device__ __forceinline__ T ge4_bits(T x0, T x1, T x2, T x3, uint32_t c) {
+  const T m0 = T(0) - T(c & 1u), m1 = T(0) - T((c >> 1) & 1u), m2 = T(0) - T((c >> 2) & 1u), m3 = T(0) - T((c >> 3) & 1u);
+  T ge = x0 | ~m0;
+  ge = (m1 & (x1 & ge)) | (~m1 & (x1 | ge));
+  ge = (m2 & (x2 & ge)) | (~m2 & (x2 | ge));
+  ge = (m3 & (x3 & ge)) | (~m3 & (x3 | ge));
+  return ge;
+}
+
 }  // namespace sigmo_k
