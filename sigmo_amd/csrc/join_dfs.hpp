@@ -76,13 +76,21 @@ __device__ __forceinline__ void join_finish(const uint64_t* __restrict__ bits, u
 
 // The reference DFS of one pair on the lane-distributed operands (see k_join_wave).  WIDE
 // handles molecules of 65..128 nodes (positions 64..127 in the `hi` words).
+//
+// The set of candidates of depth d that pass isValidMapping depends only on mapping[0..d-1],
+// which cannot change while the search is at depth >= d.  It is computed once when the search
+// descends to d (ballot over the molecule's positions) and kept in lane d; a revisit after
+// backtracking reads it back and only applies the current `visited` set and the resume
+// position, which is what the reference re-evaluates per candidate.
 template <bool FIND_FIRST, bool WIDE>
 __device__ __forceinline__ uint64_t join_dfs(const JoinPair& o, uint64_t& steps) {
   const uint32_t lane = static_cast<uint32_t>(lane_id());
   const uint32_t k = uniform32(o.k);
   uint32_t map = 0;             // lane d: mapping[d] (local id; every entry starts at node 0)
   uint64_t mslo = 0, mshi = 0;  // lane d: positions mapped at depths < d on the current path
+  uint64_t elo = 0, ehi = 0;    // lane d: valid candidates of depth d for the current mapping[0..d-1]
   uint32_t depth = 0, pos = 0;
+  bool fresh = true;            // first visit of this depth since mapping[0..depth-1] last changed
   uint64_t visited = 0, count = 0;
   while (true) {
     ++steps;
@@ -91,16 +99,14 @@ __device__ __forceinline__ uint64_t join_dfs(const JoinPair& o, uint64_t& steps)
       if (FIND_FIRST) break;
       depth = k - 1;
       pos = rl32(map, depth) + 1;
+      fresh = false;
       continue;
     }
-    const uint64_t cl = rl64(o.clo, depth) & ~visited & bits_from(pos);
-    const uint64_t ch = WIDE ? rl64(o.chi, depth) & ~visited & (pos <= 64 ? ~0ull : bits_from(pos - 64)) : 0ull;
-    uint64_t el = 0, eh = 0, Ml = 0, Mh = 0;
-    if (cl | ch) {
-      if (depth == 0) {
-        el = cl;
-        eh = ch;
-      } else {
+    uint64_t El, Eh;
+    if (fresh) {
+      El = rl64(o.clo, depth);
+      Eh = WIDE ? rl64(o.chi, depth) : 0ull;
+      if (depth > 0 && (El | Eh)) {
         // A: positions that must be adjacent (mapped query neighbours), N: must not be
         const uint32_t qad = rl32(o.qa, depth), low = (1u << depth) - 1;
         uint64_t Al = 0, Ah = 0, Nl, Nh;
@@ -109,8 +115,7 @@ __device__ __forceinline__ uint64_t join_dfs(const JoinPair& o, uint64_t& steps)
           if (!WIDE || m < 64) Al |= 1ull << (m & 63);
           else Ah |= 1ull << (m & 63);
         }
-        Ml = rl64(mslo, depth);
-        Mh = WIDE ? rl64(mshi, depth) : 0ull;
+        const uint64_t Ml = rl64(mslo, depth), Mh = WIDE ? rl64(mshi, depth) : 0ull;
         if (static_cast<uint32_t>(__popcll(Ml) + __popcll(Mh)) == depth) {  // injective: N = mapped \ A
           Nl = Ml & ~Al;
           Nh = Mh & ~Ah;
@@ -122,18 +127,29 @@ __device__ __forceinline__ uint64_t join_dfs(const JoinPair& o, uint64_t& steps)
             else Nh |= 1ull << (m & 63);
           }
         }
-        if (cl) {
+        if (El) {
           const bool ok = (o.alo & Al) == Al && !(o.alo & Nl) && (!WIDE || ((o.ahi & Ah) == Ah && !(o.ahi & Nh)));
-          el = cl & __ballot(ok);
+          El &= __ballot(ok);
         }
-        if (WIDE && !el && ch) {
+        if (WIDE && Eh) {
           const bool ok = (o.blo & Al) == Al && !(o.blo & Nl) && (o.bhi & Ah) == Ah && !(o.bhi & Nh);
-          eh = ch & __ballot(ok);
+          Eh &= __ballot(ok);
         }
       }
+      if (lane == depth) {
+        elo = El;
+        ehi = Eh;
+      }
+      fresh = false;
+    } else {
+      El = rl64(elo, depth);
+      Eh = WIDE ? rl64(ehi, depth) : 0ull;
     }
+    const uint64_t el = El & ~visited & bits_from(pos);
+    const uint64_t eh = WIDE ? Eh & ~visited & (pos <= 64 ? ~0ull : bits_from(pos - 64)) : 0ull;
     if (el | eh) {  // accept (:366-370)
       const uint32_t c = el ? static_cast<uint32_t>(__builtin_ctzll(el)) : 64u + static_cast<uint32_t>(__builtin_ctzll(eh));
+      const uint64_t Ml = rl64(mslo, depth), Mh = WIDE ? rl64(mshi, depth) : 0ull;
       if (lane == depth) map = c;
       if (lane == depth + 1) {
         mslo = Ml | (c < 64 ? 1ull << c : 0ull);
@@ -142,12 +158,14 @@ __device__ __forceinline__ uint64_t join_dfs(const JoinPair& o, uint64_t& steps)
       visited |= 1ull << (c & 63);
       ++depth;
       pos = 0;
+      fresh = true;
     } else {  // exhausted frame (:346-354)
       visited &= ~(1ull << (rl32(map, depth) & 63));
       if (depth == 0) break;
       if (depth == 1) visited = 0;
       --depth;
       pos = rl32(map, depth) + 1;
+      fresh = false;
     }
   }
   return count;
