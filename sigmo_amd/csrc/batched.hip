@@ -26,6 +26,7 @@
 #include "device_common.hpp"
 #include "host_graphs.hpp"
 #include "join_dfs.hpp"
+#include "join_lanes.hpp"
 #include "kernels.hpp"
 
 namespace sigmo_k {
@@ -705,147 +706,6 @@ __global__ __launch_bounds__(256) void k_join_arena(GraphDev q, GraphDev d, cons
   }
 }
 
-constexpr uint32_t kJoinLaneWaves = 2;  // wavefronts per block of the lane-per-pair join
-
-// Join with one lane per GMCR pair, for molecules of <= 64 nodes (the rest are listed in
-// `wide` for the wavefront-per-pair kernel).  Each lane runs the reference DFS of
-// isomorphism.hpp:270-381 for its own pair, with the same bookkeeping as join_dfs (mapping
-// starting at local node 0, stale unsets of exhausted frames, `visited` cleared when depth 1 is
-// exhausted).  The validity test of isomorphism.hpp:150-168 -- for every j < depth,
-// qAdj(j, depth) == dAdj(mapping[j], c) -- becomes set algebra on adjacency rows: adjacency is
-// symmetric, so the candidates that pass are those in the row of every mapped query neighbour
-// and in the row of no other mapped node.  The rows of the mapped positions and the mapping sit
-// in LDS (depth-major, one column per lane).
-template <bool FIND_FIRST>
-__global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, GraphDev d, const uint64_t* __restrict__ arena,
-                                                                   const uint32_t* __restrict__ mask_off, const uint32_t* __restrict__ indices,
-                                                                   const uint32_t* __restrict__ pair_graph, uint64_t P, uint32_t chunk,
-                                                                   int32_t* __restrict__ counts, unsigned long long* __restrict__ stats,
-                                                                   uint32_t* __restrict__ wide) {
-  __shared__ uint64_t rows_all[kJoinLaneWaves][32][kWave];
-  __shared__ uint8_t maps_all[kJoinLaneWaves][32][kWave];
-  const uint32_t lane = static_cast<uint32_t>(lane_id());
-  const uint32_t wave = threadIdx.x / kWave;
-  uint64_t (*rows)[kWave] = rows_all[wave];
-  uint8_t (*maps)[kWave] = maps_all[wave];
-  const uint64_t below = (1ull << lane) - 1;
-  uint64_t next = 0, end = 0;  // claimed pairs [next, end) (wave-uniform)
-  bool exhausted = false;
-  uint64_t sum = 0, matched = 0, steps = 0, overflow = 0;
-
-  bool have = false;
-  uint64_t j = 0, visited = 0, count = 0;
-  uint32_t s = 0, qoff = 0, k = 0, base = 0, depth = 0, pos = 0;
-  while (true) {
-    uint64_t need = __ballot(!have);
-    while (need && !exhausted) {
-      if (next >= end) {
-        unsigned long long b = 0;
-        if (lane == 0) b = atomicAdd(&stats[2], static_cast<unsigned long long>(chunk));
-        next = uniform64(__shfl(b, 0));
-        end = next + chunk < P ? next + chunk : P;
-        if (next >= P) {
-          exhausted = true;
-          break;
-        }
-      }
-      const uint32_t cnt = min(static_cast<uint32_t>(end - next), static_cast<uint32_t>(__popcll(need)));
-      const uint32_t rn = __popcll(need & below);
-      const bool take = ((need >> lane) & 1ull) && rn < cnt;
-      bool wide_pair = false;
-      if (take) {
-        j = next + rn;
-        const uint32_t dg = pair_graph[j], qg = indices[j];
-        base = mask_off[j];
-        s = d.go[dg];
-        const uint32_t n = d.go[dg + 1] - s;
-        qoff = q.go[qg];
-        k = q.go[qg + 1] - qoff;
-        wide_pair = n > 64;
-        if (!wide_pair) {
-          for (uint32_t t = 0; t < k; ++t) maps[t][lane] = 0;  // every mapping entry starts at local node 0
-          depth = 0;
-          pos = 0;
-          visited = 0;
-          count = 0;
-          have = true;
-        }
-      }
-      next += cnt;
-      need &= ~__ballot(take);
-      const uint64_t wm = __ballot(wide_pair);  // molecules of > 64 nodes go to the wavefront-per-pair kernel
-      if (wm) {
-        unsigned long long b = 0;
-        if (lane == 0) b = atomicAdd(&stats[5], static_cast<unsigned long long>(__popcll(wm)));
-        const uint64_t wb = uniform64(__shfl(b, 0));
-        if (wide_pair) wide[wb + __popcll(wm & below)] = static_cast<uint32_t>(j);
-      }
-      need |= wm;  // those lanes take another pair
-    }
-    if (__ballot(have) == 0) break;
-    if (have) {
-      ++steps;
-      bool done = false;
-      if (depth == k) {  // leaf (:336-344)
-        ++count;
-        if (FIND_FIRST) {
-          done = true;
-        } else {
-          depth = k - 1;
-          pos = static_cast<uint32_t>(maps[depth][lane]) + 1;
-        }
-      } else {
-        const uint64_t cand = arena[static_cast<uint64_t>(base) + depth] & ~visited & bits_from(pos);
-        const uint32_t qad = static_cast<uint32_t>(q.adj[2 * (static_cast<uint64_t>(qoff) + depth)]);
-        uint64_t must = ~0ull, mustnot = 0;
-        for (uint32_t t = 0; t < depth; ++t) {
-          const uint64_t r = rows[t][lane];
-          if ((qad >> t) & 1u) must &= r;
-          else mustnot |= r;
-        }
-        const uint64_t el = cand & must & ~mustnot;
-        if (el) {  // accept (:366-370)
-          const uint32_t c = static_cast<uint32_t>(__builtin_ctzll(el));
-          maps[depth][lane] = static_cast<uint8_t>(c);
-          rows[depth][lane] = d.adj[2 * (static_cast<uint64_t>(s) + c)];
-          visited |= 1ull << c;
-          ++depth;
-          pos = 0;
-        } else {  // exhausted frame (:346-354)
-          visited &= ~(1ull << (maps[depth][lane] & 63u));
-          if (depth == 0) {
-            done = true;
-          } else {
-            if (depth == 1) visited = 0;
-            --depth;
-            pos = static_cast<uint32_t>(maps[depth][lane]) + 1;
-          }
-        }
-      }
-      if (done) {
-        if (count > 0x7FFFFFFFull) overflow = 1;
-        counts[j] = static_cast<int32_t>(count > 0x7FFFFFFFull ? 0x7FFFFFFFull : count);
-        sum += count;
-        matched += count > 0;
-        have = false;
-      }
-    }
-  }
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    sum += __shfl_xor(sum, o);
-    matched += __shfl_xor(matched, o);
-    steps += __shfl_xor(steps, o);
-    overflow |= __shfl_xor(overflow, o);
-  }
-  if (lane == 0) {
-    if (sum) atomicAdd(&stats[0], static_cast<unsigned long long>(sum));
-    if (matched) atomicAdd(&stats[1], static_cast<unsigned long long>(matched));
-    if (overflow) atomicOr(&stats[3], 1ull);
-    atomicAdd(&stats[4], static_cast<unsigned long long>(steps));
-  }
-}
-
 }  // namespace
 
 hipError_t db_prepare(hipStream_t s, const GraphDev& g, uint32_t R, const uint32_t* cls_list, const uint32_t* pidx, const uint32_t* wide_idx,
@@ -1281,11 +1141,13 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
     const uint64_t lblocks = std::max<uint64_t>(1, std::min<uint64_t>((P + 127) / 128, static_cast<uint64_t>(cus) * 4));
     const uint32_t lchunk = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(64, P / (lblocks * sigmo_k::kJoinLaneWaves * 16))));
     if (find_first)
-      hipLaunchKernelGGL(sigmo_k::k_join_lanes<true>, dim3(static_cast<unsigned>(lblocks)), dim3(sigmo_k::kJoinLaneWaves * 64), 0, s, qd, dd,
-                         b->arena, b->mask_off, b->indices, b->pair_graph, P, lchunk, b->counts, stats, b->wide_pairs);
+      hipLaunchKernelGGL((sigmo_k::k_join_lanes<true, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(lblocks)),
+                         dim3(sigmo_k::kJoinLaneWaves * 64), 0, s, qd, dd, sigmo_k::ArenaMasks{b->arena, b->mask_off}, b->indices, b->pair_graph,
+                         P, lchunk, b->counts, stats, b->wide_pairs);
     else
-      hipLaunchKernelGGL(sigmo_k::k_join_lanes<false>, dim3(static_cast<unsigned>(lblocks)), dim3(sigmo_k::kJoinLaneWaves * 64), 0, s, qd, dd,
-                         b->arena, b->mask_off, b->indices, b->pair_graph, P, lchunk, b->counts, stats, b->wide_pairs);
+      hipLaunchKernelGGL((sigmo_k::k_join_lanes<false, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(lblocks)),
+                         dim3(sigmo_k::kJoinLaneWaves * 64), 0, s, qd, dd, sigmo_k::ArenaMasks{b->arena, b->mask_off}, b->indices, b->pair_graph,
+                         P, lchunk, b->counts, stats, b->wide_pairs);
     BATCH_HIP_TRY(hipGetLastError());
     if (db->num_wide) {
       unsigned long long nw = 0;

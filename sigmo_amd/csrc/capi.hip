@@ -230,7 +230,10 @@ struct sigmo_gmcr {
   uint64_t cap_indices = 0, cap_pair_graph = 0, cap_counts_g = 0, cap_offsets = 0, cap_pair_bits = 0, cap_scan = 0;
   int32_t* pair_counts = nullptr;
   uint64_t cap_counts = 0;
+  uint32_t* wide_pairs = nullptr;  // pairs of molecules with > 64 nodes, joined one wavefront per pair
+  uint64_t cap_wide = 0;
   ~sigmo_gmcr() {
+    dev_free(wide_pairs);
     dev_free(offsets);
     dev_free(counts);
     dev_free(indices);
@@ -809,8 +812,17 @@ int sigmo_join(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data
   hipStream_t s = ctx->stream;
   PhaseScope ps(ctx, SIGMO_PHASE_JOIN, SIGMO_KERNEL_JOIN, 0.0);
   SIGMO_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(ctx->stats), 8, 0));
-  SIGMO_HIP_TRY(sigmo_k::join(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, m->P, find_first,
-                              mm->pair_counts, ctx->stats, ctx->num_cus));
+  SIGMO_TRY(grow(&mm->wide_pairs, mm->cap_wide, m->P));
+  SIGMO_HIP_TRY(sigmo_k::join_lanes(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, m->P, find_first,
+                                    mm->pair_counts, ctx->stats, mm->wide_pairs, ctx->num_cus));
+  if (data->max_nodes > 64 && m->P) {  // pairs of molecules with > 64 nodes: one wavefront per pair
+    unsigned long long nw = 0;
+    SIGMO_HIP_TRY(hipMemcpyAsync(&nw, ctx->stats + 5, sizeof(nw), hipMemcpyDeviceToHost, s));
+    SIGMO_HIP_TRY(hipStreamSynchronize(s));
+    if (nw)
+      SIGMO_HIP_TRY(sigmo_k::join(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, nw, find_first, mm->pair_counts,
+                                  ctx->stats, ctx->num_cus, mm->wide_pairs, 6u));
+  }
   SIGMO_TRY(ps.close());
   SIGMO_HIP_TRY(hipMemcpyAsync(ctx->stats_host, ctx->stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   if (per_pair_counts && m->P)

@@ -1,0 +1,183 @@
+// Lane-per-pair join shared by the plain form (masks cut from the candidate bitset) and the
+// batched form (masks from the survivor arena).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace sigmo_k {
+namespace {
+
+// Candidate masks of the survivor arena: k consecutive words per pair (molecules of <= 64 nodes).
+struct ArenaMasks {
+  const uint64_t* arena;
+  const uint32_t* mask_off;
+  __device__ __forceinline__ uint32_t base(uint64_t j) const { return mask_off[j]; }
+  __device__ __forceinline__ uint64_t mask(uint32_t base, uint32_t, uint32_t depth, uint32_t, uint32_t) const {
+    return arena[static_cast<uint64_t>(base) + depth];
+  }
+};
+
+// Candidate masks cut from the bitset row of the query node: positions s..s+n-1 (n <= 64 spans
+// at most two 64-bit words), shifted to bit 0.
+struct BitsetMasks {
+  const uint64_t* bits;
+  uint64_t W64;
+  __device__ __forceinline__ uint32_t base(uint64_t) const { return 0; }
+  __device__ __forceinline__ uint64_t mask(uint32_t, uint32_t qrow, uint32_t, uint32_t s, uint32_t n) const {
+    const uint64_t* row = bits + static_cast<uint64_t>(qrow) * W64;
+    const uint32_t w0 = s >> 6, sh = s & 63;
+    const uint64_t lo = row[w0];
+    const uint64_t hi = (sh && w0 + 1 < W64) ? row[w0 + 1] : 0ull;
+    uint64_t m = sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
+    if (n < 64) m &= (1ull << n) - 1;
+    return m;
+  }
+};
+
+constexpr uint32_t kJoinLaneWaves = 2;  // wavefronts per block of the lane-per-pair join
+
+// Join with one lane per GMCR pair, for molecules of <= 64 nodes (the rest are listed in
+// `wide` for the wavefront-per-pair kernel).  Each lane runs the reference DFS of
+// isomorphism.hpp:270-381 for its own pair, with the same bookkeeping as join_dfs (mapping
+// starting at local node 0, stale unsets of exhausted frames, `visited` cleared when depth 1 is
+// exhausted).  The validity test of isomorphism.hpp:150-168 -- for every j < depth,
+// qAdj(j, depth) == dAdj(mapping[j], c) -- becomes set algebra on adjacency rows: adjacency is
+// symmetric, so the candidates that pass are those in the row of every mapped query neighbour
+// and in the row of no other mapped node.  The rows of the mapped positions and the mapping sit
+// in LDS (depth-major, one column per lane).
+template <bool FIND_FIRST, class Src>
+__global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, GraphDev d, Src src, const uint32_t* __restrict__ indices,
+                                                                   const uint32_t* __restrict__ pair_graph, uint64_t P, uint32_t chunk,
+                                                                   int32_t* __restrict__ counts, unsigned long long* __restrict__ stats,
+                                                                   uint32_t* __restrict__ wide) {
+  __shared__ uint64_t rows_all[kJoinLaneWaves][32][kWave];
+  __shared__ uint8_t maps_all[kJoinLaneWaves][32][kWave];
+  const uint32_t lane = static_cast<uint32_t>(lane_id());
+  const uint32_t wave = threadIdx.x / kWave;
+  uint64_t (*rows)[kWave] = rows_all[wave];
+  uint8_t (*maps)[kWave] = maps_all[wave];
+  const uint64_t below = (1ull << lane) - 1;
+  uint64_t next = 0, end = 0;  // claimed pairs [next, end) (wave-uniform)
+  bool exhausted = false;
+  uint64_t sum = 0, matched = 0, steps = 0, overflow = 0;
+
+  bool have = false;
+  uint64_t j = 0, visited = 0, count = 0;
+  uint32_t s = 0, n = 0, qoff = 0, k = 0, base = 0, depth = 0, pos = 0;
+  while (true) {
+    uint64_t need = __ballot(!have);
+    while (need && !exhausted) {
+      if (next >= end) {
+        unsigned long long b = 0;
+        if (lane == 0) b = atomicAdd(&stats[2], static_cast<unsigned long long>(chunk));
+        next = uniform64(__shfl(b, 0));
+        end = next + chunk < P ? next + chunk : P;
+        if (next >= P) {
+          exhausted = true;
+          break;
+        }
+      }
+      const uint32_t cnt = min(static_cast<uint32_t>(end - next), static_cast<uint32_t>(__popcll(need)));
+      const uint32_t rn = __popcll(need & below);
+      const bool take = ((need >> lane) & 1ull) && rn < cnt;
+      bool wide_pair = false;
+      if (take) {
+        j = next + rn;
+        const uint32_t dg = pair_graph[j], qg = indices[j];
+        base = src.base(j);
+        s = d.go[dg];
+        n = d.go[dg + 1] - s;
+        qoff = q.go[qg];
+        k = q.go[qg + 1] - qoff;
+        wide_pair = n > 64;
+        if (!wide_pair) {
+          for (uint32_t t = 0; t < k; ++t) maps[t][lane] = 0;  // every mapping entry starts at local node 0
+          depth = 0;
+          pos = 0;
+          visited = 0;
+          count = 0;
+          have = true;
+        }
+      }
+      next += cnt;
+      need &= ~__ballot(take);
+      const uint64_t wm = __ballot(wide_pair);  // molecules of > 64 nodes go to the wavefront-per-pair kernel
+      if (wm) {
+        unsigned long long b = 0;
+        if (lane == 0) b = atomicAdd(&stats[5], static_cast<unsigned long long>(__popcll(wm)));
+        const uint64_t wb = uniform64(__shfl(b, 0));
+        if (wide_pair) wide[wb + __popcll(wm & below)] = static_cast<uint32_t>(j);
+      }
+      need |= wm;  // those lanes take another pair
+    }
+    if (__ballot(have) == 0) break;
+    if (have) {
+      ++steps;
+      bool done = false;
+      if (depth == k) {  // leaf (:336-344)
+        ++count;
+        if (FIND_FIRST) {
+          done = true;
+        } else {
+          depth = k - 1;
+          pos = static_cast<uint32_t>(maps[depth][lane]) + 1;
+        }
+      } else {
+        const uint64_t cand = src.mask(base, qoff + depth, depth, s, n) & ~visited & bits_from(pos);
+        const uint32_t qad = static_cast<uint32_t>(q.adj[2 * (static_cast<uint64_t>(qoff) + depth)]);
+        uint64_t must = ~0ull, mustnot = 0;
+        for (uint32_t t = 0; t < depth; ++t) {
+          const uint64_t r = rows[t][lane];
+          if ((qad >> t) & 1u) must &= r;
+          else mustnot |= r;
+        }
+        const uint64_t el = cand & must & ~mustnot;
+        if (el) {  // accept (:366-370)
+          const uint32_t c = static_cast<uint32_t>(__builtin_ctzll(el));
+          maps[depth][lane] = static_cast<uint8_t>(c);
+          rows[depth][lane] = d.adj[2 * (static_cast<uint64_t>(s) + c)];
+          visited |= 1ull << c;
+          ++depth;
+          pos = 0;
+        } else {  // exhausted frame (:346-354)
+          visited &= ~(1ull << (maps[depth][lane] & 63u));
+          if (depth == 0) {
+            done = true;
+          } else {
+            if (depth == 1) visited = 0;
+            --depth;
+            pos = static_cast<uint32_t>(maps[depth][lane]) + 1;
+          }
+        }
+      }
+      if (done) {
+        if (count > 0x7FFFFFFFull) overflow = 1;
+        counts[j] = static_cast<int32_t>(count > 0x7FFFFFFFull ? 0x7FFFFFFFull : count);
+        sum += count;
+        matched += count > 0;
+        have = false;
+      }
+    }
+  }
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    sum += __shfl_xor(sum, o);
+    matched += __shfl_xor(matched, o);
+    steps += __shfl_xor(steps, o);
+    overflow |= __shfl_xor(overflow, o);
+  }
+  if (lane == 0) {
+    if (sum) atomicAdd(&stats[0], static_cast<unsigned long long>(sum));
+    if (matched) atomicAdd(&stats[1], static_cast<unsigned long long>(matched));
+    if (overflow) atomicOr(&stats[3], 1ull);
+    atomicAdd(&stats[4], static_cast<unsigned long long>(steps));
+  }
+}
+
+}  // namespace
+}  // namespace sigmo_k

@@ -1,3 +1,4 @@
+#include <algorithm>
 // gfx950 kernels for SIGMo's hot path: signatures, candidate filter/refine, GMCR, join.
 //
 // Wave64 throughout.  The bitset kernels are streaming (one pass over 64-bit word strips,
@@ -10,6 +11,7 @@
 #include "kernels.hpp"
 #include "device_common.hpp"
 #include "join_dfs.hpp"
+#include "join_lanes.hpp"
 
 namespace sigmo_k {
 namespace {
@@ -483,26 +485,29 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_final(const uint32_t* __res
 template <bool FIND_FIRST>
 __global__ __launch_bounds__(256) void k_join_wave(GraphDev q, GraphDev d, const uint64_t* __restrict__ bits, uint64_t W64,
                                                    const uint32_t* __restrict__ indices, const uint32_t* __restrict__ pair_graph, uint64_t P,
-                                                   uint32_t chunk, int32_t* __restrict__ counts, unsigned long long* __restrict__ stats) {
+                                                   uint32_t chunk, int32_t* __restrict__ counts, unsigned long long* __restrict__ stats,
+                                                   const uint32_t* __restrict__ list, uint32_t cursor) {
+  // pairs 0..P-1 of `list` (all GMCR pairs when list is null); stats[cursor] is the work counter
   const uint32_t lane = static_cast<uint32_t>(lane_id());
   uint64_t sum = 0, matched = 0, steps = 0, overflow = 0;
   uint64_t next = 0, end = 0;
   auto grab = [&]() {
     unsigned long long jj = 0;
-    if (lane == 0) jj = atomicAdd(&stats[2], static_cast<unsigned long long>(chunk));
+    if (lane == 0) jj = atomicAdd(&stats[cursor], static_cast<unsigned long long>(chunk));
     next = uniform64(__shfl(jj, 0));
     end = next + chunk < P ? next + chunk : P;
   };
+  auto pair_at = [&](uint64_t i) -> uint64_t { return list ? list[i] : i; };
   grab();
   if (next < P) {
     JoinPair cur, nxt;
-    join_load(q, d, bits, W64, indices, pair_graph, next, nullptr, cur);
+    join_load(q, d, bits, W64, indices, pair_graph, pair_at(next), nullptr, cur);
     join_finish(bits, W64, cur);
     while (true) {
-      const uint64_t j = next++;
+      const uint64_t j = pair_at(next++);
       if (next >= end) grab();
       const bool more = next < P;
-      if (more) join_load(q, d, bits, W64, indices, pair_graph, next, &cur, nxt);
+      if (more) join_load(q, d, bits, W64, indices, pair_graph, pair_at(next), &cur, nxt);
       const uint64_t count = cur.n > 64 ? join_dfs<FIND_FIRST, true>(cur, steps) : join_dfs<FIND_FIRST, false>(cur, steps);
       if (lane == 0) {
         if (count > 0x7FFFFFFFull) overflow = 1;
@@ -646,8 +651,26 @@ hipError_t gmcr_scatter(hipStream_t s, const uint64_t* pair_bits, uint32_t Gq, u
   return hipGetLastError();
 }
 
+hipError_t join_lanes(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
+                      const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts, unsigned long long* stats, uint32_t* wide,
+                      int num_cus) {
+  if (!P) return hipSuccess;
+  const uint64_t cus = static_cast<uint64_t>(num_cus > 0 ? num_cus : 256);
+  const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((P + 127) / 128, cus * 4));
+  const uint32_t chunk = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(64, P / (blocks * kJoinLaneWaves * 16))));
+  const BitsetMasks src{bits, W64};
+  if (find_first)
+    k_join_lanes<true, BitsetMasks><<<static_cast<unsigned>(blocks), kJoinLaneWaves * 64, 0, s>>>(q, d, src, indices, pair_graph, P, chunk,
+                                                                                                counts, stats, wide);
+  else
+    k_join_lanes<false, BitsetMasks><<<static_cast<unsigned>(blocks), kJoinLaneWaves * 64, 0, s>>>(q, d, src, indices, pair_graph, P, chunk,
+                                                                                                 counts, stats, wide);
+  return hipGetLastError();
+}
+
 hipError_t join(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
-                const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts, unsigned long long* stats, int num_cus) {
+                const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts, unsigned long long* stats, int num_cus,
+                const uint32_t* list, uint32_t cursor) {
   if (!P) return hipSuccess;
   // one wavefront per pair, persistent: 8 waves per SIMD; chunks give every wave ~8 grabs
   uint64_t blocks = (P + 3) / 4;
@@ -657,9 +680,9 @@ hipError_t join(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint6
   chunk = chunk < 1 ? 1 : (chunk > 64 ? 64 : chunk);
   const uint32_t ch = static_cast<uint32_t>(chunk);
   if (find_first)
-    k_join_wave<true><<<static_cast<unsigned>(blocks), 256, 0, s>>>(q, d, bits, W64, indices, pair_graph, P, ch, counts, stats);
+    k_join_wave<true><<<static_cast<unsigned>(blocks), 256, 0, s>>>(q, d, bits, W64, indices, pair_graph, P, ch, counts, stats, list, cursor);
   else
-    k_join_wave<false><<<static_cast<unsigned>(blocks), 256, 0, s>>>(q, d, bits, W64, indices, pair_graph, P, ch, counts, stats);
+    k_join_wave<false><<<static_cast<unsigned>(blocks), 256, 0, s>>>(q, d, bits, W64, indices, pair_graph, P, ch, counts, stats, list, cursor);
   return hipGetLastError();
 }
 

@@ -77,8 +77,13 @@ hipError_t gmcr_scatter(hipStream_t s, const uint64_t* pair_bits, uint32_t Gq, u
 
 // Join.  stats[0] += sum of counts, stats[1] += pairs with count > 0, stats[2] = work cursor
 // (zeroed by the caller), stats[3] |= 1 on int32 overflow.
-hipError_t join(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64,
-                const uint32_t* indices, const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts,
-                unsigned long long* stats, int num_cus);
+hipError_t join(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
+                const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts, unsigned long long* stats, int num_cus,
+                const uint32_t* list = nullptr, uint32_t cursor = 2);
+// Lane-per-pair join of the pairs whose molecule has <= 64 nodes; the others are appended to
+// `wide` (count in stats[5]) for join() with that list.
+hipError_t join_lanes(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
+                      const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts, unsigned long long* stats, uint32_t* wide,
+                      int num_cus);
 
 }  // namespace sigmo_k
