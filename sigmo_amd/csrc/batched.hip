@@ -128,6 +128,33 @@ struct QueryInfo {
   uint64_t first;               // first token word of the graph's first node
 };
 
+// Nibble-wise max over the 64 positions of a level signature, from its transposed planes (lane i
+// holds plane i = bit i%4 of the count of label i/4): lane 4l reads the label's other three
+// planes and takes the largest 4-bit value greedily from the top bit; four ballots then place
+// bit b of label l's max at bit 4l + b.
+__device__ __forceinline__ uint64_t planes_nib_max(uint64_t col) {
+  const uint32_t lane = static_cast<uint32_t>(lane_id());
+  const uint64_t p1 = static_cast<uint64_t>(__shfl_down(static_cast<unsigned long long>(col), 1));
+  const uint64_t p2 = static_cast<uint64_t>(__shfl_down(static_cast<unsigned long long>(col), 2));
+  const uint64_t p3 = static_cast<uint64_t>(__shfl_down(static_cast<unsigned long long>(col), 3));
+  const uint64_t planes[4] = {col, p1, p2, p3};
+  uint64_t cand = ~0ull;
+  uint32_t v = 0;
+#pragma unroll
+  for (int b = 3; b >= 0; --b) {
+    const uint64_t t = cand & planes[b];
+    if (t) {
+      v |= 1u << b;
+      cand = t;
+    }
+  }
+  const bool leader = (lane & 3u) == 0;
+  uint64_t mx = 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) mx |= __ballot(leader && ((v >> b) & 1u)) << b;
+  return mx;
+}
+
 // One wavefront per molecule: lane p is position p (and p+64 for molecules of 65..128 nodes).
 // Level planes, bit p = bit i of the level-(r+1) signature of position p for plane i =
 // 4*label + bit, go to one of two arrays by the molecule's width class: planes32[(x*R + r)*64
@@ -148,6 +175,15 @@ __global__ __launch_bounds__(256) void k_db_prepare(GraphDev g, uint32_t R, cons
   const uint64_t m = cls_list[c];
   const uint32_t s = g.go[m], n = g.go[m + 1] - s;
   const uint64_t* lm = g.lmask + m * 16;
+  // the label masks of positions 0..63 and the labels that occur there (wave-uniform): the
+  // histogram only visits those labels
+  uint64_t lmv[16];
+  uint32_t present = 0;
+#pragma unroll
+  for (int l = 0; l < 16; ++l) {
+    lmv[l] = lm[l];
+    if (lmv[l]) present |= 1u << l;
+  }
   const uint32_t halves = n > 64 ? 2 : 1;
   const uint64_t x = pidx[m];
   const uint32_t wi = wide_idx[m];
@@ -170,13 +206,19 @@ __global__ __launch_bounds__(256) void k_db_prepare(GraphDev g, uint32_t R, cons
         next &= ~ball;
         ball |= next;
         frontier = next;
-        sig = histogram16(next, lm);
+#pragma unroll
+        for (int l = 0; l < 16; ++l)
+          if ((present >> l) & 1u) {  // counts saturate at 15 (signature.hpp:44-51)
+            uint32_t cnt = __popcll(next & lmv[l]);
+            cnt = cnt < 15 ? cnt : 15;
+            sig |= static_cast<uint64_t>(cnt) << (4 * l);
+          }
       }
       const uint64_t col = transpose64(sig);
       if (h == 1) planes_hi[(static_cast<uint64_t>(wi) * R + r) * 64 + lane] = col;
       else if (n <= 32) planes32[(x * R + r) * 64 + lane] = static_cast<uint32_t>(col);
       else planes64[(x * R + r) * 64 + lane] = col;
-      const uint64_t mx = wave_nib_max(sig);
+      const uint64_t mx = planes_nib_max(col);
       if (lane == 0) dmax[c * R + r] = h == 0 ? mx : nib_max(dmax[c * R + r], mx);
     }
     if (h == 1) {
