@@ -100,22 +100,6 @@ __device__ __forceinline__ uint64_t wave_nib_max(uint64_t x) {
   return x;
 }
 
-// 64 x 64 bit-matrix transpose across the wavefront: lane p holds row p on entry, lane i
-// holds column i (bit p = bit i of lane p's row) on exit.  Six butterfly stages, each
-// exchanging the off-diagonal j x j blocks between lanes p and p ^ j.
-__device__ __forceinline__ uint64_t transpose64(uint64_t x) {
-  const uint32_t lane = static_cast<uint32_t>(lane_id());
-  constexpr uint64_t keep[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
-                                0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
-#pragma unroll
-  for (int t = 0; t < 6; ++t) {
-    const uint32_t j = 32u >> t;
-    const uint64_t L = keep[t];
-    const uint64_t y = static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(x), static_cast<int>(j)));
-    x = (lane & j) ? ((x & ~L) | ((y & ~L) >> j)) : ((x & L) | ((y & L) << j));
-  }
-  return x;
-}
 
 // Per-molecule facts the pair pre-test reads, stored in width-class order.
 struct MolInfo {

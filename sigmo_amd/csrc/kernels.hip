@@ -95,19 +95,14 @@ __device__ __forceinline__ uint64_t sig_inc(uint64_t sig, uint32_t label) {
 }
 
 // Bit-slice the packed signatures of the block's 64-node strips: plane p (= 4*label+bit)
-// of strip w is ballot(bit p of sig) over the wave that holds the strip.  Staged through LDS
+// of strip w holds bit p of the signatures of the wave that holds the strip (a 64 x 64 bit
+// transpose).  Staged through LDS
 // so each plane row is written in contiguous runs of (BLOCK/64) words.
 template <int BLOCK>
 __device__ __forceinline__ void write_planes(uint64_t sig, uint64_t W64, uint64_t* __restrict__ planes, uint64_t* tile) {
   constexpr int WPB = BLOCK / kWave;
   const int wave = threadIdx.x / kWave, lane = lane_id();
-  uint64_t mine = 0;
-#pragma unroll
-  for (int p = 0; p < 64; ++p) {
-    const uint64_t m = __ballot((sig >> p) & 1ull);
-    if (lane == p) mine = m;
-  }
-  tile[lane * WPB + wave] = mine;
+  tile[lane * WPB + wave] = transpose64(sig);  // lane p: bit j = bit p of lane j's signature
   __syncthreads();
   const uint64_t w0 = static_cast<uint64_t>(blockIdx.x) * WPB;
   const int p = threadIdx.x / WPB, j = threadIdx.x % WPB;
@@ -115,6 +110,31 @@ __device__ __forceinline__ void write_planes(uint64_t sig, uint64_t W64, uint64_
 }
 
 constexpr int kSigBlock = 512;
+
+// Union of the folded adjacency (adj_lo | adj_hi) of nodes prev + b over the set bits b of f.
+// Four positions per round, their loads issued together (a missing one repeats the first).
+__device__ __forceinline__ uint64_t expand_frontier(const uint64_t* __restrict__ adj, uint64_t prev, uint64_t f) {
+  uint64_t next = 0;
+  while (f) {
+    uint32_t b[4];
+    b[0] = __builtin_ctzll(f);
+    f &= f - 1;
+#pragma unroll
+    for (int t = 1; t < 4; ++t) {
+      b[t] = f ? static_cast<uint32_t>(__builtin_ctzll(f)) : b[0];
+      f &= f - 1;
+    }
+    uint64_t w[8];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      w[2 * t] = adj[2 * (prev + b[t])];
+      w[2 * t + 1] = adj[2 * (prev + b[t]) + 1];
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) next |= w[t];
+  }
+  return next;
+}
 
 // Saturated 16-label histogram of a position set, labels read through the graph's 64-position
 // label masks (the +1-below-15 rule of signature.hpp:44-51 applied count times).
@@ -164,13 +184,7 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_refine(GraphDev g, uint32_t v
     const uint64_t self = 1ull << ((static_cast<uint32_t>(v) - prev) & 63u);
     uint64_t ball = self, frontier = self;
     for (uint32_t it = 0; it < view_size && frontier; ++it) {
-      uint64_t next = 0, f = frontier;
-      while (f) {
-        const uint32_t b = __builtin_ctzll(f);
-        f &= f - 1;
-        const uint64_t u = static_cast<uint64_t>(prev) + b;
-        next |= g.adj[2 * u] | g.adj[2 * u + 1];
-      }
+      uint64_t next = expand_frontier(g.adj, prev, frontier);
       next &= ~ball;
       ball |= next;
       frontier = next;
@@ -198,13 +212,7 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_refine_next(GraphDev g, uint6
     const uint32_t prev = g.go[gid];
     const uint64_t self = 1ull << ((static_cast<uint32_t>(v) - prev) & 63u);
     const uint64_t old = reach[v];
-    uint64_t next = 0, f = frontier[v];
-    while (f) {
-      const uint32_t b = __builtin_ctzll(f);
-      f &= f - 1;
-      const uint64_t u = static_cast<uint64_t>(prev) + b;
-      next |= g.adj[2 * u] | g.adj[2 * u + 1];
-    }
+    uint64_t next = expand_frontier(g.adj, prev, frontier[v]);
     next &= ~(old | self);
     s = label_histogram(next, g.lmask + static_cast<uint64_t>(gid) * 16);
     sigs[v] = s;
