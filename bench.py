@@ -33,6 +33,7 @@ HBM_KERNELS = ("refine", "filter", "gmcr", "sig_refine", "sig_generate", "pair_f
 CONFIGS = {
     1: (1, 100_000, "normal", 100, 4, 20, 0.2, -1.0),
     2: (2, 1_000_000, "lognormal", 1000, 4, 32, 0.0, 0.3),
+    4: (4, 10_000_000, "lognormal", 0, 4, 32, 0.0, 0.3),  # serving: queries come with the requests
 }
 
 
@@ -93,6 +94,114 @@ def cpu_baseline(qh, dh, args) -> dict:
     }
 
 
+def _pct(x, q):
+    import numpy as np
+
+    return round(float(np.percentile(x, q)), 3) if len(x) else None
+
+
+def serve_bench(args) -> int:
+    """Serving (config 4): a resident database of LogNormal-size molecules; for each rate, a
+    Poisson stream of requests carrying U[1,50] query graphs (k ~ U[4,32], 30% with a ring) goes
+    through the continuous-batching loop.  Reports per-rate p50/p99 latency (completion minus
+    arrival) and matched pairs/s, and checks the first requests of each rate against offline
+    single-request runs."""
+    import numpy as np
+
+    from sigmo_amd import capi
+
+    seed, _, law, _, kmin, kmax, _, ring = CONFIGS[args.config]
+    dist = capi.SIZE_NORMAL if law == "normal" else capi.SIZE_LOGNORMAL
+    rates = [float(x) for x in args.qps.split(",") if x]
+    print(f"[serve] generating {args.molecules} molecules", file=sys.stderr, flush=True)
+    dh = capi.HostGraphs.molecules(seed, args.molecules, dist)
+    ctxs = [capi.Context(0) for _ in range(max(1, args.workers))]
+    dd = capi.DeviceGraph(ctxs[0], dh)
+    db = capi.Database(ctxs[0], dd, args.refinement_steps)
+    ctxs[0].sync()
+    workers = [capi.Batch(c, db) for c in ctxs]
+    dev_mem = ctxs[0].device_info()[1]
+    print(f"[serve] database resident: {dd.total_nodes} nodes, {db.device_bytes / 1e9:.2f} GB of level planes; "
+          f"{dev_mem / 1e9:.0f} GB device", file=sys.stderr, flush=True)
+    find_first = not args.find_all
+    budget = int(args.budget_gb * 1e9)
+    per_rate = []
+    total_matched, total_wall = 0, 0.0
+    for i, qps in enumerate(rates):
+        arr, per = capi.arrivals(seed + i, args.requests, qps, 1, 50)
+        rgo = np.concatenate([[0], np.cumsum(per)]).astype(np.uint32)
+        qh = capi.HostGraphs.queries(seed * 1000 + i, int(rgo[-1]), kmin, kmax, dist, ring_frac=ring)
+        # warm the buffers with the stream's largest request, untimed
+        big = int(np.argmax(per))
+        capi.serve(workers[:1], qh, rgo[big:big + 2], np.zeros(1), find_first=find_first)
+        for c in ctxs:
+            c.reset_phases()
+        print(f"[serve] {qps:g} QPS: {args.requests} requests, {int(rgo[-1])} query graphs", file=sys.stderr, flush=True)
+        out = capi.serve(workers, qh, rgo, arr, find_first=find_first, memory_budget=budget,
+                         max_batch_requests=args.max_batch_requests)
+        s = out["summary"]
+        done = out["status"] == capi.SERVE_DONE
+        lat = out["latency_ms"][done]
+        # offline check: the first requests alone, one batched run each
+        offline_ok = True
+        for r in range(min(4, args.requests)):
+            if not done[r]:
+                continue
+            one = capi.HostGraphs.slice(qh, int(rgo[r]), int(rgo[r + 1] - rgo[r]))
+            qd = capi.DeviceGraph(ctxs[0], one)
+            workers[0].run(qd, find_first)
+            st = workers[0].request_stats([0, one.num_graphs])[0]
+            offline_ok &= (st["matched_pairs"], st["embeddings"], st["hash"]) == (
+                int(out["matched_pairs"][r]), int(out["embeddings"][r]), int(out["hash"][r]))
+        pairs_s = s["matched_pairs"] / s["wall_s"] if s["wall_s"] > 0 else 0.0
+        total_matched += s["matched_pairs"]
+        total_wall += s["wall_s"]
+        kst = {}
+        for c in ctxs:
+            for name, (n, ms, by) in c.kernel_stats().items():
+                a = kst.setdefault(name, [0, 0.0, 0.0])
+                a[0] += n
+                a[1] += ms
+                a[2] += by
+        entry = {
+            "qps": qps, "requests": args.requests, "query_graphs": int(rgo[-1]), "done": s["done"], "rejected": s["rejected"],
+            "expired": s["expired"], "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99), "max_ms": _pct(lat, 100),
+            "offered_s": round(float(arr[-1]), 3), "wall_s": round(s["wall_s"], 3), "busy_frac": round(s["busy_s"] / s["wall_s"], 3),
+            "matched_pairs": s["matched_pairs"], "pairs_per_s": round(pairs_s, 1),
+            "query_graph_molecule_pairs_per_s": round(int(rgo[-1]) * dh.num_graphs / s["wall_s"], 1),
+            "batches": s["batches"], "mean_batch_requests": round(s["done"] / max(1, s["batches"]), 2),
+            "max_inflight_gb": round(s["max_inflight_bytes"] / 1e9, 2), "offline_equal": bool(offline_ok),
+            "kernel_ms": {k: round(v[1], 2) for k, v in kst.items() if v[0]},
+        }
+        per_rate.append(entry)
+        print(f"[serve] {json.dumps(entry)}", file=sys.stderr, flush=True)
+    line = {
+        "metric": METRIC,
+        "value": round(total_matched / total_wall, 1) if total_wall else 0.0,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": len(rates),
+        "warmup": 1,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"config {args.config} serving: {args.molecules} molecules seed {seed} LogNormal(ln 24,0.45) sizes "
+                        f"resident; Poisson requests of U[1,50] query graphs k~U[{kmin},{kmax}] {int(ring * 100)}% with a ring; "
+                        f"R={args.refinement_steps}, {'find-first' if find_first else 'find-all'}; {len(ctxs)} workers, "
+                        f"budget {args.budget_gb:g} GB",
+            "molecules": args.molecules,
+            "data_nodes": dd.total_nodes,
+        },
+        "serving": per_rate,
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,7 +221,16 @@ def main() -> int:
     ap.add_argument("--form", choices=("plain", "batched"), default="plain",
                     help="plain: per-phase kernels over the Q x D bitset; batched: per-pair evaluation from "
                          "per-molecule level planes (recomputed inside every step)")
+    ap.add_argument("--serve", action="store_true",
+                    help="serving mode (config 4): Poisson request streams against a resident database")
+    ap.add_argument("--qps", default="50,200,800", help="serving: comma-separated request rates")
+    ap.add_argument("--requests", type=int, default=200, help="serving: requests per rate")
+    ap.add_argument("--workers", type=int, default=2, help="serving: batches in flight (one stream each)")
+    ap.add_argument("--budget-gb", type=float, default=128.0, help="serving: device bytes for batches in flight")
+    ap.add_argument("--max-batch-requests", type=int, default=256)
     args = ap.parse_args()
+    if args.serve and args.config == 1 and "--config" not in sys.argv:
+        args.config = 4
     if args.molecules is None:
         args.molecules = CONFIGS[args.config][1]
     if args.queries is None:
@@ -130,6 +248,12 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": n * 4, "copies": 3}))
         return 0
+
+    if args.serve:
+        if args.gpus != 1:
+            print("bench.py measures one GPU; use --gpus 1.", file=sys.stderr)
+            return 2
+        return serve_bench(args)
 
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs SIGMo's hot path on a single MI355X "

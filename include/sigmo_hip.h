@@ -247,6 +247,80 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
 int sigmo_batch_pairs(sigmo_batch* b, uint32_t* data_graph_offsets, uint64_t n_offsets, uint32_t* query_graph_indices,
                       int32_t* counts, uint64_t n_pairs);
 int sigmo_batch_free(sigmo_batch* b);
+/* Per-request results of the last run, for a batch whose query graphs are the concatenation of
+ * requests: request r owns graphs [request_graph_offsets[r], request_graph_offsets[r+1]).  The
+ * hash is order-independent over the request's (molecule, request-local query graph, count)
+ * triples, so equal hashes mean equal results whatever else shared the batch.  At most 256
+ * requests per batch. */
+typedef struct sigmo_request_stats {
+  uint64_t matched_pairs;
+  uint64_t embeddings; /* sum of per-pair counts */
+  uint64_t hash;
+} sigmo_request_stats;
+int sigmo_batch_request_stats(sigmo_batch* b, const uint32_t* request_graph_offsets, uint32_t num_requests, sigmo_request_stats* out);
+/* Device bytes a run over `num_query_graphs` query graphs of `query_nodes` nodes in total is
+ * expected to hold beyond the database: the per-pair offsets (4 B per query graph x molecule),
+ * the survival bits and the first sizing of the candidate arena.  Runs whose survivors outgrow
+ * the arena's first sizing grow it (and keep it for later runs). */
+int sigmo_batch_workspace_bytes(const sigmo_batch* b, uint32_t num_query_graphs, uint64_t query_nodes, uint64_t* bytes);
+
+/* ---- serving loop (continuous batching over a resident database) ------------------------ */
+/* No counterpart in the reference, whose driver runs one query file per process
+ * (library/src/sigmo.cpp:102-159).
+ *
+ * Request r arrives at arrival_s[r] seconds after the call starts (real time; arrival_s must not
+ * decrease) and carries the query graphs [request_graph_offsets[r], request_graph_offsets[r+1]).
+ * Each worker, whenever it is free, admits in arrival order the requests that have arrived,
+ * while its batch stays within max_batch_requests and max_batch_graphs and the bytes of every
+ * batch in flight stay within memory_budget (a batch costs bytes_per_graph per query graph plus
+ * bytes_per_batch).  Requests that arrive while batches run wait for the next free worker; with
+ * several workers their batches overlap on the device.  A request whose own cost exceeds the
+ * budget is rejected; one that waited longer than deadline_ms before admission expires.  Both
+ * are reported and never run. */
+enum { SIGMO_SERVE_DONE = 0, SIGMO_SERVE_REJECTED = 1, SIGMO_SERVE_EXPIRED = 2 };
+typedef struct sigmo_serve_options {
+  uint32_t max_batch_requests; /* 0 or > 256: 256 (the per-request statistics limit) */
+  uint32_t max_batch_graphs;   /* 0: no limit; a single larger request still runs alone */
+  uint64_t memory_budget;      /* bytes over all batches in flight; 0: no limit */
+  uint64_t bytes_per_graph;    /* cost model; sigmo_serve fills 0 from sigmo_batch_workspace_bytes */
+  uint64_t bytes_per_batch;
+  double deadline_ms;          /* 0: none */
+  int find_first;
+} sigmo_serve_options;
+typedef struct sigmo_serve_result {
+  int32_t status;    /* SIGMO_SERVE_* */
+  uint32_t batch;    /* batch that served the request (dispatch order) */
+  uint32_t batch_requests;
+  uint32_t worker;
+  double admit_s;    /* from the call's start */
+  double done_s;
+  double latency_ms; /* completion (or rejection/expiry) minus arrival */
+  uint64_t matched_pairs;
+  uint64_t embeddings;
+  uint64_t hash;
+} sigmo_serve_result;
+typedef struct sigmo_serve_summary {
+  uint32_t batches;
+  uint32_t done, rejected, expired;
+  double wall_s;          /* call start to last completion */
+  double busy_s;          /* time with at least one batch in flight */
+  uint64_t max_inflight_bytes;
+  uint64_t matched_pairs, embeddings, gmcr_pairs;
+} sigmo_serve_summary;
+/* Runs one batch for the scheduler: fills out[i] for requests[0..count) and *gmcr_pairs.
+ * Returns SIGMO_OK or an error status, which stops the loop. */
+typedef int (*sigmo_serve_executor)(void* user, uint32_t worker, const uint32_t* requests, uint32_t count, sigmo_request_stats* out,
+                                    uint64_t* gmcr_pairs);
+/* The scheduler alone, with a caller-supplied executor (host-testable). */
+int sigmo_serve_schedule(uint32_t num_workers, const uint32_t* request_graph_offsets, const double* arrival_s, uint32_t num_requests,
+                         const sigmo_serve_options* options, sigmo_serve_executor executor, void* user, sigmo_serve_result* results,
+                         sigmo_serve_summary* summary);
+/* The serving loop on the device: worker w runs its batches with workers[w] (batch objects over
+ * one database, each created on its own context so that their streams overlap), uploading each
+ * batch's query graphs from `queries` into a buffer it reuses. */
+int sigmo_serve(sigmo_batch* const* workers, uint32_t num_workers, const sigmo_host_graphs* queries, const uint32_t* request_graph_offsets,
+                const double* arrival_s, uint32_t num_requests, const sigmo_serve_options* options, sigmo_serve_result* results,
+                sigmo_serve_summary* summary);
 
 #ifdef __cplusplus
 }

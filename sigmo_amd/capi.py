@@ -54,6 +54,7 @@ EXPORTED = (
     "sigmo_pipeline_create", "sigmo_pipeline_execute", "sigmo_pipeline_pairs", "sigmo_pipeline_free",
     "sigmo_kernel_stats", "sigmo_event_mark", "sigmo_event_elapsed", "sigmo_db_prepare", "sigmo_db_free",
     "sigmo_db_alloc_size", "sigmo_db_refresh", "sigmo_batch_create", "sigmo_batch_run", "sigmo_batch_pairs", "sigmo_batch_free",
+    "sigmo_batch_request_stats", "sigmo_batch_workspace_bytes", "sigmo_serve_schedule", "sigmo_serve",
 )
 KERNEL_NAMES = ("sig_generate", "sig_refine", "filter", "refine", "gmcr", "join", "pair_filter", "db_prepare")
 
@@ -86,6 +87,56 @@ class BatchResult(C.Structure):
         ("filter_scans", C.c_uint64),
     ]
 
+
+class RequestStats(C.Structure):
+    _fields_ = [("matched_pairs", C.c_uint64), ("embeddings", C.c_uint64), ("hash", C.c_uint64)]
+
+
+class ServeOptions(C.Structure):
+    _fields_ = [
+        ("max_batch_requests", C.c_uint32),
+        ("max_batch_graphs", C.c_uint32),
+        ("memory_budget", C.c_uint64),
+        ("bytes_per_graph", C.c_uint64),
+        ("bytes_per_batch", C.c_uint64),
+        ("deadline_ms", C.c_double),
+        ("find_first", C.c_int),
+    ]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32),
+        ("batch", C.c_uint32),
+        ("batch_requests", C.c_uint32),
+        ("worker", C.c_uint32),
+        ("admit_s", C.c_double),
+        ("done_s", C.c_double),
+        ("latency_ms", C.c_double),
+        ("matched_pairs", C.c_uint64),
+        ("embeddings", C.c_uint64),
+        ("hash", C.c_uint64),
+    ]
+
+
+class ServeSummary(C.Structure):
+    _fields_ = [
+        ("batches", C.c_uint32),
+        ("done", C.c_uint32),
+        ("rejected", C.c_uint32),
+        ("expired", C.c_uint32),
+        ("wall_s", C.c_double),
+        ("busy_s", C.c_double),
+        ("max_inflight_bytes", C.c_uint64),
+        ("matched_pairs", C.c_uint64),
+        ("embeddings", C.c_uint64),
+        ("gmcr_pairs", C.c_uint64),
+    ]
+
+
+SERVE_EXECUTOR = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(RequestStats),
+                             C.POINTER(C.c_uint64))
+SERVE_DONE, SERVE_REJECTED, SERVE_EXPIRED = 0, 1, 2
 
 _lib = None
 
@@ -158,6 +209,11 @@ def lib() -> C.CDLL:
             "sigmo_batch_run": ([vp, vp, i32, C.POINTER(BatchResult)], i32),
             "sigmo_batch_pairs": ([vp, vp, u64, vp, vp, u64], i32),
             "sigmo_batch_free": ([vp], i32),
+            "sigmo_batch_request_stats": ([vp, vp, u32, C.POINTER(RequestStats)], i32),
+            "sigmo_batch_workspace_bytes": ([vp, u32, u64, C.POINTER(u64)], i32),
+            "sigmo_serve_schedule": ([u32, vp, vp, u32, C.POINTER(ServeOptions), SERVE_EXECUTOR, vp, C.POINTER(ServeResult),
+                                      C.POINTER(ServeSummary)], i32),
+            "sigmo_serve": ([vp, u32, vp, vp, vp, u32, C.POINTER(ServeOptions), C.POINTER(ServeResult), C.POINTER(ServeSummary)], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -622,6 +678,20 @@ class Batch:
                      "filter_lane_steps": r.filter_lane_steps, "filter_scans": r.filter_scans}
         return self.last
 
+    def request_stats(self, request_graph_offsets) -> list[dict]:
+        """Per-request matched pairs, embeddings and order-independent hash of the last run;
+        request r owns the query graphs [offsets[r], offsets[r+1])."""
+        rgo = np.ascontiguousarray(request_graph_offsets, np.uint32)
+        n = rgo.size - 1
+        out = (RequestStats * max(n, 1))()
+        check(lib().sigmo_batch_request_stats(self._h, _ptr(rgo), n, out))
+        return [{"matched_pairs": out[i].matched_pairs, "embeddings": out[i].embeddings, "hash": out[i].hash} for i in range(n)]
+
+    def workspace_bytes(self, num_query_graphs: int, query_nodes: int) -> int:
+        v = C.c_uint64()
+        check(lib().sigmo_batch_workspace_bytes(self._h, num_query_graphs, query_nodes, C.byref(v)))
+        return v.value
+
     def pairs(self):
         """(data_graph_offsets, query_graph_indices, counts) of the last run, GMCR order."""
         P = self.last["gmcr_pairs"]
@@ -630,3 +700,63 @@ class Batch:
         cnt = np.zeros(P, np.int32)
         check(lib().sigmo_batch_pairs(self._h, _ptr(off), off.size, _ptr(idx) if P else None, _ptr(cnt) if P else None, P))
         return off, idx, cnt
+
+
+# ---- serving ------------------------------------------------------------------------------------
+
+def _serve_options(max_batch_requests=0, max_batch_graphs=0, memory_budget=0, bytes_per_graph=0, bytes_per_batch=0,
+                   deadline_ms=0.0, find_first=True) -> ServeOptions:
+    return ServeOptions(max_batch_requests, max_batch_graphs, memory_budget, bytes_per_graph, bytes_per_batch, deadline_ms,
+                        int(find_first))
+
+
+def _serve_out(res, summ) -> dict:
+    n = len(res)
+    cols = {name: np.array([getattr(res[i], name) for i in range(n)]) for name, _ in ServeResult._fields_}
+    cols["hash"] = np.array([res[i].hash for i in range(n)], np.uint64)
+    cols["summary"] = {name: getattr(summ, name) for name, _ in ServeSummary._fields_}
+    return cols
+
+
+def serve(workers: list, queries: HostGraphs, request_graph_offsets, arrival_s, **options) -> dict:
+    """Continuous-batching serving loop (sigmo_serve) over batch objects of one database, one per
+    worker; returns per-request columns (status, latency_ms, matched_pairs, embeddings, hash, ...)
+    and a 'summary' dict."""
+    rgo = np.ascontiguousarray(request_graph_offsets, np.uint32)
+    arr = np.ascontiguousarray(arrival_s, np.float64)
+    n = arr.size
+    if rgo.size != n + 1:
+        raise ValueError("request_graph_offsets needs one entry more than arrival_s")
+    handles = (C.c_void_p * len(workers))(*[w._h for w in workers])
+    res = (ServeResult * max(n, 1))()
+    summ = ServeSummary()
+    opt = _serve_options(**options)
+    check(lib().sigmo_serve(handles, len(workers), queries.handle, _ptr(rgo), _ptr(arr), n, C.byref(opt), res, C.byref(summ)))
+    return _serve_out(res[:n], summ)
+
+
+def serve_schedule(num_workers: int, request_graph_offsets, arrival_s, executor, **options) -> dict:
+    """The scheduler of the serving loop with a Python executor
+    ``executor(worker, requests) -> (list of (matched_pairs, embeddings, hash), gmcr_pairs)``."""
+    rgo = np.ascontiguousarray(request_graph_offsets, np.uint32)
+    arr = np.ascontiguousarray(arrival_s, np.float64)
+    n = arr.size
+    if rgo.size != n + 1:
+        raise ValueError("request_graph_offsets needs one entry more than arrival_s")
+
+    def _run(_user, worker, reqs, count, out, gmcr):
+        try:
+            stats, gp = executor(int(worker), [reqs[i] for i in range(count)])
+            for i, (m, e, h) in enumerate(stats):
+                out[i].matched_pairs, out[i].embeddings, out[i].hash = m, e, h
+            gmcr[0] = gp
+            return OK
+        except Exception:  # noqa: BLE001 -- reported as a failed batch
+            return ERR_INVALID
+
+    cb = SERVE_EXECUTOR(_run)
+    res = (ServeResult * max(n, 1))()
+    summ = ServeSummary()
+    opt = _serve_options(**options)
+    check(lib().sigmo_serve_schedule(num_workers, _ptr(rgo), _ptr(arr), n, C.byref(opt), cb, None, res, C.byref(summ)))
+    return _serve_out(res[:n], summ)

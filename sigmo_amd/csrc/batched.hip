@@ -732,6 +732,48 @@ __global__ __launch_bounds__(256) void k_join_arena(GraphDev q, GraphDev d, cons
   }
 }
 
+// Per-request statistics of a batch run.  Request r owns the batch's query graphs
+// [rg[r], rg[r+1]); each GMCR pair (molecule m, query graph g, count c) adds its match flag,
+// c, and mix64(mix64(((g - rg[r]) << 32) | m) + c) to its request (sums mod 2^64, mix64 being
+// the splitmix64 finaliser).  Sums do not depend on the order pairs are visited, so a request's
+// hash is the same whether it ran alone or batched with others.
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+constexpr uint32_t kMaxBatchRequests = 256;
+
+__global__ __launch_bounds__(256) void k_request_stats(uint64_t P, const uint32_t* __restrict__ indices, const uint32_t* __restrict__ pair_graph,
+                                                       const int32_t* __restrict__ counts, const uint32_t* __restrict__ rg, uint32_t nreq,
+                                                       unsigned long long* __restrict__ out) {
+  __shared__ uint32_t srg[kMaxBatchRequests + 1];
+  __shared__ unsigned long long acc[kMaxBatchRequests][3];
+  for (uint32_t t = threadIdx.x; t <= nreq; t += blockDim.x) srg[t] = rg[t];
+  for (uint32_t t = threadIdx.x; t < nreq; t += blockDim.x) acc[t][0] = acc[t][1] = acc[t][2] = 0;
+  __syncthreads();
+  for (uint64_t j = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; j < P; j += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t g = indices[j];
+    uint32_t lo = 0, hi = nreq;  // last r with srg[r] <= g
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (srg[mid] <= g) lo = mid;
+      else hi = mid;
+    }
+    const int32_t c = counts[j];
+    if (c > 0) atomicAdd(&acc[lo][0], 1ull);
+    atomicAdd(&acc[lo][1], static_cast<unsigned long long>(c));
+    const uint64_t key = (static_cast<uint64_t>(g - srg[lo]) << 32) | pair_graph[j];
+    atomicAdd(&acc[lo][2], static_cast<unsigned long long>(mix64(mix64(key) + static_cast<uint32_t>(c))));
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < nreq; t += blockDim.x)
+    for (int f = 0; f < 3; ++f)
+      if (acc[t][f]) atomicAdd(&out[3 * t + f], acc[t][f]);
+}
+
 }  // namespace
 
 hipError_t db_prepare(hipStream_t s, const GraphDev& g, uint32_t R, const uint32_t* cls_list, const uint32_t* pidx, const uint32_t* wide_idx,
@@ -808,6 +850,11 @@ uint64_t graph_edges(const sigmo_graph* g);
 
 using namespace sigmo_internal;
 
+struct sigmo_batch;
+namespace sigmo_internal {
+sigmo_ctx* batch_ctx(const sigmo_batch* b);
+}
+
 struct sigmo_db {
   sigmo_ctx* ctx = nullptr;
   const sigmo_graph* data = nullptr;
@@ -862,6 +909,8 @@ struct sigmo_batch {
   uint64_t cap_counts = 0;
   uint32_t* wide_pairs = nullptr;  // pairs of molecules with > 64 nodes, for the wavefront-per-pair join
   uint64_t cap_wide = 0;
+  uint32_t* req_offsets = nullptr;        // per-request statistics
+  unsigned long long* req_out = nullptr;
   uint64_t P = 0;
   uint32_t Gq = 0;
   bool ran = false;
@@ -882,8 +931,12 @@ struct sigmo_batch {
     bfree(total);
     bfree(counts);
     bfree(wide_pairs);
+    bfree(req_offsets);
+    bfree(req_out);
   }
 };
+
+sigmo_ctx* sigmo_internal::batch_ctx(const sigmo_batch* b) { return b->ctx; }
 
 extern "C" {
 
@@ -999,6 +1052,54 @@ int sigmo_batch_create(sigmo_ctx* ctx, const sigmo_db* db, sigmo_batch** out) {
     return stt;
   }
   *out = b;
+  return SIGMO_OK;
+}
+
+int sigmo_batch_workspace_bytes(const sigmo_batch* b, uint32_t num_query_graphs, uint64_t query_nodes, uint64_t* bytes) {
+  if (!b || !bytes) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  const sigmo_db* db = b->db;
+  const uint64_t Gd = graph_dev(db->data).G, Gq = num_query_graphs, R = db->R;
+  const uint64_t pairs = Gq * Gd, W = (Gq + 63) / 64;
+  const double mean_k = Gq ? double(query_nodes) / double(Gq) : 0.0;
+  // the arena's first sizing (see sigmo_batch_run) without the per-wavefront slack; the GMCR
+  // arrays are priced at that many pairs
+  const uint64_t guess = std::min<uint64_t>(std::max<uint64_t>(pairs / 8, 1 << 16), 1ull << 26);
+  const uint64_t arena_words = static_cast<uint64_t>(double(guess) * (mean_k + 1.0) * (db->num_wide ? 2.0 : 1.0));
+  *bytes = query_nodes * sigmo_k::token_segment(static_cast<uint32_t>(R)) * 2 + Gq * (sizeof(sigmo_k::QueryInfo) + 8 * R) + Gd * W * 8 +
+           pairs * 4 + arena_words * 8 + guess * 5 * 4;
+  return SIGMO_OK;
+}
+
+int sigmo_batch_request_stats(sigmo_batch* b, const uint32_t* request_graph_offsets, uint32_t num_requests, sigmo_request_stats* out) {
+  if (!b || !b->ran) return sigmo_err::set(SIGMO_ERR_INVALID, "batch has not run");
+  if (!request_graph_offsets || !out || num_requests == 0) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  if (num_requests > sigmo_k::kMaxBatchRequests) return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "at most 256 requests per batch");
+  if (request_graph_offsets[0] != 0 || request_graph_offsets[num_requests] != b->Gq)
+    return sigmo_err::set(SIGMO_ERR_INVALID, "request graph offsets must cover the batch's query graphs");
+  for (uint32_t r = 0; r < num_requests; ++r)
+    if (request_graph_offsets[r + 1] < request_graph_offsets[r]) return sigmo_err::set(SIGMO_ERR_INVALID, "request graph offsets must ascend");
+  int device = 0, cus = 0;
+  hipStream_t s;
+  unsigned long long *st, *sh;
+  BATCH_TRY(ctx_info(b->ctx, &device, &s, &cus, &st, &sh));
+  if (!b->req_offsets) BATCH_TRY(balloc(&b->req_offsets, sigmo_k::kMaxBatchRequests + 1));
+  if (!b->req_out) BATCH_TRY(balloc(&b->req_out, 3 * sigmo_k::kMaxBatchRequests));
+  BATCH_HIP_TRY(hipMemcpyAsync(b->req_offsets, request_graph_offsets, (num_requests + 1) * 4ull, hipMemcpyHostToDevice, s));
+  BATCH_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(b->req_out), 3ull * num_requests, 0));
+  if (b->P) {
+    const uint64_t blocks = std::min<uint64_t>((b->P + 255) / 256, static_cast<uint64_t>(cus) * 8);
+    hipLaunchKernelGGL(sigmo_k::k_request_stats, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, b->P, b->indices, b->pair_graph,
+                       b->counts, b->req_offsets, num_requests, b->req_out);
+    BATCH_HIP_TRY(hipGetLastError());
+  }
+  std::vector<unsigned long long> h(3ull * num_requests);
+  BATCH_HIP_TRY(hipMemcpyAsync(h.data(), b->req_out, h.size() * 8, hipMemcpyDeviceToHost, s));
+  BATCH_HIP_TRY(hipStreamSynchronize(s));
+  for (uint32_t r = 0; r < num_requests; ++r) {
+    out[r].matched_pairs = h[3 * r];
+    out[r].embeddings = h[3 * r + 1];
+    out[r].hash = h[3 * r + 2];
+  }
   return SIGMO_OK;
 }
 

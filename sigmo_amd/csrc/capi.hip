@@ -153,6 +153,7 @@ struct sigmo_graph {
   uint8_t *nl = nullptr, *el = nullptr;
   uint64_t *adj = nullptr, *lplanes = nullptr, *lmask = nullptr;
   sigmo_k::LabelSlots slots{};
+  uint64_t cap_G = 0, cap_N = 0, cap_E = 0;  // device buffer capacities (a re-upload reuses them)
 
   sigmo_k::GraphDev dev() const {
     sigmo_k::GraphDev g;
@@ -422,24 +423,61 @@ int sigmo_device_info(sigmo_ctx* ctx, char* name, size_t name_len, uint64_t* glo
 
 // ---- graphs ----------------------------------------------------------------------------------
 
-int sigmo_graph_upload(sigmo_ctx* ctx, const sigmo_host_graphs* host, sigmo_graph** out) {
-  SIGMO_TRY(bind(ctx));
-  if (!host || !out) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+}  // extern "C"
+
+namespace {
+
+// Fill `g` from `host`: validate, grow the device buffers where the new batch exceeds them,
+// copy the CSR arrays and derive the adjacency words and label planes.
+int graph_fill(sigmo_graph* g, const sigmo_host_graphs* host) {
   const uint32_t G = host->num_graphs();
   const uint64_t N = host->total_nodes(), E = host->total_edges();
   if (G == 0) return sigmo_err::set(SIGMO_ERR_INVALID, "empty graph batch");
   if (N >= (1ull << 32) || E >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "batch exceeds u32 node/edge ids");
   uint32_t mx = 0, mn = UINT32_MAX;
-  for (uint32_t g = 0; g < G; ++g) {
-    const uint32_t n = host->graph_offsets[g + 1] - host->graph_offsets[g];
+  for (uint32_t i = 0; i < G; ++i) {
+    const uint32_t n = host->graph_offsets[i + 1] - host->graph_offsets[i];
     mx = std::max(mx, n);
     mn = std::min(mn, n);
   }
   if (mn == 0) return sigmo_err::set(SIGMO_ERR_INVALID, "graphs with zero nodes are not supported");
   if (mx > SIGMO_MAX_DATA_GRAPH_NODES)
     return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "graph with " + std::to_string(mx) + " nodes exceeds the 128-node limit");
-  auto* g = new sigmo_graph();
-  g->ctx = ctx;
+  int st = SIGMO_OK;
+  if (G > g->cap_G) {
+    dev_free(g->go);
+    dev_free(g->lmask);
+    g->cap_G = 0;
+    if (st == SIGMO_OK) st = dev_alloc(&g->go, uint64_t(G) + 1);
+    if (st == SIGMO_OK) st = dev_alloc(&g->lmask, 16 * uint64_t(G));
+    if (st != SIGMO_OK) return st;
+    g->cap_G = G;
+  }
+  if (N > g->cap_N) {
+    dev_free(g->ro);
+    dev_free(g->nl);
+    dev_free(g->node_graph);
+    dev_free(g->adj);
+    dev_free(g->lplanes);
+    g->cap_N = 0;
+    const uint64_t W64 = (N + 63) / 64;
+    if (st == SIGMO_OK) st = dev_alloc(&g->ro, N + 1);
+    if (st == SIGMO_OK) st = dev_alloc(&g->nl, N);
+    if (st == SIGMO_OK) st = dev_alloc(&g->node_graph, N);
+    if (st == SIGMO_OK) st = dev_alloc(&g->adj, 2 * N);
+    if (st == SIGMO_OK) st = dev_alloc(&g->lplanes, 8 * W64);
+    if (st != SIGMO_OK) return st;
+    g->cap_N = N;
+  }
+  if (E > g->cap_E || !g->ci) {
+    dev_free(g->ci);
+    dev_free(g->el);
+    g->cap_E = 0;
+    if (st == SIGMO_OK) st = dev_alloc(&g->ci, E);
+    if (st == SIGMO_OK) st = dev_alloc(&g->el, E);
+    if (st != SIGMO_OK) return st;
+    g->cap_E = E;
+  }
   g->G = G;
   g->N = N;
   g->E = E;
@@ -452,22 +490,7 @@ int sigmo_graph_upload(sigmo_ctx* ctx, const sigmo_host_graphs* host, sigmo_grap
   g->slots.count = 0;
   for (int l = 0; l < 16; ++l)
     if (present[l]) g->slots.label[g->slots.count++] = static_cast<uint8_t>(l);
-  const uint64_t W64 = (N + 63) / 64;
-  int st = SIGMO_OK;
-  if (st == SIGMO_OK) st = dev_alloc(&g->go, uint64_t(G) + 1);
-  if (st == SIGMO_OK) st = dev_alloc(&g->ro, N + 1);
-  if (st == SIGMO_OK) st = dev_alloc(&g->ci, E);
-  if (st == SIGMO_OK) st = dev_alloc(&g->nl, N);
-  if (st == SIGMO_OK) st = dev_alloc(&g->el, E);
-  if (st == SIGMO_OK) st = dev_alloc(&g->node_graph, N);
-  if (st == SIGMO_OK) st = dev_alloc(&g->adj, 2 * N);
-  if (st == SIGMO_OK) st = dev_alloc(&g->lplanes, 8 * W64);
-  if (st == SIGMO_OK) st = dev_alloc(&g->lmask, 16 * uint64_t(G));
-  if (st != SIGMO_OK) {
-    delete g;
-    return st;
-  }
-  hipStream_t s = ctx->stream;
+  hipStream_t s = g->ctx->stream;
   hipError_t e = hipMemcpyAsync(g->go, host->graph_offsets.data(), (uint64_t(G) + 1) * 4, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(g->ro, host->row_offsets.data(), (N + 1) * 4, hipMemcpyHostToDevice, s);
   if (e == hipSuccess && E) e = hipMemcpyAsync(g->ci, host->column_indices.data(), E * 4, hipMemcpyHostToDevice, s);
@@ -475,9 +498,34 @@ int sigmo_graph_upload(sigmo_ctx* ctx, const sigmo_host_graphs* host, sigmo_grap
   if (e == hipSuccess && E) e = hipMemcpyAsync(g->el, host->edge_labels.data(), E, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = sigmo_k::derive_graph(s, g->go, G, g->ro, g->ci, g->nl, N, g->node_graph, g->adj, g->lplanes, g->lmask);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e != hipSuccess) {
+  if (e != hipSuccess) return sigmo_err::set(SIGMO_ERR_HIP, std::string("graph upload: ") + hipGetErrorString(e));
+  return SIGMO_OK;
+}
+
+}  // namespace
+
+namespace sigmo_internal {
+// Upload `host` into *g, reusing its device buffers when they are large enough (a serving loop
+// uploads one query batch per iteration without allocating once capacities have settled).
+int graph_reupload(sigmo_ctx* ctx, const sigmo_host_graphs* host, sigmo_graph** g) {
+  if (!*g) return sigmo_graph_upload(ctx, host, g);
+  SIGMO_TRY(bind(ctx));
+  if ((*g)->ctx != ctx) return sigmo_err::set(SIGMO_ERR_INVALID, "graph belongs to another context");
+  return graph_fill(*g, host);
+}
+}  // namespace sigmo_internal
+
+extern "C" {
+
+int sigmo_graph_upload(sigmo_ctx* ctx, const sigmo_host_graphs* host, sigmo_graph** out) {
+  SIGMO_TRY(bind(ctx));
+  if (!host || !out) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  auto* g = new sigmo_graph();
+  g->ctx = ctx;
+  const int st = graph_fill(g, host);
+  if (st != SIGMO_OK) {
     delete g;
-    return sigmo_err::set(SIGMO_ERR_HIP, std::string("graph upload: ") + hipGetErrorString(e));
+    return st;
   }
   *out = g;
   return SIGMO_OK;

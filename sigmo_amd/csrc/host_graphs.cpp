@@ -5,6 +5,7 @@
 //   "u v" (edge label 0); anything else is "Invalid graph format".
 #include "host_graphs.hpp"
 
+#include <algorithm>
 #include <cctype>
 #include <cerrno>
 #include <cstdio>
@@ -52,7 +53,6 @@ void append_graph(sigmo_host_graphs& b, const EdgeListGraph& g) {
     cur[a]++;
     cur[c]++;
   }
-  b.row_offsets.reserve(b.row_offsets.size() + n);
   for (uint32_t i = 1; i <= n; ++i) b.row_offsets.push_back(col_base + rows[i]);
   b.node_labels.insert(b.node_labels.end(), g.labels.begin(), g.labels.end());
   b.graph_offsets.push_back(node_base + n);
@@ -67,7 +67,8 @@ void append_batch(sigmo_host_graphs& dst, const sigmo_host_graphs& src, uint32_t
   const uint32_t col_base = static_cast<uint32_t>(dst.column_indices.size());
   for (uint32_t g = first; g < first + count; ++g) dst.graph_offsets.push_back(src.graph_offsets[g + 1] - node_lo + node_base);
   for (uint32_t v = node_lo; v < node_hi; ++v) dst.row_offsets.push_back(src.row_offsets[v + 1] - col_lo + col_base);
-  dst.column_indices.reserve(col_base + (col_hi - col_lo));
+  if (dst.column_indices.capacity() < col_base + (col_hi - col_lo))  // grow geometrically: appends repeat
+    dst.column_indices.reserve(std::max<size_t>(col_base + (col_hi - col_lo), 2 * dst.column_indices.capacity()));
   for (uint32_t e = col_lo; e < col_hi; ++e) dst.column_indices.push_back(src.column_indices[e] - node_lo + node_base);
   dst.edge_labels.insert(dst.edge_labels.end(), src.edge_labels.begin() + col_lo, src.edge_labels.begin() + col_hi);
   dst.node_labels.insert(dst.node_labels.end(), src.node_labels.begin() + node_lo, src.node_labels.begin() + node_hi);

@@ -80,3 +80,36 @@ def assert_phases_equal(g: dict, o: dict, modes=(True, False)) -> None:
 
 def oracle_phases(qh: capi.HostGraphs, dh: capi.HostGraphs, R: int, modes=(True, False)) -> dict:
     return O.pipeline(qh.arrays(), dh.arrays(), R, keep_levels=True, modes=list(modes))
+
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _mix64(x: np.ndarray) -> np.ndarray:
+    """splitmix64 output function on uint64 arrays (wrapping arithmetic)."""
+    with np.errstate(over="ignore"):
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return x ^ (x >> np.uint64(31))
+
+
+def request_stats(offsets, indices, counts, request_graph_offsets) -> list[tuple[int, int, int]]:
+    """Per-request (matched pairs, embeddings, hash) of a GMCR + counts result, the statistics
+    sigmo_batch_request_stats reports: request r owns query graphs [rgo[r], rgo[r+1]), and a pair
+    (molecule m, query graph g, count c) adds mix64(mix64(((g - rgo[r]) << 32) | m) + c)."""
+    rgo = np.asarray(request_graph_offsets, np.int64)
+    mol = np.repeat(np.arange(len(offsets) - 1, dtype=np.uint64), np.diff(np.asarray(offsets, np.int64)))
+    g = np.asarray(indices, np.int64)
+    c = np.asarray(counts, np.int64)
+    req = np.searchsorted(rgo, g, side="right") - 1
+    with np.errstate(over="ignore"):
+        key = ((g - rgo[req]).astype(np.uint64) << np.uint64(32)) | mol
+        h = _mix64(_mix64(key) + c.astype(np.uint32).astype(np.uint64))
+    out = []
+    for r in range(len(rgo) - 1):
+        m = req == r
+        with np.errstate(over="ignore"):
+            hs = int(np.sum(h[m], dtype=np.uint64)) if m.any() else 0
+        out.append((int((c[m] > 0).sum()), int(c[m].sum()), hs))
+    return out
