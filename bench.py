@@ -33,6 +33,7 @@ HBM_KERNELS = ("refine", "filter", "gmcr", "sig_refine", "sig_generate", "pair_f
 CONFIGS = {
     1: (1, 100_000, "normal", 100, 4, 20, 0.2, -1.0),
     2: (2, 1_000_000, "lognormal", 1000, 4, 32, 0.0, 0.3),
+    3: (3, 20_000_000, "lognormal", 2000, 4, 32, 0.0, 0.3),  # streamed in chunks
     4: (4, 10_000_000, "lognormal", 0, 4, 32, 0.0, 0.3),  # serving: queries come with the requests
 }
 
@@ -54,8 +55,12 @@ def describe(args) -> str:
     extra += f" {int(ring * 100)}% with a ring" if ring >= 0 else ""
     mode = "find-first" if not args.find_all else "find-all"
     form = "plain per-phase form" if args.form == "plain" else "batched pair form"
+    stream = ""
+    if args.config == 3:
+        stream = (f", streamed from host memory in chunks of {args.chunk_molecules} molecules"
+                  f"{' (next chunk loaded on a side stream)' if not args.no_overlap else ''}")
     return (f"config {args.config} ({form}): {args.molecules} molecules seed {seed} {sizes}, "
-            f"{args.queries} queries k~U[{kmin},{kmax}]{extra}, R={args.refinement_steps}, {mode}")
+            f"{args.queries} queries k~U[{kmin},{kmax}]{extra}, R={args.refinement_steps}, {mode}{stream}")
 
 
 CPU_SAMPLE_PAIRS = 10_000_000  # ~3 s of oracle work on 16 cores at config 1 rates; whole workload for config 1
@@ -228,6 +233,10 @@ def main() -> int:
     ap.add_argument("--workers", type=int, default=2, help="serving: batches in flight (one stream each)")
     ap.add_argument("--budget-gb", type=float, default=128.0, help="serving: device bytes for batches in flight")
     ap.add_argument("--max-batch-requests", type=int, default=256)
+    ap.add_argument("--chunk-gb", type=float, default=128.0,
+                    help="config 3: size chunks so the plain form's candidate bitset of one chunk fits this many GB")
+    ap.add_argument("--chunk-molecules", type=int, default=0, help="config 3: molecules per chunk (overrides --chunk-gb)")
+    ap.add_argument("--no-overlap", action="store_true", help="config 3: load chunks inline instead of on a side stream")
     args = ap.parse_args()
     if args.serve and args.config == 1 and "--config" not in sys.argv:
         args.config = 4
@@ -265,21 +274,37 @@ def main() -> int:
     print(f"[bench] generating workload: {args.molecules} molecules, {args.queries} queries", file=sys.stderr, flush=True)
     qh, dh = workload(args)
     ctx = capi.Context(0)
-    qd, dd = capi.DeviceGraph(ctx, qh), capi.DeviceGraph(ctx, dh)
     find_first = not args.find_all
-    if args.form == "plain":
+    qd = capi.DeviceGraph(ctx, qh)
+    if args.config == 3:
+        # the shard stays in host memory; one step streams it through the device in chunks
+        if not args.chunk_molecules:
+            # the plain form's Q x D candidate bitset of one chunk within --chunk-gb; both forms use
+            # the same chunks
+            mean_n = dh.total_nodes / dh.num_graphs
+            args.chunk_molecules = int(args.chunk_gb * 1e9 / (qd.total_nodes / 8.0) / mean_n)
+        data_nodes = dh.total_nodes
+
+        def step():
+            return capi.stream_run(ctx, qd, dh, chunk_molecules=args.chunk_molecules, form=args.form,
+                                   refinement_steps=args.refinement_steps, find_first=find_first, overlap=not args.no_overlap)
+    elif args.form == "plain":
+        dd = capi.DeviceGraph(ctx, dh)
+        data_nodes = dd.total_nodes
         pipe = capi.Pipeline(ctx, qd, dd)
 
         def step():
             return pipe.execute(args.refinement_steps, find_first)
     else:
+        dd = capi.DeviceGraph(ctx, dh)
+        data_nodes = dd.total_nodes
         db = capi.Database(ctx, dd, args.refinement_steps)
         batch = capi.Batch(ctx, db)
 
         def step():
             db.refresh()  # the data side is recomputed every step, as the plain form does
             return batch.run(qd, find_first)
-    print(f"[bench] uploaded: Q={qd.total_nodes} query nodes, D={dd.total_nodes} data nodes; warmup {args.warmup}",
+    print(f"[bench] uploaded: Q={qd.total_nodes} query nodes, D={data_nodes} data nodes; warmup {args.warmup}",
           file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         step()
@@ -350,14 +375,16 @@ def main() -> int:
             "molecules": args.molecules,
             "queries": args.queries,
             "query_nodes": qd.total_nodes,
-            "data_nodes": dd.total_nodes,
+            "data_nodes": data_nodes,
             "refinement_steps": args.refinement_steps,
             "gmcr_pairs": results[-1]["gmcr_pairs"],
             "matched_pairs": matched,
             "embeddings": results[-1]["num_matches"] if args.find_all else None,
             "dfs_steps": results[-1]["dfs_steps"],
             **({k: results[-1][k] for k in ("filter_wave_steps", "filter_lane_steps", "filter_scans")}
-               if args.form == "batched" else {}),
+               if args.form == "batched" and args.config != 3 else {}),
+            **({k: results[-1][k] for k in ("chunks", "wall_ms", "compute_ms", "load_ms", "load_wait_ms")}
+               if args.config == 3 else {}),
             "form": args.form,
         },
         "roofline": roofline,

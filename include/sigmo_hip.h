@@ -322,6 +322,37 @@ int sigmo_serve(sigmo_batch* const* workers, uint32_t num_workers, const sigmo_h
                 const double* arrival_s, uint32_t num_requests, const sigmo_serve_options* options, sigmo_serve_result* results,
                 sigmo_serve_summary* summary);
 
+/* ---- streaming (a molecule shard larger than one device pass) ---------------------------- */
+/* Replaces one rank's run of the reference's multi-node driver (library/src/sigmo_mpi.cpp:
+ * 110-159) when its shard does not fit the candidate memory: the shard stays in host memory
+ * and runs in chunks of chunk_molecules molecules.  With `overlap`, a loader thread slices and
+ * uploads chunk i+1 on its own stream while chunk i runs on the context's stream.  Each chunk's
+ * GMCR (chunk-local molecule ids) and per-pair counts are handed to on_chunk (when given); their
+ * concatenation equals one run over the whole shard. */
+enum { SIGMO_FORM_PLAIN = 0, SIGMO_FORM_BATCHED = 1 };
+typedef struct sigmo_stream_options {
+  uint32_t chunk_molecules; /* 0: the whole shard in one chunk */
+  uint32_t refinement_steps;
+  int find_first;
+  int form;                 /* SIGMO_FORM_PLAIN: per-phase kernels over the chunk's bitset; SIGMO_FORM_BATCHED */
+  int overlap;              /* nonzero: load the next chunk while the current one runs */
+} sigmo_stream_options;
+typedef struct sigmo_stream_result {
+  uint64_t num_matches, matched_pairs, gmcr_pairs, dfs_steps;
+  uint32_t chunks;
+  double wall_ms;      /* first load to last chunk's results */
+  double compute_ms;   /* chunks' device work, host-timed */
+  double load_ms;      /* chunk slicing and upload */
+  double load_wait_ms; /* time the compute side waited for a chunk */
+} sigmo_stream_result;
+/* Receives chunk `chunk` (molecules [first_molecule, first_molecule + num_molecules)): GMCR
+ * offsets[num_molecules + 1], query graph indices and counts of its num_pairs pairs.  Returns 0
+ * to go on. */
+typedef int (*sigmo_stream_chunk_fn)(void* user, uint32_t chunk, uint32_t first_molecule, uint32_t num_molecules, const uint32_t* offsets,
+                                     const uint32_t* indices, const int32_t* counts, uint64_t num_pairs);
+int sigmo_stream_run(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_host_graphs* data, const sigmo_stream_options* options,
+                     sigmo_stream_chunk_fn on_chunk, void* user, sigmo_stream_result* result);
+
 #ifdef __cplusplus
 }
 #endif

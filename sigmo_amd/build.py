@@ -40,7 +40,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wextra", "-march=x86-64-v2"]
 
 HIP_SOURCES = ["kernels.hip", "capi.hip", "batched.hip"]
-HOST_SOURCES = ["host_graphs.cpp", "generator.cpp", "serve.cpp"]
+HOST_SOURCES = ["host_graphs.cpp", "generator.cpp", "serve.cpp", "stream.cpp"]
 
 
 def _run(cmd: list[str], verbose: bool) -> None:

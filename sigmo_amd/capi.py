@@ -54,7 +54,7 @@ EXPORTED = (
     "sigmo_pipeline_create", "sigmo_pipeline_execute", "sigmo_pipeline_pairs", "sigmo_pipeline_free",
     "sigmo_kernel_stats", "sigmo_event_mark", "sigmo_event_elapsed", "sigmo_db_prepare", "sigmo_db_free",
     "sigmo_db_alloc_size", "sigmo_db_refresh", "sigmo_batch_create", "sigmo_batch_run", "sigmo_batch_pairs", "sigmo_batch_free",
-    "sigmo_batch_request_stats", "sigmo_batch_workspace_bytes", "sigmo_serve_schedule", "sigmo_serve",
+    "sigmo_batch_request_stats", "sigmo_batch_workspace_bytes", "sigmo_serve_schedule", "sigmo_serve", "sigmo_stream_run",
 )
 KERNEL_NAMES = ("sig_generate", "sig_refine", "filter", "refine", "gmcr", "join", "pair_filter", "db_prepare")
 
@@ -138,6 +138,30 @@ SERVE_EXECUTOR = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint
                              C.POINTER(C.c_uint64))
 SERVE_DONE, SERVE_REJECTED, SERVE_EXPIRED = 0, 1, 2
 
+
+class StreamOptions(C.Structure):
+    _fields_ = [("chunk_molecules", C.c_uint32), ("refinement_steps", C.c_uint32), ("find_first", C.c_int), ("form", C.c_int),
+                ("overlap", C.c_int)]
+
+
+class StreamResult(C.Structure):
+    _fields_ = [
+        ("num_matches", C.c_uint64),
+        ("matched_pairs", C.c_uint64),
+        ("gmcr_pairs", C.c_uint64),
+        ("dfs_steps", C.c_uint64),
+        ("chunks", C.c_uint32),
+        ("wall_ms", C.c_double),
+        ("compute_ms", C.c_double),
+        ("load_ms", C.c_double),
+        ("load_wait_ms", C.c_double),
+    ]
+
+
+STREAM_CHUNK_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                              C.POINTER(C.c_int32), C.c_uint64)
+FORM_PLAIN, FORM_BATCHED = 0, 1
+
 _lib = None
 
 
@@ -214,6 +238,7 @@ def lib() -> C.CDLL:
             "sigmo_serve_schedule": ([u32, vp, vp, u32, C.POINTER(ServeOptions), SERVE_EXECUTOR, vp, C.POINTER(ServeResult),
                                       C.POINTER(ServeSummary)], i32),
             "sigmo_serve": ([vp, u32, vp, vp, vp, u32, C.POINTER(ServeOptions), C.POINTER(ServeResult), C.POINTER(ServeSummary)], i32),
+            "sigmo_stream_run": ([vp, vp, vp, C.POINTER(StreamOptions), STREAM_CHUNK_FN, vp, C.POINTER(StreamResult)], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -760,3 +785,40 @@ def serve_schedule(num_workers: int, request_graph_offsets, arrival_s, executor,
     opt = _serve_options(**options)
     check(lib().sigmo_serve_schedule(num_workers, _ptr(rgo), _ptr(arr), n, C.byref(opt), cb, None, res, C.byref(summ)))
     return _serve_out(res[:n], summ)
+
+
+# ---- streaming ----------------------------------------------------------------------------------
+
+def stream_run(ctx: Context, query: DeviceGraph, data: HostGraphs, chunk_molecules: int = 0, form: str = "batched",
+               refinement_steps: int = 6, find_first: bool = True, overlap: bool = True, collect: bool = False) -> dict:
+    """Run a host-resident molecule shard in chunks (sigmo_stream_run).  With collect, also
+    returns the whole shard's GMCR (offsets over all molecules, indices) and per-pair counts,
+    assembled from the chunks."""
+    opt = StreamOptions(chunk_molecules, refinement_steps, int(find_first), FORM_PLAIN if form == "plain" else FORM_BATCHED,
+                        int(overlap))
+    parts = []
+
+    def _chunk(_user, chunk, first, n, off, idx, cnt, P):
+        try:
+            parts.append((first, np.ctypeslib.as_array(off, (n + 1,)).copy(),
+                          np.ctypeslib.as_array(idx, (P,)).copy() if P else np.zeros(0, np.uint32),
+                          np.ctypeslib.as_array(cnt, (P,)).copy() if P else np.zeros(0, np.int32)))
+            return 0
+        except Exception:  # noqa: BLE001 -- reported as a failed chunk
+            return 1
+
+    cb = STREAM_CHUNK_FN(_chunk) if collect else STREAM_CHUNK_FN()
+    r = StreamResult()
+    check(lib().sigmo_stream_run(ctx.handle, query.handle, data.handle, C.byref(opt), cb, None, C.byref(r)))
+    out = {name: getattr(r, name) for name, _ in StreamResult._fields_}
+    if collect:
+        offs, idxs, cnts, base = [np.zeros(1, np.uint32)], [], [], 0
+        for first, off, idx, cnt in sorted(parts, key=lambda p: p[0]):
+            offs.append(off[1:] + np.uint32(base))
+            idxs.append(idx)
+            cnts.append(cnt)
+            base += int(off[-1])
+        out["offsets"] = np.concatenate(offs).astype(np.uint32)
+        out["indices"] = np.concatenate(idxs).astype(np.uint32) if idxs else np.zeros(0, np.uint32)
+        out["counts"] = np.concatenate(cnts).astype(np.int32) if cnts else np.zeros(0, np.int32)
+    return out

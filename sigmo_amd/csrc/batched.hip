@@ -868,6 +868,7 @@ struct sigmo_db {
   uint64_t *planes64 = nullptr, *planes_hi = nullptr, *lab_hi = nullptr, *dmax = nullptr;  // dmax in width-class order
   sigmo_k::MolInfo* minfo = nullptr;                                                       // width-class order
   uint64_t bytes = 0;
+  uint64_t cap_G = 0, cap_32 = 0, cap_64 = 0, cap_hi = 0, cap_dmax = 0, cap_lab = 0;  // allocated elements
   ~sigmo_db() {
     bfree(cls_list);
     bfree(pidx);
@@ -938,16 +939,14 @@ struct sigmo_batch {
 
 sigmo_ctx* sigmo_internal::batch_ctx(const sigmo_batch* b) { return b->ctx; }
 
-extern "C" {
-
-int sigmo_db_prepare(sigmo_ctx* ctx, const sigmo_graph* data, uint32_t refinement_steps, sigmo_db** out) {
+namespace {
+// (Re)build the database's class lists and level planes for `data`; buffers are reused when
+// the new graph fits them (a streaming chunk), grown otherwise.
+int db_fill(sigmo_db* db, const sigmo_graph* data) {
   int device = 0, cus = 0;
   hipStream_t s;
   unsigned long long *st, *sh;
-  BATCH_TRY(ctx_info(ctx, &device, &s, &cus, &st, &sh));
-  if (!data || !out) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
-  if (refinement_steps == 0 || refinement_steps > sigmo_k::kMaxLevels)
-    return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "the batched form supports 1..8 refinement levels");
+  BATCH_TRY(ctx_info(db->ctx, &device, &s, &cus, &st, &sh));
   const auto& go = graph_host_offsets(data);
   const uint32_t G = static_cast<uint32_t>(go.size() - 1);
   std::vector<uint32_t> widx(G, sigmo_k::kNoWide), pidx(G, 0), lists[3];
@@ -962,38 +961,57 @@ int sigmo_db_prepare(sigmo_ctx* ctx, const sigmo_graph* data, uint32_t refinemen
   std::vector<uint32_t> cls;
   cls.reserve(G);
   for (int c = 0; c < 3; ++c) cls.insert(cls.end(), lists[c].begin(), lists[c].end());
-  auto* db = new sigmo_db();
-  db->ctx = ctx;
+  const uint64_t R = db->R;
+  uint64_t cap_g = db->cap_G;
+  BATCH_TRY(bgrow(&db->cls_list, cap_g, G));
+  cap_g = db->cap_G;
+  BATCH_TRY(bgrow(&db->pidx, cap_g, G));
+  cap_g = db->cap_G;
+  BATCH_TRY(bgrow(&db->wide_idx, cap_g, G));
+  cap_g = db->cap_G;
+  BATCH_TRY(bgrow(&db->minfo, cap_g, G));
+  db->cap_G = cap_g;
+  BATCH_TRY(bgrow(&db->dmax, db->cap_dmax, uint64_t(G) * R));
+  BATCH_TRY(bgrow(&db->planes32, db->cap_32, uint64_t(n32) * R * 64));
+  BATCH_TRY(bgrow(&db->planes64, db->cap_64, uint64_t(n64) * R * 64));
+  BATCH_TRY(bgrow(&db->planes_hi, db->cap_hi, uint64_t(nw) * R * 64));
+  BATCH_TRY(bgrow(&db->lab_hi, db->cap_lab, uint64_t(nw) * 16));
   db->data = data;
-  db->R = refinement_steps;
   db->num_wide = nw;
   for (int c = 0; c < 3; ++c) db->cls_count[c] = static_cast<uint32_t>(lists[c].size());
-  const uint64_t R = refinement_steps;
-  int stt = balloc(&db->cls_list, G);
-  if (stt == SIGMO_OK) stt = balloc(&db->pidx, G);
-  if (stt == SIGMO_OK) stt = balloc(&db->wide_idx, G);
-  if (stt == SIGMO_OK) stt = balloc(&db->planes32, uint64_t(n32) * R * 64);
-  if (stt == SIGMO_OK) stt = balloc(&db->planes64, uint64_t(n64) * R * 64);
-  if (stt == SIGMO_OK) stt = balloc(&db->planes_hi, uint64_t(nw) * R * 64);
-  if (stt == SIGMO_OK) stt = balloc(&db->lab_hi, uint64_t(nw) * 16);
-  if (stt == SIGMO_OK) stt = balloc(&db->dmax, uint64_t(G) * R);
-  if (stt == SIGMO_OK) stt = balloc(&db->minfo, G);
-  if (stt != SIGMO_OK) {
-    delete db;
-    return stt;
-  }
   db->bytes = G * (12ull + sizeof(sigmo_k::MolInfo)) + (uint64_t(n32) * 256 + (uint64_t(n64) + nw) * 512) * R + nw * 128ull +
               uint64_t(G) * R * 8;
-  hipError_t e = hipMemcpyAsync(db->wide_idx, widx.data(), G * 4ull, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(db->pidx, pidx.data(), G * 4ull, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(db->cls_list, cls.data(), G * 4ull, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e != hipSuccess) {
-    delete db;
-    return sigmo_err::set(SIGMO_ERR_HIP, std::string("db prepare: ") + hipGetErrorString(e));
-  }
-  stt = sigmo_db_refresh(db);
-  if (stt == SIGMO_OK && hipStreamSynchronize(s) != hipSuccess) stt = sigmo_err::set(SIGMO_ERR_HIP, "db prepare failed");
+  BATCH_HIP_TRY(hipMemcpyAsync(db->wide_idx, widx.data(), G * 4ull, hipMemcpyHostToDevice, s));
+  BATCH_HIP_TRY(hipMemcpyAsync(db->pidx, pidx.data(), G * 4ull, hipMemcpyHostToDevice, s));
+  BATCH_HIP_TRY(hipMemcpyAsync(db->cls_list, cls.data(), G * 4ull, hipMemcpyHostToDevice, s));
+  BATCH_HIP_TRY(hipStreamSynchronize(s));  // the host vectors go out of scope
+  BATCH_TRY(sigmo_db_refresh(db));
+  BATCH_HIP_TRY(hipStreamSynchronize(s));
+  return SIGMO_OK;
+}
+}  // namespace
+
+namespace sigmo_internal {
+int db_rebind(sigmo_db* db, const sigmo_graph* data) {
+  if (!db || !data) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  return db_fill(db, data);
+}
+}  // namespace sigmo_internal
+
+extern "C" {
+
+int sigmo_db_prepare(sigmo_ctx* ctx, const sigmo_graph* data, uint32_t refinement_steps, sigmo_db** out) {
+  int device = 0, cus = 0;
+  hipStream_t s;
+  unsigned long long *st, *sh;
+  BATCH_TRY(ctx_info(ctx, &device, &s, &cus, &st, &sh));
+  if (!data || !out) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  if (refinement_steps == 0 || refinement_steps > sigmo_k::kMaxLevels)
+    return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "the batched form supports 1..8 refinement levels");
+  auto* db = new sigmo_db();
+  db->ctx = ctx;
+  db->R = refinement_steps;
+  const int stt = db_fill(db, data);
   if (stt != SIGMO_OK) {
     delete db;
     return stt;
@@ -1234,7 +1252,9 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
     ctx_add_bytes(b->ctx, SIGMO_KERNEL_PAIR_FILTER, double(cnt[1]) * 8.0 + double(Gd) * W * 8.0);
     if (!cnt[3]) break;
     if (attempt == 2) return sigmo_err::set(SIGMO_ERR_NOMEM, "candidate arena overflowed repeatedly");
-    want_arena = std::max<uint64_t>(want_arena * 2, cnt[1] + slack_arena);  // replay with the exact demand
+    // replay with the demand just measured; the claim order (and so the space left unused at
+    // sub-chunk ends) varies between runs, hence the 1/8 margin
+    want_arena = std::max<uint64_t>(want_arena + 1, cnt[1] + cnt[1] / 8 + slack_arena);
   }
   // canonical GMCR: count survivors per molecule, scan, scatter in (molecule, query graph) order
   void* tok = nullptr;

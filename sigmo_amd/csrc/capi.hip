@@ -200,6 +200,7 @@ struct RefineState {
 struct sigmo_signatures {
   sigmo_ctx* ctx = nullptr;
   uint64_t D = 0, Q = 0;
+  uint64_t capD = 0;  // data-side capacity in nodes (a rebind to a smaller data graph reuses it)
   uint64_t *dsig = nullptr, *qsig = nullptr, *dreach = nullptr, *qreach = nullptr, *dplanes = nullptr;
   uint64_t *dfront = nullptr, *qfront = nullptr;
   RefineState dstate, qstate;
@@ -217,6 +218,7 @@ struct sigmo_signatures {
 struct sigmo_candidates {
   sigmo_ctx* ctx = nullptr;
   uint64_t Q = 0, D = 0, W64 = 0;
+  uint64_t cap_words = 0;  // allocated words (rows x W64 of the largest target so far)
   uint64_t* bits = nullptr;
   ~sigmo_candidates() { dev_free(bits); }
 };
@@ -427,16 +429,35 @@ int sigmo_device_info(sigmo_ctx* ctx, char* name, size_t name_len, uint64_t* glo
 
 namespace {
 
-// Fill `g` from `host`: validate, grow the device buffers where the new batch exceeds them,
-// copy the CSR arrays and derive the adjacency words and label planes.
-int graph_fill(sigmo_graph* g, const sigmo_host_graphs* host) {
-  const uint32_t G = host->num_graphs();
-  const uint64_t N = host->total_nodes(), E = host->total_edges();
+// A run of consecutive graphs of a host batch: its arrays are sub-ranges of the batch's, with
+// ids still global to the batch (node ids from node_base, entry offsets from col_base).
+struct HostView {
+  const uint32_t* go;  // G + 1 entries
+  uint32_t G;
+  const uint32_t* ro;  // N + 1 entries
+  uint64_t N;
+  const uint32_t* ci;  // E entries
+  uint64_t E;
+  const uint8_t* nl;
+  const uint8_t* el;
+  uint32_t node_base, col_base;
+};
+
+HostView whole(const sigmo_host_graphs* h) {
+  return HostView{h->graph_offsets.data(), h->num_graphs(), h->row_offsets.data(), h->total_nodes(), h->column_indices.data(),
+                  h->total_edges(), h->node_labels.data(), h->edge_labels.data(), 0, 0};
+}
+
+// Fill `g` from a host view: validate, grow the device buffers where the new batch exceeds them,
+// copy the CSR arrays (rebasing ids on the device), derive the adjacency words and label planes.
+int graph_fill(sigmo_graph* g, const HostView& h, uint32_t reserve_G = 0, uint64_t reserve_N = 0, uint64_t reserve_E = 0) {
+  const uint32_t G = h.G;
+  const uint64_t N = h.N, E = h.E;
   if (G == 0) return sigmo_err::set(SIGMO_ERR_INVALID, "empty graph batch");
   if (N >= (1ull << 32) || E >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "batch exceeds u32 node/edge ids");
   uint32_t mx = 0, mn = UINT32_MAX;
   for (uint32_t i = 0; i < G; ++i) {
-    const uint32_t n = host->graph_offsets[i + 1] - host->graph_offsets[i];
+    const uint32_t n = h.go[i + 1] - h.go[i];
     mx = std::max(mx, n);
     mn = std::min(mn, n);
   }
@@ -448,10 +469,11 @@ int graph_fill(sigmo_graph* g, const sigmo_host_graphs* host) {
     dev_free(g->go);
     dev_free(g->lmask);
     g->cap_G = 0;
-    if (st == SIGMO_OK) st = dev_alloc(&g->go, uint64_t(G) + 1);
-    if (st == SIGMO_OK) st = dev_alloc(&g->lmask, 16 * uint64_t(G));
+    const uint32_t cG = std::max(G, reserve_G);
+    if (st == SIGMO_OK) st = dev_alloc(&g->go, uint64_t(cG) + 1);
+    if (st == SIGMO_OK) st = dev_alloc(&g->lmask, 16 * uint64_t(cG));
     if (st != SIGMO_OK) return st;
-    g->cap_G = G;
+    g->cap_G = cG;
   }
   if (N > g->cap_N) {
     dev_free(g->ro);
@@ -460,42 +482,47 @@ int graph_fill(sigmo_graph* g, const sigmo_host_graphs* host) {
     dev_free(g->adj);
     dev_free(g->lplanes);
     g->cap_N = 0;
-    const uint64_t W64 = (N + 63) / 64;
-    if (st == SIGMO_OK) st = dev_alloc(&g->ro, N + 1);
-    if (st == SIGMO_OK) st = dev_alloc(&g->nl, N);
-    if (st == SIGMO_OK) st = dev_alloc(&g->node_graph, N);
-    if (st == SIGMO_OK) st = dev_alloc(&g->adj, 2 * N);
+    const uint64_t cN = std::max(N, reserve_N), W64 = (cN + 63) / 64;
+    if (st == SIGMO_OK) st = dev_alloc(&g->ro, cN + 1);
+    if (st == SIGMO_OK) st = dev_alloc(&g->nl, cN);
+    if (st == SIGMO_OK) st = dev_alloc(&g->node_graph, cN);
+    if (st == SIGMO_OK) st = dev_alloc(&g->adj, 2 * cN);
     if (st == SIGMO_OK) st = dev_alloc(&g->lplanes, 8 * W64);
     if (st != SIGMO_OK) return st;
-    g->cap_N = N;
+    g->cap_N = cN;
   }
   if (E > g->cap_E || !g->ci) {
     dev_free(g->ci);
     dev_free(g->el);
     g->cap_E = 0;
-    if (st == SIGMO_OK) st = dev_alloc(&g->ci, E);
-    if (st == SIGMO_OK) st = dev_alloc(&g->el, E);
+    const uint64_t cE = std::max(E, reserve_E);
+    if (st == SIGMO_OK) st = dev_alloc(&g->ci, cE);
+    if (st == SIGMO_OK) st = dev_alloc(&g->el, cE);
     if (st != SIGMO_OK) return st;
-    g->cap_E = E;
+    g->cap_E = cE;
   }
   g->G = G;
   g->N = N;
   g->E = E;
   g->max_nodes = mx;
   g->min_nodes = mn;
-  g->host_go = host->graph_offsets;
+  g->host_go.resize(uint64_t(G) + 1);
+  for (uint32_t i = 0; i <= G; ++i) g->host_go[i] = h.go[i] - h.node_base;
   bool present[16] = {false};
-  for (uint8_t l : host->node_labels)
-    if (l < 16) present[l] = true;
+  for (uint64_t v = 0; v < N; ++v)
+    if (h.nl[v] < 16) present[h.nl[v]] = true;
   g->slots.count = 0;
   for (int l = 0; l < 16; ++l)
     if (present[l]) g->slots.label[g->slots.count++] = static_cast<uint8_t>(l);
   hipStream_t s = g->ctx->stream;
-  hipError_t e = hipMemcpyAsync(g->go, host->graph_offsets.data(), (uint64_t(G) + 1) * 4, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(g->ro, host->row_offsets.data(), (N + 1) * 4, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess && E) e = hipMemcpyAsync(g->ci, host->column_indices.data(), E * 4, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(g->nl, host->node_labels.data(), N, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess && E) e = hipMemcpyAsync(g->el, host->edge_labels.data(), E, hipMemcpyHostToDevice, s);
+  hipError_t e = hipMemcpyAsync(g->go, h.go, (uint64_t(G) + 1) * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(g->ro, h.ro, (N + 1) * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && E) e = hipMemcpyAsync(g->ci, h.ci, E * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(g->nl, h.nl, N, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && E) e = hipMemcpyAsync(g->el, h.el, E, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = sigmo_k::sub_u32(s, g->go, uint64_t(G) + 1, h.node_base);
+  if (e == hipSuccess) e = sigmo_k::sub_u32(s, g->ci, E, h.node_base);
+  if (e == hipSuccess) e = sigmo_k::sub_u32(s, g->ro, N + 1, h.col_base);
   if (e == hipSuccess) e = sigmo_k::derive_graph(s, g->go, G, g->ro, g->ci, g->nl, N, g->node_graph, g->adj, g->lplanes, g->lmask);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return sigmo_err::set(SIGMO_ERR_HIP, std::string("graph upload: ") + hipGetErrorString(e));
@@ -511,7 +538,34 @@ int graph_reupload(sigmo_ctx* ctx, const sigmo_host_graphs* host, sigmo_graph** 
   if (!*g) return sigmo_graph_upload(ctx, host, g);
   SIGMO_TRY(bind(ctx));
   if ((*g)->ctx != ctx) return sigmo_err::set(SIGMO_ERR_INVALID, "graph belongs to another context");
-  return graph_fill(*g, host);
+  return graph_fill(*g, whole(host));
+}
+// Upload graphs [first, first + count) of `host` into *g (created on first use), reusing its
+// buffers: the sub-arrays go up as they are and their ids are rebased on the device.  Buffers
+// that must grow are sized for at least reserve_{G,N,E} (the largest chunk a caller will load),
+// so a stream of chunks allocates once.
+int graph_upload_range(sigmo_ctx* ctx, const sigmo_host_graphs* host, uint32_t first, uint32_t count, uint32_t reserve_G, uint64_t reserve_N,
+                       uint64_t reserve_E, sigmo_graph** g) {
+  SIGMO_TRY(bind(ctx));
+  if (!host || !g) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  if (count == 0 || uint64_t(first) + count > host->num_graphs()) return sigmo_err::set(SIGMO_ERR_INVALID, "graph range out of bounds");
+  if (*g && (*g)->ctx != ctx) return sigmo_err::set(SIGMO_ERR_INVALID, "graph belongs to another context");
+  const uint32_t node_lo = host->graph_offsets[first], node_hi = host->graph_offsets[first + count];
+  const uint32_t col_lo = host->row_offsets[node_lo], col_hi = host->row_offsets[node_hi];
+  const HostView v{host->graph_offsets.data() + first, count, host->row_offsets.data() + node_lo, uint64_t(node_hi - node_lo),
+                   host->column_indices.data() + col_lo, uint64_t(col_hi - col_lo), host->node_labels.data() + node_lo,
+                   host->edge_labels.data() + col_lo, node_lo, col_lo};
+  const bool fresh = !*g;
+  if (fresh) {
+    *g = new sigmo_graph();
+    (*g)->ctx = ctx;
+  }
+  const int st = graph_fill(*g, v, reserve_G, reserve_N, reserve_E);
+  if (st != SIGMO_OK && fresh) {
+    delete *g;
+    *g = nullptr;
+  }
+  return st;
 }
 }  // namespace sigmo_internal
 
@@ -522,7 +576,7 @@ int sigmo_graph_upload(sigmo_ctx* ctx, const sigmo_host_graphs* host, sigmo_grap
   if (!host || !out) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
   auto* g = new sigmo_graph();
   g->ctx = ctx;
-  const int st = graph_fill(g, host);
+  const int st = graph_fill(g, whole(host));
   if (st != SIGMO_OK) {
     delete g;
     return st;
@@ -562,6 +616,7 @@ int sigmo_signatures_create(sigmo_ctx* ctx, uint64_t data_nodes, uint64_t query_
   auto* s = new sigmo_signatures();
   s->ctx = ctx;
   s->D = data_nodes;
+  s->capD = data_nodes;
   s->Q = query_nodes;
   const uint64_t W64 = (data_nodes + 63) / 64;
   int st = dev_alloc(&s->dsig, data_nodes);
@@ -676,6 +731,7 @@ int sigmo_candidates_create(sigmo_ctx* ctx, uint64_t source_nodes, uint64_t targ
   c->Q = source_nodes;
   c->D = target_nodes;
   c->W64 = (target_nodes + 63) / 64;
+  c->cap_words = c->Q * c->W64;
   int st = dev_alloc(&c->bits, c->Q * c->W64);
   if (st == SIGMO_OK) {
     hipError_t e = sigmo_k::fill_u64(ctx->stream, c->bits, c->Q * c->W64, 0);  // candidates.hpp:169-178
@@ -984,6 +1040,54 @@ int sigmo_pipeline_pairs(sigmo_pipeline* p, uint32_t* data_graph_offsets, uint64
   }
   return SIGMO_OK;
 }
+
+}  // extern "C"
+
+namespace sigmo_internal {
+// Point a pipeline at another data graph over the same queries (a streaming chunk): the
+// data-side signature and candidate buffers are reused when the new graph fits them, and grown
+// to at least reserve_N data nodes otherwise.  With *p null the pipeline is created that way.
+int pipeline_rebind(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data, uint64_t reserve_N, sigmo_pipeline** pp) {
+  if (!pp || !query || !data) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  SIGMO_TRY(bind(ctx));
+  if (!*pp) {
+    SIGMO_TRY(check_queries(query));
+    auto* np = new sigmo_pipeline();
+    np->ctx = ctx;
+    np->q = query;
+    np->d = data;
+    int st = sigmo_gmcr_create(ctx, &np->gm);
+    if (st != SIGMO_OK) {
+      delete np;
+      return st;
+    }
+    *pp = np;
+  }
+  sigmo_pipeline* p = *pp;
+  if (p->ctx != ctx || p->q != query) return sigmo_err::set(SIGMO_ERR_INVALID, "pipeline bound to other queries or context");
+  const uint64_t N = data->N, W64 = (N + 63) / 64, cN = std::max(N, reserve_N);
+  if (!p->sig || N > p->sig->capD) {
+    sigmo_signatures_free(p->sig);
+    p->sig = nullptr;
+    SIGMO_TRY(sigmo_signatures_create(ctx, cN, query->N, &p->sig));
+  }
+  p->sig->D = N;
+  SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, p->sig->dplanes, 64 * W64, 0));
+  if (!p->cand || query->N * W64 > p->cand->cap_words) {
+    sigmo_candidates_free(p->cand);
+    p->cand = nullptr;
+    SIGMO_TRY(sigmo_candidates_create(ctx, query->N, cN, &p->cand));
+  }
+  p->cand->D = N;
+  p->cand->W64 = W64;
+  p->d = data;
+  p->has_counts = false;
+  return SIGMO_OK;
+}
+int ctx_device(const sigmo_ctx* ctx) { return ctx->device; }
+}  // namespace sigmo_internal
+
+extern "C" {
 
 int sigmo_pipeline_free(sigmo_pipeline* p) {
   if (!p) return SIGMO_OK;

@@ -36,6 +36,10 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
   for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
     p[i] = v;
 }
+__global__ void k_sub_u32(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] -= v;
+}
 
 inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 20) {
   uint64_t g = (n + block - 1) / block;
@@ -548,6 +552,11 @@ hipError_t fill_u64(hipStream_t s, uint64_t* p, uint64_t n, uint64_t v) {
 hipError_t fill_u32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v) {
   if (!n) return hipSuccess;
   k_fill_u32<<<grid_for(n, 256, 16384), 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+hipError_t sub_u32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v) {
+  if (!n || !v) return hipSuccess;
+  k_sub_u32<<<grid_for(n, 256, 16384), 256, 0, s>>>(p, n, v);
   return hipGetLastError();
 }
 

@@ -40,6 +40,8 @@ struct LabelSlots {
 
 hipError_t fill_u64(hipStream_t s, uint64_t* p, uint64_t n, uint64_t v);
 hipError_t fill_u32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v);
+// p[i] -= v (rebasing ids of a sub-range of a host batch)
+hipError_t sub_u32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v);
 
 // Derived per-node / per-graph arrays of an uploaded batch.
 hipError_t derive_graph(hipStream_t s, const uint32_t* go, uint32_t G, const uint32_t* ro, const uint32_t* ci, const uint8_t* nl,

@@ -1,0 +1,202 @@
+// Streaming: a molecule set larger than one device pass, run in chunks of molecules.
+//
+// The reference's multi-node run gives each MPI rank a byte range of the data file and runs
+// the whole driver on it (library/src/sigmo_mpi.cpp:19-80, 110-159); here one GPU takes the
+// role of a rank whose shard does not fit the candidate memory in one pass.  The shard stays in
+// host memory; chunk i+1 is uploaded (its ids rebased on the device) by a loader thread on its
+// own stream (its own context) while chunk i runs through the plain per-phase pipeline or the
+// batched form on the caller's stream.  Two device graph buffers alternate; a buffer is reloaded only after the
+// chunk that used it has returned its results.  Every chunk's GMCR and counts are those of a
+// single run over that chunk (molecule ids chunk-local), so the concatenation over chunks equals
+// a single run over the whole shard.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "host_graphs.hpp"
+
+namespace sigmo_internal {
+int graph_upload_range(sigmo_ctx* ctx, const sigmo_host_graphs* host, uint32_t first, uint32_t count, uint32_t reserve_G, uint64_t reserve_N,
+                       uint64_t reserve_E, sigmo_graph** g);
+int pipeline_rebind(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data, uint64_t reserve_N, sigmo_pipeline** p);
+int db_rebind(sigmo_db* db, const sigmo_graph* data);
+int ctx_device(const sigmo_ctx* ctx);
+}  // namespace sigmo_internal
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t) { return std::chrono::duration<double, std::milli>(Clock::now() - t).count(); }
+
+struct Loader {
+  sigmo_ctx* ctx = nullptr;  // the loader's own stream
+  const sigmo_host_graphs* data = nullptr;
+  uint32_t chunk = 0, chunks = 0, G = 0;
+  uint64_t max_N = 0, max_E = 0;  // the largest chunk: buffers are sized for it once
+  sigmo_graph* slot[2] = {nullptr, nullptr};
+  int loaded[2] = {-1, -1};  // chunk held by each slot, -1 if free
+  std::mutex mu;
+  std::condition_variable cv;
+  bool stop = false;
+  int err = SIGMO_OK;
+  std::string msg;
+  double load_ms = 0;
+
+  uint32_t first(uint32_t i) const { return i * chunk; }
+  uint32_t count(uint32_t i) const { return std::min<uint32_t>(chunk, G - first(i)); }
+
+  // Upload chunk i of the host shard into `slot[i % 2]` (ids rebased on the device).
+  int load(uint32_t i) {
+    const auto t = Clock::now();
+    const int st = sigmo_internal::graph_upload_range(ctx, data, first(i), count(i), chunk, max_N, max_E, &slot[i % 2]);
+    load_ms += ms_since(t);
+    return st;
+  }
+
+  void run() {
+    for (uint32_t i = 0; i < chunks; ++i) {
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return stop || loaded[i % 2] < 0; });
+        if (stop) return;
+      }
+      const int st = load(i);
+      std::lock_guard<std::mutex> lk(mu);
+      if (st != SIGMO_OK) {
+        err = st;
+        msg = sigmo_last_error();
+        stop = true;
+      } else {
+        loaded[i % 2] = static_cast<int>(i);
+      }
+      cv.notify_all();
+      if (stop) return;
+    }
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int sigmo_stream_run(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_host_graphs* data, const sigmo_stream_options* opt,
+                     sigmo_stream_chunk_fn on_chunk, void* user, sigmo_stream_result* res) {
+  if (!ctx || !query || !data || !opt || !res) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  if (opt->form != SIGMO_FORM_PLAIN && opt->form != SIGMO_FORM_BATCHED) return sigmo_err::set(SIGMO_ERR_INVALID, "unknown form");
+  std::memset(res, 0, sizeof(*res));
+  const uint32_t G = data->num_graphs();
+  if (G == 0) return sigmo_err::set(SIGMO_ERR_INVALID, "empty molecule shard");
+  Loader L;
+  L.data = data;
+  L.G = G;
+  L.chunk = opt->chunk_molecules ? std::min(opt->chunk_molecules, G) : G;
+  L.chunks = (G + L.chunk - 1) / L.chunk;
+  for (uint32_t i = 0; i < L.chunks; ++i) {
+    const uint32_t lo = data->graph_offsets[L.first(i)], hi = data->graph_offsets[L.first(i) + L.count(i)];
+    L.max_N = std::max<uint64_t>(L.max_N, hi - lo);
+    L.max_E = std::max<uint64_t>(L.max_E, data->row_offsets[hi] - data->row_offsets[lo]);
+  }
+  int st = sigmo_ctx_create(sigmo_internal::ctx_device(ctx), &L.ctx);
+  if (st != SIGMO_OK) return st;
+
+  sigmo_pipeline* pipe = nullptr;
+  sigmo_db* db = nullptr;
+  sigmo_batch* batch = nullptr;
+  std::vector<uint32_t> offsets, indices;
+  std::vector<int32_t> counts;
+  const auto t0 = Clock::now();
+  std::thread loader;
+  if (opt->overlap) loader = std::thread([&L] { L.run(); });
+
+  for (uint32_t i = 0; i < L.chunks && st == SIGMO_OK; ++i) {
+    const int s = static_cast<int>(i % 2);
+    if (opt->overlap) {
+      const auto tw = Clock::now();
+      std::unique_lock<std::mutex> lk(L.mu);
+      L.cv.wait(lk, [&] { return L.stop || L.loaded[s] == static_cast<int>(i); });
+      res->load_wait_ms += ms_since(tw);
+      if (L.loaded[s] != static_cast<int>(i)) {
+        st = sigmo_err::set(L.err != SIGMO_OK ? L.err : SIGMO_ERR_INVALID, "chunk load: " + L.msg);
+        break;
+      }
+    } else {
+      const auto tw = Clock::now();
+      st = L.load(i);
+      res->load_wait_ms += ms_since(tw);
+      if (st != SIGMO_OK) break;
+    }
+    const sigmo_graph* g = L.slot[s];
+    const auto tc = Clock::now();
+    uint64_t P = 0;
+    if (opt->form == SIGMO_FORM_PLAIN) {
+      st = sigmo_internal::pipeline_rebind(ctx, query, g, L.max_N, &pipe);
+      sigmo_pipeline_result pr;
+      if (st == SIGMO_OK) st = sigmo_pipeline_execute(pipe, opt->refinement_steps, opt->find_first, &pr);
+      if (st == SIGMO_OK) {
+        res->num_matches += pr.num_matches;
+        res->matched_pairs += pr.matched_pairs;
+        res->gmcr_pairs += pr.gmcr_pairs;
+        res->dfs_steps += pr.dfs_steps;
+        P = pr.gmcr_pairs;
+      }
+    } else {
+      if (!db) {
+        st = sigmo_db_prepare(ctx, g, opt->refinement_steps, &db);
+        if (st == SIGMO_OK) st = sigmo_batch_create(ctx, db, &batch);
+      } else {
+        st = sigmo_internal::db_rebind(db, g);
+      }
+      sigmo_batch_result br;
+      if (st == SIGMO_OK) st = sigmo_batch_run(batch, query, opt->find_first, &br);
+      if (st == SIGMO_OK) {
+        res->num_matches += br.num_matches;
+        res->matched_pairs += br.matched_pairs;
+        res->gmcr_pairs += br.gmcr_pairs;
+        res->dfs_steps += br.dfs_steps;
+        P = br.gmcr_pairs;
+      }
+    }
+    if (st == SIGMO_OK && on_chunk) {
+      const uint32_t n = L.count(i);
+      offsets.resize(uint64_t(n) + 1);
+      indices.resize(P);
+      counts.resize(P);
+      st = opt->form == SIGMO_FORM_PLAIN
+               ? sigmo_pipeline_pairs(pipe, offsets.data(), offsets.size(), indices.data(), counts.data(), P)
+               : sigmo_batch_pairs(batch, offsets.data(), offsets.size(), indices.data(), counts.data(), P);
+      if (st == SIGMO_OK && on_chunk(user, i, L.first(i), n, offsets.data(), indices.data(), counts.data(), P) != 0)
+        st = sigmo_err::set(SIGMO_ERR_INVALID, "chunk callback failed on chunk " + std::to_string(i));
+    }
+    res->compute_ms += ms_since(tc);
+    res->chunks = i + 1;
+    std::lock_guard<std::mutex> lk(L.mu);
+    L.loaded[s] = -1;  // results are on the host: the loader may refill this buffer
+    L.cv.notify_all();
+  }
+  std::string err_msg = st == SIGMO_OK ? std::string() : std::string(sigmo_last_error());
+  if (opt->overlap) {
+    {
+      std::lock_guard<std::mutex> lk(L.mu);
+      L.stop = true;
+      L.cv.notify_all();
+    }
+    loader.join();
+  }
+  res->wall_ms = ms_since(t0);
+  res->load_ms = L.load_ms;
+  sigmo_batch_free(batch);
+  sigmo_db_free(db);
+  sigmo_pipeline_free(pipe);
+  sigmo_graph_free(L.slot[0]);
+  sigmo_graph_free(L.slot[1]);
+  sigmo_ctx_destroy(L.ctx);
+  if (st != SIGMO_OK) return sigmo_err::set(st, err_msg);
+  return SIGMO_OK;
+}
+
+}  // extern "C"

@@ -17,6 +17,7 @@ def pytest_configure(config):
 def ctx():
     from sigmo_amd import capi
 
-    c = capi.Context(0)
-    yield c
-    c.close()
+    # No explicit close: device objects made in module fixtures hold a reference to the
+    # context and may be collected after session teardown; the context is destroyed when the
+    # last of them is.
+    yield capi.Context(0)
