@@ -25,7 +25,6 @@
 #include "../../include/sigmo_hip.h"
 #include "device_common.hpp"
 #include "host_graphs.hpp"
-#include "join_dfs.hpp"
 #include "join_lanes.hpp"
 #include "kernels.hpp"
 
@@ -643,95 +642,6 @@ __global__ __launch_bounds__(256) void k_surv_scatter(uint32_t Gd, uint32_t Gq, 
     }
 }
 
-// Join over the canonical pairs with candidate masks from the arena (the molecule-local masks
-// the plain form extracts from the bitset rows).
-__device__ __forceinline__ void join_load_arena(const GraphDev& q, const GraphDev& d, const uint64_t* __restrict__ arena,
-                                                const uint32_t* __restrict__ mask_off, const uint32_t* __restrict__ indices,
-                                                const uint32_t* __restrict__ pair_graph, uint64_t j, const JoinPair* prev, JoinPair& o) {
-  const uint32_t lane = static_cast<uint32_t>(lane_id());
-  const uint32_t dg = pair_graph[j], qg = indices[j];
-  o.dg = dg;
-  o.s = d.go[dg];
-  o.e = d.go[dg + 1];
-  o.n = o.e - o.s;
-  o.qoff = q.go[qg];
-  o.k = q.go[qg + 1] - o.qoff;
-  if (prev && prev->dg == dg) {
-    o.alo = prev->alo;
-    o.ahi = prev->ahi;
-    o.blo = prev->blo;
-    o.bhi = prev->bhi;
-  } else {
-    o.alo = o.ahi = o.blo = o.bhi = 0;
-    if (lane < o.n) {
-      o.alo = d.adj[2 * (static_cast<uint64_t>(o.s) + lane)];
-      o.ahi = d.adj[2 * (static_cast<uint64_t>(o.s) + lane) + 1];
-    }
-    if (lane + 64 < o.n) {
-      o.blo = d.adj[2 * (static_cast<uint64_t>(o.s) + lane + 64)];
-      o.bhi = d.adj[2 * (static_cast<uint64_t>(o.s) + lane + 64) + 1];
-    }
-  }
-  o.clo = o.chi = 0;
-  o.qa = 0;
-  if (lane < o.k) {
-    const uint64_t base = mask_off[j];
-    if (o.n > 64) {
-      o.clo = arena[base + 2 * lane];
-      o.chi = arena[base + 2 * lane + 1];
-    } else {
-      o.clo = arena[base + lane];
-    }
-    o.qa = static_cast<uint32_t>(q.adj[2 * static_cast<uint64_t>(o.qoff + lane)]);
-  }
-}
-
-template <bool FIND_FIRST>
-__global__ __launch_bounds__(256) void k_join_arena(GraphDev q, GraphDev d, const uint64_t* __restrict__ arena,
-                                                    const uint32_t* __restrict__ mask_off, const uint32_t* __restrict__ indices,
-                                                    const uint32_t* __restrict__ pair_graph, uint64_t P, uint32_t chunk,
-                                                    int32_t* __restrict__ counts, unsigned long long* __restrict__ stats,
-                                                    const uint32_t* __restrict__ list, uint32_t cursor) {
-  // pairs 0..P-1 of `list` (all GMCR pairs when list is null); `cursor` is the stats slot used
-  // as the work counter
-  const uint32_t lane = static_cast<uint32_t>(lane_id());
-  uint64_t sum = 0, matched = 0, steps = 0, overflow = 0;
-  uint64_t next = 0, end = 0;
-  auto grab = [&]() {
-    unsigned long long jj = 0;
-    if (lane == 0) jj = atomicAdd(&stats[cursor], static_cast<unsigned long long>(chunk));
-    next = uniform64(__shfl(jj, 0));
-    end = next + chunk < P ? next + chunk : P;
-  };
-  auto pair_at = [&](uint64_t i) -> uint64_t { return list ? list[i] : i; };
-  grab();
-  if (next < P) {
-    JoinPair cur, nxt;
-    join_load_arena(q, d, arena, mask_off, indices, pair_graph, pair_at(next), nullptr, cur);
-    while (true) {
-      const uint64_t j = pair_at(next++);
-      if (next >= end) grab();
-      const bool more = next < P;
-      if (more) join_load_arena(q, d, arena, mask_off, indices, pair_graph, pair_at(next), &cur, nxt);
-      const uint64_t count = cur.n > 64 ? join_dfs<FIND_FIRST, true>(cur, steps) : join_dfs<FIND_FIRST, false>(cur, steps);
-      if (lane == 0) {
-        if (count > 0x7FFFFFFFull) overflow = 1;
-        counts[j] = static_cast<int32_t>(count > 0x7FFFFFFFull ? 0x7FFFFFFFull : count);
-      }
-      sum += count;
-      matched += count > 0;
-      if (!more) break;
-      cur = nxt;
-    }
-  }
-  if (lane == 0) {
-    if (sum) atomicAdd(&stats[0], static_cast<unsigned long long>(sum));
-    if (matched) atomicAdd(&stats[1], static_cast<unsigned long long>(matched));
-    if (overflow) atomicOr(&stats[3], 1ull);
-    atomicAdd(&stats[4], static_cast<unsigned long long>(steps));
-  }
-}
-
 // Per-request statistics of a batch run.  Request r owns the batch's query graphs
 // [rg[r], rg[r+1]); each GMCR pair (molecule m, query graph g, count c) adds its match flag,
 // c, and mix64(mix64(((g - rg[r]) << 32) | m) + c) to its request (sums mod 2^64, mix64 being
@@ -1300,16 +1210,17 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
       unsigned long long nw = 0;
       BATCH_HIP_TRY(hipMemcpyAsync(&nw, stats + 5, sizeof(nw), hipMemcpyDeviceToHost, s));
       BATCH_HIP_TRY(hipStreamSynchronize(s));
-      if (nw) {
-        const uint64_t blocks = std::min<uint64_t>((nw + 3) / 4, static_cast<uint64_t>(cus) * 8);
-        uint64_t chunk = nw / (blocks * 4 * 8);
-        chunk = chunk < 1 ? 1 : (chunk > 64 ? 64 : chunk);
+      if (nw) {  // lane per pair over the listed pairs, two-word positions
+        uint64_t blocks = 0;
+        uint32_t chunk = 0;
+        sigmo_k::join_wide_grid(nw, cus, &blocks, &chunk);
+        const sigmo_k::ArenaMasks src{b->arena, b->mask_off};
         if (find_first)
-          hipLaunchKernelGGL(sigmo_k::k_join_arena<true>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, qd, dd, b->arena, b->mask_off,
-                             b->indices, b->pair_graph, nw, static_cast<uint32_t>(chunk), b->counts, stats, b->wide_pairs, 6u);
+          hipLaunchKernelGGL((sigmo_k::k_join_lanes_wide<true, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, qd, dd,
+                             src, b->indices, b->pair_graph, b->wide_pairs, nw, chunk, 6u, b->counts, stats);
         else
-          hipLaunchKernelGGL(sigmo_k::k_join_arena<false>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, qd, dd, b->arena, b->mask_off,
-                             b->indices, b->pair_graph, nw, static_cast<uint32_t>(chunk), b->counts, stats, b->wide_pairs, 6u);
+          hipLaunchKernelGGL((sigmo_k::k_join_lanes_wide<false, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, qd, dd,
+                             src, b->indices, b->pair_graph, b->wide_pairs, nw, chunk, 6u, b->counts, stats);
         BATCH_HIP_TRY(hipGetLastError());
       }
     }

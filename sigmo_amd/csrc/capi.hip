@@ -919,13 +919,13 @@ int sigmo_join(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data
   SIGMO_TRY(grow(&mm->wide_pairs, mm->cap_wide, m->P));
   SIGMO_HIP_TRY(sigmo_k::join_lanes(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, m->P, find_first,
                                     mm->pair_counts, ctx->stats, mm->wide_pairs, ctx->num_cus));
-  if (data->max_nodes > 64 && m->P) {  // pairs of molecules with > 64 nodes: one wavefront per pair
+  if (data->max_nodes > 64 && m->P) {  // pairs of molecules with > 64 nodes: the two-word lane join
     unsigned long long nw = 0;
     SIGMO_HIP_TRY(hipMemcpyAsync(&nw, ctx->stats + 5, sizeof(nw), hipMemcpyDeviceToHost, s));
     SIGMO_HIP_TRY(hipStreamSynchronize(s));
     if (nw)
-      SIGMO_HIP_TRY(sigmo_k::join(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, nw, find_first, mm->pair_counts,
-                                  ctx->stats, ctx->num_cus, mm->wide_pairs, 6u));
+      SIGMO_HIP_TRY(sigmo_k::join_lanes_wide(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, mm->wide_pairs, nw, 6u,
+                                             find_first, mm->pair_counts, ctx->stats, ctx->num_cus));
   }
   SIGMO_TRY(ps.close());
   SIGMO_HIP_TRY(hipMemcpyAsync(ctx->stats_host, ctx->stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "device_common.hpp"
@@ -19,6 +20,11 @@ struct ArenaMasks {
   __device__ __forceinline__ uint32_t base(uint64_t j) const { return mask_off[j]; }
   __device__ __forceinline__ uint64_t mask(uint32_t base, uint32_t, uint32_t depth, uint32_t, uint32_t) const {
     return arena[static_cast<uint64_t>(base) + depth];
+  }
+  // molecules of 65..128 nodes: two words per query node (positions 0-63, 64-127)
+  __device__ __forceinline__ void mask2(uint32_t base, uint32_t, uint32_t depth, uint32_t, uint32_t, uint64_t& lo, uint64_t& hi) const {
+    lo = arena[static_cast<uint64_t>(base) + 2 * depth];
+    hi = arena[static_cast<uint64_t>(base) + 2 * depth + 1];
   }
 };
 
@@ -37,15 +43,27 @@ struct BitsetMasks {
     if (n < 64) m &= (1ull << n) - 1;
     return m;
   }
+  // positions s..s+n-1 with 64 < n <= 128: up to three words of the row, funnel-shifted
+  __device__ __forceinline__ void mask2(uint32_t, uint32_t qrow, uint32_t, uint32_t s, uint32_t n, uint64_t& lo, uint64_t& hi) const {
+    const uint64_t* row = bits + static_cast<uint64_t>(qrow) * W64;
+    const uint32_t w0 = s >> 6, last = (s + n - 1) >> 6, sh = s & 63;
+    const uint64_t x0 = row[w0];
+    const uint64_t x1 = w0 + 1 <= last ? row[w0 + 1] : 0ull;
+    const uint64_t x2 = w0 + 2 <= last ? row[w0 + 2] : 0ull;
+    lo = sh ? (x0 >> sh) | (x1 << (64 - sh)) : x0;
+    hi = sh ? (x1 >> sh) | (x2 << (64 - sh)) : x1;
+    if (n < 128) hi &= (1ull << (n - 64)) - 1;
+  }
 };
 
 constexpr uint32_t kJoinLaneWaves = 2;  // wavefronts per block of the lane-per-pair join
 
 // Join with one lane per GMCR pair, for molecules of <= 64 nodes (the rest are listed in
 // `wide` for the wavefront-per-pair kernel).  Each lane runs the reference DFS of
-// isomorphism.hpp:270-381 for its own pair, with the same bookkeeping as join_dfs (mapping
-// starting at local node 0, stale unsets of exhausted frames, `visited` cleared when depth 1 is
-// exhausted).  The validity test of isomorphism.hpp:150-168 -- for every j < depth,
+// isomorphism.hpp:270-381 for its own pair with the reference's bookkeeping: every mapping
+// entry starts at local node 0, an exhausted frame unsets the visited bit of its (possibly
+// stale) mapping entry, `visited` is cleared when depth 1 is exhausted, and a frame resumes at
+// mapping[depth] + 1.  The validity test of isomorphism.hpp:150-168 -- for every j < depth,
 // qAdj(j, depth) == dAdj(mapping[j], c) -- becomes set algebra on adjacency rows: adjacency is
 // symmetric, so the candidates that pass are those in the row of every mapped query neighbour
 // and in the row of no other mapped node.  The rows of the mapped positions and the mapping sit
@@ -177,6 +195,143 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
     if (overflow) atomicOr(&stats[3], 1ull);
     atomicAdd(&stats[4], static_cast<unsigned long long>(steps));
   }
+}
+
+// The same lane-per-pair DFS for the pairs listed in `list` (molecules of 65..128 nodes).  The
+// positions span two words: candidates and the adjacency rows of mapped positions are (lo, hi)
+// pairs, while `visited` stays one word addressed by position mod 64, the reference's Bitset
+// aliasing (utils.hpp:71-73), exactly as the wavefront join does for these molecules.  LDS holds
+// two row words per depth (34 KB), so a block is one wavefront.
+
+template <bool FIND_FIRST, class Src>
+__global__ __launch_bounds__(64) void k_join_lanes_wide(GraphDev q, GraphDev d, Src src, const uint32_t* __restrict__ indices,
+                                                        const uint32_t* __restrict__ pair_graph, const uint32_t* __restrict__ list, uint64_t P,
+                                                        uint32_t chunk, uint32_t cursor, int32_t* __restrict__ counts,
+                                                        unsigned long long* __restrict__ stats) {
+  __shared__ uint64_t rows_lo[32][kWave];
+  __shared__ uint64_t rows_hi[32][kWave];
+  __shared__ uint8_t maps[32][kWave];
+  const uint32_t lane = static_cast<uint32_t>(lane_id());
+  const uint64_t below = (1ull << lane) - 1;
+  uint64_t next = 0, end = 0;
+  bool exhausted = false;
+  uint64_t sum = 0, matched = 0, steps = 0, overflow = 0;
+
+  bool have = false;
+  uint64_t j = 0, visited = 0, count = 0;
+  uint32_t s = 0, n = 0, qoff = 0, k = 0, base = 0, depth = 0, pos = 0;
+  while (true) {
+    uint64_t need = __ballot(!have);
+    while (need && !exhausted) {
+      if (next >= end) {
+        unsigned long long b = 0;
+        if (lane == 0) b = atomicAdd(&stats[cursor], static_cast<unsigned long long>(chunk));
+        next = uniform64(__shfl(b, 0));
+        end = next + chunk < P ? next + chunk : P;
+        if (next >= P) {
+          exhausted = true;
+          break;
+        }
+      }
+      const uint32_t cnt = min(static_cast<uint32_t>(end - next), static_cast<uint32_t>(__popcll(need)));
+      const uint32_t rn = __popcll(need & below);
+      const bool take = ((need >> lane) & 1ull) && rn < cnt;
+      if (take) {
+        j = list[next + rn];
+        const uint32_t dg = pair_graph[j], qg = indices[j];
+        base = src.base(j);
+        s = d.go[dg];
+        n = d.go[dg + 1] - s;
+        qoff = q.go[qg];
+        k = q.go[qg + 1] - qoff;
+        for (uint32_t t = 0; t < k; ++t) maps[t][lane] = 0;  // every mapping entry starts at local node 0
+        depth = 0;
+        pos = 0;
+        visited = 0;
+        count = 0;
+        have = true;
+      }
+      next += cnt;
+      need &= ~__ballot(take);
+    }
+    if (__ballot(have) == 0) break;
+    if (have) {
+      ++steps;
+      bool done = false;
+      if (depth == k) {  // leaf (:336-344)
+        ++count;
+        if (FIND_FIRST) {
+          done = true;
+        } else {
+          depth = k - 1;
+          pos = static_cast<uint32_t>(maps[depth][lane]) + 1;
+        }
+      } else {
+        uint64_t clo, chi;
+        src.mask2(base, qoff + depth, depth, s, n, clo, chi);
+        clo &= ~visited & bits_from(pos);
+        chi &= ~visited & (pos <= 64 ? ~0ull : bits_from(pos - 64));
+        const uint32_t qad = static_cast<uint32_t>(q.adj[2 * (static_cast<uint64_t>(qoff) + depth)]);
+        uint64_t must_lo = ~0ull, must_hi = ~0ull, not_lo = 0, not_hi = 0;
+        for (uint32_t t = 0; t < depth; ++t) {
+          const uint64_t rl = rows_lo[t][lane], rh = rows_hi[t][lane];
+          if ((qad >> t) & 1u) {
+            must_lo &= rl;
+            must_hi &= rh;
+          } else {
+            not_lo |= rl;
+            not_hi |= rh;
+          }
+        }
+        const uint64_t el = clo & must_lo & ~not_lo, eh = chi & must_hi & ~not_hi;
+        if (el | eh) {  // accept (:366-370)
+          const uint32_t c = el ? static_cast<uint32_t>(__builtin_ctzll(el)) : 64u + static_cast<uint32_t>(__builtin_ctzll(eh));
+          maps[depth][lane] = static_cast<uint8_t>(c);
+          rows_lo[depth][lane] = d.adj[2 * (static_cast<uint64_t>(s) + c)];
+          rows_hi[depth][lane] = d.adj[2 * (static_cast<uint64_t>(s) + c) + 1];
+          visited |= 1ull << (c & 63);
+          ++depth;
+          pos = 0;
+        } else {  // exhausted frame (:346-354)
+          visited &= ~(1ull << (maps[depth][lane] & 63u));
+          if (depth == 0) {
+            done = true;
+          } else {
+            if (depth == 1) visited = 0;
+            --depth;
+            pos = static_cast<uint32_t>(maps[depth][lane]) + 1;
+          }
+        }
+      }
+      if (done) {
+        if (count > 0x7FFFFFFFull) overflow = 1;
+        counts[j] = static_cast<int32_t>(count > 0x7FFFFFFFull ? 0x7FFFFFFFull : count);
+        sum += count;
+        matched += count > 0;
+        have = false;
+      }
+    }
+  }
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    sum += __shfl_xor(sum, o);
+    matched += __shfl_xor(matched, o);
+    steps += __shfl_xor(steps, o);
+    overflow |= __shfl_xor(overflow, o);
+  }
+  if (lane == 0) {
+    if (sum) atomicAdd(&stats[0], static_cast<unsigned long long>(sum));
+    if (matched) atomicAdd(&stats[1], static_cast<unsigned long long>(matched));
+    if (overflow) atomicOr(&stats[3], 1ull);
+    atomicAdd(&stats[4], static_cast<unsigned long long>(steps));
+  }
+}
+
+// Grid of k_join_lanes_wide for `nw` listed pairs: resident one-wavefront blocks, ~16 claims each.
+inline void join_wide_grid(uint64_t nw, int num_cus, uint64_t* blocks, uint32_t* chunk) {
+  const uint64_t cus = static_cast<uint64_t>(num_cus > 0 ? num_cus : 256);
+  *blocks = std::max<uint64_t>(1, std::min<uint64_t>((nw + 63) / 64, cus * 4));
+  *chunk = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(64, nw / (*blocks * 16))));
 }
 
 }  // namespace

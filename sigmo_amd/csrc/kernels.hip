@@ -10,7 +10,6 @@
 // segments, fresh join state per pair with mapping[] = local node 0.
 #include "kernels.hpp"
 #include "device_common.hpp"
-#include "join_dfs.hpp"
 #include "join_lanes.hpp"
 
 namespace sigmo_k {
@@ -477,69 +476,6 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_final(const uint32_t* __res
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = static_cast<uint32_t>(*total_dev);
 }
 
-// joinCandidates, isomorphism.hpp:270-381, with one wavefront per (query, molecule) pair.
-// Lane p holds molecule node p's local adjacency (and node p+64's when n > 64); lane d holds
-// query node d's candidate mask inside the molecule, its query adjacency, mapping[d] and the
-// set of positions mapped at depths < d.  One DFS frame step costs one or two ballots and no
-// memory access:
-//   A = positions mapped at depths i < d with qAdj(i,d) (scalar loop over d's query
-//   neighbours), N = positions mapped at depths i < d without it;
-//   valid(c) <=> (adj(c) & A) == A and (adj(c) & N) == 0, which is the per-i test of
-//   isValidMapping (:150-168) also when mapping[] repeats a node;
-//   the accepted candidate is the first c >= pos with cand(c), !visited(c mod 64), valid(c),
-//   exactly the one the reference's candidate-by-candidate loop (:356-370) accepts.
-// The reference's bookkeeping is kept literally (uniform scalar state): visited is a 64-bit
-// word indexed by local id mod 64; an exhausted frame unsets mapping[depth] (possibly stale),
-// pops, and clears visited when only the root remains; a frame resumes at mapping[depth]+1.
-// Pairs are taken `chunk` at a time from a global cursor (one atomic per chunk).  The next
-// pair's operands are loaded while the current pair's DFS runs; consecutive pairs share their
-// molecule (GMCR order), whose adjacency registers are then reused.
-template <bool FIND_FIRST>
-__global__ __launch_bounds__(256) void k_join_wave(GraphDev q, GraphDev d, const uint64_t* __restrict__ bits, uint64_t W64,
-                                                   const uint32_t* __restrict__ indices, const uint32_t* __restrict__ pair_graph, uint64_t P,
-                                                   uint32_t chunk, int32_t* __restrict__ counts, unsigned long long* __restrict__ stats,
-                                                   const uint32_t* __restrict__ list, uint32_t cursor) {
-  // pairs 0..P-1 of `list` (all GMCR pairs when list is null); stats[cursor] is the work counter
-  const uint32_t lane = static_cast<uint32_t>(lane_id());
-  uint64_t sum = 0, matched = 0, steps = 0, overflow = 0;
-  uint64_t next = 0, end = 0;
-  auto grab = [&]() {
-    unsigned long long jj = 0;
-    if (lane == 0) jj = atomicAdd(&stats[cursor], static_cast<unsigned long long>(chunk));
-    next = uniform64(__shfl(jj, 0));
-    end = next + chunk < P ? next + chunk : P;
-  };
-  auto pair_at = [&](uint64_t i) -> uint64_t { return list ? list[i] : i; };
-  grab();
-  if (next < P) {
-    JoinPair cur, nxt;
-    join_load(q, d, bits, W64, indices, pair_graph, pair_at(next), nullptr, cur);
-    join_finish(bits, W64, cur);
-    while (true) {
-      const uint64_t j = pair_at(next++);
-      if (next >= end) grab();
-      const bool more = next < P;
-      if (more) join_load(q, d, bits, W64, indices, pair_graph, pair_at(next), &cur, nxt);
-      const uint64_t count = cur.n > 64 ? join_dfs<FIND_FIRST, true>(cur, steps) : join_dfs<FIND_FIRST, false>(cur, steps);
-      if (lane == 0) {
-        if (count > 0x7FFFFFFFull) overflow = 1;
-        counts[j] = static_cast<int32_t>(count > 0x7FFFFFFFull ? 0x7FFFFFFFull : count);
-      }
-      sum += count;
-      matched += count > 0;
-      if (!more) break;
-      join_finish(bits, W64, nxt);
-      cur = nxt;
-    }
-  }
-  if (lane == 0) {
-    if (sum) atomicAdd(&stats[0], static_cast<unsigned long long>(sum));
-    if (matched) atomicAdd(&stats[1], static_cast<unsigned long long>(matched));
-    if (overflow) atomicOr(&stats[3], 1ull);
-    atomicAdd(&stats[4], static_cast<unsigned long long>(steps));
-  }
-}
-
 }  // namespace
 
 // ---- launchers -------------------------------------------------------------------------------
@@ -685,21 +621,20 @@ hipError_t join_lanes(hipStream_t s, const GraphDev& q, const GraphDev& d, const
   return hipGetLastError();
 }
 
-hipError_t join(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
-                const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts, unsigned long long* stats, int num_cus,
-                const uint32_t* list, uint32_t cursor) {
-  if (!P) return hipSuccess;
-  // one wavefront per pair, persistent: 8 waves per SIMD; chunks give every wave ~8 grabs
-  uint64_t blocks = (P + 3) / 4;
-  const uint64_t cap = static_cast<uint64_t>(num_cus > 0 ? num_cus : 256) * 8;
-  if (blocks > cap) blocks = cap;
-  uint64_t chunk = P / (blocks * 4 * 8);
-  chunk = chunk < 1 ? 1 : (chunk > 64 ? 64 : chunk);
-  const uint32_t ch = static_cast<uint32_t>(chunk);
+hipError_t join_lanes_wide(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
+                           const uint32_t* pair_graph, const uint32_t* list, uint64_t nw, uint32_t cursor, int find_first, int32_t* counts,
+                           unsigned long long* stats, int num_cus) {
+  if (!nw) return hipSuccess;
+  uint64_t blocks = 0;
+  uint32_t chunk = 0;
+  join_wide_grid(nw, num_cus, &blocks, &chunk);
+  const BitsetMasks src{bits, W64};
   if (find_first)
-    k_join_wave<true><<<static_cast<unsigned>(blocks), 256, 0, s>>>(q, d, bits, W64, indices, pair_graph, P, ch, counts, stats, list, cursor);
+    k_join_lanes_wide<true, BitsetMasks><<<static_cast<unsigned>(blocks), 64, 0, s>>>(q, d, src, indices, pair_graph, list, nw, chunk, cursor,
+                                                                                     counts, stats);
   else
-    k_join_wave<false><<<static_cast<unsigned>(blocks), 256, 0, s>>>(q, d, bits, W64, indices, pair_graph, P, ch, counts, stats, list, cursor);
+    k_join_lanes_wide<false, BitsetMasks><<<static_cast<unsigned>(blocks), 64, 0, s>>>(q, d, src, indices, pair_graph, list, nw, chunk, cursor,
+                                                                                      counts, stats);
   return hipGetLastError();
 }
 

@@ -79,13 +79,15 @@ hipError_t gmcr_scatter(hipStream_t s, const uint64_t* pair_bits, uint32_t Gq, u
 
 // Join.  stats[0] += sum of counts, stats[1] += pairs with count > 0, stats[2] = work cursor
 // (zeroed by the caller), stats[3] |= 1 on int32 overflow.
-hipError_t join(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
-                const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts, unsigned long long* stats, int num_cus,
-                const uint32_t* list = nullptr, uint32_t cursor = 2);
 // Lane-per-pair join of the pairs whose molecule has <= 64 nodes; the others are appended to
-// `wide` (count in stats[5]) for join() with that list.
+// `wide` (count in stats[5]) for join_lanes_wide.
 hipError_t join_lanes(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
                       const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts, unsigned long long* stats, uint32_t* wide,
                       int num_cus);
+// The lane-per-pair join over the `nw` listed pairs of molecules with 65..128 nodes;
+// stats[cursor] is the work counter.
+hipError_t join_lanes_wide(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
+                           const uint32_t* pair_graph, const uint32_t* list, uint64_t nw, uint32_t cursor, int find_first, int32_t* counts,
+                           unsigned long long* stats, int num_cus);
 
 }  // namespace sigmo_k
