@@ -381,6 +381,8 @@ def main() -> int:
             "matched_pairs": matched,
             "embeddings": results[-1]["num_matches"] if args.find_all else None,
             "dfs_steps": results[-1]["dfs_steps"],
+            "join_lane_use": (round(results[-1]["dfs_steps"] / (64.0 * results[-1]["join_wave_steps"]), 4)
+                              if results[-1].get("join_wave_steps") else None),
             **({k: results[-1][k] for k in ("filter_wave_steps", "filter_lane_steps", "filter_scans")}
                if args.form == "batched" and args.config != 3 else {}),
             **({k: results[-1][k] for k in ("chunks", "wall_ms", "compute_ms", "load_ms", "load_wait_ms")}

@@ -185,6 +185,7 @@ typedef struct sigmo_pipeline_result {
   double phase_ms[SIGMO_PHASE_COUNT];
   double total_ms;         /* device time from first signature kernel to join completion */
   uint64_t dfs_steps;      /* join DFS frame steps (accepts + backtracks + leaves)         */
+  uint64_t join_wave_steps; /* join loop iterations summed over wavefronts (lane use = dfs_steps / (64 x this)) */
 } sigmo_pipeline_result;
 /* Runs gen(D), gen(Q), filter, R x {refsig(D,r), refsig(Q,r), refine}, GMCR, join on
  * device-resident graphs, with one launch per phase; allocates and frees its buffers. */
@@ -233,6 +234,7 @@ typedef struct sigmo_batch_result {
   uint64_t filter_wave_steps; /* pair evaluation: wavefront steps        */
   uint64_t filter_lane_steps; /* pair evaluation: active lanes summed    */
   uint64_t filter_scans;      /* pair evaluation: 64-pair pre-test scans */
+  uint64_t join_wave_steps;   /* join loop iterations summed over wavefronts */
 } sigmo_batch_result;
 int sigmo_db_prepare(sigmo_ctx* ctx, const sigmo_graph* data, uint32_t refinement_steps, sigmo_db** out);
 /* Recompute the level planes from the data graph's current device contents (same result as a

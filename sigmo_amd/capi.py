@@ -73,6 +73,7 @@ class PipelineResult(C.Structure):
         ("phase_ms", C.c_double * 5),
         ("total_ms", C.c_double),
         ("dfs_steps", C.c_uint64),
+        ("join_wave_steps", C.c_uint64),
     ]
 
 
@@ -85,6 +86,7 @@ class BatchResult(C.Structure):
         ("filter_wave_steps", C.c_uint64),
         ("filter_lane_steps", C.c_uint64),
         ("filter_scans", C.c_uint64),
+        ("join_wave_steps", C.c_uint64),
     ]
 
 
@@ -608,6 +610,7 @@ def _result_dict(r: PipelineResult) -> dict:
         "phase_ms": dict(zip(PHASE_NAMES, list(r.phase_ms))),
         "total_ms": r.total_ms,
         "dfs_steps": r.dfs_steps,
+        "join_wave_steps": r.join_wave_steps,
     }
 
 
@@ -700,7 +703,8 @@ class Batch:
         check(lib().sigmo_batch_run(self._h, query.handle, int(find_first), C.byref(r)))
         self.last = {"num_matches": r.num_matches, "matched_pairs": r.matched_pairs, "gmcr_pairs": r.gmcr_pairs,
                      "dfs_steps": r.dfs_steps, "filter_wave_steps": r.filter_wave_steps,
-                     "filter_lane_steps": r.filter_lane_steps, "filter_scans": r.filter_scans}
+                     "filter_lane_steps": r.filter_lane_steps, "filter_scans": r.filter_scans,
+                     "join_wave_steps": r.join_wave_steps}
         return self.last
 
     def request_stats(self, request_graph_offsets) -> list[dict]:

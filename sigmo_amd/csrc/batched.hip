@@ -1236,6 +1236,7 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
   res->matched_pairs = stats_host[1];
   res->gmcr_pairs = P;
   res->dfs_steps = stats_host[4];
+  res->join_wave_steps = stats_host[7];
   return SIGMO_OK;
 }
 

@@ -1020,6 +1020,7 @@ int sigmo_pipeline_execute(sigmo_pipeline* p, uint32_t refinement_steps, int fin
   result->gmcr_pairs = p->gm->P;
   result->matched_pairs = ctx->stats_host[1];
   result->dfs_steps = ctx->stats_host[4];
+  result->join_wave_steps = ctx->stats_host[7];
   result->total_ms = total;
   SIGMO_TRY(ctx->collect());
   for (int ph = 0; ph < SIGMO_PHASE_COUNT; ++ph) result->phase_ms[ph] = ctx->accum_ms[ph] - before[ph];
@@ -1145,6 +1146,7 @@ int sigmo_pipeline_run(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_gra
   result->gmcr_pairs = gm->P;
   result->matched_pairs = ctx->stats_host[1];
   result->dfs_steps = ctx->stats_host[4];
+  result->join_wave_steps = ctx->stats_host[7];
   result->total_ms = total;
   for (int p = 0; p < SIGMO_PHASE_COUNT; ++p) SIGMO_TRY(sigmo_phase_ms(ctx, p, &result->phase_ms[p]));
   return SIGMO_OK;

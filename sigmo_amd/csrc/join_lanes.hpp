@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
   const uint64_t below = (1ull << lane) - 1;
   uint64_t next = 0, end = 0;  // claimed pairs [next, end) (wave-uniform)
   bool exhausted = false;
-  uint64_t sum = 0, matched = 0, steps = 0, overflow = 0;
+  uint64_t sum = 0, matched = 0, steps = 0, overflow = 0, wave_iters = 0;
 
   bool have = false;
   uint64_t j = 0, visited = 0, count = 0;
@@ -134,6 +134,7 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
       need |= wm;  // those lanes take another pair
     }
     if (__ballot(have) == 0) break;
+    ++wave_iters;
     if (have) {
       ++steps;
       bool done = false;
@@ -194,6 +195,7 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
     if (matched) atomicAdd(&stats[1], static_cast<unsigned long long>(matched));
     if (overflow) atomicOr(&stats[3], 1ull);
     atomicAdd(&stats[4], static_cast<unsigned long long>(steps));
+    atomicAdd(&stats[7], static_cast<unsigned long long>(wave_iters));
   }
 }
 
@@ -215,7 +217,7 @@ __global__ __launch_bounds__(64) void k_join_lanes_wide(GraphDev q, GraphDev d, 
   const uint64_t below = (1ull << lane) - 1;
   uint64_t next = 0, end = 0;
   bool exhausted = false;
-  uint64_t sum = 0, matched = 0, steps = 0, overflow = 0;
+  uint64_t sum = 0, matched = 0, steps = 0, overflow = 0, wave_iters = 0;
 
   bool have = false;
   uint64_t j = 0, visited = 0, count = 0;
@@ -255,6 +257,7 @@ __global__ __launch_bounds__(64) void k_join_lanes_wide(GraphDev q, GraphDev d, 
       need &= ~__ballot(take);
     }
     if (__ballot(have) == 0) break;
+    ++wave_iters;
     if (have) {
       ++steps;
       bool done = false;
@@ -324,6 +327,7 @@ __global__ __launch_bounds__(64) void k_join_lanes_wide(GraphDev q, GraphDev d, 
     if (matched) atomicAdd(&stats[1], static_cast<unsigned long long>(matched));
     if (overflow) atomicOr(&stats[3], 1ull);
     atomicAdd(&stats[4], static_cast<unsigned long long>(steps));
+    atomicAdd(&stats[7], static_cast<unsigned long long>(wave_iters));
   }
 }
 
