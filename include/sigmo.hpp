@@ -14,6 +14,7 @@
 //   isomorphism.hpp:22-127     sigmo::isomorphism::filter::{filterCandidates, refineCandidates}
 //   gmcr.hpp:18-135            sigmo::isomorphism::mapping::GMCR
 //   isomorphism.hpp:270-381    sigmo::isomorphism::join::joinCandidates
+//   isomorphism.hpp:383-556    sigmo::isomorphism::join::joinWildcardCandidates
 #pragma once
 
 #include <chrono>
@@ -421,6 +422,20 @@ inline utils::BatchedEvent joinCandidates(Queue& queue, DeviceBatchedCSRGraph& q
   if (per_pair_counts) per_pair_counts->assign(gmcr.getTotalQueryIndices(), 0);
   detail::check(sigmo_join(queue.get(), query_graphs.handle, data_graphs.handle, candidates.handle(), gmcr.handle(), find_first ? 1 : 0,
                            &total, per_pair_counts ? per_pair_counts->data() : nullptr));
+  if (num_matches) *num_matches += total;
+  return ev.done();
+}
+
+// isomorphism.hpp:428-556 (with defineMatchingOrder :383-408 and the edge-label isValidMapping
+// :410-426).
+inline utils::BatchedEvent joinWildcardCandidates(Queue& queue, DeviceBatchedCSRGraph& query_graphs, DeviceBatchedCSRGraph& data_graphs,
+                                                  candidates::Candidates& candidates, mapping::GMCR& gmcr, size_t* num_matches,
+                                                  bool find_first = true, std::vector<int32_t>* per_pair_counts = nullptr) {
+  utils::EventScope ev(queue.get());
+  uint64_t total = 0;
+  if (per_pair_counts) per_pair_counts->assign(gmcr.getTotalQueryIndices(), 0);
+  detail::check(sigmo_join_wildcard(queue.get(), query_graphs.handle, data_graphs.handle, candidates.handle(), gmcr.handle(),
+                                    find_first ? 1 : 0, &total, per_pair_counts ? per_pair_counts->data() : nullptr));
   if (num_matches) *num_matches += total;
   return ev.done();
 }

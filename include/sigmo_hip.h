@@ -177,6 +177,16 @@ int sigmo_gmcr_download(sigmo_ctx* ctx, const sigmo_gmcr* m, uint32_t* data_grap
 int sigmo_join(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data, const sigmo_candidates* c,
                const sigmo_gmcr* m, int find_first, uint64_t* num_matches, int32_t* per_pair_counts);
 
+/* ---- wildcard / edge-label join (isomorphism::join::joinWildcardCandidates,
+ * isomorphism.hpp:428-556, with defineMatchingOrder :383-408 and isValidMapping :410-426) ----
+ * Per GMCR pair: the query node with most candidates in the molecule goes first (ties: larger
+ * index; all k <= 32 nodes are considered), then the others in ascending order; each candidate
+ * of the first node starts its own DFS; adjacent query/data pairs must carry equal edge labels.
+ * In find-first mode a pair counts the first-node candidates that extend to a match.  Same
+ * accumulation and per-pair output as sigmo_join. */
+int sigmo_join_wildcard(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data, const sigmo_candidates* c,
+                        const sigmo_gmcr* m, int find_first, uint64_t* num_matches, int32_t* per_pair_counts);
+
 /* ---- whole plain pipeline (sigmo.cpp:102-159 phase order) ---------------------------- */
 typedef struct sigmo_pipeline_result {
   uint64_t num_matches;    /* "# Matches" (sigmo.cpp:215)                               */

@@ -424,4 +424,118 @@ int64_t oracle_join(uint32_t Gq, const uint32_t* qgo, const uint32_t* qro, const
   return bad ? -1 : total;
 }
 
+// joinWildcardCandidates, isomorphism.hpp:428-556, with defineMatchingOrder (:383-408) and the
+// 8-argument isValidMapping (:410-426).  Per pair:
+//   * the matching order starts with the query node of most candidates in the molecule among
+//     the first `sg_size` nodes (ties: the larger index, :397-401), then every other node in
+//     ascending order (:404-407);
+//   * each candidate of that first node starts its own DFS (:413-416; the reference splits
+//     them over the sub-group's lanes), from fresh state: every mapping[] entry equal to the
+//     molecule's first node id (local id 0);
+//   * a candidate is valid when, for every earlier node of the order, query adjacency equals
+//     data adjacency and adjacent pairs carry equal edge labels (first CSR entry, graph.hpp:88-93);
+//   * an exhausted frame unsets its node's (possibly stale) mapping; when one frame remains,
+//     visited is reset to the first node's candidate (:521-526); find-first stops a DFS at its
+//     first leaf, so the pair's count is the number of first-node candidates that extend.
+// mutation 1: edge labels not compared.  mutation 2: matching order = query order.
+// mutation 3: ties of the first node go to the smaller index.
+int64_t oracle_join_wildcard(uint32_t Gq, const uint32_t* qgo, const uint32_t* qro, const uint32_t* qci, const uint8_t* qel,
+                             uint32_t Gd, const uint32_t* dgo, const uint32_t* dro, const uint32_t* dci, const uint8_t* del, uint64_t D,
+                             const uint32_t* words, const uint32_t* offsets, const uint32_t* indices, int find_first, uint32_t sg_size,
+                             int32_t* counts, int mutation) {
+  const Graph qgr = make_graph(Gq, qgo, qro, qci, nullptr);
+  const Graph dgr = make_graph(Gd, dgo, dro, dci, nullptr);
+  const Cands c{const_cast<u32*>(words), (D + 31) / 32};
+  auto edge_label = [](const u32* ro, const u32* ci, const u8* el, u32 a, u32 b) -> u8 {  // graph.hpp:88-93
+    for (u32 i = ro[a]; i < ro[a + 1]; ++i)
+      if (ci[i] == b) return el[i];
+    return static_cast<u8>(-1);
+  };
+  int64_t total = 0;
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total) reduction(| : bad)
+  for (int64_t dg = 0; dg < static_cast<int64_t>(Gd); ++dg) {
+    const u32 s = dgo[dg], e = dgo[dg + 1];
+    for (u32 j = offsets[dg]; j < offsets[dg + 1]; ++j) {
+      const u32 qg = indices[j];
+      const u32 qoff = qgr.prev(qg);
+      const u32 k = qgr.nodes(qg);
+      if (k + 1 > static_cast<u32>(kMaxQueryNodes)) {
+        bad = 1;
+        counts[j] = -1;
+        continue;
+      }
+      // defineMatchingOrder
+      u32 order[kMaxQueryNodes];
+      u32 max_candidates = 0;
+      int first = -1;
+      const u32 considered = std::min(k, sg_size);
+      for (u32 i = 0; i < considered; ++i) max_candidates = std::max(max_candidates, c.count(qoff + i, s, e));
+      for (u32 i = 0; i < considered; ++i)
+        if (c.count(qoff + i, s, e) == max_candidates && (first < 0 || mutation != 3)) first = static_cast<int>(i);
+      if (mutation == 2) {
+        first = 0;
+        max_candidates = c.count(qoff, s, e);
+      }
+      order[0] = static_cast<u32>(first);
+      for (u32 i = 0, jj = 1; i < k; ++i)
+        if (static_cast<int>(i) != first) order[jj++] = i;
+      int64_t count = 0;
+      for (u32 start = 0; start < max_candidates; ++start) {
+        struct Frame {
+          u32 depth;
+          u64 idx;
+        } stack[kMaxQueryNodes];
+        u32 mapping[kMaxQueryNodes];
+        for (auto& m : mapping) m = s;
+        Bitset64 visited{s};
+        u32 top = 0;
+        mapping[order[0]] = c.at(order[0] + qoff, start, s, e);
+        visited.set(static_cast<u8>(mapping[order[0]]));
+        stack[top++] = {1, 0};
+        while (top > 0) {
+          const Frame f = stack[top - 1];
+          if (f.depth == k) {
+            count++;
+            top--;
+            if (find_first) break;
+            continue;
+          }
+          const u32 qn = order[f.depth];
+          if (f.idx >= c.count(qn + qoff, s, e)) {
+            top--;
+            visited.unset(static_cast<u8>(mapping[qn]));
+            if (top == 1) {
+              visited.clear();
+              visited.set(static_cast<u8>(mapping[order[0]]));
+            }
+            continue;
+          }
+          const u32 cand = c.at(qn + qoff, static_cast<u32>(f.idx), s, e);
+          stack[top - 1].idx++;
+          if (visited.get(static_cast<u8>(cand))) continue;
+          bool valid = true;
+          for (u32 i = 0; i < f.depth && valid; i++) {  // isValidMapping (8 args)
+            const u32 qa_n = order[i] + qoff, qb_n = qn + qoff;
+            const bool qa = qgr.isNeighbor(qa_n, qb_n);
+            const bool da = dgr.isNeighbor(mapping[order[i]], cand);
+            if (qa != da) valid = false;
+            else if (qa && mutation != 1 && edge_label(dro, dci, del, mapping[order[i]], cand) != edge_label(qro, qci, qel, qa_n, qb_n))
+              valid = false;
+          }
+          if (valid) {
+            mapping[qn] = cand;
+            visited.set(static_cast<u8>(cand));
+            stack[top++] = {f.depth + 1, 0};
+          }
+        }
+      }
+      if (count > INT32_MAX) bad = 1;
+      counts[j] = static_cast<int32_t>(std::min<int64_t>(count, INT32_MAX));
+      total += count;
+    }
+  }
+  return bad ? -1 : total;
+}
+
 }  // extern "C"

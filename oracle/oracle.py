@@ -50,6 +50,7 @@ def lib() -> C.CDLL:
             "oracle_gmcr_offsets": ([u32, vp, u32, vp, u64, vp, vp], i64),
             "oracle_gmcr_indices": ([u32, vp, u32, vp, u64, vp, vp, vp], None),
             "oracle_join": ([u32, vp, vp, vp, u32, vp, vp, vp, u64, vp, vp, vp, i32, vp, i32], i64),
+            "oracle_join_wildcard": ([u32, vp, vp, vp, vp, u32, vp, vp, vp, vp, u64, vp, vp, vp, i32, u32, vp, i32], i64),
         }
         for name, (args, res) in sigs.items():
             fn = getattr(L, name)
@@ -169,6 +170,20 @@ def join(q: dict, d: dict, words: np.ndarray, offsets: np.ndarray, indices: np.n
                               _p(words), _p(offsets), _p(indices), int(find_first), _p(counts), mutation)
     if total < 0:
         raise ValueError("join failed: query too large or per-pair count overflow")
+    return int(total), counts
+
+
+def join_wildcard(q: dict, d: dict, words: np.ndarray, offsets: np.ndarray, indices: np.ndarray, find_first: bool = True,
+                  sg_size: int = 64, mutation: int = 0):
+    """joinWildcardCandidates (library/include/isomorphism.hpp:428-556): matching order by
+    candidate count, edge labels compared, one DFS per first-node candidate."""
+    counts = np.zeros(len(indices), np.int32)
+    total = lib().oracle_join_wildcard(_G(q), _p(q["graph_offsets"]), _p(q["row_offsets"]), _p(q["column_indices"]),
+                                       _p(q["edge_labels"]), _G(d), _p(d["graph_offsets"]), _p(d["row_offsets"]),
+                                       _p(d["column_indices"]), _p(d["edge_labels"]), _N(d), _p(words), _p(offsets), _p(indices),
+                                       int(find_first), sg_size, _p(counts), mutation)
+    if total < 0:
+        raise ValueError("wildcard join failed: query too large or per-pair count overflow")
     return int(total), counts
 
 

@@ -50,7 +50,7 @@ EXPORTED = (
     "sigmo_signatures_refine", "sigmo_signatures_download", "sigmo_signatures_alloc_size", "sigmo_candidates_create",
     "sigmo_candidates_free", "sigmo_candidates_filter", "sigmo_candidates_refine", "sigmo_candidates_download",
     "sigmo_candidates_counts", "sigmo_candidates_alloc_size", "sigmo_gmcr_create", "sigmo_gmcr_free",
-    "sigmo_gmcr_generate", "sigmo_gmcr_total", "sigmo_gmcr_download", "sigmo_join", "sigmo_pipeline_run",
+    "sigmo_gmcr_generate", "sigmo_gmcr_total", "sigmo_gmcr_download", "sigmo_join", "sigmo_join_wildcard", "sigmo_pipeline_run",
     "sigmo_pipeline_create", "sigmo_pipeline_execute", "sigmo_pipeline_pairs", "sigmo_pipeline_free",
     "sigmo_kernel_stats", "sigmo_event_mark", "sigmo_event_elapsed", "sigmo_db_prepare", "sigmo_db_free",
     "sigmo_db_alloc_size", "sigmo_db_refresh", "sigmo_batch_create", "sigmo_batch_run", "sigmo_batch_pairs", "sigmo_batch_free",
@@ -219,6 +219,7 @@ def lib() -> C.CDLL:
             "sigmo_gmcr_total": ([vp, C.POINTER(u64)], i32),
             "sigmo_gmcr_download": ([vp, vp, vp, u64, vp, u64], i32),
             "sigmo_join": ([vp, vp, vp, vp, vp, i32, C.POINTER(u64), vp], i32),
+            "sigmo_join_wildcard": ([vp, vp, vp, vp, vp, i32, C.POINTER(u64), vp], i32),
             "sigmo_pipeline_run": ([vp, vp, vp, u32, i32, C.POINTER(PipelineResult)], i32),
             "sigmo_pipeline_create": ([vp, vp, vp, pvp], i32),
             "sigmo_pipeline_execute": ([vp, u32, i32, C.POINTER(PipelineResult)], i32),
@@ -593,6 +594,15 @@ def join_candidates(ctx: Context, query: DeviceGraph, data: DeviceGraph, cand: C
     counts = np.zeros(gmcr.total, np.int32) if per_pair else None
     check(lib().sigmo_join(ctx.handle, query.handle, data.handle, cand.handle, gmcr.handle, int(find_first),
                            C.byref(total), _ptr(counts) if per_pair and counts.size else None))
+    return total.value, counts
+
+
+def join_wildcard(ctx: Context, query: DeviceGraph, data: DeviceGraph, cand: Candidates, gmcr: GMCR, find_first: bool = True):
+    """joinWildcardCandidates: (num_matches, per-pair int32 counts in GMCR order)."""
+    total = C.c_uint64(0)
+    counts = np.zeros(gmcr.total, np.int32)
+    check(lib().sigmo_join_wildcard(ctx.handle, query.handle, data.handle, cand.handle, gmcr.handle, int(find_first), C.byref(total),
+                                    _ptr(counts) if counts.size else None))
     return total.value, counts
 
 

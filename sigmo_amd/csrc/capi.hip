@@ -937,6 +937,30 @@ int sigmo_join(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data
   return SIGMO_OK;
 }
 
+int sigmo_join_wildcard(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data, const sigmo_candidates* c, const sigmo_gmcr* m,
+                        int find_first, uint64_t* num_matches, int32_t* per_pair_counts) {
+  SIGMO_TRY(bind(ctx));
+  SIGMO_TRY(check_cand(c, query, data));
+  if (!m || !m->built) return sigmo_err::set(SIGMO_ERR_INVALID, "GMCR not generated");
+  if (m->Gq != query->G || m->Gd != data->G) return sigmo_err::set(SIGMO_ERR_INVALID, "GMCR built for other graphs");
+  SIGMO_TRY(check_queries(query));
+  auto* mm = const_cast<sigmo_gmcr*>(m);
+  SIGMO_TRY(grow(&mm->pair_counts, mm->cap_counts, m->P));
+  hipStream_t s = ctx->stream;
+  PhaseScope ps(ctx, SIGMO_PHASE_JOIN, SIGMO_KERNEL_JOIN, 0.0);
+  SIGMO_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(ctx->stats), 8, 0));
+  SIGMO_HIP_TRY(sigmo_k::join_wildcard(s, query->dev(), data->dev(), query->el, data->el, c->bits, c->W64, m->indices, m->pair_graph, m->P,
+                                       find_first, mm->pair_counts, ctx->stats, ctx->num_cus));
+  SIGMO_TRY(ps.close());
+  SIGMO_HIP_TRY(hipMemcpyAsync(ctx->stats_host, ctx->stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  if (per_pair_counts && m->P)
+    SIGMO_HIP_TRY(hipMemcpyAsync(per_pair_counts, mm->pair_counts, m->P * 4, hipMemcpyDeviceToHost, s));
+  SIGMO_HIP_TRY(hipStreamSynchronize(s));
+  if (ctx->stats_host[3]) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "a per-pair match count exceeded int32");
+  if (num_matches) *num_matches += ctx->stats_host[0];
+  return SIGMO_OK;
+}
+
 // ---- plain pipeline --------------------------------------------------------------------------------
 
 }  // extern "C"

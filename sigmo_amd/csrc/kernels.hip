@@ -11,6 +11,7 @@
 #include "kernels.hpp"
 #include "device_common.hpp"
 #include "join_lanes.hpp"
+#include "join_wildcard.hpp"
 
 namespace sigmo_k {
 namespace {
@@ -618,6 +619,19 @@ hipError_t join_lanes(hipStream_t s, const GraphDev& q, const GraphDev& d, const
   else
     k_join_lanes<false, BitsetMasks><<<static_cast<unsigned>(blocks), kJoinLaneWaves * 64, 0, s>>>(q, d, src, indices, pair_graph, P, chunk,
                                                                                                  counts, stats, wide);
+  return hipGetLastError();
+}
+
+hipError_t join_wildcard(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint8_t* qel, const uint8_t* del, const uint64_t* bits,
+                         uint64_t W64, const uint32_t* indices, const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts,
+                         unsigned long long* stats, int num_cus) {
+  if (!P) return hipSuccess;
+  const uint64_t cus = static_cast<uint64_t>(num_cus > 0 ? num_cus : 256);
+  const unsigned blocks = static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>((P + 255) / 256, cus * 8)));
+  if (find_first)
+    k_join_wildcard<true><<<blocks, kWildcardWaves * 64, 0, s>>>(q, d, qel, del, bits, W64, indices, pair_graph, P, counts, stats);
+  else
+    k_join_wildcard<false><<<blocks, kWildcardWaves * 64, 0, s>>>(q, d, qel, del, bits, W64, indices, pair_graph, P, counts, stats);
   return hipGetLastError();
 }
 

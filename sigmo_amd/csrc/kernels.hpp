@@ -84,6 +84,11 @@ hipError_t gmcr_scatter(hipStream_t s, const uint64_t* pair_bits, uint32_t Gq, u
 hipError_t join_lanes(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
                       const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts, unsigned long long* stats, uint32_t* wide,
                       int num_cus);
+// joinWildcardCandidates: matching order by candidate count, edge labels compared, one DFS per
+// candidate of the first node; one lane per pair (qel/del: edge labels of the CSR entries).
+hipError_t join_wildcard(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint8_t* qel, const uint8_t* del, const uint64_t* bits,
+                         uint64_t W64, const uint32_t* indices, const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts,
+                         unsigned long long* stats, int num_cus);
 // The lane-per-pair join over the `nw` listed pairs of molecules with 65..128 nodes;
 // stats[cursor] is the work counter.
 hipError_t join_lanes_wide(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
