@@ -242,7 +242,7 @@ enum class SignatureScope { Data, Query };
 
 template <Algorithm A = Algorithm::PowerGraph, size_t Bits = 4>
 class Signature {
-  static_assert(A == Algorithm::PowerGraph && Bits == 4, "the driver path uses Signature<PowerGraph, 4>");
+  static_assert(Bits == 4, "signatures are 16 x 4-bit counts");
 
  public:
   Signature(Queue& queue, size_t data_nodes, size_t query_nodes) : queue_(queue) {
@@ -273,7 +273,10 @@ class Signature {
   }
   utils::BatchedEvent ref(DeviceBatchedCSRGraph& g, int scope, size_t view) {
     utils::EventScope ev(queue_.get());
-    detail::check(sigmo_signatures_refine(queue_.get(), s_, g.handle, scope, static_cast<uint32_t>(view)));
+    if constexpr (A == Algorithm::PowerGraph)  // signature.hpp:336-390
+      detail::check(sigmo_signatures_refine(queue_.get(), s_, g.handle, scope, static_cast<uint32_t>(view)));
+    else  // signature.hpp:298-334
+      detail::check(sigmo_signatures_refine_view_based(queue_.get(), s_, g.handle, scope, static_cast<uint32_t>(view)));
     return ev.done();
   }
   std::vector<uint64_t> download(int scope, size_t n) const {
@@ -351,19 +354,25 @@ class Candidates {
 
 namespace isomorphism {
 namespace filter {
+// CandidatesDomain::Data stores the transposed bitset: construct the candidates with the data
+// node count as source_nodes and the query node count as target_nodes (isomorphism.hpp:49-52).
 template <CandidatesDomain D = CandidatesDomain::Query>
 utils::BatchedEvent filterCandidates(Queue& queue, DeviceBatchedCSRGraph& query_graph, DeviceBatchedCSRGraph& data_graph,
                                      signature::Signature<>& signatures, candidates::Candidates& candidates) {
-  static_assert(D == CandidatesDomain::Query, "the driver path filters in the Query domain");
   utils::EventScope ev(queue.get());
-  detail::check(sigmo_candidates_filter(queue.get(), candidates.handle(), query_graph.handle, data_graph.handle, signatures.handle()));
+  if constexpr (D == CandidatesDomain::Query)
+    detail::check(sigmo_candidates_filter(queue.get(), candidates.handle(), query_graph.handle, data_graph.handle, signatures.handle()));
+  else
+    detail::check(sigmo_candidates_filter_data(queue.get(), candidates.handle(), query_graph.handle, data_graph.handle));
   return ev.done();
 }
 
 template <CandidatesDomain D = CandidatesDomain::Query>
 utils::BatchedEvent refineCandidates(Queue& queue, DeviceBatchedCSRGraph& query_graph, DeviceBatchedCSRGraph& data_graph,
                                      signature::Signature<>& signatures, candidates::Candidates& candidates) {
-  static_assert(D == CandidatesDomain::Query, "the driver path refines in the Query domain");
+  // The reference's Data instantiation runs the Query-layout loop on a transposed bitset
+  // (isomorphism.hpp:64-127 never reads D); only the Query domain is meaningful here.
+  static_assert(D == CandidatesDomain::Query, "refineCandidates works on Query-domain candidates");
   utils::EventScope ev(queue.get());
   detail::check(sigmo_candidates_refine(queue.get(), candidates.handle(), query_graph.handle, data_graph.handle, signatures.handle()));
   return ev.done();

@@ -137,6 +137,11 @@ int sigmo_signatures_generate(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_g
 /* refineCSRSignatures<PowerGraph>(view_size) (signature.hpp:336-390). */
 int sigmo_signatures_refine(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope, uint32_t view_size);
 /* Copy the packed 16x4-bit signatures (SignatureDevice::signature) of a scope. */
+/* refineCSRSignatures<Algorithm::ViewBased>(view_size) (signature.hpp:298-334): every node adds,
+ * per CSR entry to a neighbour u and label l, (uint8)(before[u][l] - (l == own label ?
+ * view_size : 0)) through incrementLabelCount's uint8 arithmetic.  Accumulates onto the current
+ * signatures, as the reference does. */
+int sigmo_signatures_refine_view_based(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope, uint32_t view_size);
 int sigmo_signatures_download(sigmo_ctx* ctx, const sigmo_signatures* s, int scope, uint64_t* out, uint64_t count);
 /* getDataSignatureAllocationSize / getQuerySignatureAllocationSize (signature.hpp:181-198). */
 int sigmo_signatures_alloc_size(const sigmo_signatures* s, int scope, uint64_t* bytes);
@@ -149,6 +154,10 @@ int sigmo_candidates_free(sigmo_candidates* c);
 int sigmo_candidates_filter(sigmo_ctx* ctx, sigmo_candidates* c, const sigmo_graph* query, const sigmo_graph* data,
                             const sigmo_signatures* s);
 /* refineCandidates<Query> (isomorphism.hpp:64-127) with the current-level signatures. */
+/* filterCandidates<CandidatesDomain::Data> (isomorphism.hpp:22-62, :49-52): the same label
+ * test stored transposed -- create the candidates with source_nodes = data nodes and
+ * target_nodes = query nodes; row d lists the query nodes data node d may match. */
+int sigmo_candidates_filter_data(sigmo_ctx* ctx, sigmo_candidates* c, const sigmo_graph* query, const sigmo_graph* data);
 int sigmo_candidates_refine(sigmo_ctx* ctx, sigmo_candidates* c, const sigmo_graph* query, const sigmo_graph* data,
                             const sigmo_signatures* s);
 /* Download in the reference layout: uint32 words, row stride ceil(target/32)

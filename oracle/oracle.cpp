@@ -251,6 +251,43 @@ void oracle_refine_signatures(uint32_t G, const uint32_t* go, const uint32_t* ro
   }
 }
 
+// refineCSRSignatures<ViewBased>, signature.hpp:298-334: signatures are copied (tmp_buff), then
+// every node adds, for each CSR entry to a neighbour u and each label l < 16, the count
+// c = (uint8)(tmp[u][l] - (l == label(v) ? view_size : 0)) when c > 0 -- through
+// incrementLabelCount, whose uint8 sum makes a wrapped c act as a decrement (:44-51).
+// mutation 1: the neighbour's count is not reduced for the node's own label.
+void oracle_refine_view_based(uint32_t G, const uint32_t* go, const uint32_t* ro, const uint32_t* ci, const uint8_t* nl,
+                              uint64_t view_size, uint64_t* sigs, int mutation) {
+  const Graph g = make_graph(G, go, ro, ci, nl);
+  const int64_t N = static_cast<int64_t>(g.N());
+  std::vector<u64> tmp(sigs, sigs + N);
+#pragma omp parallel for schedule(static)
+  for (int64_t v = 0; v < N; ++v) {
+    const u8 node_label = nl[v];
+    for (u32 i = ro[v]; i < ro[v + 1]; ++i) {
+      const u32 nb = ci[i];
+      for (u8 l = 0; l < 16; l++) {
+        u8 count = sig_get(tmp[nb], l);
+        if (l == node_label && mutation != 1) count = static_cast<u8>(count - view_size);
+        if (count > 0) sig_inc(sigs[v], l, count);
+      }
+    }
+  }
+}
+
+// filterCandidates<Data>, isomorphism.hpp:49-52: the same label test, stored transposed --
+// row = data node, column = query node (Candidates constructed with D rows and Q targets).
+void oracle_filter_data(uint64_t Q, const uint8_t* qlabels, uint64_t D, const uint8_t* dlabels, uint32_t* words) {
+  const u64 stride = (Q + 31) / 32;
+  const Cands c{words, stride};
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t d = 0; d < static_cast<int64_t>(D); ++d)
+    for (u64 q = 0; q < Q; ++q) {
+      if (qlabels[q] != dlabels[d] && qlabels[q] != kWildcard) continue;
+      c.insert(static_cast<u32>(d), static_cast<u32>(q));
+    }
+}
+
 // filterCandidates<Query>, isomorphism.hpp:22-62: d in C(q) iff the labels are equal or the
 // query label is the wildcard.  ORs into `words` (zeroed by the Candidates constructor,
 // candidates.hpp:165-179).  Rows are independent, so the loop runs over rows.

@@ -50,6 +50,8 @@ def lib() -> C.CDLL:
             "oracle_gmcr_offsets": ([u32, vp, u32, vp, u64, vp, vp], i64),
             "oracle_gmcr_indices": ([u32, vp, u32, vp, u64, vp, vp, vp], None),
             "oracle_join": ([u32, vp, vp, vp, u32, vp, vp, vp, u64, vp, vp, vp, i32, vp, i32], i64),
+            "oracle_refine_view_based": ([u32, vp, vp, vp, vp, u64, vp, i32], None),
+            "oracle_filter_data": ([u64, vp, u64, vp, vp], None),
             "oracle_join_wildcard": ([u32, vp, vp, vp, vp, u32, vp, vp, vp, vp, u64, vp, vp, vp, i32, u32, vp, i32], i64),
         }
         for name, (args, res) in sigs.items():
@@ -171,6 +173,20 @@ def join(q: dict, d: dict, words: np.ndarray, offsets: np.ndarray, indices: np.n
     if total < 0:
         raise ValueError("join failed: query too large or per-pair count overflow")
     return int(total), counts
+
+
+def refine_view_based(g: dict, view_size: int, sigs: np.ndarray, mutation: int = 0) -> None:
+    """refineCSRSignatures<ViewBased> (library/include/signature.hpp:298-334), in place."""
+    lib().oracle_refine_view_based(_G(g), _p(g["graph_offsets"]), _p(g["row_offsets"]), _p(g["column_indices"]),
+                                   _p(g["node_labels"]), view_size, _p(sigs), mutation)
+
+
+def filter_candidates_data(q: dict, d: dict) -> np.ndarray:
+    """filterCandidates<CandidatesDomain::Data>: reference-layout words, rows = data nodes."""
+    Q, D = _N(q), _N(d)
+    words = np.zeros(D * ((Q + 31) // 32), np.uint32)
+    lib().oracle_filter_data(Q, _p(q["node_labels"]), D, _p(d["node_labels"]), _p(words))
+    return words
 
 
 def join_wildcard(q: dict, d: dict, words: np.ndarray, offsets: np.ndarray, indices: np.ndarray, find_first: bool = True,

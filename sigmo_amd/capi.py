@@ -47,7 +47,7 @@ EXPORTED = (
     "sigmo_host_graphs_slice", "sigmo_host_graphs_concat", "sigmo_host_graphs_free", "sigmo_gen_molecules",
     "sigmo_gen_queries", "sigmo_gen_arrivals", "sigmo_graph_upload", "sigmo_graph_free", "sigmo_graph_info",
     "sigmo_graph_alloc_size", "sigmo_signatures_create", "sigmo_signatures_free", "sigmo_signatures_generate",
-    "sigmo_signatures_refine", "sigmo_signatures_download", "sigmo_signatures_alloc_size", "sigmo_candidates_create",
+    "sigmo_signatures_refine", "sigmo_signatures_download", "sigmo_signatures_refine_view_based", "sigmo_candidates_filter_data", "sigmo_signatures_alloc_size", "sigmo_candidates_create",
     "sigmo_candidates_free", "sigmo_candidates_filter", "sigmo_candidates_refine", "sigmo_candidates_download",
     "sigmo_candidates_counts", "sigmo_candidates_alloc_size", "sigmo_gmcr_create", "sigmo_gmcr_free",
     "sigmo_gmcr_generate", "sigmo_gmcr_total", "sigmo_gmcr_download", "sigmo_join", "sigmo_join_wildcard", "sigmo_pipeline_run",
@@ -205,6 +205,8 @@ def lib() -> C.CDLL:
             "sigmo_signatures_generate": ([vp, vp, vp, i32], i32),
             "sigmo_signatures_refine": ([vp, vp, vp, i32, u32], i32),
             "sigmo_signatures_download": ([vp, vp, i32, vp, u64], i32),
+            "sigmo_signatures_refine_view_based": ([vp, vp, vp, i32, u32], i32),
+            "sigmo_candidates_filter_data": ([vp, vp, vp, vp], i32),
             "sigmo_signatures_alloc_size": ([vp, i32, C.POINTER(u64)], i32),
             "sigmo_candidates_create": ([vp, u64, u64, pvp], i32),
             "sigmo_candidates_free": ([vp], i32),
@@ -498,6 +500,10 @@ class Signature:
     def refine_query(self, g: DeviceGraph, view_size: int = 1):
         check(lib().sigmo_signatures_refine(self.ctx.handle, self._h, g.handle, SCOPE_QUERY, view_size))
 
+    def refine_view_based(self, g: DeviceGraph, scope: int, view_size: int = 1):
+        """refineCSRSignatures<Algorithm::ViewBased> on the data or query scope."""
+        check(lib().sigmo_signatures_refine_view_based(self.ctx.handle, self._h, g.handle, scope, view_size))
+
     def download(self, scope: int) -> np.ndarray:
         n = self.data_nodes if scope == SCOPE_DATA else self.query_nodes
         out = np.zeros(n, np.uint64)
@@ -543,6 +549,11 @@ class Candidates:
 
 def filter_candidates(ctx: Context, query: DeviceGraph, data: DeviceGraph, sig: Signature, cand: Candidates):
     check(lib().sigmo_candidates_filter(ctx.handle, cand.handle, query.handle, data.handle, sig.handle))
+
+
+def filter_candidates_data(ctx: Context, query: DeviceGraph, data: DeviceGraph, cand: Candidates):
+    """filterCandidates<CandidatesDomain::Data>: cand = Candidates(ctx, data nodes, query nodes)."""
+    check(lib().sigmo_candidates_filter_data(ctx.handle, cand.handle, query.handle, data.handle))
 
 
 def refine_candidates(ctx: Context, query: DeviceGraph, data: DeviceGraph, sig: Signature, cand: Candidates):

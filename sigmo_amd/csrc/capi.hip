@@ -203,6 +203,8 @@ struct sigmo_signatures {
   uint64_t capD = 0;  // data-side capacity in nodes (a rebind to a smaller data graph reuses it)
   uint64_t *dsig = nullptr, *qsig = nullptr, *dreach = nullptr, *qreach = nullptr, *dplanes = nullptr;
   uint64_t *dfront = nullptr, *qfront = nullptr;
+  uint64_t* tmp = nullptr;  // ViewBased refinement: the signatures before the pass
+  uint64_t cap_tmp = 0;
   RefineState dstate, qstate;
   ~sigmo_signatures() {
     dev_free(dsig);
@@ -212,6 +214,7 @@ struct sigmo_signatures {
     dev_free(dplanes);
     dev_free(dfront);
     dev_free(qfront);
+    dev_free(tmp);
   }
 };
 
@@ -701,6 +704,17 @@ int sigmo_signatures_refine(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_gra
   return ps.close();
 }
 
+int sigmo_signatures_refine_view_based(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope, uint32_t view_size) {
+  SIGMO_TRY(bind(ctx));
+  uint64_t *sigs, *reach, *planes;
+  SIGMO_TRY(sig_target(s, g, scope, &sigs, &reach, &planes));
+  SIGMO_TRY(grow(&s->tmp, s->cap_tmp, g->N));
+  PhaseScope ps(ctx, scope == SIGMO_SCOPE_DATA ? SIGMO_PHASE_SIG_DATA : SIGMO_PHASE_SIG_QUERY, -1, 0.0);
+  if (g->N) SIGMO_HIP_TRY(hipMemcpyAsync(s->tmp, sigs, g->N * 8, hipMemcpyDeviceToDevice, ctx->stream));  // tmp_buff (:306-309)
+  SIGMO_HIP_TRY(sigmo_k::sig_refine_view_based(ctx->stream, g->dev(), view_size, s->tmp, sigs, planes));
+  return ps.close();
+}
+
 int sigmo_signatures_download(sigmo_ctx* ctx, const sigmo_signatures* s, int scope, uint64_t* out, uint64_t count) {
   SIGMO_TRY(bind(ctx));
   SIGMO_TRY(check_scope(scope));
@@ -766,6 +780,16 @@ int sigmo_candidates_filter(sigmo_ctx* ctx, sigmo_candidates* c, const sigmo_gra
   (void)s;  // signatures are captured but unused by the reference filter (isomorphism.hpp:39-44)
   PhaseScope ps(ctx, SIGMO_PHASE_FILTER, SIGMO_KERNEL_FILTER, 8.0 * c->Q * c->W64 + c->D + c->Q);
   SIGMO_HIP_TRY(sigmo_k::filter(ctx->stream, query->nl, c->Q, data->lplanes, c->D, c->bits, ctx->num_cus));
+  return ps.close();
+}
+
+int sigmo_candidates_filter_data(sigmo_ctx* ctx, sigmo_candidates* c, const sigmo_graph* query, const sigmo_graph* data) {
+  SIGMO_TRY(bind(ctx));
+  if (!c || !query || !data) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  if (c->Q != data->N || c->D != query->N)
+    return sigmo_err::set(SIGMO_ERR_INVALID, "data-domain candidates need data-node rows and query-node columns");
+  PhaseScope ps(ctx, SIGMO_PHASE_FILTER, -1, 0.0);
+  SIGMO_HIP_TRY(sigmo_k::filter_data(ctx->stream, query->nl, query->N, data->nl, data->N, c->bits));
   return ps.close();
 }
 

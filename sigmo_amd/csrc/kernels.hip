@@ -169,6 +169,60 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_generate(const uint32_t* __re
   if (planes) write_planes<kSigBlock>(s, (N + 63) / 64, planes, tile);
 }
 
+// incrementLabelCount(label, add) with the reference's uint8 arithmetic (signature.hpp:44-51,
+// :32-37): below 15 only, and the new count (cur + add) mod 256 is stored only if < 16, so an
+// add that wrapped below zero acts as a decrement.
+__device__ __forceinline__ uint64_t sig_add(uint64_t sig, uint32_t label, uint32_t add) {
+  const uint32_t sh = 4 * label;
+  const uint32_t c = static_cast<uint32_t>(sig >> sh) & 15u;
+  const uint32_t nv = (c + add) & 255u;
+  if (c < 15 && nv < 16) sig = (sig & ~(15ull << sh)) | (static_cast<uint64_t>(nv) << sh);
+  return sig;
+}
+
+// refineCSRSignatures<ViewBased>(view_size), signature.hpp:298-334: for each CSR entry to a
+// neighbour u and each label l < 16, c = (uint8)(tmp[u][l] - (l == label(v) ? view_size : 0))
+// is added when nonzero; `tmp` is the signatures before this pass.
+__global__ __launch_bounds__(kSigBlock) void k_sig_refine_view_based(const uint32_t* __restrict__ ro, const uint32_t* __restrict__ ci,
+                                                                     const uint8_t* __restrict__ nl, uint64_t N, uint32_t view_size,
+                                                                     const uint64_t* __restrict__ tmp, uint64_t* __restrict__ sigs,
+                                                                     uint64_t* __restrict__ planes) {
+  __shared__ uint64_t tile[64 * (kSigBlock / kWave)];
+  const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  uint64_t s = 0;
+  if (v < N) {
+    s = sigs[v];
+    const uint32_t own = nl[v];
+    for (uint32_t i = ro[v]; i < ro[v + 1]; ++i) {
+      const uint64_t t = tmp[ci[i]];
+      for (uint32_t l = 0; l < 16; ++l) {
+        uint32_t c = static_cast<uint32_t>(t >> (4 * l)) & 15u;
+        if (l == own) c = (c - view_size) & 255u;
+        if (c) s = sig_add(s, l, c);
+      }
+    }
+    sigs[v] = s;
+  }
+  if (planes) write_planes<kSigBlock>(s, (N + 63) / 64, planes, tile);
+}
+
+// filterCandidates<Data>, isomorphism.hpp:49-52: row = data node, word w of the row = query
+// nodes 64w..64w+63 whose label equals the data node's or is the wildcard.
+__global__ __launch_bounds__(256) void k_filter_data(const uint8_t* __restrict__ qlabels, uint64_t Q, const uint8_t* __restrict__ dlabels,
+                                                     uint64_t D, uint64_t* __restrict__ bits) {
+  const uint64_t WQ = (Q + 63) / 64;
+  const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (i >= D * WQ) return;
+  const uint64_t dn = i / WQ, w = i % WQ;
+  const uint8_t dl = dlabels[dn];
+  uint64_t word = 0;
+  for (uint32_t b = 0; b < 64 && w * 64 + b < Q; ++b) {
+    const uint8_t ql = qlabels[w * 64 + b];
+    if (ql == dl || ql == 255u) word |= 1ull << b;
+  }
+  bits[i] = word;
+}
+
 // refineCSRSignatures<PowerGraph>(view_size), signature.hpp:336-390.  The reference's
 // Bitset<uint64_t> holds local ids mod 64 and expands bit b as node go[g]+b, so the BFS
 // runs over the folded adjacency adj_lo|adj_hi of node go[g]+b.  The level signature is the
@@ -511,6 +565,20 @@ hipError_t sig_generate(hipStream_t s, const GraphDev& g, uint64_t* sigs, uint64
   if (!g.N) return hipSuccess;
   const unsigned blocks = static_cast<unsigned>((g.N + kSigBlock - 1) / kSigBlock);
   k_sig_generate<<<blocks, kSigBlock, 0, s>>>(g.ro, g.ci, g.nl, g.N, sigs, planes);
+  return hipGetLastError();
+}
+
+hipError_t sig_refine_view_based(hipStream_t s, const GraphDev& g, uint32_t view_size, const uint64_t* tmp, uint64_t* sigs, uint64_t* planes) {
+  if (!g.N) return hipSuccess;
+  const unsigned blocks = static_cast<unsigned>((g.N + kSigBlock - 1) / kSigBlock);
+  k_sig_refine_view_based<<<blocks, kSigBlock, 0, s>>>(g.ro, g.ci, g.nl, g.N, view_size, tmp, sigs, planes);
+  return hipGetLastError();
+}
+
+hipError_t filter_data(hipStream_t s, const uint8_t* qlabels, uint64_t Q, const uint8_t* dlabels, uint64_t D, uint64_t* bits) {
+  const uint64_t n = D * ((Q + 63) / 64);
+  if (!n) return hipSuccess;
+  k_filter_data<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(qlabels, Q, dlabels, D, bits);
   return hipGetLastError();
 }
 

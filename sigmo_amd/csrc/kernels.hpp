@@ -55,6 +55,11 @@ hipError_t sig_refine(hipStream_t s, const GraphDev& g, uint32_t view_size, uint
 // The same for view_size = M+1 when reach/frontier hold the state left by view_size = M.
 hipError_t sig_refine_next(hipStream_t s, const GraphDev& g, uint64_t* sigs, uint64_t* reach, uint64_t* frontier, uint64_t* planes);
 
+// refineCSRSignatures<ViewBased>(view_size): tmp holds the signatures before the pass.
+hipError_t sig_refine_view_based(hipStream_t s, const GraphDev& g, uint32_t view_size, const uint64_t* tmp, uint64_t* sigs, uint64_t* planes);
+// filterCandidates<Data>: the D x ceil(Q/64) transposed bitset (rows = data nodes).
+hipError_t filter_data(hipStream_t s, const uint8_t* qlabels, uint64_t Q, const uint8_t* dlabels, uint64_t D, uint64_t* bits);
+
 // filterCandidates<Query>: overwrites every word of the Q x W64 bitset.
 hipError_t filter(hipStream_t s, const uint8_t* qlabels, uint64_t Q, const uint64_t* lplanes, uint64_t D, uint64_t* bits,
                   int num_cus);
