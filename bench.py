@@ -237,6 +237,9 @@ def main() -> int:
                     help="config 3: size chunks so the plain form's candidate bitset of one chunk fits this many GB")
     ap.add_argument("--chunk-molecules", type=int, default=0, help="config 3: molecules per chunk (overrides --chunk-gb)")
     ap.add_argument("--no-overlap", action="store_true", help="config 3: load chunks inline instead of on a side stream")
+    ap.add_argument("--autotune", action="store_true", help="batched form: pick launch shapes with the autotuner before timing")
+    ap.add_argument("--graph", action="store_true",
+                    help="batched form: capture one whole step (database refresh included) as a hipGraph and time its replays")
     args = ap.parse_args()
     if args.serve and args.config == 1 and "--config" not in sys.argv:
         args.config = 4
@@ -300,10 +303,26 @@ def main() -> int:
         data_nodes = dd.total_nodes
         db = capi.Database(ctx, dd, args.refinement_steps)
         batch = capi.Batch(ctx, db)
-
-        def step():
+        if args.autotune:
+            print("[bench] autotuning", file=sys.stderr, flush=True)
+            args.tuning, _ = batch.autotune(qd, find_first)
+        def eager_step():
             db.refresh()  # the data side is recomputed every step, as the plain form does
             return batch.run(qd, find_first)
+
+        step = eager_step
+        if args.graph:
+            # per-kernel statistics come from eager steps of the same kernels (a graph has no
+            # per-kernel events); the timed steps are replays of the captured step
+            ctx.reset_phases()
+            for _ in range(max(1, args.warmup)):
+                eager_step()
+            ctx.sync()
+            graph_kstats, graph_phase_ms, graph_runs = ctx.kernel_stats(), ctx.phase_ms(), max(1, args.warmup)
+            batch.capture(qd, find_first, refresh_db=True)
+
+            def step():
+                return batch.replay()  # the captured step refreshes the data side too
     print(f"[bench] uploaded: Q={qd.total_nodes} query nodes, D={data_nodes} data nodes; warmup {args.warmup}",
           file=sys.stderr, flush=True)
     for _ in range(args.warmup):
@@ -325,6 +344,11 @@ def main() -> int:
     kstats = ctx.kernel_stats()
     phase_ms = ctx.phase_ms()
     phase_ms = {k: round(phase_ms[k] / args.steps, 4) for k in capi.PHASE_NAMES}
+    if getattr(args, "graph", False) and args.form == "batched" and args.config != 3:
+        # eager-step statistics, scaled to one step (launch counts per step stay integral)
+        kstats = {k: (n * args.steps / graph_runs, ms * args.steps / graph_runs, by * args.steps / graph_runs)
+                  for k, (n, ms, by) in graph_kstats.items()}
+        phase_ms = {k: round(graph_phase_ms[k] / graph_runs, 4) for k in capi.PHASE_NAMES}
     kernels = {}
     for name, (n, ms, by) in kstats.items():
         if n:
@@ -388,6 +412,7 @@ def main() -> int:
             **({k: results[-1][k] for k in ("chunks", "wall_ms", "compute_ms", "load_ms", "load_wait_ms")}
                if args.config == 3 else {}),
             "form": args.form,
+            **({"tuning": getattr(args, "tuning", None), "graph": bool(args.graph)} if args.form == "batched" else {}),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

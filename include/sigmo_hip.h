@@ -255,6 +255,13 @@ typedef struct sigmo_batch_result {
   uint64_t filter_scans;      /* pair evaluation: 64-pair pre-test scans */
   uint64_t join_wave_steps;   /* join loop iterations summed over wavefronts */
 } sigmo_batch_result;
+/* Launch shape of the batched form's persistent kernels; 0 fields keep the built-in choice. */
+typedef struct sigmo_batch_tuning {
+  uint32_t filter_blocks_per_cu; /* pair evaluation: 4-wavefront blocks per CU (capped by residency); 0: all that stay resident (<= 8) */
+  uint32_t filter_claim;         /* molecules a wavefront claims at a time; 0: ~count / (4 x wavefronts), 1..16 */
+  uint32_t join_blocks_per_cu;   /* lane join: 2-wavefront blocks per CU (1..4); 0: 4 */
+  uint32_t join_chunk;           /* GMCR pairs per claim of the lane join; 0: ~P / (16 x wavefronts), 64..1024 */
+} sigmo_batch_tuning;
 int sigmo_db_prepare(sigmo_ctx* ctx, const sigmo_graph* data, uint32_t refinement_steps, sigmo_db** out);
 /* Recompute the level planes from the data graph's current device contents (same result as a
  * fresh sigmo_db_prepare; no allocation). */
@@ -268,6 +275,21 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
 int sigmo_batch_pairs(sigmo_batch* b, uint32_t* data_graph_offsets, uint64_t n_offsets, uint32_t* query_graph_indices,
                       int32_t* counts, uint64_t n_pairs);
 int sigmo_batch_free(sigmo_batch* b);
+int sigmo_batch_set_tuning(sigmo_batch* b, const sigmo_batch_tuning* t);
+int sigmo_batch_get_tuning(const sigmo_batch* b, sigmo_batch_tuning* t);
+/* Autotuner: times sigmo_batch_run over a small grid of launch shapes (pair evaluation first,
+ * then the join), each the best of `repeats` runs, and keeps the fastest in the batch.  Results
+ * do not depend on the tuning.  *best_ms (nullable) receives the winner's time. */
+int sigmo_batch_autotune(sigmo_batch* b, const sigmo_graph* query, int find_first, uint32_t repeats, sigmo_batch_tuning* chosen,
+                         double* best_ms);
+/* One hipGraph for a whole run: sigmo_batch_capture runs the batch once to size every buffer,
+ * then records [the database refresh when refresh_db,] query preparation, pair evaluation,
+ * GMCR and join, with their sizes fixed, into a graph; sigmo_batch_replay launches it and reads
+ * the results (sigmo_batch_pairs works after it).  The query graph and database must stay
+ * allocated; if their contents changed so that the pair counts differ from the captured run, the
+ * replay returns SIGMO_ERR_INVALID (capture again). */
+int sigmo_batch_capture(sigmo_batch* b, const sigmo_graph* query, int find_first, int refresh_db);
+int sigmo_batch_replay(sigmo_batch* b, sigmo_batch_result* res);
 /* Per-request results of the last run, for a batch whose query graphs are the concatenation of
  * requests: request r owns graphs [request_graph_offsets[r], request_graph_offsets[r+1]).  The
  * hash is order-independent over the request's (molecule, request-local query graph, count)

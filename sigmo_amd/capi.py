@@ -55,6 +55,7 @@ EXPORTED = (
     "sigmo_kernel_stats", "sigmo_event_mark", "sigmo_event_elapsed", "sigmo_db_prepare", "sigmo_db_free",
     "sigmo_db_alloc_size", "sigmo_db_refresh", "sigmo_batch_create", "sigmo_batch_run", "sigmo_batch_pairs", "sigmo_batch_free",
     "sigmo_batch_request_stats", "sigmo_batch_workspace_bytes", "sigmo_serve_schedule", "sigmo_serve", "sigmo_stream_run",
+    "sigmo_batch_set_tuning", "sigmo_batch_get_tuning", "sigmo_batch_autotune", "sigmo_batch_capture", "sigmo_batch_replay",
 )
 KERNEL_NAMES = ("sig_generate", "sig_refine", "filter", "refine", "gmcr", "join", "pair_filter", "db_prepare")
 
@@ -88,6 +89,11 @@ class BatchResult(C.Structure):
         ("filter_scans", C.c_uint64),
         ("join_wave_steps", C.c_uint64),
     ]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("filter_blocks_per_cu", C.c_uint32), ("filter_claim", C.c_uint32), ("join_blocks_per_cu", C.c_uint32),
+                ("join_chunk", C.c_uint32)]
 
 
 class RequestStats(C.Structure):
@@ -239,6 +245,11 @@ def lib() -> C.CDLL:
             "sigmo_batch_pairs": ([vp, vp, u64, vp, vp, u64], i32),
             "sigmo_batch_free": ([vp], i32),
             "sigmo_batch_request_stats": ([vp, vp, u32, C.POINTER(RequestStats)], i32),
+            "sigmo_batch_set_tuning": ([vp, C.POINTER(BatchTuning)], i32),
+            "sigmo_batch_get_tuning": ([vp, C.POINTER(BatchTuning)], i32),
+            "sigmo_batch_autotune": ([vp, vp, i32, u32, C.POINTER(BatchTuning), C.POINTER(dbl)], i32),
+            "sigmo_batch_capture": ([vp, vp, i32, i32], i32),
+            "sigmo_batch_replay": ([vp, C.POINTER(BatchResult)], i32),
             "sigmo_batch_workspace_bytes": ([vp, u32, u64, C.POINTER(u64)], i32),
             "sigmo_serve_schedule": ([u32, vp, vp, u32, C.POINTER(ServeOptions), SERVE_EXECUTOR, vp, C.POINTER(ServeResult),
                                       C.POINTER(ServeSummary)], i32),
@@ -719,14 +730,45 @@ class Batch:
         if _lib is not None:
             self.free()
 
-    def run(self, query: DeviceGraph, find_first: bool = True) -> dict:
-        r = BatchResult()
-        check(lib().sigmo_batch_run(self._h, query.handle, int(find_first), C.byref(r)))
+    def _store(self, r: BatchResult) -> dict:
         self.last = {"num_matches": r.num_matches, "matched_pairs": r.matched_pairs, "gmcr_pairs": r.gmcr_pairs,
                      "dfs_steps": r.dfs_steps, "filter_wave_steps": r.filter_wave_steps,
                      "filter_lane_steps": r.filter_lane_steps, "filter_scans": r.filter_scans,
                      "join_wave_steps": r.join_wave_steps}
         return self.last
+
+    def run(self, query: DeviceGraph, find_first: bool = True) -> dict:
+        r = BatchResult()
+        check(lib().sigmo_batch_run(self._h, query.handle, int(find_first), C.byref(r)))
+        return self._store(r)
+
+    def capture(self, query: DeviceGraph, find_first: bool = True, refresh_db: bool = False) -> None:
+        """Record one whole run (sized by a plain run first) as a hipGraph."""
+        self._captured_query = query  # the graph holds its device addresses
+        check(lib().sigmo_batch_capture(self._h, query.handle, int(find_first), int(refresh_db)))
+
+    def replay(self) -> dict:
+        r = BatchResult()
+        check(lib().sigmo_batch_replay(self._h, C.byref(r)))
+        return self._store(r)
+
+    @property
+    def tuning(self) -> dict:
+        t = BatchTuning()
+        check(lib().sigmo_batch_get_tuning(self._h, C.byref(t)))
+        return {name: getattr(t, name) for name, _ in BatchTuning._fields_}
+
+    @tuning.setter
+    def tuning(self, values: dict) -> None:
+        t = BatchTuning(**values)
+        check(lib().sigmo_batch_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, query: DeviceGraph, find_first: bool = True, repeats: int = 2):
+        """Pick launch shapes by timing runs; returns (tuning dict, best ms)."""
+        t = BatchTuning()
+        ms = C.c_double()
+        check(lib().sigmo_batch_autotune(self._h, query.handle, int(find_first), repeats, C.byref(t), C.byref(ms)))
+        return {name: getattr(t, name) for name, _ in BatchTuning._fields_}, ms.value
 
     def request_stats(self, request_graph_offsets) -> list[dict]:
         """Per-request matched pairs, embeddings and order-independent hash of the last run;

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <chrono>
 #include <vector>
 
 #include "../../include/sigmo_hip.h"
@@ -628,16 +629,20 @@ __global__ __launch_bounds__(256) void k_surv_count(uint32_t Gd, uint32_t W, con
 __global__ __launch_bounds__(256) void k_surv_scatter(uint32_t Gd, uint32_t Gq, uint32_t W, const unsigned long long* __restrict__ surv,
                                                       const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ pair_off,
                                                       uint32_t* __restrict__ indices, uint32_t* __restrict__ pair_graph,
-                                                      uint32_t* __restrict__ mask_off) {
+                                                      uint32_t* __restrict__ mask_off, uint64_t cap) {
+  // `cap` bounds the writes: a replayed graph sized by an earlier run never writes past its
+  // arrays (the replay then reports the changed pair count)
   const uint64_t m = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   if (m >= Gd) return;
   uint32_t o = offsets[m];
   for (uint32_t w = 0; w < W; ++w)
     for (uint64_t x = surv[m * W + w]; x; x &= x - 1) {
       const uint32_t g = w * 64 + __builtin_ctzll(x);
-      indices[o] = g;
-      pair_graph[o] = static_cast<uint32_t>(m);
-      mask_off[o] = pair_off[m * Gq + g];
+      if (o < cap) {
+        indices[o] = g;
+        pair_graph[o] = static_cast<uint32_t>(m);
+        mask_off[o] = pair_off[m * Gq + g];
+      }
       ++o;
     }
 }
@@ -779,6 +784,7 @@ struct sigmo_db {
   sigmo_k::MolInfo* minfo = nullptr;                                                       // width-class order
   uint64_t bytes = 0;
   uint64_t cap_G = 0, cap_32 = 0, cap_64 = 0, cap_hi = 0, cap_dmax = 0, cap_lab = 0;  // allocated elements
+  uint64_t generation = 0;  // bumped whenever a rebind changes the data or any buffer address
   ~sigmo_db() {
     bfree(cls_list);
     bfree(pidx);
@@ -822,6 +828,27 @@ struct sigmo_batch {
   uint64_t cap_wide = 0;
   uint32_t* req_offsets = nullptr;        // per-request statistics
   unsigned long long* req_out = nullptr;
+  sigmo_batch_tuning tune{};               // 0 fields: the built-in heuristics
+  // a captured run (sigmo_batch_capture): the graph, its sizes and its pinned readback
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  unsigned long long* graph_host = nullptr;  // [0..7] filter counters, [8] GMCR total, [9..16] join stats
+  uint64_t graph_P = 0, graph_nw = 0, graph_db_generation = 0;
+  uint32_t graph_Gq = 0;
+  int graph_find_first = 0;
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+  }
+  // bgrow, dropping a captured graph whose buffer addresses a reallocation would invalidate
+  template <typename T>
+  int grow(T** p, uint64_t& cap, uint64_t need) {
+    if (*p && cap >= need) return SIGMO_OK;
+    drop_graph();
+    return bgrow(p, cap, need);
+  }
   uint64_t P = 0;
   uint32_t Gq = 0;
   bool ran = false;
@@ -844,6 +871,8 @@ struct sigmo_batch {
     bfree(wide_pairs);
     bfree(req_offsets);
     bfree(req_out);
+    drop_graph();
+    if (graph_host) (void)hipHostFree(graph_host);
   }
 };
 
@@ -887,6 +916,7 @@ int db_fill(sigmo_db* db, const sigmo_graph* data) {
   BATCH_TRY(bgrow(&db->planes_hi, db->cap_hi, uint64_t(nw) * R * 64));
   BATCH_TRY(bgrow(&db->lab_hi, db->cap_lab, uint64_t(nw) * 16));
   db->data = data;
+  db->generation++;
   db->num_wide = nw;
   for (int c = 0; c < 3; ++c) db->cls_count[c] = static_cast<uint32_t>(lists[c].size());
   db->bytes = G * (12ull + sizeof(sigmo_k::MolInfo)) + (uint64_t(n32) * 256 + (uint64_t(n64) + nw) * 512) * R + nw * 128ull +
@@ -1041,6 +1071,172 @@ int sigmo_batch_free(sigmo_batch* b) {
   return SIGMO_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Launch shape and sizes of one batched run, fixed before any launch.
+struct RunShape {
+  sigmo_k::GraphDev qd, dd;
+  uint32_t R = 0, Gq = 0, Gd = 0, W = 0;
+  uint64_t pairs = 0, ksum = 0;
+  int cus = 0;
+  uint32_t lds_bytes[3] = {0, 0, 0};
+  uint64_t cls_blocks[3] = {0, 0, 0}, waves_total = 0;
+};
+
+int shape_run(const sigmo_batch* b, const sigmo_graph* query, int cus, RunShape& L) {
+  const sigmo_db* db = b->db;
+  L.qd = graph_dev(query);
+  L.dd = graph_dev(db->data);
+  const auto& qgo = graph_host_offsets(query);
+  L.Gq = L.qd.G;
+  L.Gd = L.dd.G;
+  L.cus = cus;
+  for (uint32_t g = 0; g < L.Gq; ++g) {
+    const uint32_t k = qgo[g + 1] - qgo[g];
+    if (k < 2) return sigmo_err::set(SIGMO_ERR_INVALID, "query graph " + std::to_string(g) + " has fewer than 2 nodes");
+    if (k > SIGMO_MAX_QUERY_NODES) return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "query graph exceeds 32 nodes");
+    L.ksum += k;
+  }
+  L.R = db->R;
+  L.W = (L.Gq + 63) / 64;
+  L.pairs = uint64_t(L.Gq) * L.Gd;
+  // one persistent launch per width class, as many 4-wave blocks as registers and LDS keep
+  // resident (or the tuned count, capped by residency)
+  const void* kern[3] = {reinterpret_cast<const void*>(sigmo_k::k_pair_survive<1>), reinterpret_cast<const void*>(sigmo_k::k_pair_survive<2>),
+                         reinterpret_cast<const void*>(sigmo_k::k_pair_survive<4>)};
+  for (int c = 0; c < 3; ++c) L.lds_bytes[c] = sigmo_k::survive_lds_bytes(L.R, c == 0 ? 1 : (c == 1 ? 2 : 4));
+  for (int c = 0; c < 3; ++c) {
+    if (!db->cls_count[c]) continue;
+    if (L.lds_bytes[c] > 64 * 1024) return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "molecule slots exceed 64 KB of LDS per block");
+    int resident = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern[c], 256, L.lds_bytes[c]) != hipSuccess) {
+      (void)hipGetLastError();
+      resident = 2;
+    }
+    uint64_t per_cu = std::max<uint64_t>(1, std::min<uint64_t>(8, static_cast<uint64_t>(resident)));
+    if (b->tune.filter_blocks_per_cu) per_cu = std::min<uint64_t>(per_cu, b->tune.filter_blocks_per_cu);
+    L.cls_blocks[c] = std::max<uint64_t>(1, std::min<uint64_t>(uint64_t(cus) * per_cu, (uint64_t(db->cls_count[c]) + 3) / 4));
+    L.waves_total += L.cls_blocks[c] * sigmo_k::kSurviveWaves;
+  }
+  return SIGMO_OK;
+}
+
+// Pair evaluation: counters and survival bits cleared by kernels, one launch per width class.
+hipError_t launch_filter(sigmo_batch* b, const RunShape& L, hipStream_t s) {
+  const sigmo_db* db = b->db;
+  hipError_t e = sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(b->counters), 8, 0);
+  if (e == hipSuccess) e = sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(b->surv), uint64_t(L.Gd) * L.W, 0);
+  sigmo_k::SurviveArgs a;
+  a.q = L.qd;
+  a.d = L.dd;
+  a.R = L.R;
+  a.Gq = L.Gq;
+  a.W = L.W;
+  a.planes32 = db->planes32;
+  a.planes64 = db->planes64;
+  a.planes_hi = db->planes_hi;
+  a.lab_hi = db->lab_hi;
+  a.tok64 = reinterpret_cast<const uint64_t*>(b->tokens);
+  a.qinfo = b->qinfo;
+  a.qmax = b->qmax;
+  a.arena = b->arena;
+  a.arena_cap = b->cap_arena;
+  a.surv = b->surv;
+  a.pair_off = b->pair_off;
+  a.counters = b->counters;
+  uint64_t first = 0;
+  for (int c = 0; c < 3 && e == hipSuccess; ++c) {
+    a.minfo = db->minfo + first;
+    a.dmax = db->dmax + first * L.R;
+    a.count = db->cls_count[c];
+    first += db->cls_count[c];
+    if (!a.count || !L.Gq) continue;
+    const uint64_t auto_claim = std::max<uint64_t>(1, std::min<uint64_t>(16, a.count / (L.cls_blocks[c] * sigmo_k::kSurviveWaves * 4)));
+    a.claim = static_cast<uint32_t>(b->tune.filter_claim ? std::min<uint32_t>(b->tune.filter_claim, 64) : auto_claim);
+    e = sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(b->counters), 1, 0);  // each launch starts its own molecule cursor
+    if (e != hipSuccess) break;
+    const dim3 grid(static_cast<unsigned>(L.cls_blocks[c]));
+    if (c == 0) hipLaunchKernelGGL(sigmo_k::k_pair_survive<1>, grid, dim3(256), L.lds_bytes[0], s, a);
+    else if (c == 1) hipLaunchKernelGGL(sigmo_k::k_pair_survive<2>, grid, dim3(256), L.lds_bytes[1], s, a);
+    else hipLaunchKernelGGL(sigmo_k::k_pair_survive<4>, grid, dim3(256), L.lds_bytes[2], s, a);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+// GMCR: survivors per molecule and their exclusive scan (total in b->total).
+hipError_t launch_count_scan(sigmo_batch* b, const RunShape& L, hipStream_t s) {
+  if (L.Gd) {
+    hipLaunchKernelGGL(sigmo_k::k_surv_count, dim3((L.Gd + 255) / 256), dim3(256), 0, s, L.Gd, L.W, b->surv, b->mol_count);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return sigmo_k::exclusive_scan(s, b->mol_count, L.Gd, b->offsets, b->scan_tmp, b->total);
+}
+
+hipError_t launch_scatter(sigmo_batch* b, const RunShape& L, hipStream_t s) {
+  if (!L.Gd) return hipSuccess;
+  hipLaunchKernelGGL(sigmo_k::k_surv_scatter, dim3((L.Gd + 255) / 256), dim3(256), 0, s, L.Gd, L.Gq, L.W, b->surv, b->offsets, b->pair_off,
+                     b->indices, b->pair_graph, b->mask_off, std::min(b->cap_idx, std::min(b->cap_pg, b->cap_mo)));
+  return hipGetLastError();
+}
+
+// Lane join over P pairs (molecules of <= 64 nodes; the others are listed in wide_pairs).
+hipError_t launch_join(sigmo_batch* b, const RunShape& L, hipStream_t s, unsigned long long* stats, uint64_t P, int find_first) {
+  if (!P) return hipSuccess;
+  const uint64_t per_cu = b->tune.join_blocks_per_cu ? std::min<uint32_t>(b->tune.join_blocks_per_cu, 4) : 4;
+  const uint64_t lblocks = std::max<uint64_t>(1, std::min<uint64_t>((P + 127) / 128, static_cast<uint64_t>(L.cus) * per_cu));
+  const uint32_t lchunk = b->tune.join_chunk ? b->tune.join_chunk
+                                             : static_cast<uint32_t>(std::min<uint64_t>(
+                                                   1024, std::max<uint64_t>(64, P / (lblocks * sigmo_k::kJoinLaneWaves * 16))));
+  if (find_first)
+    hipLaunchKernelGGL((sigmo_k::k_join_lanes<true, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(lblocks)), dim3(sigmo_k::kJoinLaneWaves * 64),
+                       0, s, L.qd, L.dd, sigmo_k::ArenaMasks{b->arena, b->mask_off}, b->indices, b->pair_graph, P, lchunk, b->counts, stats,
+                       b->wide_pairs);
+  else
+    hipLaunchKernelGGL((sigmo_k::k_join_lanes<false, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(lblocks)), dim3(sigmo_k::kJoinLaneWaves * 64),
+                       0, s, L.qd, L.dd, sigmo_k::ArenaMasks{b->arena, b->mask_off}, b->indices, b->pair_graph, P, lchunk, b->counts, stats,
+                       b->wide_pairs);
+  return hipGetLastError();
+}
+
+// Lane join over the nw listed pairs of 65..128-node molecules (two-word positions).
+hipError_t launch_join_wide(sigmo_batch* b, const RunShape& L, hipStream_t s, unsigned long long* stats, uint64_t nw, int find_first) {
+  if (!nw) return hipSuccess;
+  uint64_t blocks = 0;
+  uint32_t chunk = 0;
+  sigmo_k::join_wide_grid(nw, L.cus, &blocks, &chunk);
+  const sigmo_k::ArenaMasks src{b->arena, b->mask_off};
+  if (find_first)
+    hipLaunchKernelGGL((sigmo_k::k_join_lanes_wide<true, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, L.qd, L.dd, src,
+                       b->indices, b->pair_graph, b->wide_pairs, nw, chunk, 6u, b->counts, stats);
+  else
+    hipLaunchKernelGGL((sigmo_k::k_join_lanes_wide<false, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, L.qd, L.dd,
+                       src, b->indices, b->pair_graph, b->wide_pairs, nw, chunk, 6u, b->counts, stats);
+  return hipGetLastError();
+}
+
+// Buffers of a run that do not depend on its results.
+int size_fixed_buffers(sigmo_batch* b, const RunShape& L) {
+  const uint64_t seg = sigmo_k::token_segment(L.R);
+  if (L.qd.N * seg >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "query batch too large for one batched run");
+  BATCH_TRY(b->grow(&b->tokens, b->cap_tokens, L.qd.N * seg));
+  BATCH_TRY(b->grow(&b->qinfo, b->cap_qinfo, L.Gq));
+  BATCH_TRY(b->grow(&b->qmax, b->cap_qmax, uint64_t(L.Gq) * L.R));
+  BATCH_TRY(b->grow(&b->surv, b->cap_surv, uint64_t(L.Gd) * L.W));
+  BATCH_TRY(b->grow(&b->pair_off, b->cap_pair_off, L.pairs));
+  BATCH_TRY(b->grow(&b->mol_count, b->cap_mc, uint64_t(L.Gd) + 1));
+  BATCH_TRY(b->grow(&b->offsets, b->cap_off, uint64_t(L.Gd) + 1));
+  BATCH_TRY(b->grow(&b->scan_tmp, b->cap_scan, (uint64_t(L.Gd) + 1023) / 1024 + 1));
+  return SIGMO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, sigmo_batch_result* res) {
   if (!b || !query || !res) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
   int device = 0, cus = 0;
@@ -1049,115 +1245,41 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
   BATCH_TRY(ctx_info(b->ctx, &device, &s, &cus, &stats, &stats_host));
   std::memset(res, 0, sizeof(*res));
   const sigmo_db* db = b->db;
-  const sigmo_k::GraphDev qd = graph_dev(query), dd = graph_dev(db->data);
-  const auto& qgo = graph_host_offsets(query);
-  const uint32_t Gq = qd.G, Gd = dd.G;
-  uint64_t ksum = 0;
-  for (uint32_t g = 0; g < Gq; ++g) {
-    const uint32_t k = qgo[g + 1] - qgo[g];
-    if (k < 2) return sigmo_err::set(SIGMO_ERR_INVALID, "query graph " + std::to_string(g) + " has fewer than 2 nodes");
-    if (k > SIGMO_MAX_QUERY_NODES) return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "query graph exceeds 32 nodes");
-    ksum += k;
-  }
-  const uint32_t R = db->R;
-  const uint32_t has_wide = db->num_wide ? 1u : 0u;
-  const uint32_t W = (Gq + 63) / 64;
-  const uint64_t pairs = uint64_t(Gq) * Gd;
-  // persistent grid: as many 4-wave blocks as stay resident
-  // one persistent launch per width class, as many 4-wave blocks as registers and LDS keep resident
-  const void* kern[3] = {reinterpret_cast<const void*>(sigmo_k::k_pair_survive<1>), reinterpret_cast<const void*>(sigmo_k::k_pair_survive<2>),
-                         reinterpret_cast<const void*>(sigmo_k::k_pair_survive<4>)};
-  const uint32_t lds_bytes[3] = {sigmo_k::survive_lds_bytes(R, 1), sigmo_k::survive_lds_bytes(R, 2), sigmo_k::survive_lds_bytes(R, 4)};
-  uint64_t cls_blocks[3] = {0, 0, 0}, waves_total = 0;
-  for (int c = 0; c < 3; ++c) {
-    if (!db->cls_count[c]) continue;
-    if (lds_bytes[c] > 64 * 1024) return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "molecule slots exceed 64 KB of LDS per block");
-    int resident = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern[c], 256, lds_bytes[c]) != hipSuccess) {
-      (void)hipGetLastError();
-      resident = 2;
-    }
-    const uint64_t per_cu = std::max<uint64_t>(1, std::min<uint64_t>(8, static_cast<uint64_t>(resident)));
-    cls_blocks[c] = std::max<uint64_t>(1, std::min<uint64_t>(uint64_t(cus) * per_cu, (uint64_t(db->cls_count[c]) + 3) / 4));
-    waves_total += cls_blocks[c] * sigmo_k::kSurviveWaves;
-  }
-  const uint64_t seg = sigmo_k::token_segment(R);
-  if (qd.N * seg >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "query batch too large for one batched run");
-  BATCH_TRY(bgrow(&b->tokens, b->cap_tokens, qd.N * seg));
-  BATCH_TRY(bgrow(&b->qinfo, b->cap_qinfo, Gq));
-  BATCH_TRY(bgrow(&b->qmax, b->cap_qmax, uint64_t(Gq) * R));
-  BATCH_TRY(bgrow(&b->surv, b->cap_surv, uint64_t(Gd) * W));
-  BATCH_TRY(bgrow(&b->pair_off, b->cap_pair_off, pairs));
-  BATCH_TRY(bgrow(&b->mol_count, b->cap_mc, uint64_t(Gd) + 1));
-  BATCH_TRY(bgrow(&b->offsets, b->cap_off, uint64_t(Gd) + 1));
-  BATCH_TRY(bgrow(&b->scan_tmp, b->cap_scan, (uint64_t(Gd) + 1023) / 1024 + 1));
+  RunShape L;
+  BATCH_TRY(shape_run(b, query, cus, L));
+  const uint32_t R = L.R, Gq = L.Gq, Gd = L.Gd, W = L.W;
+  BATCH_TRY(size_fixed_buffers(b, L));
   {
     void* tok = nullptr;
     BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_SIG_QUERY, -1, 0.0, true, &tok));
-    BATCH_HIP_TRY(sigmo_k::query_prepare(s, qd, R, b->tokens, b->qinfo, b->qmax));
+    BATCH_HIP_TRY(sigmo_k::query_prepare(s, L.qd, R, b->tokens, b->qinfo, b->qmax));
     BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
   }
   // arena capacity: start from 1/8 of the pairs (at most 2^26) plus every wavefront's open chunk.
   // On overflow the counter still holds the exact demand, so one replay with that size
   // suffices; the capacity is kept for later runs.
-  const double mean_k = Gq ? double(ksum) / Gq : 0.0;
+  const double mean_k = Gq ? double(L.ksum) / Gq : 0.0;
   // every wavefront may leave its chunk's tail and each lane its sub-chunk's tail unused
-  const uint64_t slack_arena = waves_total * 2 * sigmo_k::kWaveChunk;
-  const uint64_t guess = std::min<uint64_t>(std::max<uint64_t>(pairs / 8, 1 << 16), 1ull << 26);
-  uint64_t want_arena = static_cast<uint64_t>(double(guess) * (mean_k + 1.0) * (has_wide ? 2.0 : 1.0)) + slack_arena;
+  const uint64_t slack_arena = L.waves_total * 2 * sigmo_k::kWaveChunk;
+  const uint64_t guess = std::min<uint64_t>(std::max<uint64_t>(L.pairs / 8, 1 << 16), 1ull << 26);
+  uint64_t want_arena = static_cast<uint64_t>(double(guess) * (mean_k + 1.0) * (db->num_wide ? 2.0 : 1.0)) + slack_arena;
   want_arena = std::max(want_arena, b->cap_arena);
   for (int attempt = 0; attempt < 3; ++attempt) {
-    BATCH_TRY(bgrow(&b->arena, b->cap_arena, want_arena));
+    BATCH_TRY(b->grow(&b->arena, b->cap_arena, want_arena));
     if (b->cap_arena >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "candidate arena exceeds 2^32 words");
-
     // algorithmic reads: the data side once per run -- a label byte and R level signatures per
     // data node, (1 + 8R) D -- plus the query side; writes are added once they are known.
-    const double reads = (1.0 + 8.0 * R) * double(dd.N) + double(qd.N) * (1.0 + 2.0 * 16.0 * R) + double(Gq) * (4.0 + 8.0 * R);
+    const double reads = (1.0 + 8.0 * R) * double(L.dd.N) + double(L.qd.N) * (1.0 + 2.0 * 16.0 * R) + double(Gq) * (4.0 + 8.0 * R);
     void* tok = nullptr;
     BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_FILTER, SIGMO_KERNEL_PAIR_FILTER, reads, true, &tok));
-    BATCH_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(b->counters), 8, 0));
-    BATCH_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(b->surv), uint64_t(Gd) * W, 0));
-    sigmo_k::SurviveArgs a;
-    a.q = qd;
-    a.d = dd;
-    a.R = R;
-    a.Gq = Gq;
-    a.W = W;
-    a.planes32 = db->planes32;
-    a.planes64 = db->planes64;
-    a.planes_hi = db->planes_hi;
-    a.lab_hi = db->lab_hi;
-    a.tok64 = reinterpret_cast<const uint64_t*>(b->tokens);
-    a.qinfo = b->qinfo;
-    a.qmax = b->qmax;
-    a.arena = b->arena;
-    a.arena_cap = b->cap_arena;
-    a.surv = b->surv;
-    a.pair_off = b->pair_off;
-    a.counters = b->counters;
-    uint64_t first = 0;
-    for (int c = 0; c < 3; ++c) {
-      a.minfo = db->minfo + first;
-      a.dmax = db->dmax + first * R;
-      a.count = db->cls_count[c];
-      first += db->cls_count[c];
-      if (!a.count || !Gq) continue;
-      a.claim = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(16, a.count / (cls_blocks[c] * sigmo_k::kSurviveWaves * 4))));
-      // each launch starts its own molecule cursor
-      BATCH_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(b->counters), 1, 0));
-      const dim3 grid(static_cast<unsigned>(cls_blocks[c]));
-      if (c == 0) hipLaunchKernelGGL(sigmo_k::k_pair_survive<1>, grid, dim3(256), lds_bytes[0], s, a);
-      else if (c == 1) hipLaunchKernelGGL(sigmo_k::k_pair_survive<2>, grid, dim3(256), lds_bytes[1], s, a);
-      else hipLaunchKernelGGL(sigmo_k::k_pair_survive<4>, grid, dim3(256), lds_bytes[2], s, a);
-      BATCH_HIP_TRY(hipGetLastError());
-    }
+    BATCH_HIP_TRY(launch_filter(b, L, s));
     BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
     unsigned long long cnt[8];
     BATCH_HIP_TRY(hipMemcpyAsync(cnt, b->counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    BATCH_HIP_TRY(hipStreamSynchronize(s));
     res->filter_wave_steps = cnt[4];
     res->filter_lane_steps = cnt[5];
     res->filter_scans = cnt[6];
-    BATCH_HIP_TRY(hipStreamSynchronize(s));
     // writes: candidate masks, the survival bits and arena offsets
     ctx_add_bytes(b->ctx, SIGMO_KERNEL_PAIR_FILTER, double(cnt[1]) * 8.0 + double(Gd) * W * 8.0);
     if (!cnt[3]) break;
@@ -1169,61 +1291,27 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
   // canonical GMCR: count survivors per molecule, scan, scatter in (molecule, query graph) order
   void* tok = nullptr;
   BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_GMCR, -1, 0.0, true, &tok));
-  if (Gd) {
-    hipLaunchKernelGGL(sigmo_k::k_surv_count, dim3((Gd + 255) / 256), dim3(256), 0, s, Gd, W, b->surv, b->mol_count);
-    BATCH_HIP_TRY(hipGetLastError());
-  }
-  BATCH_HIP_TRY(sigmo_k::exclusive_scan(s, b->mol_count, Gd, b->offsets, b->scan_tmp, b->total));
+  BATCH_HIP_TRY(launch_count_scan(b, L, s));
   uint64_t P = 0;
   BATCH_HIP_TRY(hipMemcpyAsync(&P, b->total, 8, hipMemcpyDeviceToHost, s));
   BATCH_HIP_TRY(hipStreamSynchronize(s));
   if (P >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "GMCR pair count exceeds u32");
-  BATCH_TRY(bgrow(&b->indices, b->cap_idx, P));
-  BATCH_TRY(bgrow(&b->pair_graph, b->cap_pg, P));
-  BATCH_TRY(bgrow(&b->mask_off, b->cap_mo, P));
-  BATCH_TRY(bgrow(&b->counts, b->cap_counts, P));
-  BATCH_TRY(bgrow(&b->wide_pairs, b->cap_wide, db->num_wide ? P : 1));
-  if (Gd) {
-    hipLaunchKernelGGL(sigmo_k::k_surv_scatter, dim3((Gd + 255) / 256), dim3(256), 0, s, Gd, Gq, W, b->surv, b->offsets, b->pair_off,
-                       b->indices, b->pair_graph, b->mask_off);
-    BATCH_HIP_TRY(hipGetLastError());
-  }
+  BATCH_TRY(b->grow(&b->indices, b->cap_idx, P));
+  BATCH_TRY(b->grow(&b->pair_graph, b->cap_pg, P));
+  BATCH_TRY(b->grow(&b->mask_off, b->cap_mo, P));
+  BATCH_TRY(b->grow(&b->counts, b->cap_counts, P));
+  BATCH_TRY(b->grow(&b->wide_pairs, b->cap_wide, db->num_wide ? P : 1));
+  BATCH_HIP_TRY(launch_scatter(b, L, s));
   BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
-  // join
+  // join: lane per pair; pairs of molecules with > 64 nodes are listed for the two-word kernel
   BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_JOIN, SIGMO_KERNEL_JOIN, 0.0, true, &tok));
   BATCH_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(stats), 8, 0));
-  if (P) {
-    // lane per pair (molecules of <= 64 nodes); wider molecules' pairs are listed for the
-    // wavefront-per-pair kernel
-    const uint64_t lblocks = std::max<uint64_t>(1, std::min<uint64_t>((P + 127) / 128, static_cast<uint64_t>(cus) * 4));
-    const uint32_t lchunk = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(64, P / (lblocks * sigmo_k::kJoinLaneWaves * 16))));
-    if (find_first)
-      hipLaunchKernelGGL((sigmo_k::k_join_lanes<true, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(lblocks)),
-                         dim3(sigmo_k::kJoinLaneWaves * 64), 0, s, qd, dd, sigmo_k::ArenaMasks{b->arena, b->mask_off}, b->indices, b->pair_graph,
-                         P, lchunk, b->counts, stats, b->wide_pairs);
-    else
-      hipLaunchKernelGGL((sigmo_k::k_join_lanes<false, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(lblocks)),
-                         dim3(sigmo_k::kJoinLaneWaves * 64), 0, s, qd, dd, sigmo_k::ArenaMasks{b->arena, b->mask_off}, b->indices, b->pair_graph,
-                         P, lchunk, b->counts, stats, b->wide_pairs);
-    BATCH_HIP_TRY(hipGetLastError());
-    if (db->num_wide) {
-      unsigned long long nw = 0;
-      BATCH_HIP_TRY(hipMemcpyAsync(&nw, stats + 5, sizeof(nw), hipMemcpyDeviceToHost, s));
-      BATCH_HIP_TRY(hipStreamSynchronize(s));
-      if (nw) {  // lane per pair over the listed pairs, two-word positions
-        uint64_t blocks = 0;
-        uint32_t chunk = 0;
-        sigmo_k::join_wide_grid(nw, cus, &blocks, &chunk);
-        const sigmo_k::ArenaMasks src{b->arena, b->mask_off};
-        if (find_first)
-          hipLaunchKernelGGL((sigmo_k::k_join_lanes_wide<true, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, qd, dd,
-                             src, b->indices, b->pair_graph, b->wide_pairs, nw, chunk, 6u, b->counts, stats);
-        else
-          hipLaunchKernelGGL((sigmo_k::k_join_lanes_wide<false, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, qd, dd,
-                             src, b->indices, b->pair_graph, b->wide_pairs, nw, chunk, 6u, b->counts, stats);
-        BATCH_HIP_TRY(hipGetLastError());
-      }
-    }
+  BATCH_HIP_TRY(launch_join(b, L, s, stats, P, find_first));
+  if (P && db->num_wide) {
+    unsigned long long nw = 0;
+    BATCH_HIP_TRY(hipMemcpyAsync(&nw, stats + 5, sizeof(nw), hipMemcpyDeviceToHost, s));
+    BATCH_HIP_TRY(hipStreamSynchronize(s));
+    BATCH_HIP_TRY(launch_join_wide(b, L, s, stats, nw, find_first));
   }
   BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
   BATCH_HIP_TRY(hipMemcpyAsync(stats_host, stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
@@ -1237,6 +1325,151 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
   res->gmcr_pairs = P;
   res->dfs_steps = stats_host[4];
   res->join_wave_steps = stats_host[7];
+  return SIGMO_OK;
+}
+
+int sigmo_batch_set_tuning(sigmo_batch* b, const sigmo_batch_tuning* t) {
+  if (!b || !t) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  if (t->join_blocks_per_cu > 4) return sigmo_err::set(SIGMO_ERR_INVALID, "join_blocks_per_cu must be 0..4 (LDS keeps 4 resident)");
+  if (t->filter_claim > 64) return sigmo_err::set(SIGMO_ERR_INVALID, "filter_claim must be 0..64");
+  b->tune = *t;
+  return SIGMO_OK;
+}
+
+int sigmo_batch_get_tuning(const sigmo_batch* b, sigmo_batch_tuning* t) {
+  if (!b || !t) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  *t = b->tune;
+  return SIGMO_OK;
+}
+
+int sigmo_batch_autotune(sigmo_batch* b, const sigmo_graph* query, int find_first, uint32_t repeats, sigmo_batch_tuning* chosen,
+                         double* best_ms) {
+  if (!b || !query) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  if (repeats == 0) repeats = 2;
+  sigmo_batch_result r;
+  BATCH_TRY(sigmo_batch_run(b, query, find_first, &r));  // sizes every buffer
+  auto time_with = [&](const sigmo_batch_tuning& t, double* ms) -> int {
+    b->tune = t;
+    double best = 1e300;
+    for (uint32_t i = 0; i < repeats; ++i) {
+      const auto t0 = std::chrono::steady_clock::now();
+      BATCH_TRY(sigmo_batch_run(b, query, find_first, &r));
+      best = std::min(best, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+    *ms = best;
+    return SIGMO_OK;
+  };
+  sigmo_batch_tuning best_t{};
+  double best = 0;
+  BATCH_TRY(time_with(best_t, &best));
+  // coordinate search: pair evaluation shape, then the join's
+  const uint32_t filter_blocks[] = {1, 2, 4, 0};
+  const uint32_t filter_claims[] = {2, 8, 32, 0};
+  const uint32_t join_blocks[] = {2, 3, 4};
+  const uint32_t join_chunks[] = {16, 64, 256, 0};
+  auto sweep = [&](uint32_t sigmo_batch_tuning::*field, const uint32_t* vals, size_t n) -> int {
+    for (size_t i = 0; i < n; ++i) {
+      sigmo_batch_tuning t = best_t;
+      t.*field = vals[i];
+      if (t.*field == best_t.*field) continue;
+      double ms = 0;
+      BATCH_TRY(time_with(t, &ms));
+      if (ms < best * 0.98) {  // a 2% margin against timing noise
+        best = ms;
+        best_t = t;
+      }
+    }
+    return SIGMO_OK;
+  };
+  BATCH_TRY(sweep(&sigmo_batch_tuning::filter_blocks_per_cu, filter_blocks, 4));
+  BATCH_TRY(sweep(&sigmo_batch_tuning::filter_claim, filter_claims, 4));
+  BATCH_TRY(sweep(&sigmo_batch_tuning::join_blocks_per_cu, join_blocks, 3));
+  BATCH_TRY(sweep(&sigmo_batch_tuning::join_chunk, join_chunks, 4));
+  b->tune = best_t;
+  if (chosen) *chosen = best_t;
+  if (best_ms) *best_ms = best;
+  return SIGMO_OK;
+}
+
+int sigmo_batch_capture(sigmo_batch* b, const sigmo_graph* query, int find_first, int refresh_db) {
+  if (!b || !query) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  int device = 0, cus = 0;
+  hipStream_t s;
+  unsigned long long *stats, *stats_host;
+  BATCH_TRY(ctx_info(b->ctx, &device, &s, &cus, &stats, &stats_host));
+  // a plain run first: it sizes the arena and the GMCR arrays and gives the pair counts
+  sigmo_batch_result r;
+  BATCH_TRY(sigmo_batch_run(b, query, find_first, &r));
+  const uint64_t nw = stats_host[5];
+  RunShape L;
+  BATCH_TRY(shape_run(b, query, cus, L));
+  if (!b->graph_host) BATCH_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&b->graph_host), 32 * sizeof(unsigned long long)));
+  b->drop_graph();
+  const sigmo_db* db = b->db;
+  BATCH_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipSuccess;
+  if (refresh_db)
+    e = sigmo_k::db_prepare(s, L.dd, db->R, db->cls_list, db->pidx, db->wide_idx, db->planes32, db->planes64, db->planes_hi, db->lab_hi,
+                            db->dmax, db->minfo);
+  if (e == hipSuccess) e = sigmo_k::query_prepare(s, L.qd, L.R, b->tokens, b->qinfo, b->qmax);
+  if (e == hipSuccess) e = launch_filter(b, L, s);
+  if (e == hipSuccess) e = launch_count_scan(b, L, s);
+  if (e == hipSuccess) e = launch_scatter(b, L, s);
+  if (e == hipSuccess) e = sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(stats), 8, 0);
+  if (e == hipSuccess) e = launch_join(b, L, s, stats, b->P, find_first);
+  if (e == hipSuccess && db->num_wide) e = launch_join_wide(b, L, s, stats, nw, find_first);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->graph_host, b->counters, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->graph_host + 8, b->total, sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->graph_host + 9, stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
+  hipGraph_t g = nullptr;
+  const hipError_t end = hipStreamEndCapture(s, &g);
+  if (e != hipSuccess || end != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return sigmo_err::set(SIGMO_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : end));
+  }
+  b->graph = g;
+  BATCH_HIP_TRY(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+  b->graph_P = b->P;
+  b->graph_nw = nw;
+  b->graph_Gq = L.Gq;
+  b->graph_find_first = find_first;
+  b->graph_db_generation = db->generation;
+  return SIGMO_OK;
+}
+
+int sigmo_batch_replay(sigmo_batch* b, sigmo_batch_result* res) {
+  if (!b || !res) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  if (!b->graph_exec)
+    return sigmo_err::set(SIGMO_ERR_INVALID, "no captured graph (never captured, or a later run reallocated its buffers): capture again");
+  if (b->db->generation != b->graph_db_generation)
+    return sigmo_err::set(SIGMO_ERR_INVALID, "the database was rebound after the capture: capture again");
+  int device = 0, cus = 0;
+  hipStream_t s;
+  unsigned long long *stats, *stats_host;
+  BATCH_TRY(ctx_info(b->ctx, &device, &s, &cus, &stats, &stats_host));
+  std::memset(res, 0, sizeof(*res));
+  void* tok = nullptr;
+  BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_FILTER, -1, 0.0, true, &tok));  // the whole graph, as one timed region
+  BATCH_HIP_TRY(hipGraphLaunch(b->graph_exec, s));
+  BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
+  BATCH_HIP_TRY(hipStreamSynchronize(s));
+  const unsigned long long* h = b->graph_host;
+  if (h[3] || h[8] != b->graph_P || (b->db->num_wide && h[9 + 5] != b->graph_nw)) {
+    b->ran = false;
+    return sigmo_err::set(SIGMO_ERR_INVALID, "the captured graph no longer matches the data (arena or pair counts changed): capture again");
+  }
+  if (h[9 + 3]) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "a per-pair match count exceeded int32");
+  b->P = b->graph_P;
+  b->Gq = b->graph_Gq;
+  b->ran = true;
+  res->num_matches = h[9 + 0];
+  res->matched_pairs = h[9 + 1];
+  res->gmcr_pairs = b->graph_P;
+  res->dfs_steps = h[9 + 4];
+  res->join_wave_steps = h[9 + 7];
+  res->filter_wave_steps = h[4];
+  res->filter_lane_steps = h[5];
+  res->filter_scans = h[6];
   return SIGMO_OK;
 }
 

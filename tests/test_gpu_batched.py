@@ -168,3 +168,70 @@ def test_batched_node_with_every_label_at_one_level(ctx):
         o = O.pipeline(qh.arrays(), dh.arrays(), R, modes=[True, False])
         _assert_equal(g, o)
         assert o["counts_all"].sum() > 0
+
+
+@pytest.mark.parametrize("name", ["cfg1_slice", "lognormal_aliasing"])
+def test_batched_graph_replay_equals_eager(ctx, name):
+    """One captured hipGraph per mode, replayed three times with other eager work and
+    synchronisations between the replays; every replay equals the eager run and the oracle."""
+    qh, dh, R = _case(name)
+    qd, dd = capi.DeviceGraph(ctx, qh), capi.DeviceGraph(ctx, dh)
+    db = capi.Database(ctx, dd, R)
+    batch = capi.Batch(ctx, db)
+    other = capi.Batch(ctx, db)
+    other_q = capi.DeviceGraph(ctx, capi.HostGraphs.queries(11, 15, 4, 12, capi.SIZE_NORMAL))
+    o = O.pipeline(qh.arrays(), dh.arrays(), R, modes=[True, False])
+    for mode in (True, False):
+        key = "first" if mode else "all"
+        eager = batch.run(qd, find_first=mode)
+        e_off, e_idx, e_cnt = batch.pairs()
+        print(f"graph capture {name} find_first={mode}", flush=True)
+        batch.capture(qd, find_first=mode, refresh_db=True)
+        for rep in range(3):
+            other.run(other_q, find_first=not mode)  # eager work on the same stream between replays
+            ctx.sync()
+            r = batch.replay()
+            off, idx, cnt = batch.pairs()
+            np.testing.assert_array_equal(off, e_off)
+            np.testing.assert_array_equal(idx, e_idx)
+            np.testing.assert_array_equal(cnt, e_cnt, err_msg=f"replay {rep}")
+            assert (r["num_matches"], r["gmcr_pairs"], r["matched_pairs"]) == (eager["num_matches"], eager["gmcr_pairs"],
+                                                                               eager["matched_pairs"])
+        np.testing.assert_array_equal(cnt, o[f"counts_{key}"])
+
+
+def test_batched_graph_dropped_when_buffers_grow(ctx):
+    dh = capi.HostGraphs.molecules(12, 800)
+    dd = capi.DeviceGraph(ctx, dh)
+    batch = capi.Batch(ctx, capi.Database(ctx, dd, 4))
+    small = capi.DeviceGraph(ctx, capi.HostGraphs.queries(12, 5, 4, 8, capi.SIZE_NORMAL))
+    big = capi.DeviceGraph(ctx, capi.HostGraphs.queries(13, 120, 4, 16, capi.SIZE_NORMAL))
+    print("graph invalidation", flush=True)
+    batch.capture(small)
+    batch.replay()
+    batch.run(big)  # grows the per-pair buffers: the graph's addresses are gone
+    with pytest.raises(capi.SigmoError):
+        batch.replay()
+
+
+def test_batched_tuning_and_autotune_keep_results(ctx):
+    qh, dh, R = _case("lognormal_aliasing")
+    qd, dd = capi.DeviceGraph(ctx, qh), capi.DeviceGraph(ctx, dh)
+    batch = capi.Batch(ctx, capi.Database(ctx, dd, R))
+    base = batch.run(qd, find_first=False)
+    ref = batch.pairs()
+    for t in ({"filter_blocks_per_cu": 1, "filter_claim": 1, "join_blocks_per_cu": 1, "join_chunk": 1},
+              {"filter_blocks_per_cu": 8, "filter_claim": 64, "join_blocks_per_cu": 4, "join_chunk": 4096}):
+        print(f"tuning {t}", flush=True)
+        batch.tuning = t
+        r = batch.run(qd, find_first=False)
+        for a, b in zip(batch.pairs(), ref):
+            np.testing.assert_array_equal(a, b)
+        assert r["num_matches"] == base["num_matches"]
+    with pytest.raises(capi.SigmoError):
+        batch.tuning = {"join_blocks_per_cu": 5}
+    chosen, ms = batch.autotune(qd, find_first=False)
+    assert ms > 0 and batch.tuning == chosen
+    batch.run(qd, find_first=False)
+    for a, b in zip(batch.pairs(), ref):
+        np.testing.assert_array_equal(a, b)
