@@ -239,7 +239,9 @@ def main() -> int:
     ap.add_argument("--no-overlap", action="store_true", help="config 3: load chunks inline instead of on a side stream")
     ap.add_argument("--autotune", action="store_true", help="batched form: pick launch shapes with the autotuner before timing")
     ap.add_argument("--graph", action="store_true",
-                    help="batched form: capture one whole step (database refresh included) as a hipGraph and time its replays")
+                    help="capture one whole step (the same kernel launches and work; the batched form's includes the "
+                         "database refresh) as a hipGraph and time its replays; per-kernel figures then come from eager "
+                         "steps run just before the capture")
     args = ap.parse_args()
     if args.serve and args.config == 1 and "--config" not in sys.argv:
         args.config = 4
@@ -298,6 +300,18 @@ def main() -> int:
 
         def step():
             return pipe.execute(args.refinement_steps, find_first)
+
+        if args.graph and args.config != 3:
+            # per-kernel statistics from eager executes; the timed steps replay the captured execute
+            ctx.reset_phases()
+            for _ in range(max(1, args.warmup)):
+                step()
+            ctx.sync()
+            graph_kstats, graph_phase_ms, graph_runs = ctx.kernel_stats(), ctx.phase_ms(), max(1, args.warmup)
+            pipe.capture(args.refinement_steps, find_first)
+
+            def step():
+                return pipe.replay()
     else:
         dd = capi.DeviceGraph(ctx, dh)
         data_nodes = dd.total_nodes
@@ -344,7 +358,7 @@ def main() -> int:
     kstats = ctx.kernel_stats()
     phase_ms = ctx.phase_ms()
     phase_ms = {k: round(phase_ms[k] / args.steps, 4) for k in capi.PHASE_NAMES}
-    if getattr(args, "graph", False) and args.form == "batched" and args.config != 3:
+    if args.graph and args.config != 3:
         # eager-step statistics, scaled to one step (launch counts per step stay integral)
         kstats = {k: (n * args.steps / graph_runs, ms * args.steps / graph_runs, by * args.steps / graph_runs)
                   for k, (n, ms, by) in graph_kstats.items()}
@@ -412,7 +426,8 @@ def main() -> int:
             **({k: results[-1][k] for k in ("chunks", "wall_ms", "compute_ms", "load_ms", "load_wait_ms")}
                if args.config == 3 else {}),
             "form": args.form,
-            **({"tuning": getattr(args, "tuning", None), "graph": bool(args.graph)} if args.form == "batched" else {}),
+            **({"tuning": getattr(args, "tuning", None)} if args.form == "batched" else {}),
+            "graph": bool(args.graph and args.config != 3),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

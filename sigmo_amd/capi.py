@@ -52,6 +52,7 @@ EXPORTED = (
     "sigmo_candidates_counts", "sigmo_candidates_alloc_size", "sigmo_gmcr_create", "sigmo_gmcr_free",
     "sigmo_gmcr_generate", "sigmo_gmcr_total", "sigmo_gmcr_download", "sigmo_join", "sigmo_join_wildcard", "sigmo_pipeline_run",
     "sigmo_pipeline_create", "sigmo_pipeline_execute", "sigmo_pipeline_pairs", "sigmo_pipeline_free",
+    "sigmo_pipeline_capture", "sigmo_pipeline_replay",
     "sigmo_kernel_stats", "sigmo_event_mark", "sigmo_event_elapsed", "sigmo_db_prepare", "sigmo_db_free",
     "sigmo_db_alloc_size", "sigmo_db_refresh", "sigmo_batch_create", "sigmo_batch_run", "sigmo_batch_pairs", "sigmo_batch_free",
     "sigmo_batch_request_stats", "sigmo_batch_workspace_bytes", "sigmo_serve_schedule", "sigmo_serve", "sigmo_stream_run",
@@ -232,6 +233,8 @@ def lib() -> C.CDLL:
             "sigmo_pipeline_create": ([vp, vp, vp, pvp], i32),
             "sigmo_pipeline_execute": ([vp, u32, i32, C.POINTER(PipelineResult)], i32),
             "sigmo_pipeline_pairs": ([vp, vp, u64, vp, vp, u64], i32),
+            "sigmo_pipeline_capture": ([vp, u32, i32], i32),
+            "sigmo_pipeline_replay": ([vp, C.POINTER(PipelineResult)], i32),
             "sigmo_pipeline_free": ([vp], i32),
             "sigmo_kernel_stats": ([vp, i32, C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl)], i32),
             "sigmo_event_mark": ([vp, C.POINTER(u64)], i32),
@@ -667,6 +670,16 @@ class Pipeline:
     def execute(self, refinement_steps: int, find_first: bool = True) -> dict:
         r = PipelineResult()
         check(lib().sigmo_pipeline_execute(self._h, refinement_steps, int(find_first), C.byref(r)))
+        self.last = _result_dict(r)
+        return self.last
+
+    def capture(self, refinement_steps: int, find_first: bool = True) -> None:
+        """Record one whole execute as a hipGraph (after an execute that sizes the buffers)."""
+        check(lib().sigmo_pipeline_capture(self._h, refinement_steps, int(find_first)))
+
+    def replay(self) -> dict:
+        r = PipelineResult()
+        check(lib().sigmo_pipeline_replay(self._h, C.byref(r)))
         self.last = _result_dict(r)
         return self.last
 

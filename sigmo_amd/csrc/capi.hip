@@ -74,6 +74,7 @@ struct sigmo_ctx {
   double k_ms[SIGMO_KERNEL_COUNT] = {};
   double k_bytes[SIGMO_KERNEL_COUNT] = {};
   std::vector<hipEvent_t> pool;
+  bool capturing = false;  // recording a graph: no timing events, no host synchronisation
   unsigned long long* stats = nullptr;       // join counters [8]: matches, matched pairs, cursor, overflow, steps
   unsigned long long* stats_host = nullptr;  // pinned mirror
 
@@ -126,10 +127,12 @@ struct PhaseScope {
   hipEvent_t a = nullptr;
   int status = SIGMO_OK;
   PhaseScope(sigmo_ctx* ctx, int ph, int k = -1, double by = 0) : c(ctx), phase(ph), kernel(k), bytes(by) {
+    if (c->capturing) return;  // a graph records no timing events
     status = c->event(&a);
     if (status == SIGMO_OK && hipEventRecord(a, c->stream) != hipSuccess) status = SIGMO_ERR_HIP;
   }
   int close() {
+    if (c->capturing) return SIGMO_OK;
     if (status != SIGMO_OK) return status;
     hipEvent_t b;
     SIGMO_TRY(c->event(&b));
@@ -236,8 +239,9 @@ struct sigmo_gmcr {
   uint64_t cap_indices = 0, cap_pair_graph = 0, cap_counts_g = 0, cap_offsets = 0, cap_pair_bits = 0, cap_scan = 0;
   int32_t* pair_counts = nullptr;
   uint64_t cap_counts = 0;
-  uint32_t* wide_pairs = nullptr;  // pairs of molecules with > 64 nodes, joined one wavefront per pair
+  uint32_t* wide_pairs = nullptr;  // pairs of molecules with > 64 nodes, for the two-word lane join
   uint64_t cap_wide = 0;
+  uint64_t last_wide = 0;  // their count in the last join
   ~sigmo_gmcr() {
     dev_free(wide_pairs);
     dev_free(offsets);
@@ -887,14 +891,18 @@ int sigmo_gmcr_generate(sigmo_ctx* ctx, sigmo_gmcr* m, const sigmo_graph* query,
   SIGMO_HIP_TRY(sigmo_k::gmcr_mark(s, query->dev(), data->dev(), c->bits, c->W64, m->pair_bits));
   SIGMO_HIP_TRY(sigmo_k::gmcr_count(s, m->pair_bits, Gq, Gd, m->counts));
   SIGMO_HIP_TRY(sigmo_k::exclusive_scan(s, m->counts, Gd, m->offsets, m->scan_tmp, m->total));
-  // gmcr.hpp:79-84 reads the total back to size the index array
-  uint64_t P = 0;
-  SIGMO_HIP_TRY(hipMemcpyAsync(&P, m->total, 8, hipMemcpyDeviceToHost, s));
-  SIGMO_HIP_TRY(hipStreamSynchronize(s));
-  if (P >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "GMCR pair count exceeds u32");
-  SIGMO_TRY(grow(&m->indices, m->cap_indices, P));
-  SIGMO_TRY(grow(&m->pair_graph, m->cap_pair_graph, P));
-  SIGMO_HIP_TRY(sigmo_k::gmcr_scatter(s, m->pair_bits, Gq, Gd, m->offsets, m->indices, m->pair_graph));
+  // gmcr.hpp:79-84 reads the total back to size the index array; a graph being recorded keeps
+  // the previous run's count and sizes (the replay checks the total)
+  uint64_t P = m->P;
+  if (!ctx->capturing) {
+    SIGMO_HIP_TRY(hipMemcpyAsync(&P, m->total, 8, hipMemcpyDeviceToHost, s));
+    SIGMO_HIP_TRY(hipStreamSynchronize(s));
+    if (P >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "GMCR pair count exceeds u32");
+    SIGMO_TRY(grow(&m->indices, m->cap_indices, P));
+    SIGMO_TRY(grow(&m->pair_graph, m->cap_pair_graph, P));
+  }
+  SIGMO_HIP_TRY(sigmo_k::gmcr_scatter(s, m->pair_bits, Gq, Gd, m->offsets, m->indices, m->pair_graph,
+                                      std::min(m->cap_indices, m->cap_pair_graph)));
   SIGMO_TRY(ps.close());
   m->Gq = Gq;
   m->Gd = Gd;
@@ -944,15 +952,19 @@ int sigmo_join(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data
   SIGMO_HIP_TRY(sigmo_k::join_lanes(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, m->P, find_first,
                                     mm->pair_counts, ctx->stats, mm->wide_pairs, ctx->num_cus));
   if (data->max_nodes > 64 && m->P) {  // pairs of molecules with > 64 nodes: the two-word lane join
-    unsigned long long nw = 0;
-    SIGMO_HIP_TRY(hipMemcpyAsync(&nw, ctx->stats + 5, sizeof(nw), hipMemcpyDeviceToHost, s));
-    SIGMO_HIP_TRY(hipStreamSynchronize(s));
+    unsigned long long nw = mm->last_wide;  // a graph being recorded keeps the previous run's count
+    if (!ctx->capturing) {
+      SIGMO_HIP_TRY(hipMemcpyAsync(&nw, ctx->stats + 5, sizeof(nw), hipMemcpyDeviceToHost, s));
+      SIGMO_HIP_TRY(hipStreamSynchronize(s));
+      mm->last_wide = nw;
+    }
     if (nw)
       SIGMO_HIP_TRY(sigmo_k::join_lanes_wide(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, mm->wide_pairs, nw, 6u,
                                              find_first, mm->pair_counts, ctx->stats, ctx->num_cus));
   }
   SIGMO_TRY(ps.close());
   SIGMO_HIP_TRY(hipMemcpyAsync(ctx->stats_host, ctx->stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  if (ctx->capturing) return SIGMO_OK;  // the replay reads the pinned mirror after the graph
   if (per_pair_counts && m->P)
     SIGMO_HIP_TRY(hipMemcpyAsync(per_pair_counts, mm->pair_counts, m->P * 4, hipMemcpyDeviceToHost, s));
   SIGMO_HIP_TRY(hipStreamSynchronize(s));
@@ -997,12 +1009,50 @@ struct sigmo_pipeline {
   sigmo_candidates* cand = nullptr;
   sigmo_gmcr* gm = nullptr;
   bool has_counts = false;
+  // a captured execute (sigmo_pipeline_capture)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  uint64_t* replay_host = nullptr;  // pinned: the GMCR total of the last replay
+  uint64_t graph_P = 0, graph_nw = 0;
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+  }
   ~sigmo_pipeline() {
+    drop_graph();
+    if (replay_host) (void)hipHostFree(replay_host);
     sigmo_signatures_free(sig);
     sigmo_candidates_free(cand);
     sigmo_gmcr_free(gm);
   }
 };
+
+namespace {
+// The pipeline's launches in the driver's order (sigmo.cpp:102-159), from fresh signature state
+// (signature.hpp:166-167; reachables empty).  Also what a capture records.
+int pipeline_enqueue(sigmo_pipeline* p, uint32_t refinement_steps, int find_first, uint64_t* matches) {
+  sigmo_ctx* ctx = p->ctx;
+  sigmo_signatures* sig = p->sig;
+  SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->dsig, sig->D, 0));
+  SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->qsig, sig->Q, 0));
+  SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->dreach, sig->D, 0));
+  SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->qreach, sig->Q, 0));
+  sig->dstate.reset();
+  sig->qstate.reset();
+  SIGMO_TRY(sigmo_signatures_generate(ctx, sig, p->d, SIGMO_SCOPE_DATA));
+  SIGMO_TRY(sigmo_signatures_generate(ctx, sig, p->q, SIGMO_SCOPE_QUERY));
+  SIGMO_TRY(sigmo_candidates_filter(ctx, p->cand, p->q, p->d, sig));
+  for (uint32_t r = 1; r <= refinement_steps; ++r) {
+    SIGMO_TRY(sigmo_signatures_refine(ctx, sig, p->d, SIGMO_SCOPE_DATA, r));
+    SIGMO_TRY(sigmo_signatures_refine(ctx, sig, p->q, SIGMO_SCOPE_QUERY, r));
+    SIGMO_TRY(sigmo_candidates_refine(ctx, p->cand, p->q, p->d, sig));
+  }
+  SIGMO_TRY(sigmo_gmcr_generate(ctx, p->gm, p->q, p->d, p->cand));
+  return sigmo_join(ctx, p->q, p->d, p->cand, p->gm, find_first, matches, nullptr);
+}
+}  // namespace
 
 extern "C" {
 
@@ -1038,25 +1088,8 @@ int sigmo_pipeline_execute(sigmo_pipeline* p, uint32_t refinement_steps, int fin
   SIGMO_TRY(ctx->event(&t0));
   SIGMO_TRY(ctx->event(&t1));
   SIGMO_HIP_TRY(hipEventRecord(t0, ctx->stream));
-  // fresh Signature state (signature.hpp:166-167; reachables empty)
-  sigmo_signatures* sig = p->sig;
-  SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->dsig, sig->D, 0));
-  SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->qsig, sig->Q, 0));
-  SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->dreach, sig->D, 0));
-  SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->qreach, sig->Q, 0));
-  sig->dstate.reset();
-  sig->qstate.reset();
-  SIGMO_TRY(sigmo_signatures_generate(ctx, sig, p->d, SIGMO_SCOPE_DATA));
-  SIGMO_TRY(sigmo_signatures_generate(ctx, sig, p->q, SIGMO_SCOPE_QUERY));
-  SIGMO_TRY(sigmo_candidates_filter(ctx, p->cand, p->q, p->d, sig));
-  for (uint32_t r = 1; r <= refinement_steps; ++r) {
-    SIGMO_TRY(sigmo_signatures_refine(ctx, sig, p->d, SIGMO_SCOPE_DATA, r));
-    SIGMO_TRY(sigmo_signatures_refine(ctx, sig, p->q, SIGMO_SCOPE_QUERY, r));
-    SIGMO_TRY(sigmo_candidates_refine(ctx, p->cand, p->q, p->d, sig));
-  }
-  SIGMO_TRY(sigmo_gmcr_generate(ctx, p->gm, p->q, p->d, p->cand));
   uint64_t matches = 0;
-  SIGMO_TRY(sigmo_join(ctx, p->q, p->d, p->cand, p->gm, find_first, &matches, nullptr));
+  SIGMO_TRY(pipeline_enqueue(p, refinement_steps, find_first, &matches));
   SIGMO_HIP_TRY(hipEventRecord(t1, ctx->stream));
   SIGMO_HIP_TRY(hipEventSynchronize(t1));
   float total = 0;
@@ -1072,6 +1105,68 @@ int sigmo_pipeline_execute(sigmo_pipeline* p, uint32_t refinement_steps, int fin
   result->total_ms = total;
   SIGMO_TRY(ctx->collect());
   for (int ph = 0; ph < SIGMO_PHASE_COUNT; ++ph) result->phase_ms[ph] = ctx->accum_ms[ph] - before[ph];
+  return SIGMO_OK;
+}
+
+int sigmo_pipeline_capture(sigmo_pipeline* p, uint32_t refinement_steps, int find_first) {
+  if (!p) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  sigmo_ctx* ctx = p->ctx;
+  SIGMO_TRY(bind(ctx));
+  sigmo_pipeline_result r;
+  SIGMO_TRY(sigmo_pipeline_execute(p, refinement_steps, find_first, &r));  // sizes every buffer
+  p->drop_graph();
+  if (!p->replay_host) SIGMO_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&p->replay_host), 64));
+  SIGMO_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  ctx->capturing = true;
+  uint64_t unused = 0;
+  int st = pipeline_enqueue(p, refinement_steps, find_first, &unused);
+  if (st == SIGMO_OK && hipMemcpyAsync(p->replay_host, p->gm->total, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+    st = sigmo_err::set(SIGMO_ERR_HIP, "graph capture: total readback");
+  ctx->capturing = false;
+  hipGraph_t g = nullptr;
+  const hipError_t end = hipStreamEndCapture(ctx->stream, &g);
+  if (st != SIGMO_OK || end != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (st != SIGMO_OK) return st;
+    return sigmo_err::set(SIGMO_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(end));
+  }
+  p->graph = g;
+  SIGMO_HIP_TRY(hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0));
+  p->graph_P = p->gm->P;
+  p->graph_nw = p->gm->last_wide;
+  return SIGMO_OK;
+}
+
+int sigmo_pipeline_replay(sigmo_pipeline* p, sigmo_pipeline_result* result) {
+  if (!p || !result) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  if (!p->graph_exec) return sigmo_err::set(SIGMO_ERR_INVALID, "no captured graph (call sigmo_pipeline_capture)");
+  sigmo_ctx* ctx = p->ctx;
+  SIGMO_TRY(bind(ctx));
+  std::memset(result, 0, sizeof(*result));
+  hipEvent_t t0, t1;
+  SIGMO_TRY(ctx->event(&t0));
+  SIGMO_TRY(ctx->event(&t1));
+  SIGMO_HIP_TRY(hipEventRecord(t0, ctx->stream));
+  SIGMO_HIP_TRY(hipGraphLaunch(p->graph_exec, ctx->stream));
+  SIGMO_HIP_TRY(hipEventRecord(t1, ctx->stream));
+  SIGMO_HIP_TRY(hipEventSynchronize(t1));
+  float total = 0;
+  SIGMO_HIP_TRY(hipEventElapsedTime(&total, t0, t1));
+  ctx->pool.push_back(t0);
+  ctx->pool.push_back(t1);
+  const unsigned long long* st = ctx->stats_host;
+  if (p->replay_host[0] != p->graph_P || (p->d->max_nodes > 64 && st[5] != p->graph_nw)) {
+    p->has_counts = false;
+    return sigmo_err::set(SIGMO_ERR_INVALID, "the captured graph no longer matches the data (pair counts changed): capture again");
+  }
+  if (st[3]) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "a per-pair match count exceeded int32");
+  p->has_counts = true;
+  result->num_matches = st[0];
+  result->gmcr_pairs = p->graph_P;
+  result->matched_pairs = st[1];
+  result->dfs_steps = st[4];
+  result->join_wave_steps = st[7];
+  result->total_ms = total;
   return SIGMO_OK;
 }
 
@@ -1114,6 +1209,7 @@ int pipeline_rebind(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph*
   }
   sigmo_pipeline* p = *pp;
   if (p->ctx != ctx || p->q != query) return sigmo_err::set(SIGMO_ERR_INVALID, "pipeline bound to other queries or context");
+  p->drop_graph();
   const uint64_t N = data->N, W64 = (N + 63) / 64, cN = std::max(N, reserve_N);
   if (!p->sig || N > p->sig->capD) {
     sigmo_signatures_free(p->sig);

@@ -435,7 +435,9 @@ __global__ __launch_bounds__(256) void k_gmcr_count(const uint64_t* __restrict__
 // query graphs in ascending id.  Also records the molecule of each pair for the join.
 __global__ __launch_bounds__(256) void k_gmcr_scatter(const uint64_t* __restrict__ pair_bits, uint32_t Gq, uint32_t Gd,
                                                       const uint32_t* __restrict__ offsets, uint32_t* __restrict__ indices,
-                                                      uint32_t* __restrict__ pair_graph) {
+                                                      uint32_t* __restrict__ pair_graph, uint64_t cap) {
+  // writes stop at `cap` (the arrays' size): a graph replay sized by an earlier run cannot write
+  // past them, and reports the changed pair count instead
   const uint64_t dg = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   if (dg >= Gd) return;
   const uint64_t strips = (static_cast<uint64_t>(Gd) + 63) / 64;
@@ -444,8 +446,10 @@ __global__ __launch_bounds__(256) void k_gmcr_scatter(const uint64_t* __restrict
   uint32_t pos = offsets[dg];
   for (uint32_t qg = 0; qg < Gq; ++qg)
     if ((pair_bits[qg * strips + strip] >> bit) & 1ull) {
-      indices[pos] = qg;
-      pair_graph[pos] = static_cast<uint32_t>(dg);
+      if (pos < cap) {
+        indices[pos] = qg;
+        pair_graph[pos] = static_cast<uint32_t>(dg);
+      }
       ++pos;
     }
 }
@@ -667,9 +671,9 @@ hipError_t exclusive_scan(hipStream_t s, const uint32_t* counts, uint64_t n, uin
 }
 
 hipError_t gmcr_scatter(hipStream_t s, const uint64_t* pair_bits, uint32_t Gq, uint32_t Gd, const uint32_t* offsets, uint32_t* indices,
-                        uint32_t* pair_graph) {
+                        uint32_t* pair_graph, uint64_t cap) {
   if (!Gd) return hipSuccess;
-  k_gmcr_scatter<<<(Gd + 255) / 256, 256, 0, s>>>(pair_bits, Gq, Gd, offsets, indices, pair_graph);
+  k_gmcr_scatter<<<(Gd + 255) / 256, 256, 0, s>>>(pair_bits, Gq, Gd, offsets, indices, pair_graph, cap);
   return hipGetLastError();
 }
 

@@ -80,7 +80,7 @@ hipError_t gmcr_count(hipStream_t s, const uint64_t* pair_bits, uint32_t Gq, uin
 hipError_t exclusive_scan(hipStream_t s, const uint32_t* counts, uint64_t n, uint32_t* offsets, uint64_t* block_tmp,
                           uint64_t* total_dev);
 hipError_t gmcr_scatter(hipStream_t s, const uint64_t* pair_bits, uint32_t Gq, uint32_t Gd, const uint32_t* offsets,
-                        uint32_t* indices, uint32_t* pair_graph);
+                        uint32_t* indices, uint32_t* pair_graph, uint64_t cap);
 
 // Join.  stats[0] += sum of counts, stats[1] += pairs with count > 0, stats[2] = work cursor
 // (zeroed by the caller), stats[3] |= 1 on int32 overflow.

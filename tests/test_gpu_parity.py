@@ -192,3 +192,32 @@ def test_refine_with_two_graphs_of_equal_size(ctx):
         sig.refine_data(graph, view)
         O.refine_signatures(arrays, view, osig, oreach)
         np.testing.assert_array_equal(sig.download(capi.SCOPE_DATA), osig, err_msg=f"view {view}")
+
+
+@pytest.mark.parametrize("name", ["cfg1_slice", "lognormal_aliasing"])
+def test_pipeline_graph_replay_equals_execute(ctx, name):
+    """The plain pipeline captured as one hipGraph: three replays per mode, with other eager
+    work and synchronisations between them, each equal to the eager execute and the oracle."""
+    qh, dh, R = _case(name)
+    qd, dd = capi.DeviceGraph(ctx, qh), capi.DeviceGraph(ctx, dh)
+    pipe = capi.Pipeline(ctx, qd, dd)
+    other_q = capi.DeviceGraph(ctx, capi.HostGraphs.queries(21, 12, 4, 10, capi.SIZE_NORMAL))
+    other = capi.Pipeline(ctx, other_q, dd)
+    o = O.pipeline(qh.arrays(), dh.arrays(), R, modes=[True, False])
+    for mode in (True, False):
+        key = "first" if mode else "all"
+        eager = pipe.execute(R, mode)
+        e_off, e_idx, e_cnt = pipe.pairs()
+        print(f"pipeline capture {name} find_first={mode}", flush=True)
+        pipe.capture(R, mode)
+        for rep in range(3):
+            other.execute(2, not mode)  # eager work on the same stream between replays
+            ctx.sync()
+            r = pipe.replay()
+            off, idx, cnt = pipe.pairs()
+            np.testing.assert_array_equal(off, e_off)
+            np.testing.assert_array_equal(idx, e_idx)
+            np.testing.assert_array_equal(cnt, e_cnt, err_msg=f"replay {rep}")
+            assert (r["num_matches"], r["gmcr_pairs"], r["matched_pairs"], r["dfs_steps"]) == (
+                eager["num_matches"], eager["gmcr_pairs"], eager["matched_pairs"], eager["dfs_steps"])
+        np.testing.assert_array_equal(cnt, o[f"counts_{key}"])
