@@ -625,26 +625,33 @@ __global__ __launch_bounds__(256) void k_surv_count(uint32_t Gd, uint32_t W, con
 }
 
 // Canonical GMCR arrays (gmcr.hpp:73-115 order: molecules ascending, query graphs ascending
-// within a molecule) from the survival bitmask, with each pair's arena offset.
+// within a molecule) from the survival bitmask, with each pair's arena offset.  One wavefront
+// per molecule: lane l takes query graph 64w+l of word w, so each word's survivors are written
+// by one coalesced store per array.
 __global__ __launch_bounds__(256) void k_surv_scatter(uint32_t Gd, uint32_t Gq, uint32_t W, const unsigned long long* __restrict__ surv,
                                                       const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ pair_off,
                                                       uint32_t* __restrict__ indices, uint32_t* __restrict__ pair_graph,
                                                       uint32_t* __restrict__ mask_off, uint64_t cap) {
   // `cap` bounds the writes: a replayed graph sized by an earlier run never writes past its
   // arrays (the replay then reports the changed pair count)
-  const uint64_t m = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-  if (m >= Gd) return;
-  uint32_t o = offsets[m];
-  for (uint32_t w = 0; w < W; ++w)
-    for (uint64_t x = surv[m * W + w]; x; x &= x - 1) {
-      const uint32_t g = w * 64 + __builtin_ctzll(x);
-      if (o < cap) {
-        indices[o] = g;
-        pair_graph[o] = static_cast<uint32_t>(m);
-        mask_off[o] = pair_off[m * Gq + g];
+  const uint64_t m = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / kWave;
+  if (m >= Gd) return;  // wave-uniform
+  const uint32_t lane = static_cast<uint32_t>(lane_id());
+  const uint64_t below = (1ull << lane) - 1;
+  uint64_t o = offsets[m];
+  for (uint32_t w = 0; w < W; ++w) {
+    const uint64_t x = surv[static_cast<uint64_t>(m) * W + w];
+    if ((x >> lane) & 1ull) {
+      const uint64_t r = o + static_cast<uint64_t>(__popcll(x & below));
+      const uint32_t g = w * 64 + lane;
+      if (r < cap) {
+        indices[r] = g;
+        pair_graph[r] = static_cast<uint32_t>(m);
+        mask_off[r] = pair_off[static_cast<uint64_t>(m) * Gq + g];
       }
-      ++o;
     }
+    o += static_cast<uint64_t>(__popcll(x));
+  }
 }
 
 // Per-request statistics of a batch run.  Request r owns the batch's query graphs
@@ -1178,7 +1185,8 @@ hipError_t launch_count_scan(sigmo_batch* b, const RunShape& L, hipStream_t s) {
 
 hipError_t launch_scatter(sigmo_batch* b, const RunShape& L, hipStream_t s) {
   if (!L.Gd) return hipSuccess;
-  hipLaunchKernelGGL(sigmo_k::k_surv_scatter, dim3((L.Gd + 255) / 256), dim3(256), 0, s, L.Gd, L.Gq, L.W, b->surv, b->offsets, b->pair_off,
+  const uint64_t blocks = (static_cast<uint64_t>(L.Gd) * sigmo_k::kWave + 255) / 256;  // a wavefront per molecule
+  hipLaunchKernelGGL(sigmo_k::k_surv_scatter, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, L.Gd, L.Gq, L.W, b->surv, b->offsets, b->pair_off,
                      b->indices, b->pair_graph, b->mask_off, std::min(b->cap_idx, std::min(b->cap_pg, b->cap_mo)));
   return hipGetLastError();
 }
