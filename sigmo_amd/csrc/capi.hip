@@ -750,9 +750,10 @@ int sigmo_candidates_create(sigmo_ctx* ctx, uint64_t source_nodes, uint64_t targ
   c->D = target_nodes;
   c->W64 = (target_nodes + 63) / 64;
   c->cap_words = c->Q * c->W64;
-  int st = dev_alloc(&c->bits, c->Q * c->W64);
+  // one spare word past the rows: the join reads a row's words in pairs
+  int st = dev_alloc(&c->bits, c->Q * c->W64 + 1);
   if (st == SIGMO_OK) {
-    hipError_t e = sigmo_k::fill_u64(ctx->stream, c->bits, c->Q * c->W64, 0);  // candidates.hpp:169-178
+    hipError_t e = sigmo_k::fill_u64(ctx->stream, c->bits, c->Q * c->W64 + 1, 0);  // candidates.hpp:169-178
     if (e != hipSuccess) st = sigmo_err::set(SIGMO_ERR_HIP, hipGetErrorString(e));
   }
   if (st != SIGMO_OK) {

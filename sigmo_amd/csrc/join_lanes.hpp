@@ -34,11 +34,15 @@ struct BitsetMasks {
   const uint64_t* bits;
   uint64_t W64;
   __device__ __forceinline__ uint32_t base(uint64_t) const { return 0; }
+  // Both words come from one 16-byte load.  The second may be the next row's first word, or the
+  // word past the last row (the bitset allocation carries one spare word): it only fills bits
+  // at positions >= 64 - sh, which n <= 64 - sh then masks off.
   __device__ __forceinline__ uint64_t mask(uint32_t, uint32_t qrow, uint32_t, uint32_t s, uint32_t n) const {
+    typedef uint64_t u64x2 __attribute__((ext_vector_type(2))) __attribute__((aligned(8)));
     const uint64_t* row = bits + static_cast<uint64_t>(qrow) * W64;
     const uint32_t w0 = s >> 6, sh = s & 63;
-    const uint64_t lo = row[w0];
-    const uint64_t hi = (sh && w0 + 1 < W64) ? row[w0 + 1] : 0ull;
+    const u64x2 v = *reinterpret_cast<const u64x2*>(row + w0);
+    const uint64_t lo = v.x, hi = v.y;
     uint64_t m = sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
     if (n < 64) m &= (1ull << n) - 1;
     return m;
