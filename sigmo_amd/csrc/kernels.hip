@@ -142,7 +142,8 @@ __device__ __forceinline__ uint64_t expand_frontier(const uint64_t* __restrict__
 
 // Saturated 16-label histogram of a position set, labels read through the graph's 64-position
 // label masks (the +1-below-15 rule of signature.hpp:44-51 applied count times).
-__device__ __forceinline__ uint64_t label_histogram(uint64_t r, const uint64_t* __restrict__ lm) {
+template <class P>
+__device__ __forceinline__ uint64_t label_histogram(uint64_t r, P lm) {
   uint64_t s = 0;
 #pragma unroll
   for (int l = 0; l < 16; ++l) {
@@ -151,6 +152,23 @@ __device__ __forceinline__ uint64_t label_histogram(uint64_t r, const uint64_t* 
     s |= static_cast<uint64_t>(c) << (4 * l);
   }
   return s;
+}
+
+// A signature block's nodes belong to a run of consecutive graphs; when at most kStagedGraphs,
+// their label masks are staged in LDS (the histograms then read LDS instead of 16 words of
+// global memory per node).  Block-uniform result; g0 = the run's first graph.
+constexpr uint32_t kStagedGraphs = 64;
+__device__ __forceinline__ bool stage_label_masks(const GraphDev& g, uint64_t* lms, uint32_t& g0) {
+  const uint64_t v0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x;
+  if (v0 >= g.N) return false;
+  const uint64_t vl = (v0 + blockDim.x < g.N ? v0 + blockDim.x : g.N) - 1;
+  g0 = g.node_graph[v0];
+  const uint32_t ng = g.node_graph[vl] - g0 + 1;
+  const bool staged = ng <= kStagedGraphs;
+  if (staged)
+    for (uint32_t t = threadIdx.x; t < ng * 16; t += blockDim.x) lms[t] = g.lmask[static_cast<uint64_t>(g0) * 16 + t];
+  __syncthreads();
+  return staged;
 }
 
 // generateCSRSignatures, signature.hpp:125-152: one thread per node, neighbour labels
@@ -234,7 +252,10 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_refine(GraphDev g, uint32_t v
                                                           uint64_t* __restrict__ reach, uint64_t* __restrict__ frontier_out,
                                                           uint64_t* __restrict__ planes) {
   __shared__ uint64_t tile[64 * (kSigBlock / kWave)];
+  __shared__ uint64_t lms[kStagedGraphs * 16];
   const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  uint32_t g0 = 0;
+  const bool staged = stage_label_masks(g, lms, g0);
   uint64_t s = 0;
   if (v < g.N) {
     const uint32_t gid = g.node_graph[v];
@@ -249,7 +270,7 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_refine(GraphDev g, uint32_t v
     }
     const uint64_t old = reach[v];
     const uint64_t r = ball & ~self & ~old;
-    s = label_histogram(r, g.lmask + static_cast<uint64_t>(gid) * 16);
+    s = staged ? label_histogram(r, lms + (gid - g0) * 16) : label_histogram(r, g.lmask + static_cast<uint64_t>(gid) * 16);
     sigs[v] = s;
     reach[v] = r | old;
     if (frontier_out) frontier_out[v] = frontier;
@@ -263,7 +284,10 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_refine(GraphDev g, uint32_t v
 __global__ __launch_bounds__(kSigBlock) void k_sig_refine_next(GraphDev g, uint64_t* __restrict__ sigs, uint64_t* __restrict__ reach,
                                                                uint64_t* __restrict__ frontier, uint64_t* __restrict__ planes) {
   __shared__ uint64_t tile[64 * (kSigBlock / kWave)];
+  __shared__ uint64_t lms[kStagedGraphs * 16];
   const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  uint32_t g0 = 0;
+  const bool staged = stage_label_masks(g, lms, g0);
   uint64_t s = 0;
   if (v < g.N) {
     const uint32_t gid = g.node_graph[v];
@@ -272,7 +296,7 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_refine_next(GraphDev g, uint6
     const uint64_t old = reach[v];
     uint64_t next = expand_frontier(g.adj, prev, frontier[v]);
     next &= ~(old | self);
-    s = label_histogram(next, g.lmask + static_cast<uint64_t>(gid) * 16);
+    s = staged ? label_histogram(next, lms + (gid - g0) * 16) : label_histogram(next, g.lmask + static_cast<uint64_t>(gid) * 16);
     sigs[v] = s;
     reach[v] = old | next;
     frontier[v] = next;
@@ -426,8 +450,16 @@ __global__ __launch_bounds__(256) void k_gmcr_count(const uint64_t* __restrict__
   const uint64_t strips = (static_cast<uint64_t>(Gd) + 63) / 64;
   const uint64_t strip = dg >> 6;
   const uint32_t bit = dg & 63;
-  uint32_t c = 0;
-  for (uint32_t qg = 0; qg < Gq; ++qg) c += (pair_bits[qg * strips + strip] >> bit) & 1ull;
+  uint32_t c = 0, qg = 0;
+  constexpr uint32_t kBatch = 8;  // loads in flight per thread
+  for (; qg + kBatch <= Gq; qg += kBatch) {
+    uint64_t w[kBatch];
+#pragma unroll
+    for (uint32_t i = 0; i < kBatch; ++i) w[i] = pair_bits[(qg + i) * strips + strip];
+#pragma unroll
+    for (uint32_t i = 0; i < kBatch; ++i) c += (w[i] >> bit) & 1ull;
+  }
+  for (; qg < Gq; ++qg) c += (pair_bits[qg * strips + strip] >> bit) & 1ull;
   counts[dg] = c;
 }
 
@@ -444,14 +476,25 @@ __global__ __launch_bounds__(256) void k_gmcr_scatter(const uint64_t* __restrict
   const uint64_t strip = dg >> 6;
   const uint32_t bit = dg & 63;
   uint32_t pos = offsets[dg];
-  for (uint32_t qg = 0; qg < Gq; ++qg)
-    if ((pair_bits[qg * strips + strip] >> bit) & 1ull) {
+  auto emit = [&](uint32_t qg, uint64_t w) {
+    if ((w >> bit) & 1ull) {
       if (pos < cap) {
         indices[pos] = qg;
         pair_graph[pos] = static_cast<uint32_t>(dg);
       }
       ++pos;
     }
+  };
+  uint32_t qg = 0;
+  constexpr uint32_t kBatch = 8;  // loads in flight per thread
+  for (; qg + kBatch <= Gq; qg += kBatch) {
+    uint64_t w[kBatch];
+#pragma unroll
+    for (uint32_t i = 0; i < kBatch; ++i) w[i] = pair_bits[(qg + i) * strips + strip];
+#pragma unroll
+    for (uint32_t i = 0; i < kBatch; ++i) emit(qg + i, w[i]);
+  }
+  for (; qg < Gq; ++qg) emit(qg, pair_bits[qg * strips + strip]);
 }
 
 // ---- exclusive scan (u32 counts, u64 carries) -------------------------------------------------
