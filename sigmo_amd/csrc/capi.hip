@@ -64,6 +64,7 @@ struct sigmo_ctx {
     int phase, kernel;
     double bytes;
     uint64_t seq;
+    bool shared_a;  // `a` is the previous phase's end event (returned to the pool with it)
   };
   uint64_t launch_seq = 0;          // sequence number of the next timed launch
   uint64_t launch_base = 0;         // sequence number of launch_ms[0]
@@ -75,6 +76,11 @@ struct sigmo_ctx {
   double k_bytes[SIGMO_KERNEL_COUNT] = {};
   std::vector<hipEvent_t> pool;
   bool capturing = false;  // recording a graph: no timing events, no host synchronisation
+  // Back-to-back phases (the plain pipeline's, which enqueue no other work and no host wait
+  // between them) share one event at each boundary: the next phase starts at the previous
+  // phase's end event.
+  bool chain = false;
+  hipEvent_t chain_end = nullptr;
   unsigned long long* stats = nullptr;       // join counters [8]: matches, matched pairs, cursor, overflow, steps
   unsigned long long* stats_host = nullptr;  // pinned mirror
 
@@ -105,10 +111,11 @@ struct sigmo_ctx {
         k_ms[t.kernel] += ms;
         k_bytes[t.kernel] += t.bytes;
       }
-      pool.push_back(t.a);
+      if (!t.shared_a) pool.push_back(t.a);
       pool.push_back(t.b);
     }
     pending.clear();
+    chain_end = nullptr;
     constexpr size_t kKeep = 1u << 20;  // bounded history for long-running contexts
     if (launch_ms.size() > 2 * kKeep) {
       const size_t drop = launch_ms.size() - kKeep;
@@ -127,9 +134,16 @@ struct PhaseScope {
   int phase, kernel;
   double bytes;
   hipEvent_t a = nullptr;
+  bool shared_a = false;
   int status = SIGMO_OK;
   PhaseScope(sigmo_ctx* ctx, int ph, int k = -1, double by = 0) : c(ctx), phase(ph), kernel(k), bytes(by) {
     if (c->capturing) return;  // a graph records no timing events
+    if (c->chain && c->chain_end) {
+      a = c->chain_end;
+      shared_a = true;
+      c->chain_end = nullptr;
+      return;
+    }
     status = c->event(&a);
     if (status == SIGMO_OK && hipEventRecord(a, c->stream) != hipSuccess) status = SIGMO_ERR_HIP;
   }
@@ -139,12 +153,13 @@ struct PhaseScope {
     hipEvent_t b;
     SIGMO_TRY(c->event(&b));
     SIGMO_HIP_TRY(hipEventRecord(b, c->stream));
-    c->pending.push_back({a, b, phase, kernel, bytes, c->launch_seq++});
+    c->pending.push_back({a, b, phase, kernel, bytes, c->launch_seq++, shared_a});
+    if (c->chain) c->chain_end = b;
     a = nullptr;
     return SIGMO_OK;
   }
   ~PhaseScope() {
-    if (a) c->pool.push_back(a);
+    if (a && !shared_a) c->pool.push_back(a);
   }
 };
 
@@ -1038,6 +1053,17 @@ namespace {
 int pipeline_enqueue(sigmo_pipeline* p, uint32_t refinement_steps, int find_first, uint64_t* matches) {
   sigmo_ctx* ctx = p->ctx;
   sigmo_signatures* sig = p->sig;
+  struct Chain {  // the phases below run back to back: chain their timing events
+    sigmo_ctx* c;
+    explicit Chain(sigmo_ctx* x) : c(x) {
+      c->chain = true;
+      c->chain_end = nullptr;
+    }
+    ~Chain() {
+      c->chain = false;
+      c->chain_end = nullptr;
+    }
+  } chain(ctx);
   SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->dsig, sig->D, 0));
   SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->qsig, sig->Q, 0));
   SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->dreach, sig->D, 0));
