@@ -216,7 +216,6 @@ int sigmo_pipeline_run(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_gra
 typedef struct sigmo_pipeline sigmo_pipeline;
 int sigmo_pipeline_create(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data, sigmo_pipeline** out);
 int sigmo_pipeline_execute(sigmo_pipeline* p, uint32_t refinement_steps, int find_first, sigmo_pipeline_result* result);
-/* Per-pair counts of the last execute (GMCR order) and the GMCR arrays. */
 /* One hipGraph for the whole pipeline: sigmo_pipeline_capture executes once to size every
  * buffer, then records the same launches (signature resets, every phase kernel, GMCR with the
  * pair count of that execute, the join) as a graph; sigmo_pipeline_replay launches it.  Work per
@@ -225,6 +224,7 @@ int sigmo_pipeline_execute(sigmo_pipeline* p, uint32_t refinement_steps, int fin
  * SIGMO_ERR_INVALID.  phase_ms is not filled by a replay (total_ms is). */
 int sigmo_pipeline_capture(sigmo_pipeline* p, uint32_t refinement_steps, int find_first);
 int sigmo_pipeline_replay(sigmo_pipeline* p, sigmo_pipeline_result* result);
+/* Per-pair counts of the last execute or replay (GMCR order) and the GMCR arrays. */
 int sigmo_pipeline_pairs(sigmo_pipeline* p, uint32_t* data_graph_offsets, uint64_t n_offsets, uint32_t* query_graph_indices,
                          int32_t* counts, uint64_t n_pairs);
 int sigmo_pipeline_free(sigmo_pipeline* p);
