@@ -328,7 +328,7 @@ int sigmo_batch_workspace_bytes(const sigmo_batch* b, uint32_t num_query_graphs,
  * several workers their batches overlap on the device.  A request whose own cost exceeds the
  * budget is rejected; one that waited longer than deadline_ms before admission expires.  Both
  * are reported and never run. */
-enum { SIGMO_SERVE_DONE = 0, SIGMO_SERVE_REJECTED = 1, SIGMO_SERVE_EXPIRED = 2 };
+enum { SIGMO_SERVE_DONE = 0, SIGMO_SERVE_REJECTED = 1, SIGMO_SERVE_EXPIRED = 2, SIGMO_SERVE_PENDING = 3 };
 typedef struct sigmo_serve_options {
   uint32_t max_batch_requests; /* 0 or > 256: 256 (the per-request statistics limit) */
   uint32_t max_batch_graphs;   /* 0: no limit; a single larger request still runs alone */
@@ -372,6 +372,23 @@ int sigmo_serve_schedule(uint32_t num_workers, const uint32_t* request_graph_off
 int sigmo_serve(sigmo_batch* const* workers, uint32_t num_workers, const sigmo_host_graphs* queries, const uint32_t* request_graph_offsets,
                 const double* arrival_s, uint32_t num_requests, const sigmo_serve_options* options, sigmo_serve_result* results,
                 sigmo_serve_summary* summary);
+/* The same loop for requests that arrive while it runs (a service's request handler): open
+ * starts one thread per worker batch; submit queues a request (its query graphs are copied,
+ * arrival = now, ids count from 0) from any host thread; poll copies a request's result,
+ * waiting up to timeout_ms for it to settle (0: no wait, < 0: no limit; status
+ * SIGMO_SERVE_PENDING until then); close stops accepting, serves what is queued, joins the
+ * threads and frees the server.  Times in the results count from open.  With bytes_per_graph
+ * = 0 each request is priced at sigmo_batch_workspace_bytes of a batch holding just it. */
+typedef struct sigmo_server sigmo_server;
+int sigmo_server_open(sigmo_batch* const* workers, uint32_t num_workers, const sigmo_serve_options* options, sigmo_server** out);
+int sigmo_server_submit(sigmo_server* server, const sigmo_host_graphs* query_graphs, uint64_t* request_id);
+int sigmo_server_poll(sigmo_server* server, uint64_t request_id, double timeout_ms, sigmo_serve_result* result);
+int sigmo_server_close(sigmo_server* server, sigmo_serve_summary* summary);
+/* The online scheduler alone, with a caller-supplied executor (host-testable): requests are
+ * submitted as query-graph counts. */
+int sigmo_server_open_schedule(uint32_t num_workers, const sigmo_serve_options* options, sigmo_serve_executor executor, void* user,
+                               sigmo_server** out);
+int sigmo_server_submit_count(sigmo_server* server, uint32_t num_query_graphs, uint64_t* request_id);
 
 /* ---- streaming (a molecule shard larger than one device pass) ---------------------------- */
 /* Replaces one rank's run of the reference's multi-node driver (library/src/sigmo_mpi.cpp:

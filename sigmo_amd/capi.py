@@ -56,6 +56,8 @@ EXPORTED = (
     "sigmo_kernel_stats", "sigmo_event_mark", "sigmo_event_elapsed", "sigmo_db_prepare", "sigmo_db_free",
     "sigmo_db_alloc_size", "sigmo_db_refresh", "sigmo_batch_create", "sigmo_batch_run", "sigmo_batch_pairs", "sigmo_batch_free",
     "sigmo_batch_request_stats", "sigmo_batch_workspace_bytes", "sigmo_serve_schedule", "sigmo_serve", "sigmo_stream_run",
+    "sigmo_server_open", "sigmo_server_submit", "sigmo_server_poll", "sigmo_server_close", "sigmo_server_open_schedule",
+    "sigmo_server_submit_count",
     "sigmo_batch_set_tuning", "sigmo_batch_get_tuning", "sigmo_batch_autotune", "sigmo_batch_capture", "sigmo_batch_replay",
 )
 KERNEL_NAMES = ("sig_generate", "sig_refine", "filter", "refine", "gmcr", "join", "pair_filter", "db_prepare")
@@ -145,7 +147,7 @@ class ServeSummary(C.Structure):
 
 SERVE_EXECUTOR = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(RequestStats),
                              C.POINTER(C.c_uint64))
-SERVE_DONE, SERVE_REJECTED, SERVE_EXPIRED = 0, 1, 2
+SERVE_DONE, SERVE_REJECTED, SERVE_EXPIRED, SERVE_PENDING = 0, 1, 2, 3
 
 
 class StreamOptions(C.Structure):
@@ -258,6 +260,12 @@ def lib() -> C.CDLL:
                                       C.POINTER(ServeSummary)], i32),
             "sigmo_serve": ([vp, u32, vp, vp, vp, u32, C.POINTER(ServeOptions), C.POINTER(ServeResult), C.POINTER(ServeSummary)], i32),
             "sigmo_stream_run": ([vp, vp, vp, C.POINTER(StreamOptions), STREAM_CHUNK_FN, vp, C.POINTER(StreamResult)], i32),
+            "sigmo_server_open": ([vp, u32, C.POINTER(ServeOptions), C.POINTER(vp)], i32),
+            "sigmo_server_submit": ([vp, vp, C.POINTER(u64)], i32),
+            "sigmo_server_poll": ([vp, u64, dbl, C.POINTER(ServeResult)], i32),
+            "sigmo_server_close": ([vp, C.POINTER(ServeSummary)], i32),
+            "sigmo_server_open_schedule": ([u32, C.POINTER(ServeOptions), SERVE_EXECUTOR, vp, C.POINTER(vp)], i32),
+            "sigmo_server_submit_count": ([vp, u32, C.POINTER(u64)], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -840,14 +848,8 @@ def serve(workers: list, queries: HostGraphs, request_graph_offsets, arrival_s, 
     return _serve_out(res[:n], summ)
 
 
-def serve_schedule(num_workers: int, request_graph_offsets, arrival_s, executor, **options) -> dict:
-    """The scheduler of the serving loop with a Python executor
-    ``executor(worker, requests) -> (list of (matched_pairs, embeddings, hash), gmcr_pairs)``."""
-    rgo = np.ascontiguousarray(request_graph_offsets, np.uint32)
-    arr = np.ascontiguousarray(arrival_s, np.float64)
-    n = arr.size
-    if rgo.size != n + 1:
-        raise ValueError("request_graph_offsets needs one entry more than arrival_s")
+def _executor_shim(executor):
+    """A SERVE_EXECUTOR body around ``executor(worker, requests) -> (stats, gmcr_pairs)``."""
 
     def _run(_user, worker, reqs, count, out, gmcr):
         try:
@@ -859,12 +861,78 @@ def serve_schedule(num_workers: int, request_graph_offsets, arrival_s, executor,
         except Exception:  # noqa: BLE001 -- reported as a failed batch
             return ERR_INVALID
 
-    cb = SERVE_EXECUTOR(_run)
+    return _run
+
+
+def serve_schedule(num_workers: int, request_graph_offsets, arrival_s, executor, **options) -> dict:
+    """The scheduler of the serving loop with a Python executor
+    ``executor(worker, requests) -> (list of (matched_pairs, embeddings, hash), gmcr_pairs)``."""
+    rgo = np.ascontiguousarray(request_graph_offsets, np.uint32)
+    arr = np.ascontiguousarray(arrival_s, np.float64)
+    n = arr.size
+    if rgo.size != n + 1:
+        raise ValueError("request_graph_offsets needs one entry more than arrival_s")
+
+    cb = SERVE_EXECUTOR(_executor_shim(executor))
     res = (ServeResult * max(n, 1))()
     summ = ServeSummary()
     opt = _serve_options(**options)
     check(lib().sigmo_serve_schedule(num_workers, _ptr(rgo), _ptr(arr), n, C.byref(opt), cb, None, res, C.byref(summ)))
     return _serve_out(res[:n], summ)
+
+
+class Server:
+    """Online serving (sigmo_server_*): requests submitted while the loop runs, from any thread.
+
+    ``submit(HostGraphs) -> id``; ``poll(id, timeout_ms)`` returns the request's result as a dict
+    (``status`` is SERVE_PENDING until it settles); ``close()`` serves what is queued and returns
+    the summary.  Worker batches must stay alive until close."""
+
+    def __init__(self, workers: list | None = None, *, num_workers: int = 0, executor=None, **options):
+        """Device server over ``workers`` (batch objects of one database), or, with ``executor``
+        (as for serve_schedule) and ``num_workers``, the scheduler alone; its requests are
+        submitted as query-graph counts."""
+        self._h = None
+        self.workers = list(workers or [])
+        opt = _serve_options(**options)
+        h = C.c_void_p()
+        if executor is None:
+            handles = (C.c_void_p * len(self.workers))(*[w._h for w in self.workers])
+            check(lib().sigmo_server_open(handles, len(self.workers), C.byref(opt), C.byref(h)))
+        else:
+            self._cb = SERVE_EXECUTOR(_executor_shim(executor))  # kept alive while the server runs
+            check(lib().sigmo_server_open_schedule(num_workers, C.byref(opt), self._cb, None, C.byref(h)))
+        self._h = h
+
+    def submit(self, queries) -> int:
+        """Queue a request: HostGraphs for a device server, a query-graph count otherwise."""
+        rid = C.c_uint64()
+        if isinstance(queries, HostGraphs):
+            check(lib().sigmo_server_submit(self._h, queries.handle, C.byref(rid)))
+        else:
+            check(lib().sigmo_server_submit_count(self._h, int(queries), C.byref(rid)))
+        return rid.value
+
+    def poll(self, request_id: int, timeout_ms: float = -1.0) -> dict:
+        r = ServeResult()
+        check(lib().sigmo_server_poll(self._h, request_id, float(timeout_ms), C.byref(r)))
+        return {name: getattr(r, name) for name, _ in ServeResult._fields_}
+
+    def close(self) -> dict:
+        if not self._h:
+            return {}
+        summ = ServeSummary()
+        h, self._h = self._h, None
+        check(lib().sigmo_server_close(h, C.byref(summ)))
+        return {name: getattr(summ, name) for name, _ in ServeSummary._fields_}
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                lib().sigmo_server_close(self._h, None)
+            except Exception:  # noqa: BLE001 -- interpreter shutdown
+                pass
+            self._h = None
 
 
 # ---- streaming ----------------------------------------------------------------------------------

@@ -2,6 +2,8 @@
 a mixed continuous batch, equal the oracle's, and do not depend on the number of workers."""
 from __future__ import annotations
 
+import time
+
 import numpy as np
 import pytest
 
@@ -87,3 +89,25 @@ def test_request_stats_partition_the_batch(ctx, served):
     assert [(p["matched_pairs"], p["embeddings"], p["hash"]) for p in parts] == request_stats(off, idx, cnt, rgo)
     with pytest.raises(capi.SigmoError):
         batch.request_stats([0, 1])  # does not cover the batch
+
+
+def test_online_server_equals_offline(ctx, served):
+    """Requests submitted while the server runs (two workers, one stream each) give the same
+    per-request results as offline single-request runs."""
+    dh, qh, rgo, db = served
+    n = rgo.size - 1
+    alone = capi.serve([capi.Batch(ctx, db)], qh, rgo, np.zeros(n), max_batch_requests=1)
+    workers = [capi.Batch(c, db) for c in (capi.Context(0), capi.Context(0))]
+    print(f"online server: {n} requests over two workers", flush=True)
+    srv = capi.Server(workers, max_batch_requests=4)
+    ids = []
+    for r in range(n):
+        ids.append(srv.submit(qh.slice(int(rgo[r]), int(rgo[r + 1] - rgo[r]))))
+        if r % 6 == 5:
+            time.sleep(0.002)
+    res = [srv.poll(i, timeout_ms=-1) for i in ids]
+    s = srv.close()
+    assert ids == list(range(n))
+    assert all(r["status"] == capi.SERVE_DONE for r in res)
+    assert [(r["matched_pairs"], r["embeddings"], r["hash"]) for r in res] == _stats_of(alone)
+    assert s["done"] == n and s["matched_pairs"] == int(alone["matched_pairs"].sum())

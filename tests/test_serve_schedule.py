@@ -148,3 +148,84 @@ def test_invalid_requests(rgo, arr):
 def test_no_requests():
     out = capi.serve_schedule(2, [0], [], Recorder())
     assert out["summary"]["batches"] == 0
+
+
+# ---- online server (sigmo_server_open_schedule): requests submitted while the loop runs ------------
+
+def test_online_server_serves_submissions_as_they_come():
+    rec = Recorder(run_s=0.01)
+    srv = capi.Server(num_workers=1, executor=rec, max_batch_requests=16)
+    ids = []
+    for i in range(10):
+        ids.append(srv.submit(1 + i % 3))
+        time.sleep(0.003)
+    assert ids == list(range(10))
+    res = [srv.poll(i, timeout_ms=-1) for i in ids]
+    s = srv.close()
+    assert all(r["status"] == capi.SERVE_DONE for r in res)
+    assert [r["matched_pairs"] for r in res] == ids and [r["hash"] for r in res] == [1000 + i for i in ids]
+    served = [r for _, reqs in rec.batches for r in reqs]
+    assert served == ids  # FIFO, each request once
+    sizes = [len(reqs) for _, reqs in rec.batches]
+    assert max(sizes) >= 2  # submissions during a batch joined the next one
+    assert all(r["done_s"] >= r["admit_s"] >= 0 for r in res)
+    assert s["done"] == 10 and s["batches"] == len(rec.batches) and s["matched_pairs"] == sum(ids)
+
+
+def test_online_server_poll_without_waiting_sees_pending():
+    gate = threading.Event()
+
+    def slow(worker, reqs):
+        gate.wait(5)
+        return [(r, 0, 0) for r in reqs], 0
+
+    srv = capi.Server(num_workers=1, executor=slow)
+    rid = srv.submit(2)
+    assert srv.poll(rid, timeout_ms=0)["status"] == capi.SERVE_PENDING
+    assert srv.poll(rid, timeout_ms=20)["status"] == capi.SERVE_PENDING  # timed out
+    gate.set()
+    assert srv.poll(rid, timeout_ms=-1)["status"] == capi.SERVE_DONE
+    with pytest.raises(capi.SigmoError):
+        srv.poll(rid + 1, timeout_ms=0)  # unknown id
+    srv.close()
+
+
+def test_online_server_threads_budget_and_close_drains_the_queue():
+    rec = Recorder(run_s=0.01)
+    srv = capi.Server(num_workers=2, executor=rec, memory_budget=1000, bytes_per_graph=100, max_batch_requests=4)
+
+    def client(k):
+        return [srv.submit(1 + (k + i) % 4) for i in range(6)]
+
+    with __import__("concurrent.futures").futures.ThreadPoolExecutor(3) as ex:
+        ids = sorted(i for part in ex.map(client, range(3)) for i in part)
+    big = srv.submit(11)  # 1100 bytes > the budget: rejected
+    s = srv.close()  # serves everything already queued
+    assert ids == list(range(18))
+    assert s["done"] == 18 and s["rejected"] == 1
+    assert s["max_inflight_bytes"] <= 1000
+    served = sorted(r for _, reqs in rec.batches for r in reqs)
+    assert served == ids and big not in served
+
+
+def test_online_server_executor_failure_is_reported():
+    rec = Recorder(fail_on_batch=0)
+    srv = capi.Server(num_workers=1, executor=rec)
+    rid = srv.submit(1)
+    with pytest.raises(capi.SigmoError, match="serving batch 0"):
+        srv.poll(rid, timeout_ms=-1)
+    with pytest.raises(capi.SigmoError):
+        srv.submit(1)
+    with pytest.raises(capi.SigmoError):
+        srv.close()
+
+
+def test_online_server_argument_checks():
+    srv = capi.Server(num_workers=1, executor=Recorder())
+    with pytest.raises(capi.SigmoError):
+        srv.submit(0)  # no query graph
+    with pytest.raises(capi.SigmoError):
+        srv.submit(capi.HostGraphs.queries(1, 2, 4, 8))  # a scheduling server takes counts
+    srv.close()
+    with pytest.raises(capi.SigmoError):
+        capi.Server(num_workers=0, executor=Recorder())
