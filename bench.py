@@ -105,6 +105,30 @@ def _pct(x, q):
     return round(float(np.percentile(x, q)), 3) if len(x) else None
 
 
+def _serve_online(capi, workers, qh, rgo, arr, **options) -> dict:
+    """The same request stream through the online server: a client submits each request at its
+    arrival time (sigmo_server_submit), then polls every result; latency is completion minus
+    submission, measured by the server."""
+    import numpy as np
+
+    n = len(arr)
+    parts = [qh.slice(int(rgo[r]), int(rgo[r + 1] - rgo[r])) for r in range(n)]  # prepared before the clock starts
+    srv = capi.Server(workers, **options)
+    t0 = time.perf_counter()
+    ids = []
+    for r in range(n):
+        wait = arr[r] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        ids.append(srv.submit(parts[r]))
+    res = [srv.poll(i, -1.0) for i in ids]
+    summary = srv.close()
+    out = {k: np.array([x[k] for x in res]) for k in ("status", "latency_ms", "matched_pairs", "embeddings")}
+    out["hash"] = np.array([x["hash"] for x in res], np.uint64)
+    out["summary"] = summary
+    return out
+
+
 def serve_bench(args) -> int:
     """Serving (config 4): a resident database of LogNormal-size molecules; for each rate, a
     Poisson stream of requests carrying U[1,50] query graphs (k ~ U[4,32], 30% with a ring) goes
@@ -142,8 +166,12 @@ def serve_bench(args) -> int:
         for c in ctxs:
             c.reset_phases()
         print(f"[serve] {qps:g} QPS: {args.requests} requests, {int(rgo[-1])} query graphs", file=sys.stderr, flush=True)
-        out = capi.serve(workers, qh, rgo, arr, find_first=find_first, memory_budget=budget,
-                         max_batch_requests=args.max_batch_requests)
+        if args.online:
+            out = _serve_online(capi, workers, qh, rgo, arr, find_first=find_first, memory_budget=budget,
+                                max_batch_requests=args.max_batch_requests)
+        else:
+            out = capi.serve(workers, qh, rgo, arr, find_first=find_first, memory_budget=budget,
+                             max_batch_requests=args.max_batch_requests)
         s = out["summary"]
         done = out["status"] == capi.SERVE_DONE
         lat = out["latency_ms"][done]
@@ -197,7 +225,7 @@ def serve_bench(args) -> int:
             "workload": f"config {args.config} serving: {args.molecules} molecules seed {seed} LogNormal(ln 24,0.45) sizes "
                         f"resident; Poisson requests of U[1,50] query graphs k~U[{kmin},{kmax}] {int(ring * 100)}% with a ring; "
                         f"R={args.refinement_steps}, {'find-first' if find_first else 'find-all'}; {len(ctxs)} workers, "
-                        f"budget {args.budget_gb:g} GB",
+                        f"budget {args.budget_gb:g} GB" + ("; online submissions" if args.online else ""),
             "molecules": args.molecules,
             "data_nodes": dd.total_nodes,
         },
@@ -233,6 +261,9 @@ def main() -> int:
     ap.add_argument("--workers", type=int, default=2, help="serving: batches in flight (one stream each)")
     ap.add_argument("--budget-gb", type=float, default=128.0, help="serving: device bytes for batches in flight")
     ap.add_argument("--max-batch-requests", type=int, default=256)
+    ap.add_argument("--online", action="store_true",
+                    help="serving: submit each request at its arrival time through the online server (sigmo_server_*) "
+                         "instead of handing the loop the whole arrival schedule")
     ap.add_argument("--chunk-gb", type=float, default=128.0,
                     help="config 3: size chunks so the plain form's candidate bitset of one chunk fits this many GB")
     ap.add_argument("--chunk-molecules", type=int, default=0, help="config 3: molecules per chunk (overrides --chunk-gb)")
