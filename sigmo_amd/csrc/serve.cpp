@@ -347,6 +347,12 @@ struct sigmo_server {
       sigmo_host::append_batch(hb, *g, 0, g->num_graphs());
       lo.push_back(hb.num_graphs());
     }
+    {
+      // the batch holds its own copy now: release the requests' graphs (a long-running server
+      // keeps only each request's result)
+      std::lock_guard<std::mutex> lk(self->sch.mu);
+      for (uint32_t i = 0; i < count; ++i) self->graphs[requests[i]].reset();
+    }
     // sch.opt is fixed once the server is open
     return run_device_batch(self->workers[w], &self->dev[w], hb, lo, self->sch.opt.find_first, count, out, gmcr_pairs);
   }
