@@ -668,7 +668,8 @@ hipError_t refine(hipStream_t s, const uint64_t* qsig, uint64_t Q, const uint64_
   const uint64_t cols = (W64 + kBitsBlock - 1) / kBitsBlock;
   // 32 B of planes per label and word against 16 B of row traffic per row.  When the column
   // strips alone give every CU 4 blocks (bitsets of tens of GB), split the rows in two as well:
-  // 8 blocks per CU keep more loads in flight (config 2: 27.7 -> 24.2 ms per pass)
+  // 8 blocks per CU keep more loads in flight (1,000,000 molecules x 1,000 queries: 27.7 -> 24.2
+  // ms per pass)
   const uint64_t cus = static_cast<uint64_t>(num_cus > 0 ? num_cus : 256);
   const unsigned rpb = rows_per_block(Q, cols, num_cus, 16 * static_cast<uint64_t>(slots.count), cols >= cus * 4 ? 8 : 4);
   const dim3 grid(static_cast<unsigned>(cols), static_cast<unsigned>((Q + rpb - 1) / rpb));
