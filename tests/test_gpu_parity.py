@@ -115,6 +115,33 @@ def test_pipeline_entry_point_matches_phases(ctx):
     assert r1["total_ms"] > 0
 
 
+def test_phase_and_kernel_timing(ctx):
+    """Phase and per-kernel times of executes: the pipeline's phases share their boundary events,
+    so the phases tile the execute (their sum stays within its total), every launch is counted
+    once and every kernel time is positive."""
+    qh, dh, R = _case("cfg1_slice")
+    qd, dd = capi.DeviceGraph(ctx, qh), capi.DeviceGraph(ctx, dh)
+    pipe = capi.Pipeline(ctx, qd, dd)
+    pipe.execute(R, True)
+    ctx.sync()
+    ctx.reset_phases()
+    runs = [pipe.execute(R, True) for _ in range(3)]
+    ctx.sync()
+    phases = ctx.phase_ms()
+    ks = ctx.kernel_stats()
+    total = sum(r["total_ms"] for r in runs)
+    assert all(v >= 0 for v in phases.values())
+    assert 0.5 * total <= sum(phases.values()) <= 1.02 * total + 0.05
+    n = len(runs)
+    expect = {"sig_generate": n, "sig_refine": n * R, "filter": n, "refine": n * R, "gmcr": n, "join": n}
+    for name, launches in expect.items():
+        got, ms, by = ks[name]
+        assert got == launches, name
+        assert ms > 0, name
+    for r in runs:
+        assert sum(r["phase_ms"].values()) <= 1.02 * r["total_ms"] + 0.05
+
+
 def test_config1_full_against_golden(ctx):
     """Config 1 at full size (100,000 molecules, seed 1) against the oracle's committed digest."""
     path = GOLDEN / "config1.json"
