@@ -688,17 +688,30 @@ static int sig_target(sigmo_signatures* s, const sigmo_graph* g, int scope, uint
 // Algorithmic HBM bytes of one signature pass: CSR + labels read, 8-byte signature written.
 static double sig_bytes(const sigmo_graph* g) { return 4.0 * (g->N + 1) + 4.0 * g->E + g->N + 8.0 * g->N; }
 
-int sigmo_signatures_generate(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope) {
+// `fresh` (the pipeline's first calls): the signatures, resp. the reachables, are taken as
+// zero without a clearing pass -- what the reference's constructor fill leaves
+// (signature.hpp:166-167; reachables start empty, A3).
+static int signatures_generate(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope, bool fresh) {
   SIGMO_TRY(bind(ctx));
   uint64_t *sigs, *reach, *planes;
   SIGMO_TRY(sig_target(s, g, scope, &sigs, &reach, &planes));
   PhaseScope ps(ctx, scope == SIGMO_SCOPE_DATA ? SIGMO_PHASE_SIG_DATA : SIGMO_PHASE_SIG_QUERY,
                 scope == SIGMO_SCOPE_DATA ? SIGMO_KERNEL_SIG_GENERATE : -1, sig_bytes(g));
-  SIGMO_HIP_TRY(sigmo_k::sig_generate(ctx->stream, g->dev(), sigs, planes));
+  SIGMO_HIP_TRY(sigmo_k::sig_generate(ctx->stream, g->dev(), sigs, planes, fresh));
   return ps.close();
 }
 
+int sigmo_signatures_generate(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope) {
+  return signatures_generate(ctx, s, g, scope, false);
+}
+
+static int signatures_refine(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope, uint32_t view_size, bool fresh_reach);
+
 int sigmo_signatures_refine(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope, uint32_t view_size) {
+  return signatures_refine(ctx, s, g, scope, view_size, false);
+}
+
+static int signatures_refine(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope, uint32_t view_size, bool fresh_reach) {
   SIGMO_TRY(bind(ctx));
   uint64_t *sigs, *reach, *planes;
   SIGMO_TRY(sig_target(s, g, scope, &sigs, &reach, &planes));
@@ -716,7 +729,7 @@ int sigmo_signatures_refine(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_gra
     st.max_view = view_size;
   } else {
     const bool keep = !st.tainted && view_size > st.max_view;
-    SIGMO_HIP_TRY(sigmo_k::sig_refine(ctx->stream, g->dev(), view_size, sigs, reach, keep ? front : nullptr, planes));
+    SIGMO_HIP_TRY(sigmo_k::sig_refine(ctx->stream, g->dev(), view_size, sigs, reach, keep ? front : nullptr, planes, fresh_reach));
     if (view_size > st.max_view) {
       st.max_view = view_size;
       st.frontier_valid = keep;
@@ -1064,18 +1077,16 @@ int pipeline_enqueue(sigmo_pipeline* p, uint32_t refinement_steps, int find_firs
       c->chain_end = nullptr;
     }
   } chain(ctx);
-  SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->dsig, sig->D, 0));
-  SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->qsig, sig->Q, 0));
-  SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->dreach, sig->D, 0));
-  SIGMO_HIP_TRY(sigmo_k::fill_u64(ctx->stream, sig->qreach, sig->Q, 0));
+  // every run starts from the constructor's state: zero signatures and empty reachables, taken
+  // as such by the first generate and the first refinement (no clearing pass)
   sig->dstate.reset();
   sig->qstate.reset();
-  SIGMO_TRY(sigmo_signatures_generate(ctx, sig, p->d, SIGMO_SCOPE_DATA));
-  SIGMO_TRY(sigmo_signatures_generate(ctx, sig, p->q, SIGMO_SCOPE_QUERY));
+  SIGMO_TRY(signatures_generate(ctx, sig, p->d, SIGMO_SCOPE_DATA, true));
+  SIGMO_TRY(signatures_generate(ctx, sig, p->q, SIGMO_SCOPE_QUERY, true));
   SIGMO_TRY(sigmo_candidates_filter(ctx, p->cand, p->q, p->d, sig));
   for (uint32_t r = 1; r <= refinement_steps; ++r) {
-    SIGMO_TRY(sigmo_signatures_refine(ctx, sig, p->d, SIGMO_SCOPE_DATA, r));
-    SIGMO_TRY(sigmo_signatures_refine(ctx, sig, p->q, SIGMO_SCOPE_QUERY, r));
+    SIGMO_TRY(signatures_refine(ctx, sig, p->d, SIGMO_SCOPE_DATA, r, r == 1));
+    SIGMO_TRY(signatures_refine(ctx, sig, p->q, SIGMO_SCOPE_QUERY, r, r == 1));
     SIGMO_TRY(sigmo_candidates_refine(ctx, p->cand, p->q, p->d, sig));
   }
   SIGMO_TRY(sigmo_gmcr_generate(ctx, p->gm, p->q, p->d, p->cand));

@@ -172,15 +172,16 @@ __device__ __forceinline__ bool stage_label_masks(const GraphDev& g, uint64_t* l
 }
 
 // generateCSRSignatures, signature.hpp:125-152: one thread per node, neighbour labels
-// incremented onto the existing value.
+// incremented onto the existing value (onto zero with `fresh`: the constructor's fill,
+// signature.hpp:166-167, without a separate pass).
 __global__ __launch_bounds__(kSigBlock) void k_sig_generate(const uint32_t* __restrict__ ro, const uint32_t* __restrict__ ci,
                                                             const uint8_t* __restrict__ nl, uint64_t N, uint64_t* __restrict__ sigs,
-                                                            uint64_t* __restrict__ planes) {
+                                                            uint64_t* __restrict__ planes, int fresh) {
   __shared__ uint64_t tile[64 * (kSigBlock / kWave)];
   const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   uint64_t s = 0;
   if (v < N) {
-    s = sigs[v];
+    s = fresh ? 0ull : sigs[v];
     for (uint32_t i = ro[v]; i < ro[v + 1]; ++i) s = sig_inc(s, nl[ci[i]]);
     sigs[v] = s;
   }
@@ -250,7 +251,7 @@ __global__ __launch_bounds__(256) void k_filter_data(const uint8_t* __restrict__
 // k_sig_refine_next.
 __global__ __launch_bounds__(kSigBlock) void k_sig_refine(GraphDev g, uint32_t view_size, uint64_t* __restrict__ sigs,
                                                           uint64_t* __restrict__ reach, uint64_t* __restrict__ frontier_out,
-                                                          uint64_t* __restrict__ planes) {
+                                                          uint64_t* __restrict__ planes, int fresh_reach) {
   __shared__ uint64_t tile[64 * (kSigBlock / kWave)];
   __shared__ uint64_t lms[kStagedGraphs * 16];
   const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
@@ -268,7 +269,7 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_refine(GraphDev g, uint32_t v
       ball |= next;
       frontier = next;
     }
-    const uint64_t old = reach[v];
+    const uint64_t old = fresh_reach ? 0ull : reach[v];  // fresh: the reachables start empty
     const uint64_t r = ball & ~self & ~old;
     s = staged ? label_histogram(r, lms + (gid - g0) * 16) : label_histogram(r, g.lmask + static_cast<uint64_t>(gid) * 16);
     sigs[v] = s;
@@ -608,10 +609,10 @@ hipError_t derive_graph(hipStream_t s, const uint32_t* go, uint32_t G, const uin
   return hipGetLastError();
 }
 
-hipError_t sig_generate(hipStream_t s, const GraphDev& g, uint64_t* sigs, uint64_t* planes) {
+hipError_t sig_generate(hipStream_t s, const GraphDev& g, uint64_t* sigs, uint64_t* planes, bool fresh) {
   if (!g.N) return hipSuccess;
   const unsigned blocks = static_cast<unsigned>((g.N + kSigBlock - 1) / kSigBlock);
-  k_sig_generate<<<blocks, kSigBlock, 0, s>>>(g.ro, g.ci, g.nl, g.N, sigs, planes);
+  k_sig_generate<<<blocks, kSigBlock, 0, s>>>(g.ro, g.ci, g.nl, g.N, sigs, planes, fresh ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -630,10 +631,10 @@ hipError_t filter_data(hipStream_t s, const uint8_t* qlabels, uint64_t Q, const 
 }
 
 hipError_t sig_refine(hipStream_t s, const GraphDev& g, uint32_t view_size, uint64_t* sigs, uint64_t* reach, uint64_t* frontier_out,
-                      uint64_t* planes) {
+                      uint64_t* planes, bool fresh_reach) {
   if (!g.N) return hipSuccess;
   const unsigned blocks = static_cast<unsigned>((g.N + kSigBlock - 1) / kSigBlock);
-  k_sig_refine<<<blocks, kSigBlock, 0, s>>>(g, view_size, sigs, reach, frontier_out, planes);
+  k_sig_refine<<<blocks, kSigBlock, 0, s>>>(g, view_size, sigs, reach, frontier_out, planes, fresh_reach ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -47,11 +47,13 @@ hipError_t sub_u32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v);
 hipError_t derive_graph(hipStream_t s, const uint32_t* go, uint32_t G, const uint32_t* ro, const uint32_t* ci, const uint8_t* nl,
                         uint64_t N, uint32_t* node_graph, uint64_t* adj, uint64_t* lplanes, uint64_t* lmask);
 
-// generateCSRSignatures: increments the neighbour-label histogram onto sigs.
-hipError_t sig_generate(hipStream_t s, const GraphDev& g, uint64_t* sigs, uint64_t* planes);
+// generateCSRSignatures: increments the neighbour-label histogram onto sigs (onto zero with
+// `fresh`, whatever sigs holds).
+hipError_t sig_generate(hipStream_t s, const GraphDev& g, uint64_t* sigs, uint64_t* planes, bool fresh = false);
 // refineCSRSignatures<PowerGraph>(view_size); frontier_out (nullable) receives the last BFS layer.
+// With `fresh_reach` the reachables are taken as empty whatever `reach` holds.
 hipError_t sig_refine(hipStream_t s, const GraphDev& g, uint32_t view_size, uint64_t* sigs, uint64_t* reach, uint64_t* frontier_out,
-                      uint64_t* planes);
+                      uint64_t* planes, bool fresh_reach = false);
 // The same for view_size = M+1 when reach/frontier hold the state left by view_size = M.
 hipError_t sig_refine_next(hipStream_t s, const GraphDev& g, uint64_t* sigs, uint64_t* reach, uint64_t* frontier, uint64_t* planes);
 
