@@ -154,21 +154,35 @@ __device__ __forceinline__ uint64_t label_histogram(uint64_t r, P lm) {
   return s;
 }
 
-// A signature block's nodes belong to a run of consecutive graphs; when at most kStagedGraphs,
-// their label masks are staged in LDS (the histograms then read LDS instead of 16 words of
-// global memory per node).  Block-uniform result; g0 = the run's first graph.
+// A signature block's nodes belong to a run of consecutive graphs.  When the run has at most
+// kStagedGraphs graphs and kStagedRows nodes, its label masks and the folded adjacency
+// (adj_lo | adj_hi) of its nodes are staged in LDS: the BFS expansions and the histograms then
+// read LDS instead of global memory.  Block-uniform result; g0 = the run's first graph, a0 its
+// first node.
 constexpr uint32_t kStagedGraphs = 64;
-__device__ __forceinline__ bool stage_label_masks(const GraphDev& g, uint64_t* lms, uint32_t& g0) {
+constexpr uint32_t kStagedRows = 1024;
+__device__ __forceinline__ bool stage_block(const GraphDev& g, uint64_t* lms, uint64_t* fold, uint32_t& g0, uint64_t& a0) {
   const uint64_t v0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x;
   if (v0 >= g.N) return false;
   const uint64_t vl = (v0 + blockDim.x < g.N ? v0 + blockDim.x : g.N) - 1;
   g0 = g.node_graph[v0];
-  const uint32_t ng = g.node_graph[vl] - g0 + 1;
-  const bool staged = ng <= kStagedGraphs;
-  if (staged)
-    for (uint32_t t = threadIdx.x; t < ng * 16; t += blockDim.x) lms[t] = g.lmask[static_cast<uint64_t>(g0) * 16 + t];
+  const uint32_t g1 = g.node_graph[vl];
+  a0 = g.go[g0];
+  const uint64_t na = static_cast<uint64_t>(g.go[g1 + 1]) - a0;
+  const bool staged = g1 - g0 + 1 <= kStagedGraphs && na <= kStagedRows;
+  if (staged) {
+    for (uint32_t t = threadIdx.x; t < (g1 - g0 + 1) * 16; t += blockDim.x) lms[t] = g.lmask[static_cast<uint64_t>(g0) * 16 + t];
+    for (uint32_t t = threadIdx.x; t < na; t += blockDim.x) fold[t] = g.adj[2 * (a0 + t)] | g.adj[2 * (a0 + t) + 1];
+  }
   __syncthreads();
   return staged;
+}
+
+// expand_frontier over the staged folded adjacency; `at` = the graph's first node - a0.
+__device__ __forceinline__ uint64_t expand_frontier_lds(const uint64_t* fold, uint64_t at, uint64_t f) {
+  uint64_t next = 0;
+  for (; f; f &= f - 1) next |= fold[at + __builtin_ctzll(f)];
+  return next;
 }
 
 // generateCSRSignatures, signature.hpp:125-152: one thread per node, neighbour labels
@@ -254,22 +268,26 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_refine(GraphDev g, uint32_t v
                                                           uint64_t* __restrict__ planes, int fresh_reach) {
   __shared__ uint64_t tile[64 * (kSigBlock / kWave)];
   __shared__ uint64_t lms[kStagedGraphs * 16];
+  __shared__ uint64_t fold[kStagedRows];
   const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  // the node's own loads go out before the block's staging barrier
+  const bool live = v < g.N;
+  const uint32_t gid = live ? g.node_graph[v] : 0u;
+  const uint32_t prev = live ? g.go[gid] : 0u;
+  const uint64_t old = live && !fresh_reach ? reach[v] : 0ull;  // fresh: the reachables start empty
   uint32_t g0 = 0;
-  const bool staged = stage_label_masks(g, lms, g0);
+  uint64_t a0 = 0;
+  const bool staged = stage_block(g, lms, fold, g0, a0);
   uint64_t s = 0;
-  if (v < g.N) {
-    const uint32_t gid = g.node_graph[v];
-    const uint32_t prev = g.go[gid];
+  if (live) {
     const uint64_t self = 1ull << ((static_cast<uint32_t>(v) - prev) & 63u);
     uint64_t ball = self, frontier = self;
     for (uint32_t it = 0; it < view_size && frontier; ++it) {
-      uint64_t next = expand_frontier(g.adj, prev, frontier);
+      uint64_t next = staged ? expand_frontier_lds(fold, prev - a0, frontier) : expand_frontier(g.adj, prev, frontier);
       next &= ~ball;
       ball |= next;
       frontier = next;
     }
-    const uint64_t old = fresh_reach ? 0ull : reach[v];  // fresh: the reachables start empty
     const uint64_t r = ball & ~self & ~old;
     s = staged ? label_histogram(r, lms + (gid - g0) * 16) : label_histogram(r, g.lmask + static_cast<uint64_t>(gid) * 16);
     sigs[v] = s;
@@ -286,16 +304,21 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_refine_next(GraphDev g, uint6
                                                                uint64_t* __restrict__ frontier, uint64_t* __restrict__ planes) {
   __shared__ uint64_t tile[64 * (kSigBlock / kWave)];
   __shared__ uint64_t lms[kStagedGraphs * 16];
+  __shared__ uint64_t fold[kStagedRows];
   const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  // the node's own loads go out before the block's staging barrier
+  const bool live = v < g.N;
+  const uint32_t gid = live ? g.node_graph[v] : 0u;
+  const uint32_t prev = live ? g.go[gid] : 0u;
+  const uint64_t old = live ? reach[v] : 0ull;
+  const uint64_t f = live ? frontier[v] : 0ull;
   uint32_t g0 = 0;
-  const bool staged = stage_label_masks(g, lms, g0);
+  uint64_t a0 = 0;
+  const bool staged = stage_block(g, lms, fold, g0, a0);
   uint64_t s = 0;
-  if (v < g.N) {
-    const uint32_t gid = g.node_graph[v];
-    const uint32_t prev = g.go[gid];
+  if (live) {
     const uint64_t self = 1ull << ((static_cast<uint32_t>(v) - prev) & 63u);
-    const uint64_t old = reach[v];
-    uint64_t next = expand_frontier(g.adj, prev, frontier[v]);
+    uint64_t next = staged ? expand_frontier_lds(fold, prev - a0, f) : expand_frontier(g.adj, prev, f);
     next &= ~(old | self);
     s = staged ? label_histogram(next, lms + (gid - g0) * 16) : label_histogram(next, g.lmask + static_cast<uint64_t>(gid) * 16);
     sigs[v] = s;
