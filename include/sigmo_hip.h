@@ -377,8 +377,10 @@ int sigmo_serve(sigmo_batch* const* workers, uint32_t num_workers, const sigmo_h
  * arrival = now, ids count from 0) from any host thread; poll copies a request's result,
  * waiting up to timeout_ms for it to settle (0: no wait, < 0: no limit; status
  * SIGMO_SERVE_PENDING until then); close stops accepting, serves what is queued, joins the
- * threads and frees the server.  Times in the results count from open.  With bytes_per_graph
- * = 0 each request is priced at sigmo_batch_workspace_bytes of a batch holding just it. */
+ * threads and frees the server (no other call on the server may overlap close).  Times in the
+ * results count from open.  With bytes_per_graph = 0 each request is priced at
+ * sigmo_batch_workspace_bytes of a batch holding just it.  After a batch fails, submit and the
+ * polls of unserved requests return its error. */
 typedef struct sigmo_server sigmo_server;
 int sigmo_server_open(sigmo_batch* const* workers, uint32_t num_workers, const sigmo_serve_options* options, sigmo_server** out);
 int sigmo_server_submit(sigmo_server* server, const sigmo_host_graphs* query_graphs, uint64_t* request_id);
