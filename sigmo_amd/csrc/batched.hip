@@ -1202,11 +1202,11 @@ hipError_t launch_join(sigmo_batch* b, const RunShape& L, hipStream_t s, unsigne
   if (find_first)
     hipLaunchKernelGGL((sigmo_k::k_join_lanes<true, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(lblocks)), dim3(sigmo_k::kJoinLaneWaves * 64),
                        0, s, L.qd, L.dd, sigmo_k::ArenaMasks{b->arena, b->mask_off}, b->indices, b->pair_graph, P, lchunk, b->counts, stats,
-                       b->wide_pairs);
+                       b->wide_pairs, nullptr);
   else
     hipLaunchKernelGGL((sigmo_k::k_join_lanes<false, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(lblocks)), dim3(sigmo_k::kJoinLaneWaves * 64),
                        0, s, L.qd, L.dd, sigmo_k::ArenaMasks{b->arena, b->mask_off}, b->indices, b->pair_graph, P, lchunk, b->counts, stats,
-                       b->wide_pairs);
+                       b->wide_pairs, nullptr);
   return hipGetLastError();
 }
 

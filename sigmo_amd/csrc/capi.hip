@@ -259,7 +259,12 @@ struct sigmo_gmcr {
   uint32_t* wide_pairs = nullptr;  // pairs of molecules with > 64 nodes, for the two-word lane join
   uint64_t cap_wide = 0;
   uint64_t last_wide = 0;  // their count in the last join
+  // The plain pipeline hands the pair count to the join on the device: P then holds an upper
+  // bound (Gq x Gd) until the join's synchronisation reads the exact count from total_host.
+  bool deferred = false;
+  unsigned long long* total_host = nullptr;  // pinned
   ~sigmo_gmcr() {
+    if (total_host) (void)hipHostFree(total_host);
     dev_free(wide_pairs);
     dev_free(offsets);
     dev_free(counts);
@@ -906,7 +911,29 @@ static int check_queries(const sigmo_graph* q) {
   return SIGMO_OK;
 }
 
+// Pair spaces up to this size let the plain pipeline skip the host round trip for the GMCR
+// total: the index arrays are sized for every pair (4 x 4 B per pair at most).
+constexpr uint64_t kDeferredPairs = 1ull << 25;
+
+static int gmcr_generate(sigmo_ctx* ctx, sigmo_gmcr* m, const sigmo_graph* query, const sigmo_graph* data, const sigmo_candidates* c,
+                         bool defer);
+
 int sigmo_gmcr_generate(sigmo_ctx* ctx, sigmo_gmcr* m, const sigmo_graph* query, const sigmo_graph* data, const sigmo_candidates* c) {
+  return gmcr_generate(ctx, m, query, data, c, false);
+}
+
+// The exact pair count of a deferred GMCR (a no-op otherwise).
+static int gmcr_resolve(sigmo_gmcr* m, hipStream_t s) {
+  if (!m->deferred) return SIGMO_OK;
+  SIGMO_HIP_TRY(hipStreamSynchronize(s));
+  m->P = *m->total_host;
+  m->deferred = false;
+  if (m->P >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "GMCR pair count exceeds u32");
+  return SIGMO_OK;
+}
+
+static int gmcr_generate(sigmo_ctx* ctx, sigmo_gmcr* m, const sigmo_graph* query, const sigmo_graph* data, const sigmo_candidates* c,
+                         bool defer) {
   SIGMO_TRY(bind(ctx));
   if (!m) return sigmo_err::set(SIGMO_ERR_INVALID, "null gmcr");
   SIGMO_TRY(check_cand(c, query, data));
@@ -923,9 +950,18 @@ int sigmo_gmcr_generate(sigmo_ctx* ctx, sigmo_gmcr* m, const sigmo_graph* query,
   SIGMO_HIP_TRY(sigmo_k::gmcr_count(s, m->pair_bits, Gq, Gd, m->counts));
   SIGMO_HIP_TRY(sigmo_k::exclusive_scan(s, m->counts, Gd, m->offsets, m->scan_tmp, m->total));
   // gmcr.hpp:79-84 reads the total back to size the index array; a graph being recorded keeps
-  // the previous run's count and sizes (the replay checks the total)
+  // the previous run's count and sizes (the replay checks the total).  A deferred GMCR sizes the
+  // arrays for every pair instead and leaves the total on the device for the join.
   uint64_t P = m->P;
-  if (!ctx->capturing) {
+  const uint64_t bound = uint64_t(Gq) * Gd;
+  m->deferred = defer && !ctx->capturing && data->max_nodes <= 64 && bound <= kDeferredPairs;
+  if (m->deferred) {
+    SIGMO_TRY(grow(&m->indices, m->cap_indices, bound));
+    SIGMO_TRY(grow(&m->pair_graph, m->cap_pair_graph, bound));
+    if (!m->total_host) SIGMO_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&m->total_host), sizeof(unsigned long long)));
+    SIGMO_HIP_TRY(hipMemcpyAsync(m->total_host, m->total, 8, hipMemcpyDeviceToHost, s));  // read after the join's sync
+    P = bound;
+  } else if (!ctx->capturing) {
     SIGMO_HIP_TRY(hipMemcpyAsync(&P, m->total, 8, hipMemcpyDeviceToHost, s));
     SIGMO_HIP_TRY(hipStreamSynchronize(s));
     if (P >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "GMCR pair count exceeds u32");
@@ -943,6 +979,7 @@ int sigmo_gmcr_generate(sigmo_ctx* ctx, sigmo_gmcr* m, const sigmo_graph* query,
 }
 
 int sigmo_gmcr_total(const sigmo_gmcr* m, uint64_t* total_query_indices) {
+  if (m && m->deferred) SIGMO_TRY(gmcr_resolve(const_cast<sigmo_gmcr*>(m), m->ctx->stream));
   if (!m || !total_query_indices) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
   *total_query_indices = m->P;
   return SIGMO_OK;
@@ -952,6 +989,7 @@ int sigmo_gmcr_download(sigmo_ctx* ctx, const sigmo_gmcr* m, uint32_t* data_grap
                         uint32_t* query_graph_indices, uint64_t n_indices) {
   SIGMO_TRY(bind(ctx));
   if (!m || !m->built) return sigmo_err::set(SIGMO_ERR_INVALID, "GMCR not generated");
+  SIGMO_TRY(gmcr_resolve(const_cast<sigmo_gmcr*>(m), ctx->stream));
   if (data_graph_offsets) {
     if (n_offsets != uint64_t(m->Gd) + 1) return sigmo_err::set(SIGMO_ERR_INVALID, "n_offsets must be Gd+1");
     SIGMO_HIP_TRY(hipMemcpyAsync(data_graph_offsets, m->offsets, n_offsets * 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -980,8 +1018,10 @@ int sigmo_join(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data
   PhaseScope ps(ctx, SIGMO_PHASE_JOIN, SIGMO_KERNEL_JOIN, 0.0);
   SIGMO_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(ctx->stats), 8, 0));
   SIGMO_TRY(grow(&mm->wide_pairs, mm->cap_wide, m->P));
+  // a deferred GMCR: P bounds the grid and the kernel reads the exact count on the device
   SIGMO_HIP_TRY(sigmo_k::join_lanes(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, m->P, find_first,
-                                    mm->pair_counts, ctx->stats, mm->wide_pairs, ctx->num_cus));
+                                    mm->pair_counts, ctx->stats, mm->wide_pairs, ctx->num_cus,
+                                    m->deferred ? reinterpret_cast<const unsigned long long*>(m->total) : nullptr));
   if (data->max_nodes > 64 && m->P) {  // pairs of molecules with > 64 nodes: the two-word lane join
     unsigned long long nw = mm->last_wide;  // a graph being recorded keeps the previous run's count
     if (!ctx->capturing) {
@@ -996,6 +1036,7 @@ int sigmo_join(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data
   SIGMO_TRY(ps.close());
   SIGMO_HIP_TRY(hipMemcpyAsync(ctx->stats_host, ctx->stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   if (ctx->capturing) return SIGMO_OK;  // the replay reads the pinned mirror after the graph
+  SIGMO_TRY(gmcr_resolve(mm, s));  // the exact pair count, before any per-pair copy
   if (per_pair_counts && m->P)
     SIGMO_HIP_TRY(hipMemcpyAsync(per_pair_counts, mm->pair_counts, m->P * 4, hipMemcpyDeviceToHost, s));
   SIGMO_HIP_TRY(hipStreamSynchronize(s));
@@ -1012,6 +1053,7 @@ int sigmo_join_wildcard(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_gr
   if (m->Gq != query->G || m->Gd != data->G) return sigmo_err::set(SIGMO_ERR_INVALID, "GMCR built for other graphs");
   SIGMO_TRY(check_queries(query));
   auto* mm = const_cast<sigmo_gmcr*>(m);
+  SIGMO_TRY(gmcr_resolve(mm, ctx->stream));
   SIGMO_TRY(grow(&mm->pair_counts, mm->cap_counts, m->P));
   hipStream_t s = ctx->stream;
   PhaseScope ps(ctx, SIGMO_PHASE_JOIN, SIGMO_KERNEL_JOIN, 0.0);
@@ -1089,7 +1131,7 @@ int pipeline_enqueue(sigmo_pipeline* p, uint32_t refinement_steps, int find_firs
     SIGMO_TRY(signatures_refine(ctx, sig, p->q, SIGMO_SCOPE_QUERY, r, r == 1));
     SIGMO_TRY(sigmo_candidates_refine(ctx, p->cand, p->q, p->d, sig));
   }
-  SIGMO_TRY(sigmo_gmcr_generate(ctx, p->gm, p->q, p->d, p->cand));
+  SIGMO_TRY(gmcr_generate(ctx, p->gm, p->q, p->d, p->cand, true));
   return sigmo_join(ctx, p->q, p->d, p->cand, p->gm, find_first, matches, nullptr);
 }
 }  // namespace

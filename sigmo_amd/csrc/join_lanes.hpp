@@ -76,8 +76,9 @@ template <bool FIND_FIRST, class Src>
 __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, GraphDev d, Src src, const uint32_t* __restrict__ indices,
                                                                    const uint32_t* __restrict__ pair_graph, uint64_t P, uint32_t chunk,
                                                                    int32_t* __restrict__ counts, unsigned long long* __restrict__ stats,
-                                                                   uint32_t* __restrict__ wide) {
+                                                                   uint32_t* __restrict__ wide, const unsigned long long* __restrict__ P_dev) {
   __shared__ uint64_t rows_all[kJoinLaneWaves][32][kWave];
+  if (P_dev) P = *P_dev;  // the pair count the GMCR scan left on the device (P: the grid's bound)
   __shared__ uint8_t maps_all[kJoinLaneWaves][32][kWave];
   const uint32_t lane = static_cast<uint32_t>(lane_id());
   const uint32_t wave = threadIdx.x / kWave;

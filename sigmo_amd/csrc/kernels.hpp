@@ -87,10 +87,11 @@ hipError_t gmcr_scatter(hipStream_t s, const uint64_t* pair_bits, uint32_t Gq, u
 // Join.  stats[0] += sum of counts, stats[1] += pairs with count > 0, stats[2] = work cursor
 // (zeroed by the caller), stats[3] |= 1 on int32 overflow.
 // Lane-per-pair join of the pairs whose molecule has <= 64 nodes; the others are appended to
-// `wide` (count in stats[5]) for join_lanes_wide.
+// `wide` (count in stats[5]) for join_lanes_wide.  With P_dev the kernel reads the pair count
+// from the device and P only bounds the grid.
 hipError_t join_lanes(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
                       const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts, unsigned long long* stats, uint32_t* wide,
-                      int num_cus);
+                      int num_cus, const unsigned long long* P_dev = nullptr);
 // joinWildcardCandidates: matching order by candidate count, edge labels compared, one DFS per
 // candidate of the first node; one lane per pair (qel/del: edge labels of the CSR entries).
 hipError_t join_wildcard(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint8_t* qel, const uint8_t* del, const uint64_t* bits,
