@@ -695,11 +695,13 @@ static double sig_bytes(const sigmo_graph* g) { return 4.0 * (g->N + 1) + 4.0 * 
 
 // `fresh` (the pipeline's first calls): the signatures, resp. the reachables, are taken as
 // zero without a clearing pass -- what the reference's constructor fill leaves
-// (signature.hpp:166-167; reachables start empty, A3).
-static int signatures_generate(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope, bool fresh) {
+// (signature.hpp:166-167; reachables start empty, A3).  `planes_only` (the pipeline's data
+// side, whose packed signatures nothing reads): only the bit-sliced planes are written.
+static int signatures_generate(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope, bool fresh, bool planes_only = false) {
   SIGMO_TRY(bind(ctx));
   uint64_t *sigs, *reach, *planes;
   SIGMO_TRY(sig_target(s, g, scope, &sigs, &reach, &planes));
+  if (planes_only && planes && fresh) sigs = nullptr;
   PhaseScope ps(ctx, scope == SIGMO_SCOPE_DATA ? SIGMO_PHASE_SIG_DATA : SIGMO_PHASE_SIG_QUERY,
                 scope == SIGMO_SCOPE_DATA ? SIGMO_KERNEL_SIG_GENERATE : -1, sig_bytes(g));
   SIGMO_HIP_TRY(sigmo_k::sig_generate(ctx->stream, g->dev(), sigs, planes, fresh));
@@ -710,16 +712,19 @@ int sigmo_signatures_generate(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_g
   return signatures_generate(ctx, s, g, scope, false);
 }
 
-static int signatures_refine(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope, uint32_t view_size, bool fresh_reach);
+static int signatures_refine(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope, uint32_t view_size, bool fresh_reach,
+                             bool planes_only = false);
 
 int sigmo_signatures_refine(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope, uint32_t view_size) {
   return signatures_refine(ctx, s, g, scope, view_size, false);
 }
 
-static int signatures_refine(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope, uint32_t view_size, bool fresh_reach) {
+static int signatures_refine(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope, uint32_t view_size, bool fresh_reach,
+                             bool planes_only) {
   SIGMO_TRY(bind(ctx));
   uint64_t *sigs, *reach, *planes;
   SIGMO_TRY(sig_target(s, g, scope, &sigs, &reach, &planes));
+  if (planes_only && planes) sigs = nullptr;
   RefineState& st = scope == SIGMO_SCOPE_DATA ? s->dstate : s->qstate;
   uint64_t* front = scope == SIGMO_SCOPE_DATA ? s->dfront : s->qfront;
   if (st.graph && st.graph != g) {
@@ -1123,11 +1128,12 @@ int pipeline_enqueue(sigmo_pipeline* p, uint32_t refinement_steps, int find_firs
   // as such by the first generate and the first refinement (no clearing pass)
   sig->dstate.reset();
   sig->qstate.reset();
-  SIGMO_TRY(signatures_generate(ctx, sig, p->d, SIGMO_SCOPE_DATA, true));
+  // the data side's packed signatures are never read here (the refine reads its planes)
+  SIGMO_TRY(signatures_generate(ctx, sig, p->d, SIGMO_SCOPE_DATA, true, true));
   SIGMO_TRY(signatures_generate(ctx, sig, p->q, SIGMO_SCOPE_QUERY, true));
   SIGMO_TRY(sigmo_candidates_filter(ctx, p->cand, p->q, p->d, sig));
   for (uint32_t r = 1; r <= refinement_steps; ++r) {
-    SIGMO_TRY(signatures_refine(ctx, sig, p->d, SIGMO_SCOPE_DATA, r, r == 1));
+    SIGMO_TRY(signatures_refine(ctx, sig, p->d, SIGMO_SCOPE_DATA, r, r == 1, true));
     SIGMO_TRY(signatures_refine(ctx, sig, p->q, SIGMO_SCOPE_QUERY, r, r == 1));
     SIGMO_TRY(sigmo_candidates_refine(ctx, p->cand, p->q, p->d, sig));
   }

@@ -197,7 +197,7 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_generate(const uint32_t* __re
   if (v < N) {
     s = fresh ? 0ull : sigs[v];
     for (uint32_t i = ro[v]; i < ro[v + 1]; ++i) s = sig_inc(s, nl[ci[i]]);
-    sigs[v] = s;
+    if (sigs) sigs[v] = s;  // null: the planes are the only output
   }
   if (planes) write_planes<kSigBlock>(s, (N + 63) / 64, planes, tile);
 }
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_refine(GraphDev g, uint32_t v
     }
     const uint64_t r = ball & ~self & ~old;
     s = staged ? label_histogram(r, lms + (gid - g0) * 16) : label_histogram(r, g.lmask + static_cast<uint64_t>(gid) * 16);
-    sigs[v] = s;
+    if (sigs) sigs[v] = s;  // null: the planes are the only output
     reach[v] = r | old;
     if (frontier_out) frontier_out[v] = frontier;
   }
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_refine_next(GraphDev g, uint6
     uint64_t next = staged ? expand_frontier_lds(fold, prev - a0, f) : expand_frontier(g.adj, prev, f);
     next &= ~(old | self);
     s = staged ? label_histogram(next, lms + (gid - g0) * 16) : label_histogram(next, g.lmask + static_cast<uint64_t>(gid) * 16);
-    sigs[v] = s;
+    if (sigs) sigs[v] = s;  // null: the planes are the only output
     reach[v] = old | next;
     frontier[v] = next;
   }

@@ -48,7 +48,8 @@ hipError_t derive_graph(hipStream_t s, const uint32_t* go, uint32_t G, const uin
                         uint64_t N, uint32_t* node_graph, uint64_t* adj, uint64_t* lplanes, uint64_t* lmask);
 
 // generateCSRSignatures: increments the neighbour-label histogram onto sigs (onto zero with
-// `fresh`, whatever sigs holds).
+// `fresh`, whatever sigs holds).  For the signature kernels below, sigs may be null when planes
+// is not (and, for sig_generate, `fresh` is set): only the bit-sliced planes are written.
 hipError_t sig_generate(hipStream_t s, const GraphDev& g, uint64_t* sigs, uint64_t* planes, bool fresh = false);
 // refineCSRSignatures<PowerGraph>(view_size); frontier_out (nullable) receives the last BFS layer.
 // With `fresh_reach` the reachables are taken as empty whatever `reach` holds.
