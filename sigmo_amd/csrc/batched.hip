@@ -833,6 +833,7 @@ struct sigmo_batch {
   uint64_t cap_counts = 0;
   uint32_t* wide_pairs = nullptr;  // pairs of molecules with > 64 nodes, for the wavefront-per-pair join
   uint64_t cap_wide = 0;
+  unsigned long long* run_host = nullptr;  // pinned: a one-sync run's filter counters [0..7] and GMCR total [8]
   uint32_t* req_offsets = nullptr;        // per-request statistics
   unsigned long long* req_out = nullptr;
   sigmo_batch_tuning tune{};               // 0 fields: the built-in heuristics
@@ -880,6 +881,7 @@ struct sigmo_batch {
     bfree(req_out);
     drop_graph();
     if (graph_host) (void)hipHostFree(graph_host);
+    if (run_host) (void)hipHostFree(run_host);
   }
 };
 
@@ -1130,6 +1132,9 @@ int shape_run(const sigmo_batch* b, const sigmo_graph* query, int cus, RunShape&
   return SIGMO_OK;
 }
 
+// Pair spaces up to this size run with one host synchronisation (sigmo_batch_run).
+constexpr uint64_t kOneSyncPairs = 1ull << 25;
+
 // Pair evaluation: counters and survival bits cleared by kernels, one launch per width class.
 hipError_t launch_filter(sigmo_batch* b, const RunShape& L, hipStream_t s) {
   const sigmo_db* db = b->db;
@@ -1192,7 +1197,8 @@ hipError_t launch_scatter(sigmo_batch* b, const RunShape& L, hipStream_t s) {
 }
 
 // Lane join over P pairs (molecules of <= 64 nodes; the others are listed in wide_pairs).
-hipError_t launch_join(sigmo_batch* b, const RunShape& L, hipStream_t s, unsigned long long* stats, uint64_t P, int find_first) {
+hipError_t launch_join(sigmo_batch* b, const RunShape& L, hipStream_t s, unsigned long long* stats, uint64_t P, int find_first,
+                       const unsigned long long* P_dev = nullptr) {
   if (!P) return hipSuccess;
   const uint64_t per_cu = b->tune.join_blocks_per_cu ? std::min<uint32_t>(b->tune.join_blocks_per_cu, 4) : 4;
   const uint64_t lblocks = std::max<uint64_t>(1, std::min<uint64_t>((P + 127) / 128, static_cast<uint64_t>(L.cus) * per_cu));
@@ -1202,11 +1208,11 @@ hipError_t launch_join(sigmo_batch* b, const RunShape& L, hipStream_t s, unsigne
   if (find_first)
     hipLaunchKernelGGL((sigmo_k::k_join_lanes<true, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(lblocks)), dim3(sigmo_k::kJoinLaneWaves * 64),
                        0, s, L.qd, L.dd, sigmo_k::ArenaMasks{b->arena, b->mask_off}, b->indices, b->pair_graph, P, lchunk, b->counts, stats,
-                       b->wide_pairs, nullptr);
+                       b->wide_pairs, P_dev);
   else
     hipLaunchKernelGGL((sigmo_k::k_join_lanes<false, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(lblocks)), dim3(sigmo_k::kJoinLaneWaves * 64),
                        0, s, L.qd, L.dd, sigmo_k::ArenaMasks{b->arena, b->mask_off}, b->indices, b->pair_graph, P, lchunk, b->counts, stats,
-                       b->wide_pairs, nullptr);
+                       b->wide_pairs, P_dev);
   return hipGetLastError();
 }
 
@@ -1272,6 +1278,55 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
   const uint64_t guess = std::min<uint64_t>(std::max<uint64_t>(L.pairs / 8, 1 << 16), 1ull << 26);
   uint64_t want_arena = static_cast<uint64_t>(double(guess) * (mean_k + 1.0) * (db->num_wide ? 2.0 : 1.0)) + slack_arena;
   want_arena = std::max(want_arena, b->cap_arena);
+  // One synchronisation per run once an earlier run has sized the arena: with no molecule of
+  // more than 64 nodes and a pair space of at most kOneSyncPairs, the GMCR arrays are sized for
+  // every pair, the join reads the pair count on the device, and the filter's counters and the
+  // total are read after the join.  An arena overflow (data that changed) falls back to the
+  // sized path below, which repeats the run.
+  const uint64_t bound = uint64_t(Gq) * Gd;
+  if (b->ran && !db->num_wide && bound && bound <= kOneSyncPairs && b->arena && b->cap_arena >= want_arena) {
+    if (!b->run_host) BATCH_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&b->run_host), 16 * sizeof(unsigned long long)));
+    BATCH_TRY(b->grow(&b->indices, b->cap_idx, bound));
+    BATCH_TRY(b->grow(&b->pair_graph, b->cap_pg, bound));
+    BATCH_TRY(b->grow(&b->mask_off, b->cap_mo, bound));
+    BATCH_TRY(b->grow(&b->counts, b->cap_counts, bound));
+    const double reads = (1.0 + 8.0 * R) * double(L.dd.N) + double(L.qd.N) * (1.0 + 2.0 * 16.0 * R) + double(Gq) * (4.0 + 8.0 * R);
+    void* tok = nullptr;
+    BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_FILTER, SIGMO_KERNEL_PAIR_FILTER, reads, true, &tok));
+    BATCH_HIP_TRY(launch_filter(b, L, s));
+    BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
+    BATCH_HIP_TRY(hipMemcpyAsync(b->run_host, b->counters, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_GMCR, -1, 0.0, true, &tok));
+    BATCH_HIP_TRY(launch_count_scan(b, L, s));
+    BATCH_HIP_TRY(launch_scatter(b, L, s));
+    BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
+    BATCH_HIP_TRY(hipMemcpyAsync(b->run_host + 8, b->total, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_JOIN, SIGMO_KERNEL_JOIN, 0.0, true, &tok));
+    BATCH_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(stats), 8, 0));
+    BATCH_HIP_TRY(launch_join(b, L, s, stats, bound, find_first, reinterpret_cast<const unsigned long long*>(b->total)));
+    BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
+    BATCH_HIP_TRY(hipMemcpyAsync(stats_host, stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    BATCH_HIP_TRY(hipStreamSynchronize(s));
+    const unsigned long long* cnt = b->run_host;
+    if (!cnt[3]) {
+      const uint64_t P = b->run_host[8];
+      if (P >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "GMCR pair count exceeds u32");
+      if (stats_host[3]) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "a per-pair match count exceeded int32");
+      ctx_add_bytes(b->ctx, SIGMO_KERNEL_PAIR_FILTER, double(cnt[1]) * 8.0 + double(Gd) * W * 8.0);
+      res->filter_wave_steps = cnt[4];
+      res->filter_lane_steps = cnt[5];
+      res->filter_scans = cnt[6];
+      b->P = P;
+      b->Gq = Gq;
+      res->num_matches = stats_host[0];
+      res->matched_pairs = stats_host[1];
+      res->gmcr_pairs = P;
+      res->dfs_steps = stats_host[4];
+      res->join_wave_steps = stats_host[7];
+      return SIGMO_OK;
+    }
+    want_arena = std::max<uint64_t>(want_arena + 1, cnt[1] + cnt[1] / 8 + slack_arena);  // overflowed: size and repeat
+  }
   for (int attempt = 0; attempt < 3; ++attempt) {
     BATCH_TRY(b->grow(&b->arena, b->cap_arena, want_arena));
     if (b->cap_arena >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "candidate arena exceeds 2^32 words");
