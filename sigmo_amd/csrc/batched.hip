@@ -56,7 +56,7 @@ __device__ __forceinline__ uint64_t next_layer(const GraphDev& g, uint32_t prev,
     const uint32_t b = __builtin_ctzll(f);
     f &= f - 1;
     const uint64_t u = static_cast<uint64_t>(prev) + b;
-    next |= g.adj[2 * u] | g.adj[2 * u + 1];
+    next |= g.fold[u];
   }
   next &= ~ball;
   ball |= next;
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void k_db_prepare(GraphDev g, uint32_t R, cons
   // Only positions 0..63 are ever expanded (frontier bit b expands node s + b), each by its
   // neighbours folded mod 64: stage those 64 words in LDS once.
   uint64_t* fold = fold_all[threadIdx.x / kWave];
-  fold[lane] = lane < n ? (g.adj[2 * (static_cast<uint64_t>(s) + lane)] | g.adj[2 * (static_cast<uint64_t>(s) + lane) + 1]) : 0ull;
+  fold[lane] = lane < n ? g.fold[static_cast<uint64_t>(s) + lane] : 0ull;
   wave_sync();
   uint32_t labels = 0;
   for (uint32_t h = 0; h < halves; ++h) {

@@ -171,7 +171,7 @@ struct sigmo_graph {
   std::vector<uint32_t> host_go;
   uint32_t *go = nullptr, *ro = nullptr, *ci = nullptr, *node_graph = nullptr;
   uint8_t *nl = nullptr, *el = nullptr;
-  uint64_t *adj = nullptr, *lplanes = nullptr, *lmask = nullptr;
+  uint64_t *adj = nullptr, *fold = nullptr, *lplanes = nullptr, *lmask = nullptr;
   sigmo_k::LabelSlots slots{};
   uint64_t cap_G = 0, cap_N = 0, cap_E = 0;  // device buffer capacities (a re-upload reuses them)
 
@@ -183,6 +183,7 @@ struct sigmo_graph {
     g.nl = nl;
     g.node_graph = node_graph;
     g.adj = adj;
+    g.fold = fold;
     g.lplanes = lplanes;
     g.lmask = lmask;
     g.G = G;
@@ -191,7 +192,7 @@ struct sigmo_graph {
   }
   uint64_t csr_bytes() const { return N * 1 + (uint64_t(G) + 1) * 4 + (N + 1) * 4 + E * 4 + E * 1; }
   uint64_t device_bytes() const {
-    return csr_bytes() + N * 4 + N * 16 + ((N + 63) / 64) * 64 + uint64_t(G) * 128;
+    return csr_bytes() + N * 4 + N * 24 + ((N + 63) / 64) * 64 + uint64_t(G) * 128;
   }
   ~sigmo_graph() {
     dev_free(go);
@@ -201,6 +202,7 @@ struct sigmo_graph {
     dev_free(nl);
     dev_free(el);
     dev_free(adj);
+    dev_free(fold);
     dev_free(lplanes);
     dev_free(lmask);
   }
@@ -509,6 +511,7 @@ int graph_fill(sigmo_graph* g, const HostView& h, uint32_t reserve_G = 0, uint64
     dev_free(g->nl);
     dev_free(g->node_graph);
     dev_free(g->adj);
+    dev_free(g->fold);
     dev_free(g->lplanes);
     g->cap_N = 0;
     const uint64_t cN = std::max(N, reserve_N), W64 = (cN + 63) / 64;
@@ -516,6 +519,7 @@ int graph_fill(sigmo_graph* g, const HostView& h, uint32_t reserve_G = 0, uint64
     if (st == SIGMO_OK) st = dev_alloc(&g->nl, cN);
     if (st == SIGMO_OK) st = dev_alloc(&g->node_graph, cN);
     if (st == SIGMO_OK) st = dev_alloc(&g->adj, 2 * cN);
+    if (st == SIGMO_OK) st = dev_alloc(&g->fold, cN);
     if (st == SIGMO_OK) st = dev_alloc(&g->lplanes, 8 * W64);
     if (st != SIGMO_OK) return st;
     g->cap_N = cN;
@@ -552,7 +556,7 @@ int graph_fill(sigmo_graph* g, const HostView& h, uint32_t reserve_G = 0, uint64
   if (e == hipSuccess) e = sigmo_k::sub_u32(s, g->go, uint64_t(G) + 1, h.node_base);
   if (e == hipSuccess) e = sigmo_k::sub_u32(s, g->ci, E, h.node_base);
   if (e == hipSuccess) e = sigmo_k::sub_u32(s, g->ro, N + 1, h.col_base);
-  if (e == hipSuccess) e = sigmo_k::derive_graph(s, g->go, G, g->ro, g->ci, g->nl, N, g->node_graph, g->adj, g->lplanes, g->lmask);
+  if (e == hipSuccess) e = sigmo_k::derive_graph(s, g->go, G, g->ro, g->ci, g->nl, N, g->node_graph, g->adj, g->fold, g->lplanes, g->lmask);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return sigmo_err::set(SIGMO_ERR_HIP, std::string("graph upload: ") + hipGetErrorString(e));
   return SIGMO_OK;

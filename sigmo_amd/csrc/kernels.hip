@@ -55,7 +55,7 @@ inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 20) {
 __global__ __launch_bounds__(256) void k_derive_nodes(const uint32_t* __restrict__ go, uint32_t G, const uint32_t* __restrict__ ro,
                                                       const uint32_t* __restrict__ ci, const uint8_t* __restrict__ nl, uint64_t N,
                                                       uint32_t* __restrict__ node_graph, uint64_t* __restrict__ adj,
-                                                      uint64_t* __restrict__ lplanes, unsigned long long* __restrict__ lmask) {
+                                                      uint64_t* __restrict__ fold, uint64_t* __restrict__ lplanes, unsigned long long* __restrict__ lmask) {
   const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   const bool valid = v < N;
   uint32_t lab = 0;
@@ -71,6 +71,7 @@ __global__ __launch_bounds__(256) void k_derive_nodes(const uint32_t* __restrict
     }
     adj[2 * v] = lo;
     adj[2 * v + 1] = hi;
+    fold[v] = lo | hi;
     lab = nl[v];
     const uint32_t local = static_cast<uint32_t>(v) - prev;
     if (local < 64 && lab < 16) atomicOr(&lmask[static_cast<uint64_t>(g) * 16 + lab], 1ull << local);
@@ -115,9 +116,9 @@ __device__ __forceinline__ void write_planes(uint64_t sig, uint64_t W64, uint64_
 
 constexpr int kSigBlock = 512;
 
-// Union of the folded adjacency (adj_lo | adj_hi) of nodes prev + b over the set bits b of f.
-// Four positions per round, their loads issued together (a missing one repeats the first).
-__device__ __forceinline__ uint64_t expand_frontier(const uint64_t* __restrict__ adj, uint64_t prev, uint64_t f) {
+// Union of the folded adjacency fold[prev + b] over the set bits b of f.  Four positions per
+// round, their loads issued together (a missing one repeats the first).
+__device__ __forceinline__ uint64_t expand_frontier(const uint64_t* __restrict__ fold, uint64_t prev, uint64_t f) {
   uint64_t next = 0;
   while (f) {
     uint32_t b[4];
@@ -128,14 +129,11 @@ __device__ __forceinline__ uint64_t expand_frontier(const uint64_t* __restrict__
       b[t] = f ? static_cast<uint32_t>(__builtin_ctzll(f)) : b[0];
       f &= f - 1;
     }
-    uint64_t w[8];
+    uint64_t w[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      w[2 * t] = adj[2 * (prev + b[t])];
-      w[2 * t + 1] = adj[2 * (prev + b[t]) + 1];
-    }
+    for (int t = 0; t < 4; ++t) w[t] = fold[prev + b[t]];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) next |= w[t];
+    for (int t = 0; t < 4; ++t) next |= w[t];
   }
   return next;
 }
@@ -156,7 +154,7 @@ __device__ __forceinline__ uint64_t label_histogram(uint64_t r, P lm) {
 
 // A signature block's nodes belong to a run of consecutive graphs.  When the run has at most
 // kStagedGraphs graphs and kStagedRows nodes, its label masks and the folded adjacency
-// (adj_lo | adj_hi) of its nodes are staged in LDS: the BFS expansions and the histograms then
+// (fold = adj_lo | adj_hi) of its nodes are staged in LDS: the BFS expansions and the histograms then
 // read LDS instead of global memory.  Block-uniform result; g0 = the run's first graph, a0 its
 // first node.
 constexpr uint32_t kStagedGraphs = 64;
@@ -172,7 +170,7 @@ __device__ __forceinline__ bool stage_block(const GraphDev& g, uint64_t* lms, ui
   const bool staged = g1 - g0 + 1 <= kStagedGraphs && na <= kStagedRows;
   if (staged) {
     for (uint32_t t = threadIdx.x; t < (g1 - g0 + 1) * 16; t += blockDim.x) lms[t] = g.lmask[static_cast<uint64_t>(g0) * 16 + t];
-    for (uint32_t t = threadIdx.x; t < na; t += blockDim.x) fold[t] = g.adj[2 * (a0 + t)] | g.adj[2 * (a0 + t) + 1];
+    for (uint32_t t = threadIdx.x; t < na; t += blockDim.x) fold[t] = g.fold[a0 + t];
   }
   __syncthreads();
   return staged;
@@ -283,7 +281,7 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_refine(GraphDev g, uint32_t v
     const uint64_t self = 1ull << ((static_cast<uint32_t>(v) - prev) & 63u);
     uint64_t ball = self, frontier = self;
     for (uint32_t it = 0; it < view_size && frontier; ++it) {
-      uint64_t next = staged ? expand_frontier_lds(fold, prev - a0, frontier) : expand_frontier(g.adj, prev, frontier);
+      uint64_t next = staged ? expand_frontier_lds(fold, prev - a0, frontier) : expand_frontier(g.fold, prev, frontier);
       next &= ~ball;
       ball |= next;
       frontier = next;
@@ -318,7 +316,7 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_refine_next(GraphDev g, uint6
   uint64_t s = 0;
   if (live) {
     const uint64_t self = 1ull << ((static_cast<uint32_t>(v) - prev) & 63u);
-    uint64_t next = staged ? expand_frontier_lds(fold, prev - a0, f) : expand_frontier(g.adj, prev, f);
+    uint64_t next = staged ? expand_frontier_lds(fold, prev - a0, f) : expand_frontier(g.fold, prev, f);
     next &= ~(old | self);
     s = staged ? label_histogram(next, lms + (gid - g0) * 16) : label_histogram(next, g.lmask + static_cast<uint64_t>(gid) * 16);
     if (sigs) sigs[v] = s;  // null: the planes are the only output
@@ -623,11 +621,11 @@ hipError_t sub_u32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v) {
 }
 
 hipError_t derive_graph(hipStream_t s, const uint32_t* go, uint32_t G, const uint32_t* ro, const uint32_t* ci, const uint8_t* nl,
-                        uint64_t N, uint32_t* node_graph, uint64_t* adj, uint64_t* lplanes, uint64_t* lmask) {
+                        uint64_t N, uint32_t* node_graph, uint64_t* adj, uint64_t* fold, uint64_t* lplanes, uint64_t* lmask) {
   hipError_t err = fill_u64(s, lmask, static_cast<uint64_t>(G) * 16, 0);
   if (err != hipSuccess || !N) return err;
   const unsigned blocks = static_cast<unsigned>((N + 255) / 256);
-  k_derive_nodes<<<blocks, 256, 0, s>>>(go, G, ro, ci, nl, N, node_graph, adj, lplanes,
+  k_derive_nodes<<<blocks, 256, 0, s>>>(go, G, ro, ci, nl, N, node_graph, adj, fold, lplanes,
                                         reinterpret_cast<unsigned long long*>(lmask));
   return hipGetLastError();
 }

@@ -5,7 +5,8 @@
 //                a candidate of query node q.  Byte-identical to the reference's uint32
 //                rows (candidates.hpp:18-36) whenever ceil(D/32) is even.
 //   adjacency    adj[2v], adj[2v+1]: bits of v's neighbours by local id (graph-relative,
-//                < 128), low and high half.
+//                < 128), low and high half; fold[v] = adj[2v] | adj[2v+1], the 64-position
+//                view the signature BFS expands through.
 //   label planes lplanes[w * 8 + b]: bit j = bit b of node_labels[64w + j].
 //   label masks  lmask[g * 16 + l]: bit b = (node_labels[go[g] + b] == l), b < min(n, 64),
 //                the 64-position view the reference's Bitset<uint64_t> BFS reads labels
@@ -26,6 +27,7 @@ struct GraphDev {
   const uint8_t* nl;   // node_labels   [N]
   const uint32_t* node_graph;  // [N]
   const uint64_t* adj;         // [2N]
+  const uint64_t* fold;        // [N]  adj[2v] | adj[2v+1]
   const uint64_t* lplanes;     // [8 * ceil(N/64)]
   const uint64_t* lmask;       // [16 G]
   uint32_t G;
@@ -45,7 +47,7 @@ hipError_t sub_u32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v);
 
 // Derived per-node / per-graph arrays of an uploaded batch.
 hipError_t derive_graph(hipStream_t s, const uint32_t* go, uint32_t G, const uint32_t* ro, const uint32_t* ci, const uint8_t* nl,
-                        uint64_t N, uint32_t* node_graph, uint64_t* adj, uint64_t* lplanes, uint64_t* lmask);
+                        uint64_t N, uint32_t* node_graph, uint64_t* adj, uint64_t* fold, uint64_t* lplanes, uint64_t* lmask);
 
 // generateCSRSignatures: increments the neighbour-label histogram onto sigs (onto zero with
 // `fresh`, whatever sigs holds).  For the signature kernels below, sigs may be null when planes
