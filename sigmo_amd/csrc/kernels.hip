@@ -430,36 +430,54 @@ __global__ __launch_bounds__(256) void k_row_counts(const uint64_t* __restrict__
 
 // ---- GMCR ----------------------------------------------------------------------------------
 
-// Any set bit of a bitset row inside the global node range [s, e) (e > s).
-__device__ __forceinline__ bool row_any(const uint64_t* __restrict__ row, uint32_t s, uint32_t e) {
-  const uint32_t ws = s >> 6, we = (e - 1) >> 6;
-  uint64_t any = 0;
-  for (uint32_t w = ws; w <= we; ++w) {
-    uint64_t x = row[w];
-    if (w == ws) x &= ~0ull << (s & 63);
-    if (w == we) x &= ~0ull >> (63 - ((e - 1) & 63));
-    any |= x;
-  }
-  return any != 0;
+// A molecule's node range [s, e) as bitset words ws..we, the end words masked by ms / me.  A
+// molecule has 1..128 nodes, so the range spans at most three words.
+struct RowRange {
+  uint32_t ws, we;
+  uint64_t ms, me;
+};
+__device__ __forceinline__ RowRange row_range(uint32_t s, uint32_t e) {
+  RowRange r;
+  r.ws = s >> 6;
+  r.we = (e - 1) >> 6;
+  r.ms = ~0ull << (s & 63);
+  r.me = ~0ull >> (63 - ((e - 1) & 63));
+  if (r.ws == r.we) r.ms = r.me = r.ms & r.me;
+  return r;
+}
+// The bits of a bitset row inside the range (nonzero <=> the row has a candidate there).
+__device__ __forceinline__ uint64_t row_bits(const uint64_t* __restrict__ row, const RowRange& r) {
+  uint64_t x = (row[r.ws] & r.ms) | (row[r.we] & r.me);
+  if (r.we > r.ws + 1) x |= row[r.ws + 1];
+  return x;
 }
 
 // gmcr.hpp:47-68 / :94-115 predicate for (query graph = blockIdx.y, data graph = lane):
 // every query node has a candidate inside the molecule.  One ballot word per 64 molecules.
+// The query nodes' rows are read four at a time (past the last node the last row repeats), so
+// a wave keeps several loads in flight; it stops once none of its molecules is alive.
 __global__ __launch_bounds__(256) void k_gmcr_mark(GraphDev q, GraphDev d, const uint64_t* __restrict__ bits, uint64_t W64,
                                                    uint64_t* __restrict__ pair_bits) {
   const uint32_t qg = blockIdx.y;
   const uint64_t dg = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   const uint64_t strips = (static_cast<uint64_t>(d.G) + 63) / 64;
   bool alive = dg < d.G;
-  uint32_t s = 0, e = 0;
-  if (alive) {
-    s = d.go[dg];
-    e = d.go[dg + 1];
-  }
-  const uint32_t qoff = q.go[qg], k = q.go[qg + 1] - qoff;
-  for (uint32_t i = 0; i < k; ++i) {
+  RowRange rr{};
+  if (alive) rr = row_range(d.go[dg], d.go[dg + 1]);
+  const uint32_t qoff = q.go[qg], k = q.go[qg + 1] - qoff;  // k >= 1
+  constexpr uint32_t kRows = 4;
+  for (uint32_t i = 0; i < k; i += kRows) {
     if (__ballot(alive) == 0) break;
-    if (alive) alive = row_any(bits + static_cast<uint64_t>(qoff + i) * W64, s, e);
+    if (alive) {
+      uint64_t x[kRows];
+#pragma unroll
+      for (uint32_t t = 0; t < kRows; ++t) {
+        const uint32_t qi = qoff + (i + t < k ? i + t : k - 1);
+        x[t] = row_bits(bits + static_cast<uint64_t>(qi) * W64, rr);
+      }
+#pragma unroll
+      for (uint32_t t = 0; t < kRows; ++t) alive = alive && x[t] != 0;
+    }
   }
   const uint64_t res = __ballot(alive);
   if (lane_id() == 0 && (dg >> 6) < strips) pair_bits[static_cast<uint64_t>(qg) * strips + (dg >> 6)] = res;
