@@ -1114,6 +1114,19 @@ struct sigmo_pipeline {
 namespace {
 // The pipeline's launches in the driver's order (sigmo.cpp:102-159), from fresh signature state
 // (signature.hpp:166-167; reachables empty).  Also what a capture records.
+// sigmo_candidates_filter, then sigmo_candidates_refine against the current signatures, as one
+// kernel (the pipeline's first refinement).
+int candidates_filter_refine(sigmo_ctx* ctx, sigmo_candidates* c, const sigmo_graph* query, const sigmo_graph* data,
+                             const sigmo_signatures* s) {
+  SIGMO_TRY(check_cand(c, query, data));
+  if (!s || s->D != c->D || s->Q != c->Q) return sigmo_err::set(SIGMO_ERR_INVALID, "signature shape does not match the candidates");
+  PhaseScope ps(ctx, SIGMO_PHASE_FILTER, SIGMO_KERNEL_FILTER,
+                8.0 * c->Q * c->W64 + c->D + 0.5 * query->slots.count * c->D + 9.0 * c->Q);
+  SIGMO_HIP_TRY(sigmo_k::filter_refine(ctx->stream, query->nl, s->qsig, c->Q, data->lplanes, s->dplanes, c->D, query->slots, c->bits,
+                                       ctx->num_cus));
+  return ps.close();
+}
+
 int pipeline_enqueue(sigmo_pipeline* p, uint32_t refinement_steps, int find_first, uint64_t* matches) {
   sigmo_ctx* ctx = p->ctx;
   sigmo_signatures* sig = p->sig;
@@ -1135,12 +1148,14 @@ int pipeline_enqueue(sigmo_pipeline* p, uint32_t refinement_steps, int find_firs
   // the data side's packed signatures are never read here (the refine reads its planes)
   SIGMO_TRY(signatures_generate(ctx, sig, p->d, SIGMO_SCOPE_DATA, true, true));
   SIGMO_TRY(signatures_generate(ctx, sig, p->q, SIGMO_SCOPE_QUERY, true));
-  SIGMO_TRY(sigmo_candidates_filter(ctx, p->cand, p->q, p->d, sig));
   for (uint32_t r = 1; r <= refinement_steps; ++r) {
     SIGMO_TRY(signatures_refine(ctx, sig, p->d, SIGMO_SCOPE_DATA, r, r == 1, true));
     SIGMO_TRY(signatures_refine(ctx, sig, p->q, SIGMO_SCOPE_QUERY, r, r == 1));
-    SIGMO_TRY(sigmo_candidates_refine(ctx, p->cand, p->q, p->d, sig));
+    // the filter reads no signatures and the signature passes read no candidates: the filter
+    // runs fused with the first candidate refinement, after that level's signatures
+    SIGMO_TRY(r == 1 ? candidates_filter_refine(ctx, p->cand, p->q, p->d, sig) : sigmo_candidates_refine(ctx, p->cand, p->q, p->d, sig));
   }
+  if (refinement_steps == 0) SIGMO_TRY(sigmo_candidates_filter(ctx, p->cand, p->q, p->d, sig));
   SIGMO_TRY(gmcr_generate(ctx, p->gm, p->q, p->d, p->cand, true));
   return sigmo_join(ctx, p->q, p->d, p->cand, p->gm, find_first, matches, nullptr);
 }

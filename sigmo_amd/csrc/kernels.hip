@@ -413,6 +413,57 @@ __global__ __launch_bounds__(kBitsBlock) void k_refine(const uint64_t* __restric
   }
 }
 
+// filterCandidates<Query> (isomorphism.hpp:22-62) and the first refineCandidates<Query> pass
+// (:64-127) in one sweep: word = label match of row q AND the signature test at the level that
+// pass reads.  The filter reads no signatures and the signature refinement reads no candidates,
+// so computing that level's signatures first and then running this gives the same bits as
+// filter, signatures, refine -- with each word written once and never re-read.
+template <int U>
+__global__ __launch_bounds__(kBitsBlock) void k_filter_refine(const uint8_t* __restrict__ qlabels, const uint64_t* __restrict__ qsig,
+                                                              uint64_t Q, const uint64_t* __restrict__ lplanes,
+                                                              const uint64_t* __restrict__ planes, uint64_t D, LabelSlots slots,
+                                                              uint64_t* __restrict__ bits, unsigned rpb) {
+  const uint64_t W64 = (D + 63) / 64;
+  const uint64_t w = blockIdx.x * static_cast<uint64_t>(kBitsBlock) + threadIdx.x;
+  if (w >= W64) return;
+  uint64_t L[8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) L[b] = lplanes[w * 8 + b];
+  uint64_t P[U][4];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) P[u][b] = planes[(static_cast<uint64_t>(slots.label[u]) * 4 + b) * W64 + w];
+  const uint64_t valid = (w == W64 - 1 && (D & 63)) ? (~0ull >> (64 - (D & 63))) : ~0ull;
+  const uint64_t q0 = static_cast<uint64_t>(blockIdx.y) * rpb;
+  const uint64_t q1 = q0 + rpb < Q ? q0 + rpb : Q;
+  constexpr int kRows = 4;  // the rows' label and signature loads go out together
+  for (uint64_t q = q0; q < q1; q += kRows) {
+    uint32_t ql[kRows];
+    uint64_t qs[kRows];
+#pragma unroll
+    for (int i = 0; i < kRows; ++i) {
+      const uint64_t qi = q + i < q1 ? q + i : q1 - 1;
+      ql[i] = qlabels[qi];
+      qs[i] = qsig[qi];
+    }
+#pragma unroll
+    for (int i = 0; i < kRows; ++i) {
+      uint64_t m = valid;
+      if (ql[i] != 255u) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) m &= ((ql[i] >> b) & 1u) ? L[b] : ~L[b];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t c = static_cast<uint32_t>(qs[i] >> (4 * slots.label[u])) & 15u;
+        if (c) m &= ge4(P[u], c);
+      }
+      if (q + i < q1) bits[(q + i) * W64 + w] = m;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_row_counts(const uint64_t* __restrict__ bits, uint64_t W64, uint32_t* __restrict__ counts) {
   __shared__ uint32_t part[256 / kWave];
   const uint64_t q = blockIdx.x;
@@ -721,6 +772,35 @@ hipError_t refine(hipStream_t s, const uint64_t* qsig, uint64_t Q, const uint64_
     SIGMO_REFINE_CASE(9) SIGMO_REFINE_CASE(10) SIGMO_REFINE_CASE(11) SIGMO_REFINE_CASE(12)
     SIGMO_REFINE_CASE(13) SIGMO_REFINE_CASE(14) SIGMO_REFINE_CASE(15) SIGMO_REFINE_CASE(16)
 #undef SIGMO_REFINE_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int U>
+static hipError_t launch_filter_refine(hipStream_t s, dim3 grid, const uint8_t* qlabels, const uint64_t* qsig, uint64_t Q,
+                                       const uint64_t* lplanes, const uint64_t* planes, uint64_t D, const LabelSlots& slots,
+                                       uint64_t* bits, unsigned rpb) {
+  k_filter_refine<U><<<grid, kBitsBlock, 0, s>>>(qlabels, qsig, Q, lplanes, planes, D, slots, bits, rpb);
+  return hipGetLastError();
+}
+
+hipError_t filter_refine(hipStream_t s, const uint8_t* qlabels, const uint64_t* qsig, uint64_t Q, const uint64_t* lplanes,
+                         const uint64_t* planes, uint64_t D, const LabelSlots& slots, uint64_t* bits, int num_cus) {
+  if (slots.count == 0) return filter(s, qlabels, Q, lplanes, D, bits, num_cus);  // the refine keeps every bit
+  const uint64_t W64 = (D + 63) / 64;
+  if (!Q || !W64) return hipSuccess;
+  const uint64_t cols = (W64 + kBitsBlock - 1) / kBitsBlock;
+  // store-only like the filter, with 64 + 32 U bytes of planes per word and block
+  const unsigned rpb = rows_per_block(Q, cols, num_cus, 16 * static_cast<uint64_t>(slots.count), 8);
+  const dim3 grid(static_cast<unsigned>(cols), static_cast<unsigned>((Q + rpb - 1) / rpb));
+  switch (slots.count) {
+#define SIGMO_FILTER_REFINE_CASE(u) \
+  case u: return launch_filter_refine<u>(s, grid, qlabels, qsig, Q, lplanes, planes, D, slots, bits, rpb);
+    SIGMO_FILTER_REFINE_CASE(1) SIGMO_FILTER_REFINE_CASE(2) SIGMO_FILTER_REFINE_CASE(3) SIGMO_FILTER_REFINE_CASE(4)
+    SIGMO_FILTER_REFINE_CASE(5) SIGMO_FILTER_REFINE_CASE(6) SIGMO_FILTER_REFINE_CASE(7) SIGMO_FILTER_REFINE_CASE(8)
+    SIGMO_FILTER_REFINE_CASE(9) SIGMO_FILTER_REFINE_CASE(10) SIGMO_FILTER_REFINE_CASE(11) SIGMO_FILTER_REFINE_CASE(12)
+    SIGMO_FILTER_REFINE_CASE(13) SIGMO_FILTER_REFINE_CASE(14) SIGMO_FILTER_REFINE_CASE(15) SIGMO_FILTER_REFINE_CASE(16)
+#undef SIGMO_FILTER_REFINE_CASE
     default: return hipErrorInvalidValue;
   }
 }

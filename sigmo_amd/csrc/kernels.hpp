@@ -71,6 +71,9 @@ hipError_t filter(hipStream_t s, const uint8_t* qlabels, uint64_t Q, const uint6
 // refineCandidates<Query> against bit-sliced data planes.
 hipError_t refine(hipStream_t s, const uint64_t* qsig, uint64_t Q, const uint64_t* planes, uint64_t D, const LabelSlots& slots,
                   uint64_t* bits, int num_cus);
+// filter followed by one refine against the given level's signatures, each word written once.
+hipError_t filter_refine(hipStream_t s, const uint8_t* qlabels, const uint64_t* qsig, uint64_t Q, const uint64_t* lplanes,
+                         const uint64_t* planes, uint64_t D, const LabelSlots& slots, uint64_t* bits, int num_cus);
 // Per-row popcounts.
 hipError_t row_counts(hipStream_t s, const uint64_t* bits, uint64_t Q, uint64_t W64, uint32_t* counts);
 // Copy into the reference layout (uint32 words, stride ceil(D/32)).
