@@ -26,19 +26,19 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
 // Bits at positions >= pos of a 64-bit word (0 when pos >= 64).
 __device__ __forceinline__ uint64_t bits_from(uint32_t pos) { return pos >= 64 ? 0ull : (~0ull << pos); }
 
-// x >= c for 64 four-bit values held as bit planes x[0..3] and a wave-uniform c.
+// x >= c for 64 four-bit values held as bit planes x[0..3] and a wave-uniform c in 1..15:
+// x >= c exactly when the 4-bit sum x + (16 - c) carries out.  The carry planes are built from
+// the lowest bit up, carry' = (x_b & carry) | (k_b & (x_b | carry)) with k_b a uniform all-ones
+// or all-zero mask: straight-line code, no selects.
 __device__ __forceinline__ uint64_t ge4(const uint64_t x[4], uint32_t c) {
-  uint64_t gt = 0, eq = ~0ull;
+  const uint32_t k = 16u - c;
+  uint64_t carry = x[0] & (0ull - static_cast<uint64_t>(k & 1u));
 #pragma unroll
-  for (int b = 3; b >= 0; --b) {
-    if ((c >> b) & 1u) {
-      eq &= x[b];
-    } else {
-      gt |= eq & x[b];
-      eq &= ~x[b];
-    }
+  for (int b = 1; b < 4; ++b) {
+    const uint64_t km = 0ull - static_cast<uint64_t>((k >> b) & 1u);
+    carry = (x[b] & carry) | (km & (x[b] | carry));
   }
-  return gt | eq;
+  return carry;
 }
 
 // The same with a per-lane c (no branches).
