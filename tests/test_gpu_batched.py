@@ -124,7 +124,7 @@ def test_batched_matches_plain_pipeline(ctx):
 
 
 def test_batched_config1_full_against_golden(ctx):
-    """Config 1 at full size (100,000 molecules) against the oracle's committed digests."""
+    """100,000 molecules x 100 queries (seed 1) at full size against the oracle's committed digests."""
     gold = json.loads((GOLDEN / "config1.json").read_text())
     qh = capi.HostGraphs.queries(1, 100, 4, 20, capi.SIZE_NORMAL, perturb_frac=0.2)
     dh = capi.HostGraphs.molecules(1, 100_000)

@@ -19,9 +19,9 @@ pytestmark = pytest.mark.gpu
 
 
 def _case(name):
-    if name == "cfg0":  # config 0: 2,000 molecules (seed 0), 20 queries k in U[4,12]
+    if name == "cfg0":  # 2,000 molecules (seed 0), 20 queries k in U[4,12]
         return (capi.HostGraphs.queries(0, 20, 4, 12, capi.SIZE_NORMAL), capi.HostGraphs.molecules(0, 2000), 6)
-    if name == "cfg1_slice":  # config 1 law on 5,000 molecules, 100 queries, 20% perturbed
+    if name == "cfg1_slice":  # the 100,000-molecule workload's law on 5,000 molecules, 100 queries, 20% perturbed
         return (capi.HostGraphs.queries(1, 100, 4, 20, capi.SIZE_NORMAL, perturb_frac=0.2),
                 capi.HostGraphs.molecules(1, 5000), 6)
     if name == "lognormal_aliasing":  # molecules up to 128 nodes (positions alias mod 64), k up to 32
@@ -165,7 +165,7 @@ def test_phase_and_kernel_timing(ctx):
 
 
 def test_config1_full_against_golden(ctx):
-    """Config 1 at full size (100,000 molecules, seed 1) against the oracle's committed digest."""
+    """100,000 molecules (seed 1) x 100 queries at full size against the oracle's committed digest."""
     path = GOLDEN / "config1.json"
     gold = json.loads(path.read_text())
     qh = capi.HostGraphs.queries(1, 100, 4, 20, capi.SIZE_NORMAL, perturb_frac=0.2)
