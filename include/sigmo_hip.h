@@ -207,7 +207,10 @@ typedef struct sigmo_pipeline_result {
   uint64_t join_wave_steps; /* join loop iterations summed over wavefronts (lane use = dfs_steps / (64 x this)) */
 } sigmo_pipeline_result;
 /* Runs gen(D), gen(Q), filter, R x {refsig(D,r), refsig(Q,r), refine}, GMCR, join on
- * device-resident graphs, with one launch per phase; allocates and frees its buffers. */
+ * device-resident graphs, with one launch per phase; allocates and frees its buffers.  With
+ * R >= 1 the filter runs after the level-1 signature passes, fused with the first refine into one
+ * launch (the same bits: the filter reads no signatures, the signature passes no candidates);
+ * sigmo_kernel_stats then counts that launch under SIGMO_KERNEL_FILTER and R - 1 refines. */
 int sigmo_pipeline_run(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data, uint32_t refinement_steps,
                        int find_first, sigmo_pipeline_result* result);
 
