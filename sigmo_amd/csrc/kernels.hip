@@ -764,9 +764,9 @@ hipError_t refine(hipStream_t s, const uint64_t* qsig, uint64_t Q, const uint64_
   // ms per pass)
   const uint64_t cus = static_cast<uint64_t>(num_cus > 0 ? num_cus : 256);
   const unsigned rpb = rows_per_block(Q, cols, num_cus, 16 * static_cast<uint64_t>(slots.count), cols >= cus * 4 ? 8 : 4);
-  // one block per (word strip, row chunk), row chunk fastest: a strip's chunks are dispatched
-  // together and re-read its planes from cache, not HBM (1,000,000 molecules x 1,000 queries:
-  // 25.1 -> 20.4 ms per pass; no change at 100,000 x 100)
+  // one block per (word strip, row chunk) in a 1-D grid, row chunk fastest: a strip's chunks are
+  // dispatched together and can share its planes through cache (measured no faster or slower
+  // than the strip-fastest 2-D grid, and without its 65,535-chunk limit)
   const unsigned chunks = static_cast<unsigned>((Q + rpb - 1) / rpb);
   if (cols * chunks > 0x7fffffffull) return hipErrorInvalidValue;
   const dim3 grid(static_cast<unsigned>(cols * chunks));
