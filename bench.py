@@ -1,16 +1,20 @@
 #!/usr/bin/env python3
 """Throughput bench: matched (query, molecule) pairs/s of SIGMo's hot path on one MI355X.
 
-Workload (default): 100,000 synthetic molecules (seed 1, heavy-atom count
-clamp(round(Normal(24,8)),6,64)) x 100 query graphs (k ~ U[4,20], 20% with one perturbed
-label), plain per-phase form: signatures, filter, 6 x (signature refine + candidate refine),
-GMCR, find-first join.  One step = one full pass of that pipeline on device-resident graphs
-(upload and buffer allocation happen before the timed region, as in the reference driver).
+Workload (default): one rank's shard of the multi-node run -- 20,000,000 synthetic molecules
+(seed 3, heavy-atom count clamp(round(LogNormal(ln 24, 0.45)), 6, 128)) x 2,000 query graphs
+(k ~ U[4,32], 30% with a ring), R = 6, find-first -- held in host memory and streamed through the
+device in chunks, the next chunk uploaded on a side stream while the current one runs.  One
+step = one pass over the whole shard.  The value is the batched persistent form's matched
+(query, molecule) pairs/s; the plain per-phase form (signatures, filter, 6 x refine, GMCR, join
+over a Q x D candidate bitset per chunk, chunks as large as the device holds) is timed in the
+same run on the same shard, and the two must agree on every total.
 
-Prints ONE JSON line with the throughput, a roofline entry for the dominant HBM-bound
-kernel (algorithmic bytes / measured HIP-event time) and a CPU baseline: the CPU oracle
-(a restatement of the reference kernels, OpenMP over all host cores) on a bounded molecule
-sample of the same workload.
+Prints ONE JSON line with the throughput, a roofline entry for the dominant HBM-bound kernel of
+each form (algorithmic bytes / measured HIP-event time) and a CPU baseline: the CPU oracle (a
+restatement of the reference kernels, OpenMP over the host's cores) on a bounded molecule sample
+of the same workload.  --config 1/2 time the device-resident configs instead; --serve runs the
+serving workload.
 """
 from __future__ import annotations
 
@@ -48,17 +52,21 @@ def workload(args):
     return qh, dh
 
 
-def describe(args) -> str:
+def describe(args, plan=None) -> str:
     seed, _, law, _, kmin, kmax, perturb, ring = CONFIGS[args.config]
     sizes = "Normal(24,8) sizes" if law == "normal" else "LogNormal(ln 24,0.45) sizes"
     extra = f" {int(perturb * 100)}% perturbed" if perturb > 0 else ""
     extra += f" {int(ring * 100)}% with a ring" if ring >= 0 else ""
     mode = "find-first" if not args.find_all else "find-all"
-    form = "plain per-phase form" if args.form == "plain" else "batched pair form"
+    if args.form == "both":
+        form = "batched persistent form; plain per-phase form timed in the same run"
+    else:
+        form = "plain per-phase form" if args.form == "plain" else "batched persistent form"
     stream = ""
-    if args.config == 3:
-        stream = (f", streamed from host memory in chunks of {args.chunk_molecules} molecules"
-                  f"{' (next chunk loaded on a side stream)' if not args.no_overlap else ''}")
+    if plan:
+        stream = (f", streamed from host memory in chunks (batched {plan['batched']} molecules, plain {plan['plain']} "
+                  f"molecules = a {plan['bitset_gb']} GB candidate bitset per chunk)"
+                  f"{'; next chunk loaded on a side stream' if not args.no_overlap else ''}")
     return (f"config {args.config} ({form}): {args.molecules} molecules seed {seed} {sizes}, "
             f"{args.queries} queries k~U[{kmin},{kmax}]{extra}, R={args.refinement_steps}, {mode}{stream}")
 
@@ -76,11 +84,33 @@ def _cpu_model() -> str:
     return "unknown CPU"
 
 
+def _cpu_threads() -> int:
+    """Threads for the CPU baseline: OMP_NUM_THREADS when set (the host's share of cores for this
+    process), otherwise every core the process may run on."""
+    env = os.environ.get("SIGMO_CPU_THREADS") or os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(qh, dh, args) -> dict:
-    """Oracle (CPU restatement of the reference kernels) on the first molecules of the workload."""
+    """Oracle (CPU restatement of the reference kernels) on the first molecules of the workload,
+    built with g++ -O3 -march=native -fopenmp for this host (the portable build when that fails)."""
+    import tempfile
+
     from oracle import oracle as O
 
-    threads = int(os.environ.get("SIGMO_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    build = "g++ -O3 -march=native -fopenmp"
+    tmp = tempfile.TemporaryDirectory(prefix="sigmo_oracle_")
+    try:
+        O.use_library(O.build_native(tmp.name))
+    except Exception as e:  # noqa: BLE001 -- fall back to the prebuilt portable oracle
+        print(f"[bench] native oracle build failed ({e}); using the portable build", file=sys.stderr, flush=True)
+        build = "g++ -O3 -march=x86-64-v2 -fopenmp (prebuilt)"
+    threads = _cpu_threads()
     O.set_threads(threads)
     count = args.cpu_molecules or max(1, CPU_SAMPLE_PAIRS // max(1, qh.num_graphs))
     sample = dh.slice(0, min(count, dh.num_graphs))
@@ -88,15 +118,41 @@ def cpu_baseline(qh, dh, args) -> dict:
     t0 = time.perf_counter()
     r = O.pipeline(qa, sa, args.refinement_steps, modes=[not args.find_all])
     dt = time.perf_counter() - t0
+    tmp.cleanup()
     return {
         "value": (r["total_first"] if not args.find_all else int((r["counts_all"] > 0).sum())) / dt,
         "unit": "pairs/s",
         "cores": O.threads(),
         "kind": "port",
         "sample": f"first {sample.num_graphs} of the {dh.num_graphs} molecules x all {qh.num_graphs} queries, "
-                  f"R={args.refinement_steps}, {'find-first' if not args.find_all else 'find-all'}; {dt:.2f} s on "
-                  f"{O.threads()} OpenMP threads ({_cpu_model()}, {os.cpu_count()} logical CPUs visible)",
+                  f"R={args.refinement_steps}, {'find-first' if not args.find_all else 'find-all'}, whole pipeline in one "
+                  f"pass (no chunking needed at this size); {dt:.2f} s on {O.threads()} OpenMP threads ({_cpu_model()}, "
+                  f"{os.cpu_count()} logical CPUs visible, OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}); "
+                  f"built {build}",
     }
+
+
+def calibrate_copy(mib: int) -> int:
+    """Three device-to-device copies of `mib` MiB through the HIP runtime (its copy kernel), the
+    known-size reference for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib * 1024 * 1024
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(a, 1, ctypes.c_size_t(n)) != 0:
+        raise RuntimeError("hipMemset failed")
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibrate_copy_bytes": n, "copies": 3}))
+    return 0
 
 
 def _pct(x, q):
@@ -235,25 +291,127 @@ def serve_bench(args) -> int:
     return 0
 
 
+def _device_free_bytes() -> int:
+    """Free device memory as the HIP runtime reports it (hipMemGetInfo), without PyTorch."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
+        raise RuntimeError("hipMemGetInfo failed")
+    return int(free.value)
+
+
+def _even_chunks(G: int, max_chunk: int) -> int:
+    """Molecules per chunk when G molecules are split into the fewest chunks of at most max_chunk."""
+    n = max(1, -(-G // max(1, max_chunk)))
+    return -(-G // n)
+
+
+def chunk_plan(args, qd, dh) -> dict:
+    """Config 3's chunk plans.  Plain form: the largest chunks whose Q x D candidate bitset, with
+    the rest of one chunk's device buffers, fits the device (--chunk-gb caps the bitset instead).
+    Batched form: chunks of at most --batched-chunk molecules (its candidate arena uses 32-bit
+    offsets, and the survivors of ~1 M molecules x 2,000 queries stay well inside them)."""
+    mean_n = dh.total_nodes / dh.num_graphs
+    bitset_per_mol = qd.total_nodes * mean_n / 8.0
+    if args.chunk_gb:
+        budget = args.chunk_gb * 1e9
+        source = f"--chunk-gb {args.chunk_gb:g}"
+    else:
+        free = _device_free_bytes()
+        # per data node, besides the bitset: two alternating graph buffers (~70 B each), signature
+        # state and planes (~32 B) and the GMCR arrays (pair bits and ~16 B per surviving pair)
+        per_mol_other = mean_n * (2 * 70 + 32) + args.queries * (1 / 8.0 + 0.1 * 16)
+        budget = (free - 8e9) * bitset_per_mol / (bitset_per_mol + per_mol_other)
+        source = f"{free / 1e9:.1f} GB free on the device"
+    plain = _even_chunks(dh.num_graphs, int(budget / bitset_per_mol))
+    batched = _even_chunks(dh.num_graphs, args.batched_chunk)
+    return {"plain": plain, "batched": batched, "bitset_gb": round(plain * bitset_per_mol / 1e9, 1), "source": source}
+
+
+def time_form(args, ctx, step, steps: int, warmup: int) -> dict:
+    """Warm up, then time `steps` calls of step() bracketed by device synchronisations; returns the
+    per-step time, the last result and the context's per-kernel and per-phase figures."""
+    from sigmo_amd import capi
+
+    for _ in range(warmup):
+        step()
+    ctx.sync()
+    ctx.reset_phases()
+    results = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        results.append(step())
+    ctx.sync()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1000.0 / steps
+    matched = results[-1]["matched_pairs"]
+    assert all(r["matched_pairs"] == matched for r in results), "non-deterministic match count"
+    phase = ctx.phase_ms()
+    return {"ms_per_step": ms, "matched": matched, "result": results[-1], "kstats": ctx.kernel_stats(),
+            "phase_ms": {k: round(phase[k] / steps, 4) for k in capi.PHASE_NAMES}, "steps": steps}
+
+
+def kernel_table(kstats: dict, steps: int) -> dict:
+    out = {}
+    for name, (n, ms, by) in kstats.items():
+        if n:
+            avg_ms = ms / n
+            out[name] = {"launches_per_step": n / steps, "avg_ms": round(avg_ms, 5),
+                         "gbs": round(by / n / (avg_ms / 1e3) / 1e9, 1) if by else None}
+    return out
+
+
+def roofline_of(kstats: dict, args, form: str) -> dict:
+    """The form's dominant HBM-bound kernel (by time): algorithmic bytes per launch over its HIP-event
+    launch time, with the rocprofv3-measured traffic per launch where a summary exists for it."""
+    hbm = {k: v for k, v in kstats.items() if k in HBM_KERNELS and v[0]}
+    dom = max(hbm, key=lambda k: hbm[k][1])
+    n, ms, by = hbm[dom]
+    achieved = by / n / (ms / n / 1e3) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_{dom}_config{args.config}_{form}.json"
+    if tfile.exists() and args.molecules == CONFIGS[args.config][1] and args.queries == CONFIGS[args.config][3]:
+        traffic = json.loads(tfile.read_text()).get("traffic_bytes_per_launch")
+    return {
+        "bound": "hbm",
+        "kernel": dom,
+        "form": form,
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "algorithmic_bytes_per_launch": by / n,
+        "avg_launch_ms": round(ms / n, 5),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=1, choices=sorted(CONFIGS))
+    ap.add_argument("--config", type=int, default=3, choices=sorted(CONFIGS))
     ap.add_argument("--molecules", type=int, default=None, help="override the config's molecule count")
     ap.add_argument("--queries", type=int, default=None, help="override the config's query count")
     ap.add_argument("--find-all", action="store_true", help="count every embedding instead of find-first")
     ap.add_argument("--refinement-steps", type=int, default=6)
     ap.add_argument("--cpu-molecules", type=int, default=0,
                     help="molecules in the CPU baseline sample (default: about 1e7 query x molecule pairs)")
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="only run 3 plain device copies of MIB MiB (the reference kernel for scaling the "
-                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--form", choices=("plain", "batched"), default="plain",
-                    help="plain: per-phase kernels over the Q x D bitset; batched: per-pair evaluation from "
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB through the HIP runtime (the known-size kernel "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--form", choices=("plain", "batched", "both"), default="both",
+                    help="both (default): the batched form gives the value and the plain form is timed in the same run "
+                         "for the ratio; plain: per-phase kernels over the Q x D bitset; batched: per-pair evaluation from "
                          "per-molecule level planes (recomputed inside every step)")
+    ap.add_argument("--plain-steps", type=int, default=None,
+                    help="timed steps of the plain form when --form both (default: 2 for config 3, --steps otherwise)")
+    ap.add_argument("--plain-warmup", type=int, default=None,
+                    help="untimed steps of the plain form when --form both (default: 1 for config 3, --warmup otherwise)")
     ap.add_argument("--serve", action="store_true",
                     help="serving mode (config 4): Poisson request streams against a resident database")
     ap.add_argument("--qps", default="50,200,800", help="serving: comma-separated request rates")
@@ -264,45 +422,38 @@ def main() -> int:
     ap.add_argument("--online", action="store_true",
                     help="serving: submit each request at its arrival time through the online server (sigmo_server_*) "
                          "instead of handing the loop the whole arrival schedule")
-    ap.add_argument("--chunk-gb", type=float, default=128.0,
-                    help="config 3: size chunks so the plain form's candidate bitset of one chunk fits this many GB")
-    ap.add_argument("--chunk-molecules", type=int, default=0, help="config 3: molecules per chunk (overrides --chunk-gb)")
+    ap.add_argument("--chunk-gb", type=float, default=0.0,
+                    help="config 3: cap the plain form's candidate bitset of one chunk at this many GB (default: the "
+                         "largest chunk that fits the device's free memory)")
+    ap.add_argument("--batched-chunk", type=int, default=1 << 20, help="config 3: most molecules per chunk of the batched form")
     ap.add_argument("--no-overlap", action="store_true", help="config 3: load chunks inline instead of on a side stream")
     ap.add_argument("--autotune", action="store_true", help="batched form: pick launch shapes with the autotuner before timing")
     ap.add_argument("--graph", action="store_true",
-                    help="capture one whole step (the same kernel launches and work; the batched form's includes the "
-                         "database refresh) as a hipGraph and time its replays; per-kernel figures then come from eager "
-                         "steps run just before the capture")
+                    help="configs 1-2: capture one whole step (the same kernel launches and work; the batched form's "
+                         "includes the database refresh) as a hipGraph and time its replays; per-kernel figures then "
+                         "come from eager steps run just before the capture")
     args = ap.parse_args()
-    if args.serve and args.config == 1 and "--config" not in sys.argv:
+    if args.serve and "--config" not in sys.argv:
         args.config = 4
     if args.molecules is None:
         args.molecules = CONFIGS[args.config][1]
     if args.queries is None:
         args.queries = CONFIGS[args.config][3]
+    if args.plain_steps is None:
+        args.plain_steps = 2 if args.config == 3 else args.steps
+    if args.plain_warmup is None:
+        args.plain_warmup = 1 if args.config == 3 else args.warmup
 
     if args.calibrate_copy:
-        import torch
-
-        n = args.calibrate_copy * 1024 * 1024 // 4
-        a = torch.ones(n, dtype=torch.int32, device="cuda")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"calibrate_copy_bytes": n * 4, "copies": 3}))
-        return 0
-
-    if args.serve:
-        if args.gpus != 1:
-            print("bench.py measures one GPU; use --gpus 1.", file=sys.stderr)
-            return 2
-        return serve_bench(args)
-
+        return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs SIGMo's hot path on a single MI355X "
               "(molecule shards would run one per GPU with no communication); use --gpus 1.", file=sys.stderr)
+        return 2
+    if args.serve:
+        return serve_bench(args)
+    if args.config == 4:
+        print("config 4 is the serving workload: use --serve", file=sys.stderr)
         return 2
 
     from sigmo_amd import capi
@@ -312,120 +463,107 @@ def main() -> int:
     ctx = capi.Context(0)
     find_first = not args.find_all
     qd = capi.DeviceGraph(ctx, qh)
+    forms = ["batched", "plain"] if args.form == "both" else [args.form]
+    steps_of = {f: (args.plain_steps, args.plain_warmup) if (f == "plain" and args.form == "both") else (args.steps, args.warmup)
+                for f in forms}
+    timed, graph_stats = {}, {}
+    plan = None
     if args.config == 3:
         # the shard stays in host memory; one step streams it through the device in chunks
-        if not args.chunk_molecules:
-            # the plain form's Q x D candidate bitset of one chunk within --chunk-gb; both forms use
-            # the same chunks
-            mean_n = dh.total_nodes / dh.num_graphs
-            args.chunk_molecules = int(args.chunk_gb * 1e9 / (qd.total_nodes / 8.0) / mean_n)
+        plan = chunk_plan(args, qd, dh)
         data_nodes = dh.total_nodes
-
-        def step():
-            return capi.stream_run(ctx, qd, dh, chunk_molecules=args.chunk_molecules, form=args.form,
-                                   refinement_steps=args.refinement_steps, find_first=find_first, overlap=not args.no_overlap)
-    elif args.form == "plain":
-        dd = capi.DeviceGraph(ctx, dh)
-        data_nodes = dd.total_nodes
-        pipe = capi.Pipeline(ctx, qd, dd)
-
-        def step():
-            return pipe.execute(args.refinement_steps, find_first)
-
-        if args.graph and args.config != 3:
-            # per-kernel statistics from eager executes; the timed steps replay the captured execute
-            ctx.reset_phases()
-            for _ in range(max(1, args.warmup)):
-                step()
-            ctx.sync()
-            graph_kstats, graph_phase_ms, graph_runs = ctx.kernel_stats(), ctx.phase_ms(), max(1, args.warmup)
-            pipe.capture(args.refinement_steps, find_first)
-
-            def step():
-                return pipe.replay()
+        print(f"[bench] chunk plan: {plan}", file=sys.stderr, flush=True)
+        for f in forms:
+            # one session per form: its first pass (a warmup step) allocates the device buffers,
+            # which later passes reuse, as a rank allocates once before its timed phases
+            sess = capi.Stream(ctx, qd, dh, chunk_molecules=plan[f], form=f, refinement_steps=args.refinement_steps,
+                               find_first=find_first, overlap=not args.no_overlap)
+            print(f"[bench] {f}: warmup {steps_of[f][1]}, {steps_of[f][0]} timed steps", file=sys.stderr, flush=True)
+            timed[f] = time_form(args, ctx, sess.pass_, max(1, steps_of[f][0]), max(1, steps_of[f][1]))
+            sess.close()  # frees the form's device buffers before the next form allocates
+            print(f"[bench] {f}: {timed[f]['ms_per_step']:.1f} ms/step, {timed[f]['matched']} matched pairs",
+                  file=sys.stderr, flush=True)
     else:
         dd = capi.DeviceGraph(ctx, dh)
         data_nodes = dd.total_nodes
-        db = capi.Database(ctx, dd, args.refinement_steps)
-        batch = capi.Batch(ctx, db)
-        if args.autotune:
-            print("[bench] autotuning", file=sys.stderr, flush=True)
-            args.tuning, _ = batch.autotune(qd, find_first)
-        def eager_step():
-            db.refresh()  # the data side is recomputed every step, as the plain form does
-            return batch.run(qd, find_first)
+        print(f"[bench] uploaded: Q={qd.total_nodes} query nodes, D={data_nodes} data nodes", file=sys.stderr, flush=True)
+        for f in forms:
+            if f == "plain":
+                pipe = capi.Pipeline(ctx, qd, dd)
 
-        step = eager_step
-        if args.graph:
-            # per-kernel statistics come from eager steps of the same kernels (a graph has no
-            # per-kernel events); the timed steps are replays of the captured step
-            ctx.reset_phases()
-            for _ in range(max(1, args.warmup)):
-                eager_step()
-            ctx.sync()
-            graph_kstats, graph_phase_ms, graph_runs = ctx.kernel_stats(), ctx.phase_ms(), max(1, args.warmup)
-            batch.capture(qd, find_first, refresh_db=True)
+                def step():
+                    return pipe.execute(args.refinement_steps, find_first)
 
-            def step():
-                return batch.replay()  # the captured step refreshes the data side too
-    print(f"[bench] uploaded: Q={qd.total_nodes} query nodes, D={data_nodes} data nodes; warmup {args.warmup}",
-          file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        step()
-    ctx.sync()
-    ctx.reset_phases()
-    results = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        results.append(step())
-    ctx.sync()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1000.0 / args.steps
-    matched = results[-1]["matched_pairs"]
-    assert all(r["matched_pairs"] == matched for r in results), "non-deterministic match count"
-    value = matched / (ms_per_step / 1000.0)
-    print(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step, {matched} matched pairs", file=sys.stderr, flush=True)
+                if args.graph:
+                    ctx.reset_phases()
+                    for _ in range(max(1, args.warmup)):
+                        step()
+                    ctx.sync()
+                    graph_stats[f] = (ctx.kernel_stats(), ctx.phase_ms(), max(1, args.warmup))
+                    pipe.capture(args.refinement_steps, find_first)
 
-    kstats = ctx.kernel_stats()
-    phase_ms = ctx.phase_ms()
-    phase_ms = {k: round(phase_ms[k] / args.steps, 4) for k in capi.PHASE_NAMES}
-    if args.graph and args.config != 3:
-        # eager-step statistics, scaled to one step (launch counts per step stay integral)
-        kstats = {k: (n * args.steps / graph_runs, ms * args.steps / graph_runs, by * args.steps / graph_runs)
-                  for k, (n, ms, by) in graph_kstats.items()}
-        phase_ms = {k: round(graph_phase_ms[k] / graph_runs, 4) for k in capi.PHASE_NAMES}
-    kernels = {}
-    for name, (n, ms, by) in kstats.items():
-        if n:
-            avg_ms = ms / n
-            kernels[name] = {"launches_per_step": n / args.steps, "avg_ms": round(avg_ms, 5),
-                             "gbs": round(by / n / (avg_ms / 1e3) / 1e9, 1) if by else None}
-    hbm = {k: v for k, v in kstats.items() if k in HBM_KERNELS and v[0]}
-    dom = max(hbm, key=lambda k: hbm[k][1])
-    n, ms, by = hbm[dom]
-    achieved = by / n / (ms / n / 1e3) / 1e9
-    traffic = None
-    tfile = ROOT / "profiles" / f"traffic_{dom}_config{args.config}_{args.form}.json"
-    if tfile.exists() and args.molecules == CONFIGS[args.config][1] and args.queries == CONFIGS[args.config][3]:
-        traffic = json.loads(tfile.read_text()).get("traffic_bytes_per_launch")
-    roofline = {
-        "bound": "hbm",
-        "kernel": dom,
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic,
-        "algorithmic_bytes_per_launch": by / n,
-        "avg_launch_ms": round(ms / n, 5),
-    }
-    overall = max(kstats, key=lambda k: kstats[k][1])
+                    def step():
+                        return pipe.replay()
+            else:
+                db = capi.Database(ctx, dd, args.refinement_steps)
+                batch = capi.Batch(ctx, db)
+                if args.autotune:
+                    print("[bench] autotuning", file=sys.stderr, flush=True)
+                    args.tuning, _ = batch.autotune(qd, find_first)
+
+                def step():
+                    db.refresh()  # the data side is recomputed every step, as the plain form does
+                    return batch.run(qd, find_first)
+
+                if args.graph:
+                    ctx.reset_phases()
+                    for _ in range(max(1, args.warmup)):
+                        step()
+                    ctx.sync()
+                    graph_stats[f] = (ctx.kernel_stats(), ctx.phase_ms(), max(1, args.warmup))
+                    batch.capture(qd, find_first, refresh_db=True)
+
+                    def step():
+                        return batch.replay()  # the captured step refreshes the data side too
+            timed[f] = time_form(args, ctx, step, *steps_of[f])
+            print(f"[bench] {f}: {timed[f]['ms_per_step']:.3f} ms/step, {timed[f]['matched']} matched pairs",
+                  file=sys.stderr, flush=True)
+    if len(forms) == 2:
+        assert timed["plain"]["matched"] == timed["batched"]["matched"], "the two forms disagree"
+        assert timed["plain"]["result"]["gmcr_pairs"] == timed["batched"]["result"]["gmcr_pairs"], "the two forms disagree"
+    for f, (ks, ph, runs) in graph_stats.items():
+        # eager-step statistics of the same kernels, scaled to the timed steps
+        n_steps = timed[f]["steps"]
+        timed[f]["kstats"] = {k: (n * n_steps / runs, ms * n_steps / runs, by * n_steps / runs) for k, (n, ms, by) in ks.items()}
+        timed[f]["phase_ms"] = {k: round(ph[k] / runs, 4) for k in capi.PHASE_NAMES}
+
+    head = forms[0]  # the value: the batched form when both run
+    main_t = timed[head]
+    value = main_t["matched"] / (main_t["ms_per_step"] / 1000.0)
+    res = main_t["result"]
 
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(qh, dh, args)
 
+    per_form = {}
+    for f in forms:
+        t = timed[f]
+        per_form[f] = {
+            "ms_per_step": round(t["ms_per_step"], 4),
+            "pairs_per_s": round(t["matched"] / (t["ms_per_step"] / 1000.0), 1),
+            "steps": t["steps"],
+            "roofline": roofline_of(t["kstats"], args, f),
+            "phase_ms": t["phase_ms"],
+            "kernels": kernel_table(t["kstats"], t["steps"]),
+            "dominant_kernel": max(t["kstats"], key=lambda k: t["kstats"][k][1]),
+            **({k: t["result"][k] for k in ("chunks", "wall_ms", "compute_ms", "load_ms", "load_wait_ms")}
+               if args.config == 3 else {}),
+            **({k: t["result"][k] for k in ("filter_wave_steps", "filter_lane_steps", "filter_scans")}
+               if f == "batched" and args.config != 3 else {}),
+            **({"chunk_molecules": plan[f]} if plan else {}),
+        }
     line = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -433,38 +571,34 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(main_t["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
         "config": {
-            "workload": describe(args),
+            "workload": describe(args, plan),
             "molecules": args.molecules,
             "queries": args.queries,
             "query_nodes": qd.total_nodes,
             "data_nodes": data_nodes,
             "refinement_steps": args.refinement_steps,
-            "gmcr_pairs": results[-1]["gmcr_pairs"],
-            "matched_pairs": matched,
-            "embeddings": results[-1]["num_matches"] if args.find_all else None,
-            "dfs_steps": results[-1]["dfs_steps"],
-            "join_lane_use": (round(results[-1]["dfs_steps"] / (64.0 * results[-1]["join_wave_steps"]), 4)
-                              if results[-1].get("join_wave_steps") else None),
-            **({k: results[-1][k] for k in ("filter_wave_steps", "filter_lane_steps", "filter_scans")}
-               if args.form == "batched" and args.config != 3 else {}),
-            **({k: results[-1][k] for k in ("chunks", "wall_ms", "compute_ms", "load_ms", "load_wait_ms")}
-               if args.config == 3 else {}),
-            "form": args.form,
-            **({"tuning": getattr(args, "tuning", None)} if args.form == "batched" else {}),
+            "gmcr_pairs": res["gmcr_pairs"],
+            "matched_pairs": main_t["matched"],
+            "embeddings": res["num_matches"] if args.find_all else None,
+            "dfs_steps": res["dfs_steps"],
+            "join_lane_use": (round(res["dfs_steps"] / (64.0 * res["join_wave_steps"]), 4) if res.get("join_wave_steps") else None),
+            "form": head,
+            **({"tuning": getattr(args, "tuning", None)} if "batched" in forms else {}),
             "graph": bool(args.graph and args.config != 3),
+            **({"chunk_plan": plan} if plan else {}),
         },
-        "roofline": roofline,
+        "roofline": per_form[head]["roofline"],
         "cpu_baseline": cpu,
-        "phase_ms": phase_ms,
-        "kernels": kernels,
-        "dominant_kernel": overall,
+        **({"batched_over_plain": round(timed["plain"]["ms_per_step"] / timed["batched"]["ms_per_step"], 3)}
+           if len(forms) == 2 else {}),
+        "forms": per_form,
     }
     print(json.dumps(line))
     return 0

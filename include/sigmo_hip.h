@@ -265,6 +265,8 @@ typedef struct sigmo_batch_result {
   uint64_t filter_lane_steps; /* pair evaluation: active lanes summed    */
   uint64_t filter_scans;      /* pair evaluation: 64-pair pre-test scans */
   uint64_t join_wave_steps;   /* join loop iterations summed over wavefronts */
+  uint64_t one_sync;          /* 1 if the run first took the one-synchronisation path */
+  uint64_t arena_repeats;     /* pair evaluations repeated after the candidate arena overflowed */
 } sigmo_batch_result;
 /* Launch shape of the batched form's persistent kernels; 0 fields keep the built-in choice. */
 typedef struct sigmo_batch_tuning {
@@ -423,8 +425,20 @@ typedef struct sigmo_stream_result {
  * to go on. */
 typedef int (*sigmo_stream_chunk_fn)(void* user, uint32_t chunk, uint32_t first_molecule, uint32_t num_molecules, const uint32_t* offsets,
                                      const uint32_t* indices, const int32_t* counts, uint64_t num_pairs);
+/* One pass over the shard: sigmo_stream_open, sigmo_stream_pass, sigmo_stream_close. */
 int sigmo_stream_run(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_host_graphs* data, const sigmo_stream_options* options,
                      sigmo_stream_chunk_fn on_chunk, void* user, sigmo_stream_result* result);
+/* A streaming session: the chunk plan and the loader are set up once, and the device buffers
+ * (upload slots, the plain pipeline's signatures, bitset and GMCR arrays, or the batched form's
+ * database and batch) are allocated by the first pass and kept for later passes, as one rank of
+ * the reference's multi-node run allocates once (library/src/sigmo_mpi.cpp:153-195) before its
+ * timed phases.  `data` and `query` must outlive the session; the session uses `ctx` (its stream)
+ * and a loader context of its own. */
+typedef struct sigmo_stream sigmo_stream;
+int sigmo_stream_open(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_host_graphs* data, const sigmo_stream_options* options,
+                      sigmo_stream** out);
+int sigmo_stream_pass(sigmo_stream* stream, sigmo_stream_chunk_fn on_chunk, void* user, sigmo_stream_result* result);
+int sigmo_stream_close(sigmo_stream* stream);
 
 #ifdef __cplusplus
 }

@@ -69,6 +69,25 @@ def _p(a: np.ndarray):
     return C.c_void_p(a.ctypes.data) if a.size else None
 
 
+def use_library(path) -> None:
+    """Load the oracle from another build of oracle.cpp (e.g. :func:`build_native`) from now on."""
+    global LIB_PATH, _lib
+    LIB_PATH = Path(path)
+    _lib = None
+
+
+def build_native(dest_dir) -> Path:
+    """Compile oracle.cpp with ``g++ -O3 -march=native -fopenmp`` for the host it runs on (the CPU
+    baseline's build); returns the shared library's path.  Raises CalledProcessError on failure."""
+    import subprocess
+
+    out = Path(dest_dir) / "liboracle_native.so"
+    src = Path(__file__).resolve().parent / "oracle.cpp"
+    subprocess.run(["g++", "-O3", "-march=native", "-fopenmp", "-fPIC", "-shared", "-std=c++17", str(src), "-o", str(out)],
+                   check=True, capture_output=True, text=True)
+    return out
+
+
 def set_threads(n: int) -> None:
     lib().oracle_set_threads(n)
 

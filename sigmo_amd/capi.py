@@ -59,6 +59,7 @@ EXPORTED = (
     "sigmo_server_open", "sigmo_server_submit", "sigmo_server_poll", "sigmo_server_close", "sigmo_server_open_schedule",
     "sigmo_server_submit_count",
     "sigmo_batch_set_tuning", "sigmo_batch_get_tuning", "sigmo_batch_autotune", "sigmo_batch_capture", "sigmo_batch_replay",
+    "sigmo_stream_open", "sigmo_stream_pass", "sigmo_stream_close",
 )
 KERNEL_NAMES = ("sig_generate", "sig_refine", "filter", "refine", "gmcr", "join", "pair_filter", "db_prepare")
 
@@ -91,6 +92,8 @@ class BatchResult(C.Structure):
         ("filter_lane_steps", C.c_uint64),
         ("filter_scans", C.c_uint64),
         ("join_wave_steps", C.c_uint64),
+        ("one_sync", C.c_uint64),
+        ("arena_repeats", C.c_uint64),
     ]
 
 
@@ -260,6 +263,9 @@ def lib() -> C.CDLL:
                                       C.POINTER(ServeSummary)], i32),
             "sigmo_serve": ([vp, u32, vp, vp, vp, u32, C.POINTER(ServeOptions), C.POINTER(ServeResult), C.POINTER(ServeSummary)], i32),
             "sigmo_stream_run": ([vp, vp, vp, C.POINTER(StreamOptions), STREAM_CHUNK_FN, vp, C.POINTER(StreamResult)], i32),
+            "sigmo_stream_open": ([vp, vp, vp, C.POINTER(StreamOptions), C.POINTER(vp)], i32),
+            "sigmo_stream_pass": ([vp, STREAM_CHUNK_FN, vp, C.POINTER(StreamResult)], i32),
+            "sigmo_stream_close": ([vp], i32),
             "sigmo_server_open": ([vp, u32, C.POINTER(ServeOptions), C.POINTER(vp)], i32),
             "sigmo_server_submit": ([vp, vp, C.POINTER(u64)], i32),
             "sigmo_server_poll": ([vp, u64, dbl, C.POINTER(ServeResult)], i32),
@@ -755,7 +761,7 @@ class Batch:
         self.last = {"num_matches": r.num_matches, "matched_pairs": r.matched_pairs, "gmcr_pairs": r.gmcr_pairs,
                      "dfs_steps": r.dfs_steps, "filter_wave_steps": r.filter_wave_steps,
                      "filter_lane_steps": r.filter_lane_steps, "filter_scans": r.filter_scans,
-                     "join_wave_steps": r.join_wave_steps}
+                     "join_wave_steps": r.join_wave_steps, "one_sync": r.one_sync, "arena_repeats": r.arena_repeats}
         return self.last
 
     def run(self, query: DeviceGraph, find_first: bool = True) -> dict:
@@ -938,28 +944,75 @@ class Server:
 # ---- streaming ----------------------------------------------------------------------------------
 
 def stream_run(ctx: Context, query: DeviceGraph, data: HostGraphs, chunk_molecules: int = 0, form: str = "batched",
-               refinement_steps: int = 6, find_first: bool = True, overlap: bool = True, collect: bool = False) -> dict:
-    """Run a host-resident molecule shard in chunks (sigmo_stream_run).  With collect, also
-    returns the whole shard's GMCR (offsets over all molecules, indices) and per-pair counts,
-    assembled from the chunks."""
+               refinement_steps: int = 6, find_first: bool = True, overlap: bool = True, collect: bool = False,
+               digest: bool = False) -> dict:
+    """Run a host-resident molecule shard in chunks (sigmo_stream_run: one pass of a session).  With
+    collect, also returns the whole shard's GMCR (offsets over all molecules, indices) and per-pair
+    counts, assembled from the chunks.  With digest, returns instead, per chunk in order, (first
+    molecule, molecules, pairs, a 128-bit xxh3 digest of the chunk's offsets, indices and counts)
+    under 'chunk_digests': equal chunk plans give equal digests exactly when the results are equal."""
     opt = StreamOptions(chunk_molecules, refinement_steps, int(find_first), FORM_PLAIN if form == "plain" else FORM_BATCHED,
                         int(overlap))
+    return _stream_call(lambda cb, r: lib().sigmo_stream_run(ctx.handle, query.handle, data.handle, C.byref(opt), cb, None,
+                                                             C.byref(r)), collect, digest)
+
+
+class Stream:
+    """A streaming session (sigmo_stream_open / pass / close): device buffers are allocated by the
+    first pass and kept for the later ones.  ``pass_()`` takes the same collect / digest options
+    as :func:`stream_run`."""
+
+    def __init__(self, ctx: Context, query: DeviceGraph, data: HostGraphs, chunk_molecules: int = 0, form: str = "batched",
+                 refinement_steps: int = 6, find_first: bool = True, overlap: bool = True):
+        opt = StreamOptions(chunk_molecules, refinement_steps, int(find_first), FORM_PLAIN if form == "plain" else FORM_BATCHED,
+                            int(overlap))
+        h = C.c_void_p()
+        check(lib().sigmo_stream_open(ctx.handle, query.handle, data.handle, C.byref(opt), C.byref(h)))
+        self._h, self.ctx, self.query, self.data = h, ctx, query, data
+
+    def pass_(self, collect: bool = False, digest: bool = False) -> dict:
+        return _stream_call(lambda cb, r: lib().sigmo_stream_pass(self._h, cb, None, C.byref(r)), collect, digest)
+
+    def close(self):
+        if self._h:
+            check(lib().sigmo_stream_close(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 -- interpreter shutdown
+            pass
+
+
+def _stream_call(call, collect: bool, digest: bool) -> dict:
     parts = []
 
     def _chunk(_user, chunk, first, n, off, idx, cnt, P):
         try:
-            parts.append((first, np.ctypeslib.as_array(off, (n + 1,)).copy(),
-                          np.ctypeslib.as_array(idx, (P,)).copy() if P else np.zeros(0, np.uint32),
-                          np.ctypeslib.as_array(cnt, (P,)).copy() if P else np.zeros(0, np.int32)))
+            o = np.ctypeslib.as_array(off, (n + 1,))
+            i = np.ctypeslib.as_array(idx, (P,)) if P else np.zeros(0, np.uint32)
+            c = np.ctypeslib.as_array(cnt, (P,)) if P else np.zeros(0, np.int32)
+            if digest:
+                import xxhash
+
+                h = xxhash.xxh3_128()
+                for a in (o, i, c):
+                    h.update(memoryview(np.ascontiguousarray(a)).cast("B"))
+                parts.append((first, n, P, h.hexdigest()))
+            else:
+                parts.append((first, o.copy(), i.copy(), c.copy()))
             return 0
         except Exception:  # noqa: BLE001 -- reported as a failed chunk
             return 1
 
-    cb = STREAM_CHUNK_FN(_chunk) if collect else STREAM_CHUNK_FN()
+    cb = STREAM_CHUNK_FN(_chunk) if (collect or digest) else STREAM_CHUNK_FN()
     r = StreamResult()
-    check(lib().sigmo_stream_run(ctx.handle, query.handle, data.handle, C.byref(opt), cb, None, C.byref(r)))
+    check(call(cb, r))
     out = {name: getattr(r, name) for name, _ in StreamResult._fields_}
-    if collect:
+    if digest:
+        out["chunk_digests"] = sorted(parts)
+    elif collect:
         offs, idxs, cnts, base = [np.zeros(1, np.uint32)], [], [], 0
         for first, off, idx, cnt in sorted(parts, key=lambda p: p[0]):
             offs.append(off[1:] + np.uint32(base))

@@ -756,6 +756,13 @@ int bgrow(T** p, uint64_t& cap, uint64_t need) {
   cap = need;
   return SIGMO_OK;
 }
+// Growth with an eighth to spare, for buffers whose size follows the data of a streaming chunk:
+// chunks of similar size then rarely reallocate (hipFree waits for the whole device).
+template <typename T>
+int bgrow_slack(T** p, uint64_t& cap, uint64_t need) {
+  if (*p && cap >= need) return SIGMO_OK;
+  return bgrow(p, cap, need + need / 8);
+}
 
 }  // namespace
 
@@ -771,6 +778,9 @@ uint64_t graph_edges(const sigmo_graph* g);
 }  // namespace sigmo_internal
 
 using namespace sigmo_internal;
+
+// Pair spaces up to this size run with one host synchronisation (sigmo_batch_run).
+constexpr uint64_t kOneSyncPairs = 1ull << 25;
 
 struct sigmo_batch;
 namespace sigmo_internal {
@@ -852,10 +862,10 @@ struct sigmo_batch {
   }
   // bgrow, dropping a captured graph whose buffer addresses a reallocation would invalidate
   template <typename T>
-  int grow(T** p, uint64_t& cap, uint64_t need) {
+  int grow(T** p, uint64_t& cap, uint64_t need, bool slack = false) {
     if (*p && cap >= need) return SIGMO_OK;
     drop_graph();
-    return bgrow(p, cap, need);
+    return bgrow(p, cap, slack ? need + need / 4 : need);
   }
   uint64_t P = 0;
   uint32_t Gq = 0;
@@ -920,10 +930,10 @@ int db_fill(sigmo_db* db, const sigmo_graph* data) {
   BATCH_TRY(bgrow(&db->minfo, cap_g, G));
   db->cap_G = cap_g;
   BATCH_TRY(bgrow(&db->dmax, db->cap_dmax, uint64_t(G) * R));
-  BATCH_TRY(bgrow(&db->planes32, db->cap_32, uint64_t(n32) * R * 64));
-  BATCH_TRY(bgrow(&db->planes64, db->cap_64, uint64_t(n64) * R * 64));
-  BATCH_TRY(bgrow(&db->planes_hi, db->cap_hi, uint64_t(nw) * R * 64));
-  BATCH_TRY(bgrow(&db->lab_hi, db->cap_lab, uint64_t(nw) * 16));
+  BATCH_TRY(bgrow_slack(&db->planes32, db->cap_32, uint64_t(n32) * R * 64));
+  BATCH_TRY(bgrow_slack(&db->planes64, db->cap_64, uint64_t(n64) * R * 64));
+  BATCH_TRY(bgrow_slack(&db->planes_hi, db->cap_hi, uint64_t(nw) * R * 64));
+  BATCH_TRY(bgrow_slack(&db->lab_hi, db->cap_lab, uint64_t(nw) * 16));
   db->data = data;
   db->generation++;
   db->num_wide = nw;
@@ -1028,12 +1038,19 @@ int sigmo_batch_workspace_bytes(const sigmo_batch* b, uint32_t num_query_graphs,
   const uint64_t Gd = graph_dev(db->data).G, Gq = num_query_graphs, R = db->R;
   const uint64_t pairs = Gq * Gd, W = (Gq + 63) / 64;
   const double mean_k = Gq ? double(query_nodes) / double(Gq) : 0.0;
-  // the arena's first sizing (see sigmo_batch_run) without the per-wavefront slack; the GMCR
-  // arrays are priced at that many pairs
+  // the arena's first sizing (see sigmo_batch_run) without the per-wavefront slack.  The GMCR
+  // arrays (indices, molecule, arena offset, count: 16 B per pair) are sized for every pair on
+  // the one-synchronisation path (pair space <= kOneSyncPairs, no molecule over 64 nodes), and
+  // for the pairs that survive otherwise, priced at the arena's guess of 1/8 of the pairs (plus
+  // 4 B per pair for the list of wide-molecule pairs).  A run whose survivors exceed the guess
+  // grows the arena and those arrays to the measured demand; a batch keeps its buffers at their
+  // largest size, so a caller admitting by this figure should leave room for that growth.
   const uint64_t guess = std::min<uint64_t>(std::max<uint64_t>(pairs / 8, 1 << 16), 1ull << 26);
   const uint64_t arena_words = static_cast<uint64_t>(double(guess) * (mean_k + 1.0) * (db->num_wide ? 2.0 : 1.0));
+  const bool one_sync = !db->num_wide && pairs && pairs <= kOneSyncPairs;
+  const uint64_t gmcr_pairs = one_sync ? pairs : guess;
   *bytes = query_nodes * sigmo_k::token_segment(static_cast<uint32_t>(R)) * 2 + Gq * (sizeof(sigmo_k::QueryInfo) + 8 * R) + Gd * W * 8 +
-           pairs * 4 + arena_words * 8 + guess * 5 * 4;
+           pairs * 4 + arena_words * 8 + gmcr_pairs * 16 + (db->num_wide ? guess * 4 : 0) + (Gd + 1) * 8;
   return SIGMO_OK;
 }
 
@@ -1132,9 +1149,6 @@ int shape_run(const sigmo_batch* b, const sigmo_graph* query, int cus, RunShape&
   return SIGMO_OK;
 }
 
-// Pair spaces up to this size run with one host synchronisation (sigmo_batch_run).
-constexpr uint64_t kOneSyncPairs = 1ull << 25;
-
 // Pair evaluation: counters and survival bits cleared by kernels, one launch per width class.
 hipError_t launch_filter(sigmo_batch* b, const RunShape& L, hipStream_t s) {
   const sigmo_db* db = b->db;
@@ -1207,28 +1221,29 @@ hipError_t launch_join(sigmo_batch* b, const RunShape& L, hipStream_t s, unsigne
                                                    1024, std::max<uint64_t>(64, P / (lblocks * sigmo_k::kJoinLaneWaves * 16))));
   if (find_first)
     hipLaunchKernelGGL((sigmo_k::k_join_lanes<true, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(lblocks)), dim3(sigmo_k::kJoinLaneWaves * 64),
-                       0, s, L.qd, L.dd, sigmo_k::ArenaMasks{b->arena, b->mask_off}, b->indices, b->pair_graph, P, lchunk, b->counts, stats,
+                       0, s, L.qd, L.dd, sigmo_k::ArenaMasks{b->arena, b->mask_off, b->counters + 3}, b->indices, b->pair_graph, P, lchunk, b->counts, stats,
                        b->wide_pairs, P_dev);
   else
     hipLaunchKernelGGL((sigmo_k::k_join_lanes<false, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(lblocks)), dim3(sigmo_k::kJoinLaneWaves * 64),
-                       0, s, L.qd, L.dd, sigmo_k::ArenaMasks{b->arena, b->mask_off}, b->indices, b->pair_graph, P, lchunk, b->counts, stats,
+                       0, s, L.qd, L.dd, sigmo_k::ArenaMasks{b->arena, b->mask_off, b->counters + 3}, b->indices, b->pair_graph, P, lchunk, b->counts, stats,
                        b->wide_pairs, P_dev);
   return hipGetLastError();
 }
 
 // Lane join over the nw listed pairs of 65..128-node molecules (two-word positions).
-hipError_t launch_join_wide(sigmo_batch* b, const RunShape& L, hipStream_t s, unsigned long long* stats, uint64_t nw, int find_first) {
+hipError_t launch_join_wide(sigmo_batch* b, const RunShape& L, hipStream_t s, unsigned long long* stats, uint64_t nw, int find_first,
+                            const unsigned long long* nw_dev = nullptr) {
   if (!nw) return hipSuccess;
   uint64_t blocks = 0;
   uint32_t chunk = 0;
   sigmo_k::join_wide_grid(nw, L.cus, &blocks, &chunk);
-  const sigmo_k::ArenaMasks src{b->arena, b->mask_off};
+  const sigmo_k::ArenaMasks src{b->arena, b->mask_off, b->counters + 3};
   if (find_first)
     hipLaunchKernelGGL((sigmo_k::k_join_lanes_wide<true, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, L.qd, L.dd, src,
-                       b->indices, b->pair_graph, b->wide_pairs, nw, chunk, 6u, b->counts, stats);
+                       b->indices, b->pair_graph, b->wide_pairs, nw, chunk, 6u, b->counts, stats, nw_dev);
   else
     hipLaunchKernelGGL((sigmo_k::k_join_lanes_wide<false, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, L.qd, L.dd,
-                       src, b->indices, b->pair_graph, b->wide_pairs, nw, chunk, 6u, b->counts, stats);
+                       src, b->indices, b->pair_graph, b->wide_pairs, nw, chunk, 6u, b->counts, stats, nw_dev);
   return hipGetLastError();
 }
 
@@ -1262,6 +1277,11 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
   RunShape L;
   BATCH_TRY(shape_run(b, query, cus, L));
   const uint32_t R = L.R, Gq = L.Gq, Gd = L.Gd, W = L.W;
+  // From here on the result arrays are overwritten: until this run succeeds, no earlier run's
+  // results may be read (sigmo_batch_pairs, sigmo_batch_request_stats), so `ran` is cleared and
+  // set again only on success.  `sized` remembers that an earlier run sized the arena.
+  const bool sized = b->ran;
+  b->ran = false;
   BATCH_TRY(size_fixed_buffers(b, L));
   {
     void* tok = nullptr;
@@ -1284,7 +1304,7 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
   // total are read after the join.  An arena overflow (data that changed) falls back to the
   // sized path below, which repeats the run.
   const uint64_t bound = uint64_t(Gq) * Gd;
-  if (b->ran && !db->num_wide && bound && bound <= kOneSyncPairs && b->arena && b->cap_arena >= want_arena) {
+  if (sized && !db->num_wide && bound && bound <= kOneSyncPairs && b->arena && b->cap_arena >= want_arena) {
     if (!b->run_host) BATCH_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&b->run_host), 16 * sizeof(unsigned long long)));
     BATCH_TRY(b->grow(&b->indices, b->cap_idx, bound));
     BATCH_TRY(b->grow(&b->pair_graph, b->cap_pg, bound));
@@ -1295,6 +1315,7 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
     BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_FILTER, SIGMO_KERNEL_PAIR_FILTER, reads, true, &tok));
     BATCH_HIP_TRY(launch_filter(b, L, s));
     BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
+    res->one_sync = 1;
     BATCH_HIP_TRY(hipMemcpyAsync(b->run_host, b->counters, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_GMCR, -1, 0.0, true, &tok));
     BATCH_HIP_TRY(launch_count_scan(b, L, s));
@@ -1318,6 +1339,7 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
       res->filter_scans = cnt[6];
       b->P = P;
       b->Gq = Gq;
+      b->ran = true;
       res->num_matches = stats_host[0];
       res->matched_pairs = stats_host[1];
       res->gmcr_pairs = P;
@@ -1326,6 +1348,7 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
       return SIGMO_OK;
     }
     want_arena = std::max<uint64_t>(want_arena + 1, cnt[1] + cnt[1] / 8 + slack_arena);  // overflowed: size and repeat
+    res->arena_repeats = 1;
   }
   for (int attempt = 0; attempt < 3; ++attempt) {
     BATCH_TRY(b->grow(&b->arena, b->cap_arena, want_arena));
@@ -1347,6 +1370,7 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
     ctx_add_bytes(b->ctx, SIGMO_KERNEL_PAIR_FILTER, double(cnt[1]) * 8.0 + double(Gd) * W * 8.0);
     if (!cnt[3]) break;
     if (attempt == 2) return sigmo_err::set(SIGMO_ERR_NOMEM, "candidate arena overflowed repeatedly");
+    res->arena_repeats++;
     // replay with the demand just measured; the claim order (and so the space left unused at
     // sub-chunk ends) varies between runs, hence the 1/8 margin
     want_arena = std::max<uint64_t>(want_arena + 1, cnt[1] + cnt[1] / 8 + slack_arena);
@@ -1359,11 +1383,12 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
   BATCH_HIP_TRY(hipMemcpyAsync(&P, b->total, 8, hipMemcpyDeviceToHost, s));
   BATCH_HIP_TRY(hipStreamSynchronize(s));
   if (P >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "GMCR pair count exceeds u32");
-  BATCH_TRY(b->grow(&b->indices, b->cap_idx, P));
-  BATCH_TRY(b->grow(&b->pair_graph, b->cap_pg, P));
-  BATCH_TRY(b->grow(&b->mask_off, b->cap_mo, P));
-  BATCH_TRY(b->grow(&b->counts, b->cap_counts, P));
-  BATCH_TRY(b->grow(&b->wide_pairs, b->cap_wide, db->num_wide ? P : 1));
+  // result-sized arrays regrow with a quarter to spare: each hipFree waits for the whole device
+  BATCH_TRY(b->grow(&b->indices, b->cap_idx, P, true));
+  BATCH_TRY(b->grow(&b->pair_graph, b->cap_pg, P, true));
+  BATCH_TRY(b->grow(&b->mask_off, b->cap_mo, P, true));
+  BATCH_TRY(b->grow(&b->counts, b->cap_counts, P, true));
+  BATCH_TRY(b->grow(&b->wide_pairs, b->cap_wide, db->num_wide ? P : 1, true));
   BATCH_HIP_TRY(launch_scatter(b, L, s));
   BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
   // join: lane per pair; pairs of molecules with > 64 nodes are listed for the two-word kernel
@@ -1479,8 +1504,10 @@ int sigmo_batch_capture(sigmo_batch* b, const sigmo_graph* query, int find_first
   if (e == hipSuccess) e = launch_count_scan(b, L, s);
   if (e == hipSuccess) e = launch_scatter(b, L, s);
   if (e == hipSuccess) e = sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(stats), 8, 0);
-  if (e == hipSuccess) e = launch_join(b, L, s, stats, b->P, find_first);
-  if (e == hipSuccess && db->num_wide) e = launch_join_wide(b, L, s, stats, nw, find_first);
+  // the joins read this replay's pair counts on the device (bounded by the captured sizes), so a
+  // replay on changed data reads only entries the replay itself wrote
+  if (e == hipSuccess) e = launch_join(b, L, s, stats, b->P, find_first, reinterpret_cast<const unsigned long long*>(b->total));
+  if (e == hipSuccess && db->num_wide) e = launch_join_wide(b, L, s, stats, nw, find_first, stats + 5);
   if (e == hipSuccess) e = hipMemcpyAsync(b->graph_host, b->counters, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(b->graph_host + 8, b->total, sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(b->graph_host + 9, stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
@@ -1511,6 +1538,7 @@ int sigmo_batch_replay(sigmo_batch* b, sigmo_batch_result* res) {
   unsigned long long *stats, *stats_host;
   BATCH_TRY(ctx_info(b->ctx, &device, &s, &cus, &stats, &stats_host));
   std::memset(res, 0, sizeof(*res));
+  b->ran = false;  // the replay overwrites the result arrays
   void* tok = nullptr;
   BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_FILTER, -1, 0.0, true, &tok));  // the whole graph, as one timed region
   BATCH_HIP_TRY(hipGraphLaunch(b->graph_exec, s));
@@ -1518,7 +1546,6 @@ int sigmo_batch_replay(sigmo_batch* b, sigmo_batch_result* res) {
   BATCH_HIP_TRY(hipStreamSynchronize(s));
   const unsigned long long* h = b->graph_host;
   if (h[3] || h[8] != b->graph_P || (b->db->num_wide && h[9 + 5] != b->graph_nw)) {
-    b->ran = false;
     return sigmo_err::set(SIGMO_ERR_INVALID, "the captured graph no longer matches the data (arena or pair counts changed): capture again");
   }
   if (h[9 + 3]) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "a per-pair match count exceeded int32");

@@ -302,6 +302,15 @@ int grow(T** p, uint64_t& cap, uint64_t need) {
   return SIGMO_OK;
 }
 
+// The same for arrays sized by a result (GMCR pairs): a regrowth takes a quarter more, so a run
+// over successive chunks reallocates rarely (hipFree waits for the whole device, including the
+// next chunk's upload on the loader's stream).
+template <typename T>
+int grow_slack(T** p, uint64_t& cap, uint64_t need) {
+  if (*p && cap >= need) return SIGMO_OK;
+  return grow(p, cap, need + need / 4);
+}
+
 }  // namespace
 
 // Accessors used by the batched form (batched.hip).
@@ -974,8 +983,8 @@ static int gmcr_generate(sigmo_ctx* ctx, sigmo_gmcr* m, const sigmo_graph* query
     SIGMO_HIP_TRY(hipMemcpyAsync(&P, m->total, 8, hipMemcpyDeviceToHost, s));
     SIGMO_HIP_TRY(hipStreamSynchronize(s));
     if (P >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "GMCR pair count exceeds u32");
-    SIGMO_TRY(grow(&m->indices, m->cap_indices, P));
-    SIGMO_TRY(grow(&m->pair_graph, m->cap_pair_graph, P));
+    SIGMO_TRY(grow_slack(&m->indices, m->cap_indices, P));
+    SIGMO_TRY(grow_slack(&m->pair_graph, m->cap_pair_graph, P));
   }
   SIGMO_HIP_TRY(sigmo_k::gmcr_scatter(s, m->pair_bits, Gq, Gd, m->offsets, m->indices, m->pair_graph,
                                       std::min(m->cap_indices, m->cap_pair_graph)));
@@ -1022,15 +1031,15 @@ int sigmo_join(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data
   if (m->Gq != query->G || m->Gd != data->G) return sigmo_err::set(SIGMO_ERR_INVALID, "GMCR built for other graphs");
   SIGMO_TRY(check_queries(query));
   auto* mm = const_cast<sigmo_gmcr*>(m);
-  SIGMO_TRY(grow(&mm->pair_counts, mm->cap_counts, m->P));
+  SIGMO_TRY(grow_slack(&mm->pair_counts, mm->cap_counts, m->P));
   hipStream_t s = ctx->stream;
   PhaseScope ps(ctx, SIGMO_PHASE_JOIN, SIGMO_KERNEL_JOIN, 0.0);
   SIGMO_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(ctx->stats), 8, 0));
-  SIGMO_TRY(grow(&mm->wide_pairs, mm->cap_wide, m->P));
+  SIGMO_TRY(grow_slack(&mm->wide_pairs, mm->cap_wide, m->P));
   // a deferred GMCR: P bounds the grid and the kernel reads the exact count on the device
   SIGMO_HIP_TRY(sigmo_k::join_lanes(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, m->P, find_first,
                                     mm->pair_counts, ctx->stats, mm->wide_pairs, ctx->num_cus,
-                                    m->deferred ? reinterpret_cast<const unsigned long long*>(m->total) : nullptr));
+                                    m->deferred || ctx->capturing ? reinterpret_cast<const unsigned long long*>(m->total) : nullptr));
   if (data->max_nodes > 64 && m->P) {  // pairs of molecules with > 64 nodes: the two-word lane join
     unsigned long long nw = mm->last_wide;  // a graph being recorded keeps the previous run's count
     if (!ctx->capturing) {
@@ -1040,7 +1049,8 @@ int sigmo_join(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data
     }
     if (nw)
       SIGMO_HIP_TRY(sigmo_k::join_lanes_wide(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, mm->wide_pairs, nw, 6u,
-                                             find_first, mm->pair_counts, ctx->stats, ctx->num_cus));
+                                             find_first, mm->pair_counts, ctx->stats, ctx->num_cus,
+                                             ctx->capturing ? ctx->stats + 5 : nullptr));
   }
   SIGMO_TRY(ps.close());
   SIGMO_HIP_TRY(hipMemcpyAsync(ctx->stats_host, ctx->stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));

@@ -91,6 +91,17 @@ std::string validate(const sigmo_host_graphs& g) {
       for (uint32_t e = g.row_offsets[v]; e < g.row_offsets[v + 1]; ++e)
         if (g.column_indices[e] < lo || g.column_indices[e] >= hi) return "edge leaves its graph (graph " + std::to_string(gi) + ")";
   }
+  // Adjacency must be symmetric, as the reference's CSR always is (toCSRGraph appends every
+  // undirected edge in both directions, graph.hpp:216-247): the validity test of
+  // isomorphism.hpp:150-168 reads query row i at column `depth` and data row mapping[i] at column
+  // c, and the kernels read each pair of rows in whichever direction suits their layout.
+  for (uint64_t v = 0; v < N; ++v)
+    for (uint32_t e = g.row_offsets[v]; e < g.row_offsets[v + 1]; ++e) {
+      const uint32_t u = g.column_indices[e];
+      bool back = false;
+      for (uint32_t f = g.row_offsets[u]; f < g.row_offsets[u + 1] && !back; ++f) back = g.column_indices[f] == v;
+      if (!back) return "adjacency is not symmetric: node " + std::to_string(u) + " does not list neighbour " + std::to_string(v);
+    }
   return "";
 }
 

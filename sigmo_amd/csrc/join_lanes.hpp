@@ -14,9 +14,14 @@ namespace sigmo_k {
 namespace {
 
 // Candidate masks of the survivor arena: k consecutive words per pair (molecules of <= 64 nodes).
+// `overflow` is the pair evaluation's arena-overflow flag: once set, offsets past the arena were
+// handed out and no mask may be read, so the join does nothing and the host repeats the run with
+// a larger arena.
 struct ArenaMasks {
   const uint64_t* arena;
   const uint32_t* mask_off;
+  const unsigned long long* overflow;
+  __device__ __forceinline__ bool aborted() const { return uniform32(static_cast<uint32_t>(*overflow != 0ull)) != 0u; }
   __device__ __forceinline__ uint32_t base(uint64_t j) const { return mask_off[j]; }
   __device__ __forceinline__ uint64_t mask(uint32_t base, uint32_t, uint32_t depth, uint32_t, uint32_t) const {
     return arena[static_cast<uint64_t>(base) + depth];
@@ -33,6 +38,7 @@ struct ArenaMasks {
 struct BitsetMasks {
   const uint64_t* bits;
   uint64_t W64;
+  __device__ __forceinline__ bool aborted() const { return false; }
   __device__ __forceinline__ uint32_t base(uint64_t) const { return 0; }
   // Both words come from one 16-byte load.  The second may be the next row's first word, or the
   // word past the last row (the bitset allocation carries one spare word): it only fills bits
@@ -78,7 +84,9 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
                                                                    int32_t* __restrict__ counts, unsigned long long* __restrict__ stats,
                                                                    uint32_t* __restrict__ wide, const unsigned long long* __restrict__ P_dev) {
   __shared__ uint64_t rows_all[kJoinLaneWaves][32][kWave];
-  if (P_dev) P = *P_dev;  // the pair count the GMCR scan left on the device (P: the grid's bound)
+  if (src.aborted()) return;  // block-uniform: no barrier follows
+  // the pair count the GMCR scan left on the device; P, what the arrays were sized for, bounds it
+  if (P_dev) P = min(P, static_cast<uint64_t>(*P_dev));
   __shared__ uint8_t maps_all[kJoinLaneWaves][32][kWave];
   const uint32_t lane = static_cast<uint32_t>(lane_id());
   const uint32_t wave = threadIdx.x / kWave;
@@ -214,10 +222,14 @@ template <bool FIND_FIRST, class Src>
 __global__ __launch_bounds__(64) void k_join_lanes_wide(GraphDev q, GraphDev d, Src src, const uint32_t* __restrict__ indices,
                                                         const uint32_t* __restrict__ pair_graph, const uint32_t* __restrict__ list, uint64_t P,
                                                         uint32_t chunk, uint32_t cursor, int32_t* __restrict__ counts,
-                                                        unsigned long long* __restrict__ stats) {
+                                                        unsigned long long* __restrict__ stats,
+                                                        const unsigned long long* __restrict__ P_dev = nullptr) {
   __shared__ uint64_t rows_lo[32][kWave];
   __shared__ uint64_t rows_hi[32][kWave];
   __shared__ uint8_t maps[32][kWave];
+  if (src.aborted()) return;
+  // the list's length as the lane join left it on the device, bounded by what the caller sized
+  if (P_dev) P = min(P, static_cast<uint64_t>(*P_dev));
   const uint32_t lane = static_cast<uint32_t>(lane_id());
   const uint64_t below = (1ull << lane) - 1;
   uint64_t next = 0, end = 0;

@@ -886,7 +886,7 @@ hipError_t join_wildcard(hipStream_t s, const GraphDev& q, const GraphDev& d, co
 
 hipError_t join_lanes_wide(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
                            const uint32_t* pair_graph, const uint32_t* list, uint64_t nw, uint32_t cursor, int find_first, int32_t* counts,
-                           unsigned long long* stats, int num_cus) {
+                           unsigned long long* stats, int num_cus, const unsigned long long* nw_dev) {
   if (!nw) return hipSuccess;
   uint64_t blocks = 0;
   uint32_t chunk = 0;
@@ -894,10 +894,10 @@ hipError_t join_lanes_wide(hipStream_t s, const GraphDev& q, const GraphDev& d, 
   const BitsetMasks src{bits, W64};
   if (find_first)
     k_join_lanes_wide<true, BitsetMasks><<<static_cast<unsigned>(blocks), 64, 0, s>>>(q, d, src, indices, pair_graph, list, nw, chunk, cursor,
-                                                                                     counts, stats);
+                                                                                     counts, stats, nw_dev);
   else
     k_join_lanes_wide<false, BitsetMasks><<<static_cast<unsigned>(blocks), 64, 0, s>>>(q, d, src, indices, pair_graph, list, nw, chunk, cursor,
-                                                                                      counts, stats);
+                                                                                      counts, stats, nw_dev);
   return hipGetLastError();
 }
 

@@ -104,9 +104,10 @@ hipError_t join_wildcard(hipStream_t s, const GraphDev& q, const GraphDev& d, co
                          uint64_t W64, const uint32_t* indices, const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts,
                          unsigned long long* stats, int num_cus);
 // The lane-per-pair join over the `nw` listed pairs of molecules with 65..128 nodes;
-// stats[cursor] is the work counter.
+// stats[cursor] is the work counter.  With nw_dev the kernel reads the list's length on the
+// device, bounded by nw.
 hipError_t join_lanes_wide(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
                            const uint32_t* pair_graph, const uint32_t* list, uint64_t nw, uint32_t cursor, int find_first, int32_t* counts,
-                           unsigned long long* stats, int num_cus);
+                           unsigned long long* stats, int num_cus, const unsigned long long* nw_dev = nullptr);
 
 }  // namespace sigmo_k

@@ -281,13 +281,31 @@ int sigmo_serve_schedule(uint32_t num_workers, const uint32_t* request_graph_off
   return SIGMO_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Workers run concurrently, each on its batch's context (its stream, join statistics and pinned
+// readback): two workers on one context, or one batch listed twice, would share those.
+int check_workers(sigmo_batch* const* workers, uint32_t n) {
+  for (uint32_t w = 0; w < n; ++w) {
+    if (!workers[w]) return sigmo_err::set(SIGMO_ERR_INVALID, "null worker batch");
+    for (uint32_t v = 0; v < w; ++v)
+      if (workers[v] == workers[w] || sigmo_internal::batch_ctx(workers[v]) == sigmo_internal::batch_ctx(workers[w]))
+        return sigmo_err::set(SIGMO_ERR_INVALID, "workers " + std::to_string(v) + " and " + std::to_string(w) +
+                                                     " share a context: each worker needs its own");
+  }
+  return SIGMO_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int sigmo_serve(sigmo_batch* const* workers, uint32_t num_workers, const sigmo_host_graphs* queries, const uint32_t* request_graph_offsets,
                 const double* arrival_s, uint32_t num_requests, const sigmo_serve_options* options, sigmo_serve_result* results,
                 sigmo_serve_summary* summary) {
   if (!workers || !queries || !options) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
   if (num_workers == 0 || num_workers > 16) return sigmo_err::set(SIGMO_ERR_INVALID, "1..16 workers");
-  for (uint32_t w = 0; w < num_workers; ++w)
-    if (!workers[w]) return sigmo_err::set(SIGMO_ERR_INVALID, "null worker batch");
+  if (const int wk = check_workers(workers, num_workers); wk != SIGMO_OK) return wk;
   const int chk = check_inputs(request_graph_offsets, arrival_s, num_requests, options, results);
   if (chk != SIGMO_OK) return chk;
   if (num_requests && request_graph_offsets[num_requests] > queries->num_graphs())
@@ -385,8 +403,7 @@ extern "C" {
 int sigmo_server_open(sigmo_batch* const* workers, uint32_t num_workers, const sigmo_serve_options* options, sigmo_server** out) {
   if (!workers || !options || !out) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
   if (num_workers == 0 || num_workers > 16) return sigmo_err::set(SIGMO_ERR_INVALID, "1..16 workers");
-  for (uint32_t w = 0; w < num_workers; ++w)
-    if (!workers[w]) return sigmo_err::set(SIGMO_ERR_INVALID, "null worker batch");
+  if (const int wk = check_workers(workers, num_workers); wk != SIGMO_OK) return wk;
   auto* srv = new sigmo_server();
   srv->workers.assign(workers, workers + num_workers);
   srv->dev.assign(num_workers, nullptr);

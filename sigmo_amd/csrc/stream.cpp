@@ -50,6 +50,15 @@ struct Loader {
   uint32_t first(uint32_t i) const { return i * chunk; }
   uint32_t count(uint32_t i) const { return std::min<uint32_t>(chunk, G - first(i)); }
 
+  // A pass starts with both slots free.
+  void reset() {
+    loaded[0] = loaded[1] = -1;
+    stop = false;
+    err = SIGMO_OK;
+    msg.clear();
+    load_ms = 0;
+  }
+
   // Upload chunk i of the host shard into `slot[i % 2]` (ids rebased on the device).
   int load(uint32_t i) {
     const auto t = Clock::now();
@@ -82,16 +91,41 @@ struct Loader {
 
 }  // namespace
 
+// A streaming session: the chunk plan, the loader's context and upload slots, and the form's
+// device objects (plain pipeline, or database and batch), kept from one pass to the next.
+struct sigmo_stream {
+  sigmo_ctx* ctx = nullptr;
+  const sigmo_graph* query = nullptr;
+  sigmo_stream_options opt{};
+  Loader L;
+  sigmo_pipeline* pipe = nullptr;
+  sigmo_db* db = nullptr;
+  sigmo_batch* batch = nullptr;
+  std::vector<uint32_t> offsets, indices;
+  std::vector<int32_t> counts;
+  ~sigmo_stream() {
+    sigmo_batch_free(batch);
+    sigmo_db_free(db);
+    sigmo_pipeline_free(pipe);
+    sigmo_graph_free(L.slot[0]);
+    sigmo_graph_free(L.slot[1]);
+    sigmo_ctx_destroy(L.ctx);
+  }
+};
+
 extern "C" {
 
-int sigmo_stream_run(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_host_graphs* data, const sigmo_stream_options* opt,
-                     sigmo_stream_chunk_fn on_chunk, void* user, sigmo_stream_result* res) {
-  if (!ctx || !query || !data || !opt || !res) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+int sigmo_stream_open(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_host_graphs* data, const sigmo_stream_options* opt,
+                      sigmo_stream** out) {
+  if (!ctx || !query || !data || !opt || !out) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
   if (opt->form != SIGMO_FORM_PLAIN && opt->form != SIGMO_FORM_BATCHED) return sigmo_err::set(SIGMO_ERR_INVALID, "unknown form");
-  std::memset(res, 0, sizeof(*res));
   const uint32_t G = data->num_graphs();
   if (G == 0) return sigmo_err::set(SIGMO_ERR_INVALID, "empty molecule shard");
-  Loader L;
+  auto* st = new sigmo_stream();
+  st->ctx = ctx;
+  st->query = query;
+  st->opt = *opt;
+  Loader& L = st->L;
   L.data = data;
   L.G = G;
   L.chunk = opt->chunk_molecules ? std::min(opt->chunk_molecules, G) : G;
@@ -101,14 +135,24 @@ int sigmo_stream_run(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_host_
     L.max_N = std::max<uint64_t>(L.max_N, hi - lo);
     L.max_E = std::max<uint64_t>(L.max_E, data->row_offsets[hi] - data->row_offsets[lo]);
   }
-  int st = sigmo_ctx_create(sigmo_internal::ctx_device(ctx), &L.ctx);
-  if (st != SIGMO_OK) return st;
+  const int rc = sigmo_ctx_create(sigmo_internal::ctx_device(ctx), &L.ctx);
+  if (rc != SIGMO_OK) {
+    delete st;
+    return rc;
+  }
+  *out = st;
+  return SIGMO_OK;
+}
 
-  sigmo_pipeline* pipe = nullptr;
-  sigmo_db* db = nullptr;
-  sigmo_batch* batch = nullptr;
-  std::vector<uint32_t> offsets, indices;
-  std::vector<int32_t> counts;
+int sigmo_stream_pass(sigmo_stream* S, sigmo_stream_chunk_fn on_chunk, void* user, sigmo_stream_result* res) {
+  if (!S || !res) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  std::memset(res, 0, sizeof(*res));
+  sigmo_ctx* ctx = S->ctx;
+  const sigmo_graph* query = S->query;
+  const sigmo_stream_options* opt = &S->opt;
+  Loader& L = S->L;
+  L.reset();
+  int st = SIGMO_OK;
   const auto t0 = Clock::now();
   std::thread loader;
   if (opt->overlap) loader = std::thread([&L] { L.run(); });
@@ -134,9 +178,9 @@ int sigmo_stream_run(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_host_
     const auto tc = Clock::now();
     uint64_t P = 0;
     if (opt->form == SIGMO_FORM_PLAIN) {
-      st = sigmo_internal::pipeline_rebind(ctx, query, g, L.max_N, &pipe);
+      st = sigmo_internal::pipeline_rebind(ctx, query, g, L.max_N, &S->pipe);
       sigmo_pipeline_result pr;
-      if (st == SIGMO_OK) st = sigmo_pipeline_execute(pipe, opt->refinement_steps, opt->find_first, &pr);
+      if (st == SIGMO_OK) st = sigmo_pipeline_execute(S->pipe, opt->refinement_steps, opt->find_first, &pr);
       if (st == SIGMO_OK) {
         res->num_matches += pr.num_matches;
         res->matched_pairs += pr.matched_pairs;
@@ -145,14 +189,14 @@ int sigmo_stream_run(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_host_
         P = pr.gmcr_pairs;
       }
     } else {
-      if (!db) {
-        st = sigmo_db_prepare(ctx, g, opt->refinement_steps, &db);
-        if (st == SIGMO_OK) st = sigmo_batch_create(ctx, db, &batch);
+      if (!S->db) {
+        st = sigmo_db_prepare(ctx, g, opt->refinement_steps, &S->db);
+        if (st == SIGMO_OK) st = sigmo_batch_create(ctx, S->db, &S->batch);
       } else {
-        st = sigmo_internal::db_rebind(db, g);
+        st = sigmo_internal::db_rebind(S->db, g);
       }
       sigmo_batch_result br;
-      if (st == SIGMO_OK) st = sigmo_batch_run(batch, query, opt->find_first, &br);
+      if (st == SIGMO_OK) st = sigmo_batch_run(S->batch, query, opt->find_first, &br);
       if (st == SIGMO_OK) {
         res->num_matches += br.num_matches;
         res->matched_pairs += br.matched_pairs;
@@ -163,13 +207,13 @@ int sigmo_stream_run(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_host_
     }
     if (st == SIGMO_OK && on_chunk) {
       const uint32_t n = L.count(i);
-      offsets.resize(uint64_t(n) + 1);
-      indices.resize(P);
-      counts.resize(P);
+      S->offsets.resize(uint64_t(n) + 1);
+      S->indices.resize(P);
+      S->counts.resize(P);
       st = opt->form == SIGMO_FORM_PLAIN
-               ? sigmo_pipeline_pairs(pipe, offsets.data(), offsets.size(), indices.data(), counts.data(), P)
-               : sigmo_batch_pairs(batch, offsets.data(), offsets.size(), indices.data(), counts.data(), P);
-      if (st == SIGMO_OK && on_chunk(user, i, L.first(i), n, offsets.data(), indices.data(), counts.data(), P) != 0)
+               ? sigmo_pipeline_pairs(S->pipe, S->offsets.data(), S->offsets.size(), S->indices.data(), S->counts.data(), P)
+               : sigmo_batch_pairs(S->batch, S->offsets.data(), S->offsets.size(), S->indices.data(), S->counts.data(), P);
+      if (st == SIGMO_OK && on_chunk(user, i, L.first(i), n, S->offsets.data(), S->indices.data(), S->counts.data(), P) != 0)
         st = sigmo_err::set(SIGMO_ERR_INVALID, "chunk callback failed on chunk " + std::to_string(i));
     }
     res->compute_ms += ms_since(tc);
@@ -189,12 +233,25 @@ int sigmo_stream_run(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_host_
   }
   res->wall_ms = ms_since(t0);
   res->load_ms = L.load_ms;
-  sigmo_batch_free(batch);
-  sigmo_db_free(db);
-  sigmo_pipeline_free(pipe);
-  sigmo_graph_free(L.slot[0]);
-  sigmo_graph_free(L.slot[1]);
-  sigmo_ctx_destroy(L.ctx);
+  if (st != SIGMO_OK) return sigmo_err::set(st, err_msg);
+  return SIGMO_OK;
+}
+
+int sigmo_stream_close(sigmo_stream* S) {
+  delete S;
+  return SIGMO_OK;
+}
+
+int sigmo_stream_run(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_host_graphs* data, const sigmo_stream_options* opt,
+                     sigmo_stream_chunk_fn on_chunk, void* user, sigmo_stream_result* res) {
+  if (!res) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  std::memset(res, 0, sizeof(*res));
+  sigmo_stream* S = nullptr;
+  int st = sigmo_stream_open(ctx, query, data, opt, &S);
+  if (st != SIGMO_OK) return st;
+  st = sigmo_stream_pass(S, on_chunk, user, res);
+  const std::string err_msg = st == SIGMO_OK ? std::string() : std::string(sigmo_last_error());
+  sigmo_stream_close(S);
   if (st != SIGMO_OK) return sigmo_err::set(st, err_msg);
   return SIGMO_OK;
 }

@@ -1,12 +1,14 @@
 """Write the golden fixtures of tests/golden/ from the CPU oracle.
 
-    python tests/golden/make_golden.py [--config1]
+    python tests/golden/make_golden.py [--config1] [--config2]
 
 * config0.json -- config 0 (2,000 molecules seed 0, 20 queries k in U[4,12], R = 6):
   per-level signature digests, candidate digests, the full GMCR and per-pair counts for
   find-first and find-all.
 * config1.json -- config 1 (100,000 molecules seed 1, 100 queries k in U[4,20] with 20%
   perturbed, R = 6): input digests, candidate/GMCR/count digests and match totals.
+* config2_subset.json -- config 2's exact 1,000-query set (seed 2) against the first 20,000
+  molecules of seed 2: input digests, GMCR/count digests and match totals.
 
 The oracle itself is pinned by tests/test_oracle_pinning.py; these fixtures freeze its
 output so the GPU tests can compare against it without re-running the CPU path.
@@ -76,9 +78,38 @@ def config1() -> dict:
     }
 
 
+CONFIG2_SUBSET_MOLECULES = 20_000
+
+
+def config2_subset() -> dict:
+    """Config 2's exact query set (1,000 queries, seed 2, k~U[4,32], 30% with a ring) against the
+    first 20,000 molecules of its seed-2 LogNormal molecule stream (a prefix of the 1,000,000)."""
+    qh = capi.HostGraphs.queries(2, 1000, 4, 32, capi.SIZE_LOGNORMAL, ring_frac=0.3)
+    dh = capi.HostGraphs.molecules(2, CONFIG2_SUBSET_MOLECULES, capi.SIZE_LOGNORMAL)
+    qa, da = qh.arrays(), dh.arrays()
+    t = time.time()
+    r = O.pipeline(qa, da, 6, modes=[True, False])
+    return {
+        "workload": f"config 2 subset: config 2's 1000 queries (seed 2, k~U[4,32], 30% with a ring) x the first "
+                    f"{CONFIG2_SUBSET_MOLECULES} molecules of seed 2 (LogNormal(ln 24,0.45) sizes), R=6",
+        "molecules": CONFIG2_SUBSET_MOLECULES,
+        "oracle_seconds": round(time.time() - t, 1),
+        "oracle_threads": O.threads(),
+        "query_digest": digest(*qa.values()),
+        "data_digest": digest(*da.values()),
+        "gmcr_digest": digest(r["offsets"], r["indices"]),
+        "gmcr_pairs": int(len(r["indices"])),
+        "counts_first_digest": digest(r["counts_first"]),
+        "counts_all_digest": digest(r["counts_all"]),
+        "matches_first": r["total_first"],
+        "matches_all": r["total_all"],
+    }
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--config1", action="store_true", help="also write config1.json (minutes of CPU)")
+    ap.add_argument("--config2", action="store_true", help="also write config2_subset.json (minutes of CPU)")
     args = ap.parse_args()
     (HERE / "config0.json").write_text(json.dumps(config0()) + "\n")
     print("wrote config0.json")
@@ -86,6 +117,10 @@ def main() -> None:
         c1 = config1()
         (HERE / "config1.json").write_text(json.dumps(c1, indent=1) + "\n")
         print("wrote config1.json", c1["oracle_seconds"], "s")
+    if args.config2:
+        c2 = config2_subset()
+        (HERE / "config2_subset.json").write_text(json.dumps(c2, indent=1) + "\n")
+        print("wrote config2_subset.json", c2["oracle_seconds"], "s")
 
 
 if __name__ == "__main__":

@@ -92,6 +92,30 @@ def test_batched_wildcards_overflow_and_replay(ctx):
     assert len(o["indices"]) > 65536  # beyond the initial record capacity
 
 
+def test_batched_one_sync_arena_overflow_is_repeated(ctx):
+    """A batch sized by one query set, then run on a wildcard-heavy set whose arena demand exceeds
+    that size while its estimate does not: the second run takes the one-synchronisation path,
+    overflows there, skips the join (no arena offset past the end is read) and repeats sized.
+    Sizes: 200 molecules; 10,000 ordinary queries (k in 4..12, ~15% of 2 M pairs survive) size the
+    arena at ~3.9 M words; 10,000 three-node wildcard paths keep all 2 M pairs (6 M words)."""
+    dh = capi.HostGraphs.molecules(4, 200)
+    q1 = capi.HostGraphs.queries(11, 10_000, 4, 12, capi.SIZE_NORMAL)
+    q2 = capi.HostGraphs.from_edge_lists([([255, 255, 255], [(0, 1), (1, 2)])] * 10_000)
+    dd = capi.DeviceGraph(ctx, dh)
+    db = capi.Database(ctx, dd, 6)
+    batch = capi.Batch(ctx, db)
+    batch.tuning = {"filter_blocks_per_cu": 1}
+    print("one-sync overflow: sizing run", flush=True)
+    first = batch.run(capi.DeviceGraph(ctx, q1), find_first=True)
+    assert first["one_sync"] == 0
+    o = O.pipeline(q2.arrays(), dh.arrays(), 6, modes=[True, False])
+    print("one-sync overflow: wildcard run", flush=True)
+    g = _batched(ctx, q2, dh, 6, db=db, batch=batch)
+    r = g["result_first"]
+    assert r["one_sync"] == 1 and r["arena_repeats"] >= 1, r
+    _assert_equal(g, o)
+
+
 def test_batched_reuse_across_query_sets(ctx):
     """One database and one batch object serve successive query sets, larger then smaller."""
     dh = capi.HostGraphs.molecules(6, 2500)

@@ -54,3 +54,19 @@ def test_stream_growing_chunks_and_forms_agree(ctx, shard):
         np.testing.assert_array_equal(res["plain"][k], res["batched"][k])
     o = O.pipeline(qh.arrays(), dh.arrays(), 6, modes=[False])
     np.testing.assert_array_equal(res["batched"]["counts"], o["counts_all"])
+
+
+@pytest.mark.parametrize("form", ["plain", "batched"])
+def test_stream_session_passes_repeat_the_result(ctx, shard, form):
+    """A session (sigmo_stream_open / pass / close) keeps its device buffers between passes; every
+    pass gives the oracle's GMCR and counts."""
+    dh, qh, qd, o = shard
+    sess = capi.Stream(ctx, qd, dh, chunk_molecules=997, form=form, find_first=False)
+    for p in range(3):
+        print(f"stream session form={form} pass {p}", flush=True)
+        r = sess.pass_(collect=True)
+        np.testing.assert_array_equal(r["offsets"], o["offsets"], err_msg="GMCR offsets")
+        np.testing.assert_array_equal(r["indices"], o["indices"], err_msg="GMCR indices")
+        np.testing.assert_array_equal(r["counts"], o["counts_all"], err_msg="counts")
+        assert r["chunks"] == 4
+    sess.close()

@@ -178,6 +178,16 @@ def test_from_arrays_validates_graph_membership():
         capi.HostGraphs.from_arrays(**bad)
 
 
+def test_from_arrays_rejects_asymmetric_adjacency():
+    # a path 0-1-2 listed in both directions is accepted; dropping 2 -> 1 leaves 1 -> 2 one-way,
+    # which the reference's CSR (every edge appended in both directions) never produces
+    ok = dict(graph_offsets=[0, 3], row_offsets=[0, 1, 3, 4], column_indices=[1, 0, 2, 1], node_labels=[8, 7, 8])
+    capi.HostGraphs.from_arrays(**ok)
+    one_way = dict(graph_offsets=[0, 3], row_offsets=[0, 1, 3, 3], column_indices=[1, 0, 2], node_labels=[8, 7, 8])
+    with pytest.raises(capi.SigmoError, match="not symmetric"):
+        capi.HostGraphs.from_arrays(**one_way)
+
+
 def test_slice_and_concat():
     g = capi.HostGraphs.molecules(4, 50)
     a, b = g.slice(0, 20), g.slice(20, 30)
