@@ -186,6 +186,18 @@ int sigmo_gmcr_download(sigmo_ctx* ctx, const sigmo_gmcr* m, uint32_t* data_grap
 int sigmo_join(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data, const sigmo_candidates* c,
                const sigmo_gmcr* m, int find_first, uint64_t* num_matches, int32_t* per_pair_counts);
 
+/* sigmo_join without allocation: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, accumulation and per-pair output as sigmo_join; the join's
+ * counters, per-pair counts and list of wide-molecule pairs live in a static device array of the
+ * library (16 MiB per device, held until the process ends) instead of allocated buffers, so the
+ * GMCR may hold at most SIGMO_JOIN_NOALLOC_MAX_PAIRS pairs (more: SIGMO_ERR_UNSUPPORTED).  All
+ * calls on one device share that array: calls that overlap on one device (several contexts or
+ * threads at once) race; calls in sequence are independent. */
+#define SIGMO_JOIN_NOALLOC_MAX_PAIRS (1u << 21)
+int sigmo_join_noalloc(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data, const sigmo_candidates* c,
+                       const sigmo_gmcr* m, int find_first, uint64_t* num_matches, int32_t* per_pair_counts);
+
 /* ---- wildcard / edge-label join (isomorphism::join::joinWildcardCandidates,
  * isomorphism.hpp:428-556, with defineMatchingOrder :383-408 and isValidMapping :410-426) ----
  * Per GMCR pair: the query node with most candidates in the molecule goes first (ties: larger

@@ -59,7 +59,7 @@ EXPORTED = (
     "sigmo_server_open", "sigmo_server_submit", "sigmo_server_poll", "sigmo_server_close", "sigmo_server_open_schedule",
     "sigmo_server_submit_count",
     "sigmo_batch_set_tuning", "sigmo_batch_get_tuning", "sigmo_batch_autotune", "sigmo_batch_capture", "sigmo_batch_replay",
-    "sigmo_stream_open", "sigmo_stream_pass", "sigmo_stream_close",
+    "sigmo_stream_open", "sigmo_stream_pass", "sigmo_stream_close", "sigmo_join_noalloc",
 )
 KERNEL_NAMES = ("sig_generate", "sig_refine", "filter", "refine", "gmcr", "join", "pair_filter", "db_prepare")
 
@@ -233,6 +233,7 @@ def lib() -> C.CDLL:
             "sigmo_gmcr_total": ([vp, C.POINTER(u64)], i32),
             "sigmo_gmcr_download": ([vp, vp, vp, u64, vp, u64], i32),
             "sigmo_join": ([vp, vp, vp, vp, vp, i32, C.POINTER(u64), vp], i32),
+            "sigmo_join_noalloc": ([vp, vp, vp, vp, vp, i32, C.POINTER(u64), vp], i32),
             "sigmo_join_wildcard": ([vp, vp, vp, vp, vp, i32, C.POINTER(u64), vp], i32),
             "sigmo_pipeline_run": ([vp, vp, vp, u32, i32, C.POINTER(PipelineResult)], i32),
             "sigmo_pipeline_create": ([vp, vp, vp, pvp], i32),
@@ -633,6 +634,20 @@ def join_candidates(ctx: Context, query: DeviceGraph, data: DeviceGraph, cand: C
     counts = np.zeros(gmcr.total, np.int32) if per_pair else None
     check(lib().sigmo_join(ctx.handle, query.handle, data.handle, cand.handle, gmcr.handle, int(find_first),
                            C.byref(total), _ptr(counts) if per_pair and counts.size else None))
+    return total.value, counts
+
+
+JOIN_NOALLOC_MAX_PAIRS = 1 << 21
+
+
+def join_candidates_noalloc(ctx: Context, query: DeviceGraph, data: DeviceGraph, cand: Candidates, gmcr: GMCR,
+                            find_first: bool = True, per_pair: bool = True):
+    """sigmo_join_noalloc: the join with its counters and per-pair counts in the library's static
+    device scratch (at most JOIN_NOALLOC_MAX_PAIRS GMCR pairs); returns what join_candidates does."""
+    total = C.c_uint64(0)
+    counts = np.zeros(gmcr.total, np.int32) if per_pair else None
+    check(lib().sigmo_join_noalloc(ctx.handle, query.handle, data.handle, cand.handle, gmcr.handle, int(find_first),
+                                   C.byref(total), _ptr(counts) if per_pair and counts.size else None))
     return total.value, counts
 
 

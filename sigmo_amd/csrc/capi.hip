@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -354,6 +355,46 @@ sigmo_ctx* graph_ctx(const sigmo_graph* g) { return g->ctx; }
 uint64_t graph_edges(const sigmo_graph* g) { return g->E; }
 }  // namespace sigmo_internal
 
+// ---- static device scratch of the allocation-free join ---------------------------------------
+//
+// sigmo_join_noalloc keeps its counters, per-pair counts and list of wide-molecule pairs in this
+// file-scope array instead of allocated buffers.  Layout (64-bit words): [0, 16) join counters,
+// then SIGMO_JOIN_NOALLOC_MAX_PAIRS int32 counts, then as many uint32 list entries: 16 MiB +
+// 128 B in all.  Each device holds its own copy for the life of the process; host code reaches it
+// only through the address hipGetSymbolAddress gives on that device (kept in g_scratch below),
+// never through the array's name, which in host code is a stand-in, not a device address.
+namespace {
+constexpr uint64_t kScratchStatWords = 16;
+constexpr uint64_t kScratchPairs = SIGMO_JOIN_NOALLOC_MAX_PAIRS;
+constexpr uint64_t kScratchWords = kScratchStatWords + kScratchPairs;  // two 4-byte arrays of kScratchPairs
+static __device__ __align__(256) uint64_t scratch[kScratchWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint64_t* g_scratch[kMaxDevices] = {};
+
+// Once per device, with that device current: the device address of `scratch`.
+int scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return SIGMO_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return sigmo_err::set(SIGMO_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  }
+  g_scratch[device] = static_cast<uint64_t*>(p);
+  return SIGMO_OK;
+}
+
+uint64_t* scratch_on(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch[device];
+}
+}  // namespace
+
 extern "C" {
 
 // ---- context ---------------------------------------------------------------------------------
@@ -379,6 +420,12 @@ int sigmo_ctx_create(int device, sigmo_ctx** out) {
   if (e != hipSuccess) {
     sigmo_ctx_destroy(c);
     return sigmo_err::set(SIGMO_ERR_HIP, std::string("context setup: ") + hipGetErrorString(e));
+  }
+  const int st = scratch_setup(device);  // the allocation-free join's scratch on this device
+  if (st != SIGMO_OK) {
+    const std::string msg = sigmo_last_error();
+    sigmo_ctx_destroy(c);
+    return sigmo_err::set(st, msg);
   }
   *out = c;
   return SIGMO_OK;
@@ -1083,6 +1130,44 @@ int sigmo_join_wildcard(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_gr
   SIGMO_HIP_TRY(hipMemcpyAsync(ctx->stats_host, ctx->stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   if (per_pair_counts && m->P)
     SIGMO_HIP_TRY(hipMemcpyAsync(per_pair_counts, mm->pair_counts, m->P * 4, hipMemcpyDeviceToHost, s));
+  SIGMO_HIP_TRY(hipStreamSynchronize(s));
+  if (ctx->stats_host[3]) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "a per-pair match count exceeded int32");
+  if (num_matches) *num_matches += ctx->stats_host[0];
+  return SIGMO_OK;
+}
+
+int sigmo_join_noalloc(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data, const sigmo_candidates* c, const sigmo_gmcr* m,
+                       int find_first, uint64_t* num_matches, int32_t* per_pair_counts) {
+  SIGMO_TRY(bind(ctx));
+  SIGMO_TRY(check_cand(c, query, data));
+  if (!m || !m->built) return sigmo_err::set(SIGMO_ERR_INVALID, "GMCR not generated");
+  if (m->Gq != query->G || m->Gd != data->G) return sigmo_err::set(SIGMO_ERR_INVALID, "GMCR built for other graphs");
+  SIGMO_TRY(check_queries(query));
+  auto* mm = const_cast<sigmo_gmcr*>(m);
+  hipStream_t s = ctx->stream;
+  SIGMO_TRY(gmcr_resolve(mm, s));  // the exact pair count (a deferred GMCR left it on the device)
+  if (m->P > kScratchPairs)
+    return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "the allocation-free join takes at most " + std::to_string(kScratchPairs) +
+                                                     " GMCR pairs (this GMCR has " + std::to_string(m->P) + "): use sigmo_join");
+  uint64_t* base = scratch_on(ctx->device);
+  if (!base) return sigmo_err::set(SIGMO_ERR_INVALID, "no scratch on this device (context set-up did not run)");
+  auto* stats = reinterpret_cast<unsigned long long*>(base);
+  auto* counts = reinterpret_cast<int32_t*>(base + kScratchStatWords);
+  auto* wide = reinterpret_cast<uint32_t*>(counts + kScratchPairs);
+  // the scratch keeps the previous call's contents: the counters (work cursors included) start at 0
+  SIGMO_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(stats), kScratchStatWords, 0));
+  SIGMO_HIP_TRY(sigmo_k::join_lanes(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, m->P, find_first, counts, stats,
+                                    wide, ctx->num_cus));
+  if (data->max_nodes > 64 && m->P) {  // pairs of molecules with > 64 nodes: the two-word lane join
+    SIGMO_HIP_TRY(hipMemcpyAsync(ctx->stats_host, stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    SIGMO_HIP_TRY(hipStreamSynchronize(s));
+    const uint64_t nw = ctx->stats_host[5];
+    if (nw)
+      SIGMO_HIP_TRY(sigmo_k::join_lanes_wide(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, wide, nw, 6u,
+                                             find_first, counts, stats, ctx->num_cus));
+  }
+  SIGMO_HIP_TRY(hipMemcpyAsync(ctx->stats_host, stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  if (per_pair_counts && m->P) SIGMO_HIP_TRY(hipMemcpyAsync(per_pair_counts, counts, m->P * 4, hipMemcpyDeviceToHost, s));
   SIGMO_HIP_TRY(hipStreamSynchronize(s));
   if (ctx->stats_host[3]) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "a per-pair match count exceeded int32");
   if (num_matches) *num_matches += ctx->stats_host[0];
