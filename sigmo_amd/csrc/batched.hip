@@ -231,8 +231,11 @@ __global__ __launch_bounds__(256) void k_db_prepare(GraphDev g, uint32_t R, cons
 }
 
 // One wavefront per query graph, lane i = node i (query graphs have <= 32 nodes, so
-// positions never alias): writes each node's token segment, the graph's label set (labels
-// < 16) and, per level, the nibble-wise max of its nodes' signatures.
+// positions never alias): writes each node's tokens, the graph's label set (labels < 16) and,
+// per level, the nibble-wise max of its nodes' signatures.  The graph's nodes' token streams
+// are packed one after another from the start of its region (token qoff * segment), each padded
+// to a whole token word (4 tokens), so a pair walks one contiguous run of words: the next node
+// starts at the word after the one holding the current node's end token.
 __global__ __launch_bounds__(256) void k_query_prepare(GraphDev q, uint32_t R, uint16_t* __restrict__ tokens,
                                                        QueryInfo* __restrict__ qinfo, uint64_t* __restrict__ qmax) {
   const uint32_t lane = static_cast<uint32_t>(lane_id());
@@ -241,29 +244,49 @@ __global__ __launch_bounds__(256) void k_query_prepare(GraphDev q, uint32_t R, u
   const uint32_t prev = q.go[g], k = q.go[g + 1] - prev;
   const bool valid = lane < k;
   const uint32_t seg = token_segment(R);
-  uint16_t* t = tokens + (static_cast<uint64_t>(prev) + lane) * seg;
   const uint32_t label = valid ? q.nl[prev + lane] : 0u;
-  if (valid) t[0] = static_cast<uint16_t>(label);
-  uint64_t first = label;  // lane 0: the first four tokens of node 0
+  // the node's level signatures, and its token count: label + one per nonzero nibble + end
+  uint64_t sig[kMaxLevels];
   uint64_t self = 1ull << (lane & 63), ball = self, frontier = self;
-  uint32_t cnt = 1;
-  for (uint32_t r = 0; r < R; ++r) {
-    uint64_t sig = 0;
-    if (valid) {
-      sig = histogram16(next_layer(q, prev, ball, frontier), q.lmask + g * 16);
-      for (uint32_t l = 0; l < 16; ++l) {
-        const uint32_t c = static_cast<uint32_t>(sig >> (4 * l)) & 15u;
-        if (c) {
-          const uint32_t tk = r | (l << 3) | (c << 7);
-          if (cnt < 4) first |= static_cast<uint64_t>(tk) << (16 * cnt);
-          t[cnt++] = static_cast<uint16_t>(tk);
-        }
+  uint32_t ntok = 2;
+#pragma unroll
+  for (uint32_t r = 0; r < kMaxLevels; ++r) {
+    sig[r] = 0;
+    if (r < R) {
+      if (valid) {
+        sig[r] = histogram16(next_layer(q, prev, ball, frontier), q.lmask + g * 16);
+        for (uint32_t l = 0; l < 16; ++l) ntok += ((sig[r] >> (4 * l)) & 15u) != 0;
       }
+      const uint64_t mx = wave_nib_max(sig[r]);
+      if (lane == 0) qmax[g * R + r] = mx;
     }
-    const uint64_t mx = wave_nib_max(sig);
-    if (lane == 0) qmax[g * R + r] = mx;
   }
-  if (valid) t[cnt] = kEndToken;
+  // exclusive scan of the padded lengths over the graph's nodes
+  const uint32_t len = valid ? (ntok + 3) & ~3u : 0u;
+  uint32_t incl = len;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), o));
+    if (lane >= static_cast<uint32_t>(o)) incl += y;
+  }
+  uint16_t* t = tokens + static_cast<uint64_t>(prev) * seg + (incl - len);
+  uint64_t first = label;  // lane 0: the first four tokens of node 0
+  uint32_t cnt = 1;
+  if (valid) {
+    t[0] = static_cast<uint16_t>(label);
+#pragma unroll
+    for (uint32_t r = 0; r < kMaxLevels; ++r)
+      if (r < R)
+        for (uint32_t l = 0; l < 16; ++l) {
+          const uint32_t c = static_cast<uint32_t>(sig[r] >> (4 * l)) & 15u;
+          if (c) {
+            const uint32_t tk = r | (l << 3) | (c << 7);
+            if (cnt < 4) first |= static_cast<uint64_t>(tk) << (16 * cnt);
+            t[cnt++] = static_cast<uint16_t>(tk);
+          }
+        }
+    t[cnt] = kEndToken;
+  }
   if (cnt < 4) first |= static_cast<uint64_t>(kEndToken) << (16 * cnt);
   uint32_t bits = valid && label < 16 ? 1u << label : 0u;
 #pragma unroll
@@ -305,6 +328,10 @@ struct SurviveArgs {
   uint32_t* pair_off;            // [Gd * Gq] arena offset of each surviving pair
   unsigned long long* counters;  // [0] molecule cursor, [1] arena top, [3] overflow, [4..6] statistics
 };
+
+// Arena words of a pair's block: its k node masks (two words each for molecules of > 64 nodes),
+// rounded up to a whole 32-byte group.
+__host__ __device__ __forceinline__ uint32_t pair_words(uint32_t k, uint32_t stride) { return (k * stride + 3) & ~3u; }
 
 constexpr uint32_t kSurviveWaves = 4;     // wavefronts per block
 constexpr uint32_t kLaneChunk = 64;       // arena words a lane owns at a time (>= the largest pair, 2 x 32)
@@ -379,6 +406,11 @@ __global__ __launch_bounds__(256) void k_pair_survive(SurviveArgs a) {
   uint64_t nodebase = 0, tw = 0;
   uint64_t lc_next = 0, lc_end = 0, base = 0;  // the lane's arena sub-chunk; the pair's first word
   Mask<NW> mask;
+  // Node masks are held in registers four words at a time and stored as one aligned 32-byte
+  // group (pair blocks start on a 4-word boundary and take a multiple of 4 words): whole sectors
+  // go out together instead of single words at scattered times, and a pair that dies never
+  // stores its open group.
+  uint64_t buf[4] = {0, 0, 0, 0};
   while (true) {
     uint64_t need = __ballot(!have);
     while (need) {
@@ -401,7 +433,7 @@ __global__ __launch_bounds__(256) void k_pair_survive(SurviveArgs a) {
         lhead += cnt;
         need &= ~__ballot(take);
         // lanes whose sub-chunk cannot hold the new pair get a fresh one from the wavefront's chunk
-        const bool fresh_chunk = take && lc_next + k * stride > lc_end;
+        const bool fresh_chunk = take && lc_next + pair_words(k, stride) > lc_end;
         const uint64_t fm = __ballot(fresh_chunk);
         if (fm) {
           const uint32_t nf = __popcll(fm);
@@ -516,7 +548,7 @@ __global__ __launch_bounds__(256) void k_pair_survive(SurviveArgs a) {
       const uint32_t e = tok[0] == kEndToken ? 0u : tok[1] == kEndToken ? 1u : tok[2] == kEndToken ? 2u : tok[3] == kEndToken ? 3u : 4u;
       const bool last = e < 4 && i + 1 == k;  // the pair's last node ends in this word
       uint64_t tw_next = 0;
-      if (!last) tw_next = a.tok64[(e < 4 ? nodebase + seg : nodebase + tp + 4) / 4];
+      if (!last) tw_next = a.tok64[(nodebase + tp + 4) / 4];  // this node's next word, or the next node's first
       // all four tokens' slot reads issued together (clamped addresses), masked afterwards
       const uint32_t* slot32 = reinterpret_cast<const uint32_t*>(slot);
       uint64_t pv[4][4], ph[4][4];
@@ -577,11 +609,22 @@ __global__ __launch_bounds__(256) void k_pair_survive(SurviveArgs a) {
       if (!mask.any()) {
         have = false;
         finished = true;
-      } else if (e < 4) {  // node done: its mask goes to the pair's arena block
-        const uint64_t at = base + static_cast<uint64_t>(i) * stride;
-        if (at + stride <= a.arena_cap) {
-          a.arena[at] = static_cast<uint64_t>(mask.lo);
-          if (NW == 4) a.arena[at + 1] = mask.hi;
+      } else if (e < 4) {  // node done: its mask joins the register group, stored when full or last
+        const uint32_t w = i * stride;  // the node's first word in the pair's block
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+          if (j == (w & 3u)) buf[j] = static_cast<uint64_t>(mask.lo);
+          if (NW == 4 && j == (w & 3u) + 1) buf[j] = mask.hi;
+        }
+        if ((w & 3u) + stride == 4u || i + 1 == k) {
+          const uint64_t at = base + (w & ~3u);
+          if (at + 4 <= a.arena_cap) {  // 32-byte aligned: two 16-byte stores
+            uint4* dst = reinterpret_cast<uint4*>(a.arena + at);
+            dst[0] = make_uint4(static_cast<uint32_t>(buf[0]), static_cast<uint32_t>(buf[0] >> 32), static_cast<uint32_t>(buf[1]),
+                                static_cast<uint32_t>(buf[1] >> 32));
+            dst[1] = make_uint4(static_cast<uint32_t>(buf[2]), static_cast<uint32_t>(buf[2] >> 32), static_cast<uint32_t>(buf[3]),
+                                static_cast<uint32_t>(buf[3] >> 32));
+          }
         }
         ++i;
         if (i == k) {  // every query node has a candidate: the pair survives GMCR
@@ -591,11 +634,11 @@ __global__ __launch_bounds__(256) void k_pair_survive(SurviveArgs a) {
             if (sl == sl_of) m = smol[sl].m;
           a.pair_off[static_cast<uint64_t>(m) * Gq + qg] = static_cast<uint32_t>(base);
           atomicOr(&a.surv[static_cast<uint64_t>(m) * a.W + (qg >> 6)], 1ull << (qg & 63));
-          lc_next = base + static_cast<uint64_t>(k) * stride;
+          lc_next = base + pair_words(k, stride);
           have = false;
           finished = true;
         } else {
-          nodebase += seg;
+          nodebase += tp + 4;  // the next node's stream starts at the next word
           tp = 0;
         }
       } else {
