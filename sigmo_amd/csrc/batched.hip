@@ -884,8 +884,10 @@ struct sigmo_batch {
   uint64_t* total = nullptr;
   int32_t* counts = nullptr;
   uint64_t cap_counts = 0;
-  uint32_t* wide_pairs = nullptr;  // pairs of molecules with > 64 nodes, for the wavefront-per-pair join
+  uint32_t* wide_pairs = nullptr;  // pairs of molecules with > 64 nodes, for the two-word lane join
   uint64_t cap_wide = 0;
+  uint32_t* deep_pairs = nullptr;  // pairs of queries deeper than the lane join's first pass holds
+  uint64_t cap_deep = 0;
   unsigned long long* run_host = nullptr;  // pinned: a one-sync run's filter counters [0..7] and GMCR total [8]
   uint32_t* req_offsets = nullptr;        // per-request statistics
   unsigned long long* req_out = nullptr;
@@ -930,6 +932,7 @@ struct sigmo_batch {
     bfree(total);
     bfree(counts);
     bfree(wide_pairs);
+    bfree(deep_pairs);
     bfree(req_offsets);
     bfree(req_out);
     drop_graph();
@@ -1253,24 +1256,14 @@ hipError_t launch_scatter(sigmo_batch* b, const RunShape& L, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Lane join over P pairs (molecules of <= 64 nodes; the others are listed in wide_pairs).
+// Lane join over P pairs (molecules of <= 64 nodes; the others are listed in wide_pairs): a
+// shallow first pass and a full-depth pass over the pairs of deeper queries (deep_pairs).
 hipError_t launch_join(sigmo_batch* b, const RunShape& L, hipStream_t s, unsigned long long* stats, uint64_t P, int find_first,
                        const unsigned long long* P_dev = nullptr) {
   if (!P) return hipSuccess;
-  const uint64_t per_cu = b->tune.join_blocks_per_cu ? std::min<uint32_t>(b->tune.join_blocks_per_cu, 4) : 4;
-  const uint64_t lblocks = std::max<uint64_t>(1, std::min<uint64_t>((P + 127) / 128, static_cast<uint64_t>(L.cus) * per_cu));
-  const uint32_t lchunk = b->tune.join_chunk ? b->tune.join_chunk
-                                             : static_cast<uint32_t>(std::min<uint64_t>(
-                                                   1024, std::max<uint64_t>(64, P / (lblocks * sigmo_k::kJoinLaneWaves * 16))));
-  if (find_first)
-    hipLaunchKernelGGL((sigmo_k::k_join_lanes<true, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(lblocks)), dim3(sigmo_k::kJoinLaneWaves * 64),
-                       0, s, L.qd, L.dd, sigmo_k::ArenaMasks{b->arena, b->mask_off, b->counters + 3}, b->indices, b->pair_graph, P, lchunk, b->counts, stats,
-                       b->wide_pairs, P_dev);
-  else
-    hipLaunchKernelGGL((sigmo_k::k_join_lanes<false, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(lblocks)), dim3(sigmo_k::kJoinLaneWaves * 64),
-                       0, s, L.qd, L.dd, sigmo_k::ArenaMasks{b->arena, b->mask_off, b->counters + 3}, b->indices, b->pair_graph, P, lchunk, b->counts, stats,
-                       b->wide_pairs, P_dev);
-  return hipGetLastError();
+  const uint32_t per_cu = b->tune.join_blocks_per_cu ? std::min<uint32_t>(b->tune.join_blocks_per_cu, 4) : 4;
+  return sigmo_k::launch_join_lanes(s, L.qd, L.dd, sigmo_k::ArenaMasks{b->arena, b->mask_off, b->counters + 3}, b->indices, b->pair_graph, P,
+                                    find_first, b->counts, stats, b->wide_pairs, L.cus, P_dev, b->deep_pairs, per_cu, b->tune.join_chunk);
 }
 
 // Lane join over the nw listed pairs of 65..128-node molecules (two-word positions).
@@ -1353,6 +1346,7 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
     BATCH_TRY(b->grow(&b->pair_graph, b->cap_pg, bound));
     BATCH_TRY(b->grow(&b->mask_off, b->cap_mo, bound));
     BATCH_TRY(b->grow(&b->counts, b->cap_counts, bound));
+    BATCH_TRY(b->grow(&b->deep_pairs, b->cap_deep, bound));
     const double reads = (1.0 + 8.0 * R) * double(L.dd.N) + double(L.qd.N) * (1.0 + 2.0 * 16.0 * R) + double(Gq) * (4.0 + 8.0 * R);
     void* tok = nullptr;
     BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_FILTER, SIGMO_KERNEL_PAIR_FILTER, reads, true, &tok));
@@ -1366,7 +1360,7 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
     BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
     BATCH_HIP_TRY(hipMemcpyAsync(b->run_host + 8, b->total, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_JOIN, SIGMO_KERNEL_JOIN, 0.0, true, &tok));
-    BATCH_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(stats), 8, 0));
+    BATCH_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(stats), sigmo_k::kJoinStatWords, 0));
     BATCH_HIP_TRY(launch_join(b, L, s, stats, bound, find_first, reinterpret_cast<const unsigned long long*>(b->total)));
     BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
     BATCH_HIP_TRY(hipMemcpyAsync(stats_host, stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
@@ -1432,11 +1426,12 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
   BATCH_TRY(b->grow(&b->mask_off, b->cap_mo, P, true));
   BATCH_TRY(b->grow(&b->counts, b->cap_counts, P, true));
   BATCH_TRY(b->grow(&b->wide_pairs, b->cap_wide, db->num_wide ? P : 1, true));
+  BATCH_TRY(b->grow(&b->deep_pairs, b->cap_deep, P, true));
   BATCH_HIP_TRY(launch_scatter(b, L, s));
   BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
   // join: lane per pair; pairs of molecules with > 64 nodes are listed for the two-word kernel
   BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_JOIN, SIGMO_KERNEL_JOIN, 0.0, true, &tok));
-  BATCH_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(stats), 8, 0));
+  BATCH_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(stats), sigmo_k::kJoinStatWords, 0));
   BATCH_HIP_TRY(launch_join(b, L, s, stats, P, find_first));
   if (P && db->num_wide) {
     unsigned long long nw = 0;
@@ -1546,7 +1541,7 @@ int sigmo_batch_capture(sigmo_batch* b, const sigmo_graph* query, int find_first
   if (e == hipSuccess) e = launch_filter(b, L, s);
   if (e == hipSuccess) e = launch_count_scan(b, L, s);
   if (e == hipSuccess) e = launch_scatter(b, L, s);
-  if (e == hipSuccess) e = sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(stats), 8, 0);
+  if (e == hipSuccess) e = sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(stats), sigmo_k::kJoinStatWords, 0);
   // the joins read this replay's pair counts on the device (bounded by the captured sizes), so a
   // replay on changed data reads only entries the replay itself wrote
   if (e == hipSuccess) e = launch_join(b, L, s, stats, b->P, find_first, reinterpret_cast<const unsigned long long*>(b->total));

@@ -261,6 +261,8 @@ struct sigmo_gmcr {
   uint64_t cap_counts = 0;
   uint32_t* wide_pairs = nullptr;  // pairs of molecules with > 64 nodes, for the two-word lane join
   uint64_t cap_wide = 0;
+  uint32_t* deep_pairs = nullptr;  // pairs of queries deeper than the lane join's first pass holds
+  uint64_t cap_deep = 0;
   uint64_t last_wide = 0;  // their count in the last join
   // The plain pipeline hands the pair count to the join on the device: P then holds an upper
   // bound (Gq x Gd) until the join's synchronisation reads the exact count from total_host.
@@ -269,6 +271,7 @@ struct sigmo_gmcr {
   ~sigmo_gmcr() {
     if (total_host) (void)hipHostFree(total_host);
     dev_free(wide_pairs);
+    dev_free(deep_pairs);
     dev_free(offsets);
     dev_free(counts);
     dev_free(indices);
@@ -415,8 +418,9 @@ int sigmo_ctx_create(int device, sigmo_ctx** out) {
     c->global_mem = prop.totalGlobalMem;
   }
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->stats), 8 * sizeof(unsigned long long));
-  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->stats_host), 8 * sizeof(unsigned long long), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->stats), sigmo_k::kJoinStatWords * sizeof(unsigned long long));
+  if (e == hipSuccess)
+    e = hipHostMalloc(reinterpret_cast<void**>(&c->stats_host), sigmo_k::kJoinStatWords * sizeof(unsigned long long), hipHostMallocDefault);
   if (e != hipSuccess) {
     sigmo_ctx_destroy(c);
     return sigmo_err::set(SIGMO_ERR_HIP, std::string("context setup: ") + hipGetErrorString(e));
@@ -1081,12 +1085,14 @@ int sigmo_join(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data
   SIGMO_TRY(grow_slack(&mm->pair_counts, mm->cap_counts, m->P));
   hipStream_t s = ctx->stream;
   PhaseScope ps(ctx, SIGMO_PHASE_JOIN, SIGMO_KERNEL_JOIN, 0.0);
-  SIGMO_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(ctx->stats), 8, 0));
+  SIGMO_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(ctx->stats), sigmo_k::kJoinStatWords, 0));
   SIGMO_TRY(grow_slack(&mm->wide_pairs, mm->cap_wide, m->P));
+  SIGMO_TRY(grow_slack(&mm->deep_pairs, mm->cap_deep, m->P));
   // a deferred GMCR: P bounds the grid and the kernel reads the exact count on the device
   SIGMO_HIP_TRY(sigmo_k::join_lanes(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, m->P, find_first,
                                     mm->pair_counts, ctx->stats, mm->wide_pairs, ctx->num_cus,
-                                    m->deferred || ctx->capturing ? reinterpret_cast<const unsigned long long*>(m->total) : nullptr));
+                                    m->deferred || ctx->capturing ? reinterpret_cast<const unsigned long long*>(m->total) : nullptr,
+                                    mm->deep_pairs));
   if (data->max_nodes > 64 && m->P) {  // pairs of molecules with > 64 nodes: the two-word lane join
     unsigned long long nw = mm->last_wide;  // a graph being recorded keeps the previous run's count
     if (!ctx->capturing) {
@@ -1123,7 +1129,7 @@ int sigmo_join_wildcard(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_gr
   SIGMO_TRY(grow(&mm->pair_counts, mm->cap_counts, m->P));
   hipStream_t s = ctx->stream;
   PhaseScope ps(ctx, SIGMO_PHASE_JOIN, SIGMO_KERNEL_JOIN, 0.0);
-  SIGMO_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(ctx->stats), 8, 0));
+  SIGMO_HIP_TRY(sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(ctx->stats), sigmo_k::kJoinStatWords, 0));
   SIGMO_HIP_TRY(sigmo_k::join_wildcard(s, query->dev(), data->dev(), query->el, data->el, c->bits, c->W64, m->indices, m->pair_graph, m->P,
                                        find_first, mm->pair_counts, ctx->stats, ctx->num_cus));
   SIGMO_TRY(ps.close());

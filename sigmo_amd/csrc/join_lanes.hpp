@@ -68,26 +68,38 @@ struct BitsetMasks {
 
 constexpr uint32_t kJoinLaneWaves = 2;  // wavefronts per block of the lane-per-pair join
 
+// Query depth the first lane-join pass holds in LDS.  Surviving pairs are mostly small queries
+// (on 20 M molecules x 2,000 queries of U[4,32] nodes: 89% have k <= 8, 98.5% k <= 12), so the
+// first pass keeps 12 levels per lane (6.75 KB per wavefront instead of 18 KB, 2.5x the resident
+// wavefronts) and lists the deeper pairs for a second pass that keeps all 32.
+constexpr int kJoinShallow = 12;
+constexpr uint64_t kJoinTwoPassPairs = 1ull << 24;  // pair counts from which the lane join runs in two passes
+
 // Join with one lane per GMCR pair, for molecules of <= 64 nodes (the rest are listed in
-// `wide` for the wavefront-per-pair kernel).  Each lane runs the reference DFS of
-// isomorphism.hpp:270-381 for its own pair with the reference's bookkeeping: every mapping
-// entry starts at local node 0, an exhausted frame unsets the visited bit of its (possibly
-// stale) mapping entry, `visited` is cleared when depth 1 is exhausted, and a frame resumes at
-// mapping[depth] + 1.  The validity test of isomorphism.hpp:150-168 -- for every j < depth,
-// qAdj(j, depth) == dAdj(mapping[j], c) -- becomes set algebra on adjacency rows: adjacency is
-// symmetric, so the candidates that pass are those in the row of every mapped query neighbour
-// and in the row of no other mapped node.  The rows of the mapped positions and the mapping sit
-// in LDS (depth-major, one column per lane).
-template <bool FIND_FIRST, class Src>
+// `wide` for the two-word kernel) and queries of <= KMAX nodes (the rest are listed in `deep`
+// for a second pass of this kernel with KMAX = 32 over that list).  Each lane runs the reference
+// DFS of isomorphism.hpp:270-381 for its own pair with the reference's bookkeeping: every
+// mapping entry starts at local node 0, an exhausted frame unsets the visited bit of its
+// (possibly stale) mapping entry, `visited` is cleared when depth 1 is exhausted, and a frame
+// resumes at mapping[depth] + 1.  The validity test of isomorphism.hpp:150-168 -- for every
+// j < depth, qAdj(j, depth) == dAdj(mapping[j], c) -- becomes set algebra on adjacency rows:
+// adjacency is symmetric, so the candidates that pass are those in the row of every mapped query
+// neighbour and in the row of no other mapped node.  The rows of the mapped positions and the
+// mapping sit in LDS (depth-major, one column per lane).  With `list` the pairs are list[0..P),
+// claimed through stats[cursor]; without it they are 0..P through stats[kStatCursor].
+template <bool FIND_FIRST, class Src, int KMAX = 32>
 __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, GraphDev d, Src src, const uint32_t* __restrict__ indices,
                                                                    const uint32_t* __restrict__ pair_graph, uint64_t P, uint32_t chunk,
                                                                    int32_t* __restrict__ counts, unsigned long long* __restrict__ stats,
-                                                                   uint32_t* __restrict__ wide, const unsigned long long* __restrict__ P_dev) {
-  __shared__ uint64_t rows_all[kJoinLaneWaves][32][kWave];
+                                                                   uint32_t* __restrict__ wide, const unsigned long long* __restrict__ P_dev,
+                                                                   const uint32_t* __restrict__ list = nullptr, uint32_t* __restrict__ deep = nullptr,
+                                                                   uint32_t cursor = kStatCursor) {
+  static_assert(KMAX >= 2 && KMAX <= 32, "query depth");
+  __shared__ uint64_t rows_all[kJoinLaneWaves][KMAX][kWave];
   if (src.aborted()) return;  // block-uniform: no barrier follows
-  // the pair count the GMCR scan left on the device; P, what the arrays were sized for, bounds it
+  // the pair (or list) count left on the device; P, what the arrays were sized for, bounds it
   if (P_dev) P = min(P, static_cast<uint64_t>(*P_dev));
-  __shared__ uint8_t maps_all[kJoinLaneWaves][32][kWave];
+  __shared__ uint8_t maps_all[kJoinLaneWaves][KMAX][kWave];
   const uint32_t lane = static_cast<uint32_t>(lane_id());
   const uint32_t wave = threadIdx.x / kWave;
   uint64_t (*rows)[kWave] = rows_all[wave];
@@ -105,7 +117,7 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
     while (need && !exhausted) {
       if (next >= end) {
         unsigned long long b = 0;
-        if (lane == 0) b = atomicAdd(&stats[2], static_cast<unsigned long long>(chunk));
+        if (lane == 0) b = atomicAdd(&stats[cursor], static_cast<unsigned long long>(chunk));
         next = uniform64(__shfl(b, 0));
         end = next + chunk < P ? next + chunk : P;
         if (next >= P) {
@@ -116,9 +128,9 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
       const uint32_t cnt = min(static_cast<uint32_t>(end - next), static_cast<uint32_t>(__popcll(need)));
       const uint32_t rn = __popcll(need & below);
       const bool take = ((need >> lane) & 1ull) && rn < cnt;
-      bool wide_pair = false;
+      bool wide_pair = false, deep_pair = false;
       if (take) {
-        j = next + rn;
+        j = list ? list[next + rn] : next + rn;
         const uint32_t dg = pair_graph[j], qg = indices[j];
         base = src.base(j);
         s = d.go[dg];
@@ -126,7 +138,8 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
         qoff = q.go[qg];
         k = q.go[qg + 1] - qoff;
         wide_pair = n > 64;
-        if (!wide_pair) {
+        deep_pair = KMAX < 32 && !wide_pair && k > static_cast<uint32_t>(KMAX);
+        if (!wide_pair && !deep_pair) {
           for (uint32_t t = 0; t < k; ++t) maps[t][lane] = 0;  // every mapping entry starts at local node 0
           depth = 0;
           pos = 0;
@@ -137,14 +150,21 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
       }
       next += cnt;
       need &= ~__ballot(take);
-      const uint64_t wm = __ballot(wide_pair);  // molecules of > 64 nodes go to the wavefront-per-pair kernel
+      const uint64_t wm = __ballot(wide_pair);  // molecules of > 64 nodes go to the two-word kernel
       if (wm) {
         unsigned long long b = 0;
-        if (lane == 0) b = atomicAdd(&stats[5], static_cast<unsigned long long>(__popcll(wm)));
+        if (lane == 0) b = atomicAdd(&stats[kStatWide], static_cast<unsigned long long>(__popcll(wm)));
         const uint64_t wb = uniform64(__shfl(b, 0));
         if (wide_pair) wide[wb + __popcll(wm & below)] = static_cast<uint32_t>(j);
       }
-      need |= wm;  // those lanes take another pair
+      const uint64_t dm = KMAX < 32 ? __ballot(deep_pair) : 0ull;  // queries deeper than KMAX go to the second pass
+      if (dm) {
+        unsigned long long b = 0;
+        if (lane == 0) b = atomicAdd(&stats[kStatDeep], static_cast<unsigned long long>(__popcll(dm)));
+        const uint64_t db = uniform64(__shfl(b, 0));
+        if (deep_pair) deep[db + __popcll(dm & below)] = static_cast<uint32_t>(j);
+      }
+      need |= wm | dm;  // those lanes take another pair
     }
     if (__ballot(have) == 0) break;
     ++wave_iters;
@@ -346,6 +366,57 @@ __global__ __launch_bounds__(64) void k_join_lanes_wide(GraphDev q, GraphDev d, 
     atomicAdd(&stats[4], static_cast<unsigned long long>(steps));
     atomicAdd(&stats[7], static_cast<unsigned long long>(wave_iters));
   }
+}
+
+// The lane join over P pairs: with `deep`, a kJoinShallow-level pass over all of them that lists
+// the deeper queries, then a full-depth pass over that list (its length read on the device);
+// without it, one full-depth pass.  blocks_per_cu (<= 4 by LDS at full depth) and chunk (0: ~16
+// claims per wavefront) shape the first pass.
+template <class Src>
+hipError_t launch_join_lanes(hipStream_t s, const GraphDev& q, const GraphDev& d, const Src& src, const uint32_t* indices,
+                             const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts, unsigned long long* stats,
+                             uint32_t* wide, int num_cus, const unsigned long long* P_dev, uint32_t* deep, uint32_t blocks_per_cu,
+                             uint32_t chunk) {
+  if (!P) return hipSuccess;
+  // Two passes pay off once the pairs fill the device many times over (1.4 x 10^8 pairs of a
+  // 1 M-molecule x 2,000-query chunk: 50.6 -> 45.4 ms); on 1.5 x 10^6 pairs the second pass's
+  // serial tail costs more than the first pass gains (0.53 -> 0.83 ms), so small joins run one
+  // full-depth pass.
+  if (P < kJoinTwoPassPairs) deep = nullptr;
+  const uint64_t cus = static_cast<uint64_t>(num_cus > 0 ? num_cus : 256);
+  // the shallow pass keeps 2.5x the wavefronts resident: up to 10 blocks per CU
+  const uint64_t per_cu = deep ? static_cast<uint64_t>(blocks_per_cu) * 5 / 2 : blocks_per_cu;
+  const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((P + 127) / 128, cus * per_cu));
+  const uint32_t ch = chunk ? chunk
+                            : static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(64, P / (blocks * kJoinLaneWaves * 16))));
+  const dim3 grid(static_cast<unsigned>(blocks)), block(kJoinLaneWaves * 64);
+  if (!deep) {
+    if (find_first)
+      hipLaunchKernelGGL((k_join_lanes<true, Src, 32>), grid, block, 0, s, q, d, src, indices, pair_graph, P, ch, counts, stats, wide, P_dev,
+                         nullptr, nullptr, kStatCursor);
+    else
+      hipLaunchKernelGGL((k_join_lanes<false, Src, 32>), grid, block, 0, s, q, d, src, indices, pair_graph, P, ch, counts, stats, wide, P_dev,
+                         nullptr, nullptr, kStatCursor);
+    return hipGetLastError();
+  }
+  if (find_first)
+    hipLaunchKernelGGL((k_join_lanes<true, Src, kJoinShallow>), grid, block, 0, s, q, d, src, indices, pair_graph, P, ch, counts, stats, wide,
+                       P_dev, nullptr, deep, kStatCursor);
+  else
+    hipLaunchKernelGGL((k_join_lanes<false, Src, kJoinShallow>), grid, block, 0, s, q, d, src, indices, pair_graph, P, ch, counts, stats, wide,
+                       P_dev, nullptr, deep, kStatCursor);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // the deep list is short (queries of more than kJoinShallow nodes): small claims, full depth
+  const uint64_t dblocks = std::max<uint64_t>(1, std::min<uint64_t>((P + 127) / 128, cus * 4));
+  const dim3 dgrid(static_cast<unsigned>(dblocks));
+  if (find_first)
+    hipLaunchKernelGGL((k_join_lanes<true, Src, 32>), dgrid, block, 0, s, q, d, src, indices, pair_graph, P, 16u, counts, stats, wide,
+                       stats + kStatDeep, deep, nullptr, kStatDeepCursor);
+  else
+    hipLaunchKernelGGL((k_join_lanes<false, Src, 32>), dgrid, block, 0, s, q, d, src, indices, pair_graph, P, 16u, counts, stats, wide,
+                       stats + kStatDeep, deep, nullptr, kStatDeepCursor);
+  return hipGetLastError();
 }
 
 // Grid of k_join_lanes_wide for `nw` listed pairs: resident one-wavefront blocks, ~16 claims each.

@@ -856,19 +856,10 @@ hipError_t gmcr_scatter(hipStream_t s, const uint64_t* pair_bits, uint32_t Gq, u
 
 hipError_t join_lanes(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
                       const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts, unsigned long long* stats, uint32_t* wide,
-                      int num_cus, const unsigned long long* P_dev) {
+                      int num_cus, const unsigned long long* P_dev, uint32_t* deep) {
   if (!P) return hipSuccess;
-  const uint64_t cus = static_cast<uint64_t>(num_cus > 0 ? num_cus : 256);
-  const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((P + 127) / 128, cus * 4));
-  const uint32_t chunk = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(64, P / (blocks * kJoinLaneWaves * 16))));
   const BitsetMasks src{bits, W64};
-  if (find_first)
-    k_join_lanes<true, BitsetMasks><<<static_cast<unsigned>(blocks), kJoinLaneWaves * 64, 0, s>>>(q, d, src, indices, pair_graph, P, chunk,
-                                                                                                counts, stats, wide, P_dev);
-  else
-    k_join_lanes<false, BitsetMasks><<<static_cast<unsigned>(blocks), kJoinLaneWaves * 64, 0, s>>>(q, d, src, indices, pair_graph, P, chunk,
-                                                                                                 counts, stats, wide, P_dev);
-  return hipGetLastError();
+  return launch_join_lanes(s, q, d, src, indices, pair_graph, P, find_first, counts, stats, wide, num_cus, P_dev, deep, 4, 0);
 }
 
 hipError_t join_wildcard(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint8_t* qel, const uint8_t* del, const uint64_t* bits,

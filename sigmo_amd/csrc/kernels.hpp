@@ -90,14 +90,32 @@ hipError_t exclusive_scan(hipStream_t s, const uint32_t* counts, uint64_t n, uin
 hipError_t gmcr_scatter(hipStream_t s, const uint64_t* pair_bits, uint32_t Gq, uint32_t Gd, const uint32_t* offsets,
                         uint32_t* indices, uint32_t* pair_graph, uint64_t cap);
 
+// Slots of the join's statistics array (kJoinStatWords 64-bit words, zeroed before a join).
+enum : uint32_t {
+  kStatSum = 0,        // sum of per-pair counts
+  kStatMatched = 1,    // pairs with count > 0
+  kStatCursor = 2,     // work cursor of the lane join
+  kStatOverflow = 3,   // 1 if a count exceeded int32
+  kStatSteps = 4,      // DFS steps
+  kStatWide = 5,       // pairs listed for the two-word join (molecules of > 64 nodes)
+  kStatWideCursor = 6, // its work cursor
+  kStatWaveIters = 7,  // loop iterations summed over wavefronts
+  kStatDeep = 8,       // pairs listed for the full-depth pass of the lane join
+  kStatDeepCursor = 9  // its work cursor
+};
+constexpr uint32_t kJoinStatWords = 16;
+
 // Join.  stats[0] += sum of counts, stats[1] += pairs with count > 0, stats[2] = work cursor
-// (zeroed by the caller), stats[3] |= 1 on int32 overflow.
+// (zeroed by the caller), stats[3] |= 1 on int32 overflow.  With `deep` (capacity P) the join
+// runs in two passes: a first that holds kJoinShallow query levels per lane in LDS and lists
+// the pairs of deeper queries in `deep` (count in stats[8]), then a full-depth pass over that
+// list; without it, one full-depth pass.
 // Lane-per-pair join of the pairs whose molecule has <= 64 nodes; the others are appended to
 // `wide` (count in stats[5]) for join_lanes_wide.  With P_dev the kernel reads the pair count
 // from the device and P only bounds the grid.
 hipError_t join_lanes(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
                       const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts, unsigned long long* stats, uint32_t* wide,
-                      int num_cus, const unsigned long long* P_dev = nullptr);
+                      int num_cus, const unsigned long long* P_dev = nullptr, uint32_t* deep = nullptr);
 // joinWildcardCandidates: matching order by candidate count, edge labels compared, one DFS per
 // candidate of the first node; one lane per pair (qel/del: edge labels of the CSR entries).
 hipError_t join_wildcard(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint8_t* qel, const uint8_t* del, const uint64_t* bits,
