@@ -20,7 +20,7 @@ from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "liboracle.so"
+LIB_PATH = Path(os.environ.get("SIGMO_ORACLE_LIB", Path(__file__).resolve().parent / "liboracle.so"))
 _lib = None
 
 

@@ -180,15 +180,11 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
           pos = static_cast<uint32_t>(maps[depth][lane]) + 1;
         }
       } else {
-        const uint64_t cand = src.mask(base, qoff + depth, depth, s, n) & ~visited & bits_from(pos);
         const uint32_t qad = static_cast<uint32_t>(q.adj[2 * (static_cast<uint64_t>(qoff) + depth)]);
-        uint64_t must = ~0ull, mustnot = 0;
-        for (uint32_t t = 0; t < depth; ++t) {
-          const uint64_t r = rows[t][lane];
-          if ((qad >> t) & 1u) must &= r;
-          else mustnot |= r;
-        }
-        const uint64_t el = cand & must & ~mustnot;
+        // candidates in the row of every mapped query neighbour and in no row of another mapped
+        // node: AND over t of row_t, complemented where t is not adjacent to `depth`
+        uint64_t el = src.mask(base, qoff + depth, depth, s, n) & ~visited & bits_from(pos);
+        for (uint32_t t = 0; t < depth; ++t) el &= rows[t][lane] ^ (static_cast<uint64_t>(((qad >> t) & 1u)) - 1ull);
         if (el) {  // accept (:366-370)
           const uint32_t c = static_cast<uint32_t>(__builtin_ctzll(el));
           maps[depth][lane] = static_cast<uint8_t>(c);
@@ -312,18 +308,12 @@ __global__ __launch_bounds__(64) void k_join_lanes_wide(GraphDev q, GraphDev d, 
         clo &= ~visited & bits_from(pos);
         chi &= ~visited & (pos <= 64 ? ~0ull : bits_from(pos - 64));
         const uint32_t qad = static_cast<uint32_t>(q.adj[2 * (static_cast<uint64_t>(qoff) + depth)]);
-        uint64_t must_lo = ~0ull, must_hi = ~0ull, not_lo = 0, not_hi = 0;
+        uint64_t el = clo, eh = chi;
         for (uint32_t t = 0; t < depth; ++t) {
-          const uint64_t rl = rows_lo[t][lane], rh = rows_hi[t][lane];
-          if ((qad >> t) & 1u) {
-            must_lo &= rl;
-            must_hi &= rh;
-          } else {
-            not_lo |= rl;
-            not_hi |= rh;
-          }
+          const uint64_t flip = static_cast<uint64_t>(((qad >> t) & 1u)) - 1ull;  // all ones where t is not adjacent
+          el &= rows_lo[t][lane] ^ flip;
+          eh &= rows_hi[t][lane] ^ flip;
         }
-        const uint64_t el = clo & must_lo & ~not_lo, eh = chi & must_hi & ~not_hi;
         if (el | eh) {  // accept (:366-370)
           const uint32_t c = el ? static_cast<uint32_t>(__builtin_ctzll(el)) : 64u + static_cast<uint32_t>(__builtin_ctzll(eh));
           maps[depth][lane] = static_cast<uint8_t>(c);

@@ -1,13 +1,15 @@
-#include <algorithm>
 // gfx950 kernels for SIGMo's hot path: signatures, candidate filter/refine, GMCR, join.
 //
 // Wave64 throughout.  The bitset kernels are streaming (one pass over 64-bit word strips,
-// coalesced across lanes) and bound by HBM; the join is a persistent one-wavefront-per-pair
-// DFS whose state is distributed over the lanes' registers.  Reference semantics are cited per kernel (paths relative
-// to the SIGMo repository); the integer conventions for the reference's undefined
-// behaviour are: shift counts masked to width-1, Bitset<uint64_t> positions taken mod 64,
+// coalesced across lanes) and bound by HBM; the join (join_lanes.hpp) is a persistent
+// lane-per-pair DFS: each lane runs one GMCR pair's search, its mapping and the adjacency rows of
+// its mapped positions in LDS, and takes the next pair when it finishes.  Reference semantics
+// are cited per kernel (paths relative to the SIGMo repository); the integer conventions for the
+// reference's undefined behaviour are: shift counts masked to width-1, Bitset<uint64_t> positions taken mod 64,
 // PowerGraph reachables starting at zero, per-row refine, ascending query ids inside GMCR
 // segments, fresh join state per pair with mapping[] = local node 0.
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "device_common.hpp"
 #include "join_lanes.hpp"

@@ -11,6 +11,7 @@ if str(ROOT) not in sys.path:
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) and the built HIP library")
     config.addinivalue_line("markers", "slow: long-running CPU test")
+    config.addinivalue_line("markers", "sanitize: runs the CPU suite or a harness under a sanitizer build")
 
 
 @pytest.fixture(scope="session")
