@@ -180,7 +180,9 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
           pos = static_cast<uint32_t>(maps[depth][lane]) + 1;
         }
       } else {
-        const uint32_t qad = static_cast<uint32_t>(q.adj[2 * (static_cast<uint64_t>(qoff) + depth)]);
+        // adjacency rows as 8-byte folded words: query graphs (<= 32 nodes) and these molecules
+        // (<= 64 nodes) have no neighbour at positions >= 64, so fold = the low row word
+        const uint32_t qad = static_cast<uint32_t>(q.fold[static_cast<uint64_t>(qoff) + depth]);
         // candidates in the row of every mapped query neighbour and in no row of another mapped
         // node: AND over t of row_t, complemented where t is not adjacent to `depth`
         uint64_t el = src.mask(base, qoff + depth, depth, s, n) & ~visited & bits_from(pos);
@@ -188,7 +190,7 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
         if (el) {  // accept (:366-370)
           const uint32_t c = static_cast<uint32_t>(__builtin_ctzll(el));
           maps[depth][lane] = static_cast<uint8_t>(c);
-          rows[depth][lane] = d.adj[2 * (static_cast<uint64_t>(s) + c)];
+          rows[depth][lane] = d.fold[static_cast<uint64_t>(s) + c];
           visited |= 1ull << c;
           ++depth;
           pos = 0;
