@@ -275,7 +275,7 @@ typedef struct sigmo_batch_result {
   uint64_t dfs_steps;     /* join DFS frame steps                         */
   uint64_t filter_wave_steps; /* pair evaluation: wavefront steps        */
   uint64_t filter_lane_steps; /* pair evaluation: active lanes summed    */
-  uint64_t filter_scans;      /* pair evaluation: 64-pair pre-test scans */
+  uint64_t filter_scans;      /* pair evaluation: pairs that passed the pre-test */
   uint64_t join_wave_steps;   /* join loop iterations summed over wavefronts */
   uint64_t one_sync;          /* 1 if the run first took the one-synchronisation path */
   uint64_t arena_repeats;     /* pair evaluations repeated after the candidate arena overflowed */
@@ -283,9 +283,12 @@ typedef struct sigmo_batch_result {
 /* Launch shape of the batched form's persistent kernels; 0 fields keep the built-in choice. */
 typedef struct sigmo_batch_tuning {
   uint32_t filter_blocks_per_cu; /* pair evaluation: 4-wavefront blocks per CU (capped by residency); 0: all that stay resident (<= 8) */
-  uint32_t filter_claim;         /* molecules a wavefront claims at a time; 0: ~count / (4 x wavefronts), 1..16 */
+  uint32_t filter_claim;         /* molecule mode: molecules a wavefront claims at a time (0: ~count / (4 x wavefronts), 1..64);
+                                    list mode: 64-pair groups a wavefront claims at a time (0: 2) */
   uint32_t join_blocks_per_cu;   /* lane join: 2-wavefront blocks per CU (1..4); 0: 4 */
   uint32_t join_chunk;           /* GMCR pairs per claim of the lane join; 0: ~P / (16 x wavefronts), 64..1024 */
+  uint32_t filter_mode;          /* pair evaluation: 1 molecule by molecule from LDS-staged planes, 2 pair by pair from the
+                                    pre-tested pair list; 0: list mode below 512 query graphs */
 } sigmo_batch_tuning;
 int sigmo_db_prepare(sigmo_ctx* ctx, const sigmo_graph* data, uint32_t refinement_steps, sigmo_db** out);
 /* Recompute the level planes from the data graph's current device contents (same result as a
@@ -302,7 +305,7 @@ int sigmo_batch_pairs(sigmo_batch* b, uint32_t* data_graph_offsets, uint64_t n_o
 int sigmo_batch_free(sigmo_batch* b);
 int sigmo_batch_set_tuning(sigmo_batch* b, const sigmo_batch_tuning* t);
 int sigmo_batch_get_tuning(const sigmo_batch* b, sigmo_batch_tuning* t);
-/* Autotuner: times sigmo_batch_run over a small grid of launch shapes (pair evaluation first,
+/* Autotuner: times sigmo_batch_run over a small grid of launch shapes (pair evaluation mode and shape first,
  * then the join), each the best of `repeats` runs, and keeps the fastest in the batch.  Results
  * do not depend on the tuning.  *best_ms (nullable) receives the winner's time. */
 int sigmo_batch_autotune(sigmo_batch* b, const sigmo_graph* query, int find_first, uint32_t repeats, sigmo_batch_tuning* chosen,

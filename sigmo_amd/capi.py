@@ -99,7 +99,7 @@ class BatchResult(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("filter_blocks_per_cu", C.c_uint32), ("filter_claim", C.c_uint32), ("join_blocks_per_cu", C.c_uint32),
-                ("join_chunk", C.c_uint32)]
+                ("join_chunk", C.c_uint32), ("filter_mode", C.c_uint32)]
 
 
 class RequestStats(C.Structure):

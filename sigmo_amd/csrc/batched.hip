@@ -108,8 +108,15 @@ struct MolInfo {
 
 // Per-query-graph facts the pair pre-test reads.
 struct QueryInfo {
-  uint32_t qoff, k, qlab, pad;  // first node, nodes, labels < 16 present
-  uint64_t first;               // first token word of the graph's first node
+  uint32_t qoff, k, qlab, qcls;  // first node, nodes, labels < 16 present, signature classes present (bit c)
+  uint64_t first;                // first token word of the graph's first node
+};
+
+// The database's signature classes: up to three labels (< 16), the database's most frequent, for
+// which the pre-test keeps per-label maxima (l[c] = 0xFFFFFFFF: class c unused).
+constexpr uint32_t kClasses = 3;
+struct ClassLabels {
+  uint32_t l[kClasses];
 };
 
 // Nibble-wise max over the 64 positions of a level signature, from its transposed planes (lane i
@@ -145,13 +152,15 @@ __device__ __forceinline__ uint64_t planes_nib_max(uint64_t col) {
 // + i] (u32) for molecules of <= 32 nodes, planes64[(x*R + r)*64 + i] (u64) otherwise, x =
 // pidx[m]; planes_hi holds positions 64..127 and lab_hi[w*16+l] the positions 64..127 with
 // label l (w = wide_idx[m]; positions 0..63 use the graph's label masks).  Wavefront c handles
-// molecule m = cls_list[c] (width-class order) and writes minfo[c] and dmax[c*R + r], the
-// nibble-wise max of the level-(r+1) signatures over the molecule's positions.
+// molecule m = cls_list[c] (width-class order) and writes minfo[c], dmax[c*R + r], the
+// nibble-wise max of the level-(r+1) signatures over the molecule's positions, and
+// cmax[(k*R + r)*G + c], the same max over the positions labelled cl.l[k] (0 if none).
 __global__ __launch_bounds__(256) void k_db_prepare(GraphDev g, uint32_t R, const uint32_t* __restrict__ cls_list,
                                                     const uint32_t* __restrict__ pidx, const uint32_t* __restrict__ wide_idx,
                                                     uint32_t* __restrict__ planes32, uint64_t* __restrict__ planes64,
                                                     uint64_t* __restrict__ planes_hi, uint64_t* __restrict__ lab_hi,
-                                                    uint64_t* __restrict__ dmax, MolInfo* __restrict__ minfo) {
+                                                    uint64_t* __restrict__ dmax, MolInfo* __restrict__ minfo, ClassLabels cl,
+                                                    uint64_t* __restrict__ cmax) {
   __shared__ uint64_t fold_all[4][kWave];
   const uint32_t lane = static_cast<uint32_t>(lane_id());
   const uint64_t c = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
@@ -180,7 +189,11 @@ __global__ __launch_bounds__(256) void k_db_prepare(GraphDev g, uint32_t R, cons
   for (uint32_t h = 0; h < halves; ++h) {
     const uint32_t pos = lane + 64 * h;
     const bool valid = pos < n;
-    if (valid && g.nl[s + pos] < 16) labels |= 1u << g.nl[s + pos];
+    const uint32_t mylab = valid ? g.nl[s + pos] : 0x100u;
+    if (mylab < 16) labels |= 1u << mylab;
+    uint64_t cpos[kClasses];  // this half's positions of each class label
+#pragma unroll
+    for (uint32_t k = 0; k < kClasses; ++k) cpos[k] = __ballot(mylab == cl.l[k]);
     uint64_t self = 1ull << (pos & 63), ball = self, frontier = self;
     for (uint32_t r = 0; r < R; ++r) {
       uint64_t sig = 0;
@@ -204,6 +217,12 @@ __global__ __launch_bounds__(256) void k_db_prepare(GraphDev g, uint32_t R, cons
       else planes64[(x * R + r) * 64 + lane] = col;
       const uint64_t mx = planes_nib_max(col);
       if (lane == 0) dmax[c * R + r] = h == 0 ? mx : nib_max(dmax[c * R + r], mx);
+#pragma unroll
+      for (uint32_t k = 0; k < kClasses; ++k) {
+        const uint64_t cm = planes_nib_max(col & cpos[k]);
+        uint64_t* dst = cmax + (static_cast<uint64_t>(k) * R + r) * g.G + c;
+        if (lane == 0) *dst = h == 0 ? cm : nib_max(*dst, cm);
+      }
     }
     if (h == 1) {
       const uint32_t lab = valid ? g.nl[s + pos] : 0xFFu;
@@ -296,11 +315,327 @@ __global__ __launch_bounds__(256) void k_query_prepare(GraphDev q, uint32_t R, u
     info.qoff = prev;
     info.k = k;
     info.qlab = bits;
-    info.pad = 0;
+    info.qcls = 0;  // set by k_query_classes
     info.first = first;
     qinfo[g] = info;
   }
 }
+
+// One thread per query graph: the per-class signature maxima the pre-test reads, from the graph's
+// token streams (k_query_prepare's output: each requirement token carries one nonzero count, of
+// label l at level r, of its node's signature).  qcmax[(g*3 + c)*R + r] is the nibble-wise max of
+// the level-(r+1) signatures of the graph's nodes labelled cl.l[c], and bit c of qinfo[g].qcls is
+// set when there is such a node.
+__global__ __launch_bounds__(256) void k_query_classes(uint32_t Gq, uint32_t R, const uint16_t* __restrict__ tokens, ClassLabels cl,
+                                                       QueryInfo* __restrict__ qinfo, uint64_t* __restrict__ qcmax) {
+  const uint64_t g = blockIdx.x * 256ull + threadIdx.x;
+  if (g >= Gq) return;
+  const QueryInfo qi = qinfo[g];
+  const uint32_t seg = token_segment(R);
+  uint64_t cm[kClasses][kMaxLevels];
+#pragma unroll
+  for (uint32_t c = 0; c < kClasses; ++c)
+#pragma unroll
+    for (uint32_t r = 0; r < kMaxLevels; ++r) cm[c][r] = 0;
+  uint32_t qcls = 0;
+  uint64_t pos = static_cast<uint64_t>(qi.qoff) * seg;
+  for (uint32_t i = 0; i < qi.k; ++i) {
+    const uint32_t label = tokens[pos];
+    uint32_t c = kClasses;
+#pragma unroll
+    for (uint32_t cc = 0; cc < kClasses; ++cc)
+      if (c == kClasses && label == cl.l[cc]) c = cc;
+    uint64_t j = pos + 1;
+    for (uint32_t t = tokens[j]; t != kEndToken; t = tokens[++j]) {
+      const uint32_t r = t & 7u, l = (t >> 3) & 15u;
+      const uint64_t v = static_cast<uint64_t>((t >> 7) & 15u) << (4 * l);
+#pragma unroll
+      for (uint32_t cc = 0; cc < kClasses; ++cc)
+#pragma unroll
+        for (uint32_t rr = 0; rr < kMaxLevels; ++rr)
+          if (cc == c && rr == r) cm[cc][rr] = nib_max(cm[cc][rr], v);
+    }
+    if (c < kClasses) qcls |= 1u << c;
+    pos += (j - pos + 1 + 3) & ~3ull;  // the node's padded stream: label, requirements, end
+  }
+#pragma unroll
+  for (uint32_t c = 0; c < kClasses; ++c)
+#pragma unroll
+    for (uint32_t r = 0; r < kMaxLevels; ++r)
+      if (r < R) qcmax[(g * kClasses + c) * R + r] = cm[c][r];
+  qinfo[g].qcls = qcls;
+}
+
+// ---- pair pre-test ------------------------------------------------------------------------------
+//
+// Two exact necessary conditions for a (query graph, molecule) pair to survive GMCR, tested for
+// every pair before any candidate mask is built: (1) the graph's label set (labels < 16) is
+// contained in the molecule's, and at every level the graph's nibble-wise signature max is <=
+// the molecule's (a node whose signature exceeds the max fits no position); (2) for each class
+// label present in the graph, the same holds between the max over the graph's nodes of that label
+// and the max over the molecule's positions of that label (a node's candidates carry its label,
+// isomorphism.hpp:49).  Query nodes of label 255 only enter (1).
+
+struct PretestArgs {
+  uint32_t R, Gq, W, G;          // W: bitmask words per molecule; G: molecules
+  const MolInfo* minfo;          // width-class order
+  const uint64_t* dmax;          // [c * R + r]
+  const uint64_t* cmax;          // [(k * R + r) * G + c]
+  const QueryInfo* qinfo;
+  const uint64_t* qmax;          // [g * R + r]
+  const uint64_t* qcmax;         // [(g * 3 + k) * R + r]
+  unsigned long long* surv;      // [m * W + w] out: bit g of word w = pair (g, m) passes
+  uint32_t* pcnt;                // [c] out: passing pairs of the molecule
+};
+
+constexpr uint32_t kPretestWords = 1 + (1 + kClasses) * kMaxLevels;  // per query graph in the LDS tile
+
+// One thread per molecule (width-class order), the query graphs 64 at a time from an LDS tile
+// (every lane reads the same graph: broadcast reads).  A graph whose first test fails on all 64
+// molecules of a wavefront skips the rest.
+__global__ __launch_bounds__(256) void k_pair_pretest(PretestArgs a) {
+  __shared__ uint64_t tile[kWave * kPretestWords];
+  const uint32_t R = a.R, stride = 1 + (1 + kClasses) * R;
+  const uint64_t c = blockIdx.x * 256ull + threadIdx.x;
+  const bool valid = c < a.G;
+  uint32_t mlab = 0, m = 0;
+  uint64_t dm[kMaxLevels], cm[kClasses][kMaxLevels];
+  if (valid) {
+    const MolInfo mi = a.minfo[c];
+    mlab = mi.mlab;
+    m = mi.m;
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < kMaxLevels; ++r) {
+    dm[r] = valid && r < R ? a.dmax[c * R + r] : 0ull;
+#pragma unroll
+    for (uint32_t k = 0; k < kClasses; ++k) cm[k][r] = valid && r < R ? a.cmax[(static_cast<uint64_t>(k) * R + r) * a.G + c] : 0ull;
+  }
+  uint32_t cnt = 0;
+  for (uint32_t w = 0; w < a.W; ++w) {
+    __syncthreads();  // the previous tile has been read
+    for (uint32_t t = threadIdx.x; t < kWave * stride; t += 256) {
+      const uint32_t j = t / stride, f = t - j * stride;
+      const uint32_t g = w * kWave + j;
+      uint64_t v = 0;
+      if (g < a.Gq) {
+        if (f == 0) {
+          const QueryInfo qi = a.qinfo[g];
+          v = qi.qlab | (static_cast<uint64_t>(qi.qcls) << 32);
+        } else if (f <= R) {
+          v = a.qmax[static_cast<uint64_t>(g) * R + f - 1];
+        } else {
+          v = a.qcmax[static_cast<uint64_t>(g) * kClasses * R + f - 1 - R];
+        }
+      }
+      tile[j * stride + f] = v;
+    }
+    __syncthreads();
+    const uint32_t nj = min(static_cast<uint32_t>(kWave), a.Gq - w * kWave);
+    uint64_t word = 0;
+    for (uint32_t j = 0; j < nj; ++j) {
+      const uint64_t* e = tile + j * stride;
+      const uint64_t h = e[0];
+      bool fits = valid && (static_cast<uint32_t>(h) & ~mlab) == 0u;
+#pragma unroll
+      for (uint32_t r = 0; r < kMaxLevels; ++r)
+        if (r < R) fits = fits & nib_le_all(e[1 + r], dm[r]);
+      const uint32_t qcls = static_cast<uint32_t>(h >> 32);
+      if (qcls && __ballot(fits)) {  // wave-uniform
+#pragma unroll
+        for (uint32_t k = 0; k < kClasses; ++k)
+          if ((qcls >> k) & 1u) {
+#pragma unroll
+            for (uint32_t r = 0; r < kMaxLevels; ++r)
+              if (r < R) fits = fits & nib_le_all(e[1 + R + k * R + r], cm[k][r]);
+          }
+      }
+      word |= static_cast<uint64_t>(fits) << j;
+    }
+    if (valid) a.surv[static_cast<uint64_t>(m) * a.W + w] = word;
+    cnt += static_cast<uint32_t>(__popcll(word));
+  }
+  if (valid) a.pcnt[c] = cnt;
+}
+
+// The pre-tested pairs listed in (width-class order, query graph) order: entry = c | g << 32,
+// molecule c's pairs from poff[c] (the exclusive scan of pcnt).  Entries past `cap` are not
+// written; a list longer than `cap` sets the overflow flag (counters[3]) and the host repeats the
+// run with the demand (the scan's total, counters[7]).
+__global__ __launch_bounds__(256) void k_pair_expand(uint32_t G, uint32_t W, const MolInfo* __restrict__ minfo,
+                                                     const unsigned long long* __restrict__ surv, const uint32_t* __restrict__ poff,
+                                                     uint64_t* __restrict__ list, uint64_t cap, unsigned long long* __restrict__ counters) {
+  const uint64_t c = blockIdx.x * 256ull + threadIdx.x;
+  if (c == 0 && counters[7] > cap) atomicOr(&counters[3], 1ull);
+  if (c >= G) return;
+  uint64_t pos = poff[c];
+  const uint64_t end = poff[c + 1];
+  if (pos == end) return;
+  const uint64_t m = minfo[c].m;
+  for (uint32_t w = 0; w < W && pos < end; ++w) {
+    uint64_t x = surv[m * W + w];
+    while (x) {
+      const uint32_t g = w * kWave + static_cast<uint32_t>(__builtin_ctzll(x));
+      x &= x - 1;
+      if (pos < cap) list[pos] = c | (static_cast<uint64_t>(g) << 32);
+      ++pos;
+    }
+  }
+}
+
+// ---- pair evaluation ----------------------------------------------------------------------------
+
+// Arena words of a pair's block: its k node masks (two words each for molecules of > 64 nodes),
+// rounded up to a whole 32-byte group.
+__host__ __device__ __forceinline__ uint32_t pair_words(uint32_t k, uint32_t stride) { return (k * stride + 3) & ~3u; }
+
+constexpr uint32_t kSurviveWaves = 4;     // wavefronts per block
+constexpr uint32_t kLaneChunk = 64;       // arena words a lane owns at a time (>= the largest pair, 2 x 32)
+constexpr uint32_t kWaveChunk = 64 * kLaneChunk;
+
+// Candidate masks of one width class: NW = 1 (<= 32 positions, u32), 2 (<= 64, u64) or
+// 4 (<= 128, two u64 words).
+template <int NW>
+struct Mask {
+  using T = typename std::conditional<NW == 1, uint32_t, uint64_t>::type;
+  T lo = 0;
+  uint64_t hi = 0;  // NW == 4 only
+  __device__ __forceinline__ bool any() const { return NW == 4 ? ((static_cast<uint64_t>(lo) | hi) != 0) : (lo != 0); }
+};
+
+// One lane's walk over a pair's token words.
+template <int NW>
+struct PairWalk {
+  Mask<NW> mask;
+  uint64_t tw = 0;        // the current token word
+  uint64_t nodebase = 0;  // token index of the current node's stream
+  uint64_t base = 0;      // the pair's first arena word
+  uint32_t i = 0, k = 0, tp = 0;  // current node, nodes, token offset within the node's stream
+  // node masks are held four words at a time and stored as one aligned 32-byte group (pair
+  // blocks start on a 4-word boundary and take a multiple of 4 words): whole sectors go out
+  // together, and a pair that dies never stores its open group
+  uint64_t buf[4] = {0, 0, 0, 0};
+};
+
+enum : int { kWalkOn = 0, kWalkDied = 1, kWalkSurvived = 2 };
+
+// One token word of a pair: a node's label token sets its mask to the filter's label test
+// (isomorphism.hpp:49), each requirement token ANDs one bit-sliced refine test
+// (isomorphism.hpp:103-110), and the end token stores the node's mask.  The pair dies at the
+// first node without candidates (the GMCR predicate, gmcr.hpp:56-61) and survives when its last
+// node ends.  `P` gives the four plane words of (level r, label l) and the position masks of a
+// label; `ns()` gives the molecule's node count and first node (read only for labels >= 16).  All
+// four tokens' plane reads and the next word's load are issued together; requirements of a node
+// commute, so their order within the word is free.
+template <int NW, class Planes, class NS>
+__device__ __forceinline__ int walk_step(const Planes& P, NS ns, const uint8_t* __restrict__ nl, const uint64_t* __restrict__ tok64,
+                                         uint32_t R, PairWalk<NW>& w, uint64_t* __restrict__ arena, uint64_t arena_cap) {
+  using T = typename Mask<NW>::T;
+  constexpr uint32_t stride = NW == 4 ? 2u : 1u;
+  uint32_t tok[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) tok[j] = static_cast<uint32_t>(w.tw >> (16 * j)) & 0xFFFFu;
+  const uint32_t e = tok[0] == kEndToken ? 0u : tok[1] == kEndToken ? 1u : tok[2] == kEndToken ? 2u : tok[3] == kEndToken ? 3u : 4u;
+  const bool last = e < 4 && w.i + 1 == w.k;  // the pair's last node ends in this word
+  uint64_t tw_next = 0;
+  if (!last) tw_next = tok64[(w.nodebase + w.tp + 4) / 4];  // this node's next word, or the next node's first
+  uint64_t pv[4][4], ph[4][4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) P.load(min(tok[j] & 7u, R - 1), (tok[j] >> 3) & 15u, pv[j], ph[j]);  // clamped, masked below
+  const uint32_t ql = tok[0];
+  if (w.tp == 0) {  // the node's label token: the filter's label test (isomorphism.hpp:49)
+    if (ql < 16u) {
+      uint64_t lo = 0, hi = 0;
+      P.lab(ql, lo, hi);
+      w.mask.lo = static_cast<T>(lo);
+      if (NW == 4) w.mask.hi = hi;
+    } else {
+      const uint2 mns = ns();  // (n, s)
+      if (ql == 255u) {
+        w.mask.lo = static_cast<T>(mns.x >= 64 ? ~0ull : ((1ull << mns.x) - 1));
+        if (NW == 4) w.mask.hi = mns.x >= 128 ? ~0ull : ((1ull << (mns.x - 64)) - 1);
+      } else {
+        uint64_t l0 = 0, l1 = 0;
+        for (uint32_t p = 0; p < mns.x; ++p)
+          if (nl[mns.y + p] == ql) {
+            if (p < 64) l0 |= 1ull << p;
+            else l1 |= 1ull << (p - 64);
+          }
+        w.mask.lo = static_cast<T>(l0);
+        if (NW == 4) w.mask.hi = l1;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {  // count_q(l) <= count_p(l) at level r (isomorphism.hpp:103-110)
+    const bool req = static_cast<uint32_t>(j) < e && (w.tp != 0 || j != 0);
+    const uint32_t c = req ? (tok[j] >> 7) & 15u : 0u;  // c = 0: always true
+    w.mask.lo &= static_cast<T>(ge4_bits<T>(static_cast<T>(pv[j][0]), static_cast<T>(pv[j][1]), static_cast<T>(pv[j][2]),
+                                            static_cast<T>(pv[j][3]), c));
+    if (NW == 4) w.mask.hi &= ge4_bits<uint64_t>(ph[j][0], ph[j][1], ph[j][2], ph[j][3], c);
+  }
+  int out = kWalkOn;
+  if (!w.mask.any()) {
+    out = kWalkDied;
+  } else if (e < 4) {  // node done: its mask joins the register group, stored when full or last
+    const uint32_t wd = w.i * stride;  // the node's first word in the pair's block
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+      if (j == (wd & 3u)) w.buf[j] = static_cast<uint64_t>(w.mask.lo);
+      if (NW == 4 && j == (wd & 3u) + 1) w.buf[j] = w.mask.hi;
+    }
+    if ((wd & 3u) + stride == 4u || w.i + 1 == w.k) {
+      const uint64_t at = w.base + (wd & ~3u);
+      if (at + 4 <= arena_cap) {  // 32-byte aligned: two 16-byte stores
+        uint4* dst = reinterpret_cast<uint4*>(arena + at);
+        dst[0] = make_uint4(static_cast<uint32_t>(w.buf[0]), static_cast<uint32_t>(w.buf[0] >> 32), static_cast<uint32_t>(w.buf[1]),
+                            static_cast<uint32_t>(w.buf[1] >> 32));
+        dst[1] = make_uint4(static_cast<uint32_t>(w.buf[2]), static_cast<uint32_t>(w.buf[2] >> 32), static_cast<uint32_t>(w.buf[3]),
+                            static_cast<uint32_t>(w.buf[3] >> 32));
+      }
+    }
+    ++w.i;
+    if (w.i == w.k) {
+      out = kWalkSurvived;
+    } else {
+      w.nodebase += w.tp + 4;  // the next node's stream starts at the next word
+      w.tp = 0;
+    }
+  } else {
+    w.tp += 4;
+  }
+  w.tw = tw_next;
+  return out;
+}
+
+// A lane's arena sub-chunk and its wavefront's chunk: lanes whose sub-chunk cannot hold a new
+// pair of `need` words get a fresh one from the wavefront's chunk, which claims kWaveChunk words
+// of the arena at a time (setting the overflow flag, counters[3], past the arena's end).
+struct ArenaCursor {
+  uint64_t wc_next = 0, wc_end = 0;  // the wavefront's chunk (wave-uniform)
+  uint64_t lc_next = 0, lc_end = 0;  // the lane's sub-chunk
+  __device__ __forceinline__ void reserve(bool take, uint32_t need, uint64_t below, unsigned long long* counters, uint64_t arena_cap) {
+    const bool fresh = take && lc_next + need > lc_end;
+    const uint64_t fm = __ballot(fresh);
+    if (fm) {
+      const uint32_t nf = __popcll(fm);
+      if (wc_next + static_cast<uint64_t>(nf) * kLaneChunk > wc_end) {
+        unsigned long long b = 0;
+        if (lane_id() == 0) {
+          b = atomicAdd(&counters[1], static_cast<unsigned long long>(kWaveChunk));
+          if (b + kWaveChunk > arena_cap) atomicOr(&counters[3], 1ull);
+        }
+        wc_next = uniform64(__shfl(b, 0));
+        wc_end = wc_next + kWaveChunk;
+      }
+      if (fresh) {
+        lc_next = wc_next + static_cast<uint64_t>(__popcll(fm & below)) * kLaneChunk;
+        lc_end = lc_next + kLaneChunk;
+      }
+      wc_next += static_cast<uint64_t>(nf) * kLaneChunk;
+    }
+  }
+};
 
 // A pre-tested (query graph, molecule) pair waiting for a lane.
 struct PairEntry {
@@ -314,28 +649,19 @@ struct SurviveArgs {
   uint32_t count;          // molecules of the width class
   uint32_t claim;          // molecules a wavefront claims at a time
   const MolInfo* minfo;    // the width class's molecules, ascending
-  const uint64_t* dmax;    // the width class's [molecule][R] signature maxima
+  const uint32_t* pcnt;    // the width class's pre-tested pair counts
   const uint32_t* planes32;
   const uint64_t* planes64;
   const uint64_t* planes_hi;
   const uint64_t* lab_hi;
   const uint64_t* tok64;   // token segments as 64-bit words
   const QueryInfo* qinfo;
-  const uint64_t* qmax;
   uint64_t* arena;         // survivor masks: k words (2k, interleaved, for molecules of > 64 nodes)
   uint64_t arena_cap;
-  unsigned long long* surv;      // [Gd][W] bit qg of row m: the pair survives GMCR
+  unsigned long long* surv;      // [Gd][W] in: the pre-test's pairs; out: bit qg of row m = the pair survives GMCR
   uint32_t* pair_off;            // [Gd * Gq] arena offset of each surviving pair
-  unsigned long long* counters;  // [0] molecule cursor, [1] arena top, [3] overflow, [4..6] statistics
+  unsigned long long* counters;  // [0] molecule cursor, [1] arena top, [3] overflow, [4..5] statistics
 };
-
-// Arena words of a pair's block: its k node masks (two words each for molecules of > 64 nodes),
-// rounded up to a whole 32-byte group.
-__host__ __device__ __forceinline__ uint32_t pair_words(uint32_t k, uint32_t stride) { return (k * stride + 3) & ~3u; }
-
-constexpr uint32_t kSurviveWaves = 4;     // wavefronts per block
-constexpr uint32_t kLaneChunk = 64;       // arena words a lane owns at a time (>= the largest pair, 2 x 32)
-constexpr uint32_t kWaveChunk = 64 * kLaneChunk;
 
 // Molecule slots per wavefront and LDS words (u64) per slot, by width class.
 __host__ __device__ constexpr uint32_t ring_slots(int nw) { return nw == 1 ? 4u : (nw == 2 ? 3u : 2u); }
@@ -347,33 +673,82 @@ __host__ __device__ constexpr uint32_t survive_lds_bytes(uint32_t R, int nw) {
   return kSurviveWaves * (ring_slots(nw) * slot_words(R, nw) + plist_words()) * 8;
 }
 
-// Candidate masks of one width class: NW = 1 (<= 32 positions, u32), 2 (<= 64, u64) or
-// 4 (<= 128, two u64 words).
+// The plane words of a molecule staged in an LDS slot (layout of k_pair_survive).
 template <int NW>
-struct Mask {
-  using T = typename std::conditional<NW == 1, uint32_t, uint64_t>::type;
-  T lo = 0;
-  uint64_t hi = 0;  // NW == 4 only
-  __device__ __forceinline__ bool any() const { return NW == 4 ? ((static_cast<uint64_t>(lo) | hi) != 0) : (lo != 0); }
+struct SlotPlanes {
+  const uint64_t* slot;
+  uint32_t R;
+  __device__ __forceinline__ void load(uint32_t r, uint32_t l, uint64_t pv[4], uint64_t ph[4]) const {
+    if (NW == 1) {
+      const uint4 v = *reinterpret_cast<const uint4*>(reinterpret_cast<const uint32_t*>(slot) + r * 64 + l * 4);
+      pv[0] = v.x;
+      pv[1] = v.y;
+      pv[2] = v.z;
+      pv[3] = v.w;
+    } else {
+#pragma unroll
+      for (int b = 0; b < 4; ++b) pv[b] = slot[r * 64 + l * 4 + b];
+      if (NW == 4) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) ph[b] = slot[R * 64 + r * 64 + l * 4 + b];
+      }
+    }
+  }
+  __device__ __forceinline__ void lab(uint32_t l, uint64_t& lo, uint64_t& hi) const {
+    const uint32_t lm_at = NW == 1 ? 2 * R * 32 : (NW == 4 ? 2 * R * 64 : R * 64);  // u32 index for NW == 1
+    lo = NW == 1 ? reinterpret_cast<const uint32_t*>(slot)[lm_at + l] : slot[lm_at + l];
+    hi = NW == 4 ? slot[lm_at + 16 + l] : 0ull;
+  }
 };
 
-// Persistent pair evaluation of one width class.  Each wavefront keeps a ring of molecule slots
-// in LDS -- the molecule's level planes and label masks, staged with coalesced loads when the
-// wavefront claims it -- and pre-tests the open molecule against 64 query graphs at a time with
-// two exact necessary tests: the query graph's label set (labels < 16) is contained in the
-// molecule's, and at every level the graph's nibble-wise signature max is <= the molecule's (a
-// node fits nowhere if it exceeds the max).  Passing pairs wait in an LDS list and go to idle
-// lanes, which may hold pairs of any slot; a lane runs its pair over the query's token
-// segments, one token word per step: a node's label token sets its mask to the filter's label
-// test (isomorphism.hpp:49), each requirement token ANDs one bit-sliced refine test
-// (isomorphism.hpp:103-110) read from the slot, and the end token stores the node's mask.  The
-// pair stops at the first node without candidates, which is the GMCR predicate
-// (gmcr.hpp:56-61).  A slot is reused once its molecule is fully pre-tested and none of its
-// pairs is pending.  Each lane owns an arena sub-chunk: node masks go straight to the pair's
-// final place, and a pair that dies leaves its space to the lane's next pair.
+// The plane words of a molecule read from the database in device memory.
+template <int NW>
+struct GlobalPlanes {
+  const uint32_t* p32;  // the molecule's planes (NW == 1)
+  const uint64_t* p64;  // the molecule's planes (NW >= 2)
+  const uint64_t* phi;  // positions 64..127 (NW == 4)
+  const uint64_t* lm;   // label masks of positions 0..63
+  const uint64_t* lmh;  // label masks of positions 64..127 (NW == 4)
+  __device__ __forceinline__ void load(uint32_t r, uint32_t l, uint64_t pv[4], uint64_t ph[4]) const {
+    if (NW == 1) {
+      const uint4 v = *reinterpret_cast<const uint4*>(p32 + r * 64 + l * 4);
+      pv[0] = v.x;
+      pv[1] = v.y;
+      pv[2] = v.z;
+      pv[3] = v.w;
+    } else {
+      const uint4* s = reinterpret_cast<const uint4*>(p64 + r * 64 + l * 4);
+      const uint4 v0 = s[0], v1 = s[1];
+      pv[0] = v0.x | (static_cast<uint64_t>(v0.y) << 32);
+      pv[1] = v0.z | (static_cast<uint64_t>(v0.w) << 32);
+      pv[2] = v1.x | (static_cast<uint64_t>(v1.y) << 32);
+      pv[3] = v1.z | (static_cast<uint64_t>(v1.w) << 32);
+      if (NW == 4) {
+        const uint4* t = reinterpret_cast<const uint4*>(phi + r * 64 + l * 4);
+        const uint4 u0 = t[0], u1 = t[1];
+        ph[0] = u0.x | (static_cast<uint64_t>(u0.y) << 32);
+        ph[1] = u0.z | (static_cast<uint64_t>(u0.w) << 32);
+        ph[2] = u1.x | (static_cast<uint64_t>(u1.y) << 32);
+        ph[3] = u1.z | (static_cast<uint64_t>(u1.w) << 32);
+      }
+    }
+  }
+  __device__ __forceinline__ void lab(uint32_t l, uint64_t& lo, uint64_t& hi) const {
+    lo = lm[l];
+    hi = NW == 4 ? lmh[l] : 0ull;
+  }
+};
+
+// Persistent pair evaluation of one width class, molecule by molecule.  Each wavefront keeps a
+// ring of molecule slots in LDS -- the molecule's level planes and label masks, staged with
+// coalesced loads when the wavefront claims it (molecules with no pre-tested pair are skipped) --
+// and lists the open molecule's pre-tested pairs, 64 query graphs at a time, in an LDS list from
+// which idle lanes take pairs of any slot.  A slot is reused once its molecule's list is done and
+// none of its pairs is pending.  Each lane owns an arena sub-chunk: node masks go straight to the
+// pair's final place, and a pair that dies leaves its space to the lane's next pair (and clears
+// its pre-test bit).  Suited to many pairs per molecule (large query batches).
 template <int NW>
 __global__ __launch_bounds__(256) void k_pair_survive(SurviveArgs a) {
-  using T = typename Mask<NW>::T;
   constexpr uint32_t stride = NW == 4 ? 2u : 1u;
   constexpr uint32_t S = ring_slots(NW);
   extern __shared__ uint64_t lds[];
@@ -387,30 +762,22 @@ __global__ __launch_bounds__(256) void k_pair_survive(SurviveArgs a) {
   const uint64_t below = (1ull << lane) - 1;
 
   // wave-uniform state
-  uint64_t wc_next = 0, wc_end = 0;           // claimed arena chunk
+  ArenaCursor ac;
   uint32_t lhead = 0, lcount = 0;             // pending list entries [lhead, lcount)
-  uint32_t pending[S];                        // passing pairs of the slot's molecule not yet finished
+  uint32_t pending[S];                        // listed pairs of the slot's molecule not yet finished
   MolInfo smol[S];                            // the slot's molecule
-  uint32_t open = S, scan_qg = 0;             // slot being pre-tested (S: none) and its next query graph
-  uint64_t omax[kMaxLevels];                  // the open molecule's signature maxima
-  uint32_t omlab = 0;                         // the open molecule's labels < 16
+  uint32_t open = S, scan_qg = 0;             // slot being listed (S: none) and its next query graph
+  uint32_t om = 0;                            // the open molecule
   bool exhausted = false;
   uint64_t mc_next = 0, mc_end = 0;           // claimed molecules [mc_next, mc_end)
 #pragma unroll
   for (uint32_t sl = 0; sl < S; ++sl) pending[sl] = 0;
-  uint64_t wave_steps = 0, lane_steps = 0, scans = 0;  // statistics
+  uint64_t wave_steps = 0, lane_steps = 0;    // statistics
 
   // per-lane pair state
   bool have = false;
-  uint32_t sl_of = 0, qg = 0, k = 0, i = 0, tp = 0;
-  uint64_t nodebase = 0, tw = 0;
-  uint64_t lc_next = 0, lc_end = 0, base = 0;  // the lane's arena sub-chunk; the pair's first word
-  Mask<NW> mask;
-  // Node masks are held in registers four words at a time and stored as one aligned 32-byte
-  // group (pair blocks start on a 4-word boundary and take a multiple of 4 words): whole sectors
-  // go out together instead of single words at scattered times, and a pair that dies never
-  // stores its open group.
-  uint64_t buf[4] = {0, 0, 0, 0};
+  uint32_t sl_of = 0, qg = 0;
+  PairWalk<NW> pw;
   while (true) {
     uint64_t need = __ballot(!have);
     while (need) {
@@ -422,56 +789,27 @@ __global__ __launch_bounds__(256) void k_pair_survive(SurviveArgs a) {
           const PairEntry& e = plist[lhead + rn];
           sl_of = e.slot;
           qg = e.qg;
-          k = e.k;
-          tw = e.tw;
-          nodebase = static_cast<uint64_t>(e.qoff) * seg;
-          i = 0;
-          tp = 0;
+          pw.k = e.k;
+          pw.tw = e.tw;
+          pw.nodebase = static_cast<uint64_t>(e.qoff) * seg;
+          pw.i = 0;
+          pw.tp = 0;
           have = true;
         }
         wave_sync();
         lhead += cnt;
         need &= ~__ballot(take);
-        // lanes whose sub-chunk cannot hold the new pair get a fresh one from the wavefront's chunk
-        const bool fresh_chunk = take && lc_next + pair_words(k, stride) > lc_end;
-        const uint64_t fm = __ballot(fresh_chunk);
-        if (fm) {
-          const uint32_t nf = __popcll(fm);
-          if (wc_next + static_cast<uint64_t>(nf) * kLaneChunk > wc_end) {
-            unsigned long long b = 0;
-            if (lane == 0) {
-              b = atomicAdd(&a.counters[1], static_cast<unsigned long long>(kWaveChunk));
-              if (b + kWaveChunk > a.arena_cap) atomicOr(&a.counters[3], 1ull);
-            }
-            wc_next = uniform64(__shfl(b, 0));
-            wc_end = wc_next + kWaveChunk;
-          }
-          if (fresh_chunk) {
-            lc_next = wc_next + static_cast<uint64_t>(__popcll(fm & below)) * kLaneChunk;
-            lc_end = lc_next + kLaneChunk;
-          }
-          wc_next += static_cast<uint64_t>(nf) * kLaneChunk;
-        }
-        if (take) base = lc_next;
+        ac.reserve(take, pair_words(pw.k, stride), below, a.counters, a.arena_cap);
+        if (take) pw.base = ac.lc_next;
         continue;
       }
-      if (open < S && scan_qg < Gq) {  // pre-test the open molecule against the next 64 query graphs
+      if (open < S && scan_qg < Gq) {  // list the open molecule's pre-tested pairs of the next 64 query graphs
+        const uint64_t word = a.surv[static_cast<uint64_t>(om) * a.W + scan_qg / kWave];
         const uint32_t g = scan_qg + lane;
-        bool ok = g < Gq;
-        // every load issued before any use: clamped indices, results masked afterwards
-        const uint32_t gs = ok ? g : Gq - 1;
-        const QueryInfo qi = a.qinfo[gs];
-        uint64_t qm[kMaxLevels];
-#pragma unroll
-        for (uint32_t r = 0; r < kMaxLevels; ++r) qm[r] = a.qmax[static_cast<uint64_t>(gs) * R + (r < R ? r : R - 1)];
-        bool fits = (qi.qlab & ~omlab) == 0u;
-#pragma unroll
-        for (uint32_t r = 0; r < kMaxLevels; ++r) fits = fits & nib_le_all(qm[r], omax[r]);
-        ok = ok && fits;
-        const uint64_t okm = __ballot(ok);
-        ++scans;
+        const bool ok = (word >> lane) & 1ull;
         if (ok) {
-          PairEntry& e = plist[__popcll(okm & below)];
+          const QueryInfo qi = a.qinfo[g];
+          PairEntry& e = plist[__popcll(word & below)];
           e.slot = open;
           e.qg = g;
           e.qoff = qi.qoff;
@@ -480,14 +818,14 @@ __global__ __launch_bounds__(256) void k_pair_survive(SurviveArgs a) {
         }
         wave_sync();
         lhead = 0;
-        lcount = __popcll(okm);
+        lcount = __popcll(word);
 #pragma unroll
         for (uint32_t sl = 0; sl < S; ++sl)
           if (sl == open) pending[sl] += lcount;
         scan_qg += kWave;
         continue;
       }
-      // the open molecule is fully pre-tested: open the next one in a free slot
+      // the open molecule is fully listed: open the next one with pairs in a free slot
       if (exhausted) break;
       uint32_t free_slot = S;
 #pragma unroll
@@ -506,9 +844,8 @@ __global__ __launch_bounds__(256) void k_pair_survive(SurviveArgs a) {
         break;
       }
       const uint64_t c = mc_next++;
+      if (a.pcnt[c] == 0) continue;  // nothing passed the pre-test: not staged
       const MolInfo mi = a.minfo[c];
-#pragma unroll
-      for (uint32_t r = 0; r < kMaxLevels; ++r) omax[r] = a.dmax[c * R + (r < R ? r : R - 1)];
       uint64_t* slot = ring + free_slot * sw;
       if (NW == 1) {
         const uint64_t* src = reinterpret_cast<const uint64_t*>(a.planes32 + static_cast<uint64_t>(mi.x) * R * 64);
@@ -530,7 +867,7 @@ __global__ __launch_bounds__(256) void k_pair_survive(SurviveArgs a) {
       for (uint32_t sl = 0; sl < S; ++sl)
         if (sl == free_slot) smol[sl] = mi;
       open = free_slot;
-      omlab = mi.mlab;
+      om = mi.m;
       scan_qg = 0;
     }
     const uint64_t active = __ballot(have);
@@ -539,112 +876,29 @@ __global__ __launch_bounds__(256) void k_pair_survive(SurviveArgs a) {
     lane_steps += __popcll(active);
     bool finished = false;
     if (have) {
-      // one token word per step: its label / requirement reads (LDS) and the next word's load are
-      // all in flight together; requirements of a node commute, so their order within the word is free
-      const uint64_t* slot = ring + sl_of * sw;
-      uint32_t tok[4];
+      const SlotPlanes<NW> P{ring + sl_of * sw, R};
+      auto ns = [&]() {
+        MolInfo mi = {};
 #pragma unroll
-      for (int j = 0; j < 4; ++j) tok[j] = static_cast<uint32_t>(tw >> (16 * j)) & 0xFFFFu;
-      const uint32_t e = tok[0] == kEndToken ? 0u : tok[1] == kEndToken ? 1u : tok[2] == kEndToken ? 2u : tok[3] == kEndToken ? 3u : 4u;
-      const bool last = e < 4 && i + 1 == k;  // the pair's last node ends in this word
-      uint64_t tw_next = 0;
-      if (!last) tw_next = a.tok64[(nodebase + tp + 4) / 4];  // this node's next word, or the next node's first
-      // all four tokens' slot reads issued together (clamped addresses), masked afterwards
-      const uint32_t* slot32 = reinterpret_cast<const uint32_t*>(slot);
-      uint64_t pv[4][4], ph[4][4];
+        for (uint32_t sl = 0; sl < S; ++sl)
+          if (sl == sl_of) mi = smol[sl];
+        return make_uint2(mi.n, mi.s);
+      };
+      const int st = walk_step<NW>(P, ns, a.d.nl, a.tok64, R, pw, a.arena, a.arena_cap);
+      if (st != kWalkOn) {
+        uint32_t m = 0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint32_t r = min(tok[j] & 7u, R - 1), l = (tok[j] >> 3) & 15u;
-        if (NW == 1) {
-          const uint4 v = *reinterpret_cast<const uint4*>(slot32 + r * 64 + l * 4);
-          pv[j][0] = v.x;
-          pv[j][1] = v.y;
-          pv[j][2] = v.z;
-          pv[j][3] = v.w;
+        for (uint32_t sl = 0; sl < S; ++sl)
+          if (sl == sl_of) m = smol[sl].m;
+        if (st == kWalkSurvived) {  // every query node has a candidate: the pair survives GMCR
+          a.pair_off[static_cast<uint64_t>(m) * Gq + qg] = static_cast<uint32_t>(pw.base);
+          ac.lc_next = pw.base + pair_words(pw.k, stride);
         } else {
-#pragma unroll
-          for (int b = 0; b < 4; ++b) pv[j][b] = slot[r * 64 + l * 4 + b];
-          if (NW == 4) {
-#pragma unroll
-            for (int b = 0; b < 4; ++b) ph[j][b] = slot[R * 64 + r * 64 + l * 4 + b];
-          }
+          atomicAnd(&a.surv[static_cast<uint64_t>(m) * a.W + (qg >> 6)], ~(1ull << (qg & 63)));
         }
-      }
-      const uint32_t ql = tok[0];
-      const uint32_t lm_at = NW == 1 ? 2 * R * 32 : (NW == 4 ? 2 * R * 64 : R * 64);  // u32 index for NW == 1
-      uint64_t lab_lo = NW == 1 ? slot32[lm_at + (ql & 15u)] : slot[lm_at + (ql & 15u)];
-      const uint64_t lab_hi_w = NW == 4 ? slot[lm_at + 16 + (ql & 15u)] : 0ull;
-      if (tp == 0) {  // the node's label token: the filter's label test (isomorphism.hpp:49)
-        if (ql < 16u) {
-          mask.lo = static_cast<T>(lab_lo);
-          if (NW == 4) mask.hi = lab_hi_w;
-        } else {
-          MolInfo mi = {};
-#pragma unroll
-          for (uint32_t sl = 0; sl < S; ++sl)
-            if (sl == sl_of) mi = smol[sl];
-          if (ql == 255u) {
-            mask.lo = static_cast<T>(mi.n >= 64 ? ~0ull : ((1ull << mi.n) - 1));
-            if (NW == 4) mask.hi = mi.n >= 128 ? ~0ull : ((1ull << (mi.n - 64)) - 1);
-          } else {
-            uint64_t l0 = 0, l1 = 0;
-            for (uint32_t p = 0; p < mi.n; ++p)
-              if (a.d.nl[mi.s + p] == ql) {
-                if (p < 64) l0 |= 1ull << p;
-                else l1 |= 1ull << (p - 64);
-              }
-            mask.lo = static_cast<T>(l0);
-            if (NW == 4) mask.hi = l1;
-          }
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {  // count_q(l) <= count_p(l) at level r (isomorphism.hpp:103-110)
-        const bool req = static_cast<uint32_t>(j) < e && (tp != 0 || j != 0);
-        const uint32_t c = req ? (tok[j] >> 7) & 15u : 0u;  // c = 0: always true
-        mask.lo &= static_cast<T>(ge4_bits<T>(static_cast<T>(pv[j][0]), static_cast<T>(pv[j][1]), static_cast<T>(pv[j][2]),
-                                              static_cast<T>(pv[j][3]), c));
-        if (NW == 4) mask.hi &= ge4_bits<uint64_t>(ph[j][0], ph[j][1], ph[j][2], ph[j][3], c);
-      }
-      if (!mask.any()) {
         have = false;
         finished = true;
-      } else if (e < 4) {  // node done: its mask joins the register group, stored when full or last
-        const uint32_t w = i * stride;  // the node's first word in the pair's block
-#pragma unroll
-        for (uint32_t j = 0; j < 4; ++j) {
-          if (j == (w & 3u)) buf[j] = static_cast<uint64_t>(mask.lo);
-          if (NW == 4 && j == (w & 3u) + 1) buf[j] = mask.hi;
-        }
-        if ((w & 3u) + stride == 4u || i + 1 == k) {
-          const uint64_t at = base + (w & ~3u);
-          if (at + 4 <= a.arena_cap) {  // 32-byte aligned: two 16-byte stores
-            uint4* dst = reinterpret_cast<uint4*>(a.arena + at);
-            dst[0] = make_uint4(static_cast<uint32_t>(buf[0]), static_cast<uint32_t>(buf[0] >> 32), static_cast<uint32_t>(buf[1]),
-                                static_cast<uint32_t>(buf[1] >> 32));
-            dst[1] = make_uint4(static_cast<uint32_t>(buf[2]), static_cast<uint32_t>(buf[2] >> 32), static_cast<uint32_t>(buf[3]),
-                                static_cast<uint32_t>(buf[3] >> 32));
-          }
-        }
-        ++i;
-        if (i == k) {  // every query node has a candidate: the pair survives GMCR
-          uint32_t m = 0;
-#pragma unroll
-          for (uint32_t sl = 0; sl < S; ++sl)
-            if (sl == sl_of) m = smol[sl].m;
-          a.pair_off[static_cast<uint64_t>(m) * Gq + qg] = static_cast<uint32_t>(base);
-          atomicOr(&a.surv[static_cast<uint64_t>(m) * a.W + (qg >> 6)], 1ull << (qg & 63));
-          lc_next = base + pair_words(k, stride);
-          have = false;
-          finished = true;
-        } else {
-          nodebase += tp + 4;  // the next node's stream starts at the next word
-          tp = 0;
-        }
-      } else {
-        tp += 4;
       }
-      tw = tw_next;
     }
     // pairs finished this step release their slot's count
 #pragma unroll
@@ -653,7 +907,115 @@ __global__ __launch_bounds__(256) void k_pair_survive(SurviveArgs a) {
   if (lane == 0) {
     atomicAdd(&a.counters[4], static_cast<unsigned long long>(wave_steps));
     atomicAdd(&a.counters[5], static_cast<unsigned long long>(lane_steps));
-    atomicAdd(&a.counters[6], static_cast<unsigned long long>(scans));
+  }
+}
+
+struct EvalArgs {
+  GraphDev d;
+  uint32_t R, Gq, W;
+  const MolInfo* minfo;    // width-class order (all classes)
+  const uint32_t* planes32;
+  const uint64_t* planes64;
+  const uint64_t* planes_hi;
+  const uint64_t* lab_hi;
+  const uint64_t* tok64;
+  const QueryInfo* qinfo;
+  const uint64_t* list;    // pre-tested pairs, c | g << 32
+  const uint32_t* poff;    // list offset of each molecule (width-class order)
+  uint64_t list_cap;
+  uint32_t first, count;   // the width class's molecules [first, first + count)
+  uint32_t chunk;          // list entries a wavefront claims at a time
+  uint64_t* arena;
+  uint64_t arena_cap;
+  unsigned long long* surv;
+  uint32_t* pair_off;
+  unsigned long long* counters;  // [0] list cursor, [1] arena top, [3] overflow, [4..5] statistics
+};
+
+// Persistent pair evaluation of one width class, pair by pair: lanes take the class's entries of
+// the pre-tested pair list as they become idle and read the molecule's planes from device memory
+// (neighbouring entries mostly share a molecule, so the reads share cache lines).  No molecule is
+// staged, so every lane has a pair however few pairs a molecule has (small query batches).
+// Survivors and deaths are recorded as in k_pair_survive.
+template <int NW>
+__global__ __launch_bounds__(256) void k_pair_eval(EvalArgs a) {
+  constexpr uint32_t stride = NW == 4 ? 2u : 1u;
+  const uint32_t lane = static_cast<uint32_t>(lane_id());
+  const uint32_t R = a.R, Gq = a.Gq;
+  const uint32_t seg = token_segment(R);
+  const uint64_t below = (1ull << lane) - 1;
+  const uint64_t lo = a.poff[a.first];
+  const uint64_t hi = min(static_cast<uint64_t>(a.poff[a.first + a.count]), a.list_cap);
+  ArenaCursor ac;
+  uint64_t next = 0, end = 0;  // claimed entries [next, end) (wave-uniform)
+  bool exhausted = lo >= hi;
+  uint64_t wave_steps = 0, lane_steps = 0;
+
+  bool have = false;
+  uint32_t qg = 0, m = 0, n = 0, s = 0;
+  GlobalPlanes<NW> P{};
+  PairWalk<NW> pw;
+  while (true) {
+    uint64_t need = __ballot(!have);
+    while (need && !exhausted) {
+      if (next >= end) {
+        unsigned long long b = 0;
+        if (lane == 0) b = atomicAdd(&a.counters[0], static_cast<unsigned long long>(a.chunk));
+        next = lo + uniform64(__shfl(b, 0));
+        end = next + a.chunk < hi ? next + a.chunk : hi;
+        if (next >= hi) {
+          exhausted = true;
+          break;
+        }
+      }
+      const uint32_t cnt = min(static_cast<uint32_t>(end - next), static_cast<uint32_t>(__popcll(need)));
+      const uint32_t rn = __popcll(need & below);
+      const bool take = ((need >> lane) & 1ull) && rn < cnt;
+      if (take) {
+        const uint64_t e = a.list[next + rn];
+        const MolInfo mi = a.minfo[static_cast<uint32_t>(e)];
+        qg = static_cast<uint32_t>(e >> 32);
+        const QueryInfo qi = a.qinfo[qg];
+        m = mi.m;
+        n = mi.n;
+        s = mi.s;
+        P.p32 = a.planes32 + static_cast<uint64_t>(mi.x) * R * 64;
+        P.p64 = a.planes64 + static_cast<uint64_t>(mi.x) * R * 64;
+        P.phi = a.planes_hi + static_cast<uint64_t>(NW == 4 ? mi.wi : 0u) * R * 64;
+        P.lm = a.d.lmask + static_cast<uint64_t>(mi.m) * 16;
+        P.lmh = a.lab_hi + static_cast<uint64_t>(NW == 4 ? mi.wi : 0u) * 16;
+        pw.k = qi.k;
+        pw.tw = qi.first;
+        pw.nodebase = static_cast<uint64_t>(qi.qoff) * seg;
+        pw.i = 0;
+        pw.tp = 0;
+        have = true;
+      }
+      next += cnt;
+      need &= ~__ballot(take);
+      ac.reserve(take, pair_words(pw.k, stride), below, a.counters, a.arena_cap);
+      if (take) pw.base = ac.lc_next;
+    }
+    const uint64_t active = __ballot(have);
+    if (active == 0) break;
+    ++wave_steps;
+    lane_steps += __popcll(active);
+    if (have) {
+      auto ns = [&]() { return make_uint2(n, s); };
+      const int st = walk_step<NW>(P, ns, a.d.nl, a.tok64, R, pw, a.arena, a.arena_cap);
+      if (st == kWalkSurvived) {
+        a.pair_off[static_cast<uint64_t>(m) * Gq + qg] = static_cast<uint32_t>(pw.base);
+        ac.lc_next = pw.base + pair_words(pw.k, stride);
+        have = false;
+      } else if (st == kWalkDied) {
+        atomicAnd(&a.surv[static_cast<uint64_t>(m) * a.W + (qg >> 6)], ~(1ull << (qg & 63)));
+        have = false;
+      }
+    }
+  }
+  if (lane == 0) {
+    atomicAdd(&a.counters[4], static_cast<unsigned long long>(wave_steps));
+    atomicAdd(&a.counters[5], static_cast<unsigned long long>(lane_steps));
   }
 }
 
@@ -808,15 +1170,38 @@ __global__ __launch_bounds__(256) void k_request_stats(uint64_t P, const uint32_
 }  // namespace
 
 hipError_t db_prepare(hipStream_t s, const GraphDev& g, uint32_t R, const uint32_t* cls_list, const uint32_t* pidx, const uint32_t* wide_idx,
-                      uint32_t* planes32, uint64_t* planes64, uint64_t* planes_hi, uint64_t* lab_hi, uint64_t* dmax, MolInfo* minfo) {
+                      uint32_t* planes32, uint64_t* planes64, uint64_t* planes_hi, uint64_t* lab_hi, uint64_t* dmax, MolInfo* minfo,
+                      const ClassLabels& cl, uint64_t* cmax) {
   if (!g.G) return hipSuccess;
-  k_db_prepare<<<(g.G + 3) / 4, 256, 0, s>>>(g, R, cls_list, pidx, wide_idx, planes32, planes64, planes_hi, lab_hi, dmax, minfo);
+  k_db_prepare<<<(g.G + 3) / 4, 256, 0, s>>>(g, R, cls_list, pidx, wide_idx, planes32, planes64, planes_hi, lab_hi, dmax, minfo, cl, cmax);
   return hipGetLastError();
 }
 
-hipError_t query_prepare(hipStream_t s, const GraphDev& q, uint32_t R, uint16_t* tokens, QueryInfo* qinfo, uint64_t* qmax) {
+hipError_t query_prepare(hipStream_t s, const GraphDev& q, uint32_t R, uint16_t* tokens, QueryInfo* qinfo, uint64_t* qmax, const ClassLabels& cl,
+                         uint64_t* qcmax) {
   if (!q.G) return hipSuccess;
   k_query_prepare<<<(q.G + 3) / 4, 256, 0, s>>>(q, R, tokens, qinfo, qmax);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_query_classes<<<(q.G + 255) / 256, 256, 0, s>>>(q.G, R, tokens, cl, qinfo, qcmax);
+  return hipGetLastError();
+}
+
+// Node-label histogram (256 bins) of a graph batch.
+__global__ __launch_bounds__(256) void k_label_hist(const uint8_t* __restrict__ nl, uint64_t N, unsigned long long* __restrict__ hist) {
+  __shared__ unsigned int bins[256];
+  bins[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint64_t v = blockIdx.x * 256ull + threadIdx.x; v < N; v += static_cast<uint64_t>(gridDim.x) * 256) atomicAdd(&bins[nl[v]], 1u);
+  __syncthreads();
+  if (bins[threadIdx.x]) atomicAdd(&hist[threadIdx.x], static_cast<unsigned long long>(bins[threadIdx.x]));
+}
+
+hipError_t label_hist(hipStream_t s, const uint8_t* nl, uint64_t N, unsigned long long* hist) {
+  hipError_t e = fill_u64(s, reinterpret_cast<uint64_t*>(hist), 256, 0);
+  if (e != hipSuccess || !N) return e;
+  const uint64_t blocks = std::min<uint64_t>(1024, (N + 255) / 256);
+  k_label_hist<<<static_cast<unsigned>(blocks), 256, 0, s>>>(nl, N, hist);
   return hipGetLastError();
 }
 
@@ -890,6 +1275,10 @@ using namespace sigmo_internal;
 
 // Pair spaces up to this size run with one host synchronisation (sigmo_batch_run).
 constexpr uint64_t kOneSyncPairs = 1ull << 25;
+// Query graphs per batch below which the pair evaluation runs pair by pair from the list: with
+// fewer, a molecule has too few pre-tested pairs to keep a wavefront's lanes busy from the few
+// molecules its LDS ring holds.
+constexpr uint32_t kListModeGraphs = 512;
 
 struct sigmo_batch;
 namespace sigmo_internal {
@@ -907,9 +1296,11 @@ struct sigmo_db {
   uint32_t* wide_idx = nullptr;
   uint32_t* planes32 = nullptr;
   uint64_t *planes64 = nullptr, *planes_hi = nullptr, *lab_hi = nullptr, *dmax = nullptr;  // dmax in width-class order
+  uint64_t* cmax = nullptr;                                                                // per-class maxima, [(k*R + r)*G + c]
   sigmo_k::MolInfo* minfo = nullptr;                                                       // width-class order
+  sigmo_k::ClassLabels cls{{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}};                        // the pre-test's class labels
   uint64_t bytes = 0;
-  uint64_t cap_G = 0, cap_32 = 0, cap_64 = 0, cap_hi = 0, cap_dmax = 0, cap_lab = 0;  // allocated elements
+  uint64_t cap_G = 0, cap_32 = 0, cap_64 = 0, cap_hi = 0, cap_dmax = 0, cap_lab = 0, cap_cmax = 0;  // allocated elements
   uint64_t generation = 0;  // bumped whenever a rebind changes the data or any buffer address
   ~sigmo_db() {
     bfree(cls_list);
@@ -920,6 +1311,7 @@ struct sigmo_db {
     bfree(planes_hi);
     bfree(lab_hi);
     bfree(dmax);
+    bfree(cmax);
     bfree(minfo);
   }
 };
@@ -931,7 +1323,13 @@ struct sigmo_batch {
   uint16_t* tokens = nullptr;
   sigmo_k::QueryInfo* qinfo = nullptr;
   uint64_t* qmax = nullptr;
-  uint64_t cap_tokens = 0, cap_qinfo = 0, cap_qmax = 0;
+  uint64_t* qcmax = nullptr;  // per-class maxima, [(g*3 + k)*R + r]
+  uint64_t cap_tokens = 0, cap_qinfo = 0, cap_qmax = 0, cap_qcmax = 0;
+  // pre-test: passing pairs per molecule (width-class order), their scan and the pair list
+  uint32_t *pcnt = nullptr, *poff = nullptr;
+  uint64_t cap_pcnt = 0, cap_poff = 0;
+  uint64_t* list = nullptr;
+  uint64_t cap_list = 0;
   // pair evaluation
   uint64_t* arena = nullptr;
   uint64_t cap_arena = 0;
@@ -990,6 +1388,10 @@ struct sigmo_batch {
     bfree(tokens);
     bfree(qinfo);
     bfree(qmax);
+    bfree(qcmax);
+    bfree(pcnt);
+    bfree(poff);
+    bfree(list);
     bfree(arena);
     bfree(surv);
     bfree(pair_off);
@@ -1019,6 +1421,29 @@ struct sigmo_batch {
 sigmo_ctx* sigmo_internal::batch_ctx(const sigmo_batch* b) { return b->ctx; }
 
 namespace {
+// The pre-test's class labels: the three most frequent node labels < 16 of the data (ties to the
+// smaller label), unused slots where fewer occur.  Any choice gives the same results; frequent
+// labels prune most.
+int choose_classes(sigmo_db* db, const sigmo_graph* data, hipStream_t s) {
+  const sigmo_k::GraphDev g = graph_dev(data);
+  unsigned long long* hist = nullptr;
+  BATCH_TRY(balloc(&hist, 256));
+  std::vector<unsigned long long> h(256, 0);
+  hipError_t e = sigmo_k::label_hist(s, g.nl, g.N, hist);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), hist, 256 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  bfree(hist);
+  if (e != hipSuccess) return sigmo_err::set(SIGMO_ERR_HIP, std::string("label histogram: ") + hipGetErrorString(e));
+  for (uint32_t k = 0; k < sigmo_k::kClasses; ++k) {
+    uint32_t best = 0xFFFFFFFFu;
+    for (uint32_t l = 0; l < 16; ++l)
+      if (h[l] && (best == 0xFFFFFFFFu || h[l] > h[best])) best = l;
+    db->cls.l[k] = best;
+    if (best != 0xFFFFFFFFu) h[best] = 0;
+  }
+  return SIGMO_OK;
+}
+
 // (Re)build the database's class lists and level planes for `data`; buffers are reused when
 // the new graph fits them (a streaming chunk), grown otherwise.
 int db_fill(sigmo_db* db, const sigmo_graph* data) {
@@ -1051,6 +1476,7 @@ int db_fill(sigmo_db* db, const sigmo_graph* data) {
   BATCH_TRY(bgrow(&db->minfo, cap_g, G));
   db->cap_G = cap_g;
   BATCH_TRY(bgrow(&db->dmax, db->cap_dmax, uint64_t(G) * R));
+  BATCH_TRY(bgrow(&db->cmax, db->cap_cmax, uint64_t(G) * R * sigmo_k::kClasses));
   BATCH_TRY(bgrow_slack(&db->planes32, db->cap_32, uint64_t(n32) * R * 64));
   BATCH_TRY(bgrow_slack(&db->planes64, db->cap_64, uint64_t(n64) * R * 64));
   BATCH_TRY(bgrow_slack(&db->planes_hi, db->cap_hi, uint64_t(nw) * R * 64));
@@ -1060,7 +1486,8 @@ int db_fill(sigmo_db* db, const sigmo_graph* data) {
   db->num_wide = nw;
   for (int c = 0; c < 3; ++c) db->cls_count[c] = static_cast<uint32_t>(lists[c].size());
   db->bytes = G * (12ull + sizeof(sigmo_k::MolInfo)) + (uint64_t(n32) * 256 + (uint64_t(n64) + nw) * 512) * R + nw * 128ull +
-              uint64_t(G) * R * 8;
+              uint64_t(G) * R * 8 * (1 + sigmo_k::kClasses);
+  BATCH_TRY(choose_classes(db, data, s));
   BATCH_HIP_TRY(hipMemcpyAsync(db->wide_idx, widx.data(), G * 4ull, hipMemcpyHostToDevice, s));
   BATCH_HIP_TRY(hipMemcpyAsync(db->pidx, pidx.data(), G * 4ull, hipMemcpyHostToDevice, s));
   BATCH_HIP_TRY(hipMemcpyAsync(db->cls_list, cls.data(), G * 4ull, hipMemcpyHostToDevice, s));
@@ -1113,7 +1540,7 @@ int sigmo_db_refresh(sigmo_db* db) {
   void* tok = nullptr;
   BATCH_TRY(ctx_timed(db->ctx, SIGMO_PHASE_SIG_DATA, SIGMO_KERNEL_DB_PREPARE, bytes, true, &tok));
   BATCH_HIP_TRY(sigmo_k::db_prepare(s, g, db->R, db->cls_list, db->pidx, db->wide_idx, db->planes32, db->planes64, db->planes_hi, db->lab_hi,
-                                    db->dmax, db->minfo));
+                                    db->dmax, db->minfo, db->cls, db->cmax));
   BATCH_TRY(ctx_timed(db->ctx, 0, 0, 0.0, false, &tok));
   return SIGMO_OK;
 }
@@ -1143,7 +1570,7 @@ int sigmo_batch_create(sigmo_ctx* ctx, const sigmo_db* db, sigmo_batch** out) {
   auto* b = new sigmo_batch();
   b->ctx = ctx;
   b->db = db;
-  int stt = balloc(&b->counters, 8);
+  int stt = balloc(&b->counters, 16);
   if (stt == SIGMO_OK) stt = balloc(&b->total, 1);
   if (stt != SIGMO_OK) {
     delete b;
@@ -1165,13 +1592,17 @@ int sigmo_batch_workspace_bytes(const sigmo_batch* b, uint32_t num_query_graphs,
   // for the pairs that survive otherwise, priced at the arena's guess of 1/8 of the pairs (plus
   // 4 B per pair for the list of wide-molecule pairs).  A run whose survivors exceed the guess
   // grows the arena and those arrays to the measured demand; a batch keeps its buffers at their
-  // largest size, so a caller admitting by this figure should leave room for that growth.
+  // largest size, so a caller admitting by this figure should leave room for that growth.  In
+  // list mode (below 512 query graphs by default) the pre-tested pair list takes 8 B per listed
+  // pair, priced at its first sizing (a quarter of the pairs; every pair on the one-sync path).
   const uint64_t guess = std::min<uint64_t>(std::max<uint64_t>(pairs / 8, 1 << 16), 1ull << 26);
   const uint64_t arena_words = static_cast<uint64_t>(double(guess) * (mean_k + 1.0) * (db->num_wide ? 2.0 : 1.0));
   const bool one_sync = !db->num_wide && pairs && pairs <= kOneSyncPairs;
   const uint64_t gmcr_pairs = one_sync ? pairs : guess;
-  *bytes = query_nodes * sigmo_k::token_segment(static_cast<uint32_t>(R)) * 2 + Gq * (sizeof(sigmo_k::QueryInfo) + 8 * R) + Gd * W * 8 +
-           pairs * 4 + arena_words * 8 + gmcr_pairs * 16 + (db->num_wide ? guess * 4 : 0) + (Gd + 1) * 8;
+  const bool list_mode = b->tune.filter_mode == 2 || (b->tune.filter_mode == 0 && Gq < kListModeGraphs);
+  const uint64_t list = !list_mode ? 0 : (one_sync ? pairs : std::min<uint64_t>(pairs, pairs / 4 + 65536));
+  *bytes = query_nodes * sigmo_k::token_segment(static_cast<uint32_t>(R)) * 2 + Gq * (sizeof(sigmo_k::QueryInfo) + 8 * R * (1 + sigmo_k::kClasses)) +
+           Gd * W * 8 + pairs * 4 + arena_words * 8 + gmcr_pairs * 16 + (db->num_wide ? guess * 4 : 0) + (Gd + 1) * 16 + list * 8;
   return SIGMO_OK;
 }
 
@@ -1228,9 +1659,11 @@ struct RunShape {
   uint32_t R = 0, Gq = 0, Gd = 0, W = 0;
   uint64_t pairs = 0, ksum = 0;
   int cus = 0;
+  bool list_mode = false;  // pair evaluation from the pre-tested pair list (k_pair_eval)
   uint32_t lds_bytes[3] = {0, 0, 0};
   uint64_t cls_blocks[3] = {0, 0, 0}, waves_total = 0;
 };
+
 
 int shape_run(const sigmo_batch* b, const sigmo_graph* query, int cus, RunShape& L) {
   const sigmo_db* db = b->db;
@@ -1249,13 +1682,28 @@ int shape_run(const sigmo_batch* b, const sigmo_graph* query, int cus, RunShape&
   L.R = db->R;
   L.W = (L.Gq + 63) / 64;
   L.pairs = uint64_t(L.Gq) * L.Gd;
+  L.list_mode = b->tune.filter_mode == 2 || (b->tune.filter_mode == 0 && L.Gq < kListModeGraphs);
   // one persistent launch per width class, as many 4-wave blocks as registers and LDS keep
   // resident (or the tuned count, capped by residency)
   const void* kern[3] = {reinterpret_cast<const void*>(sigmo_k::k_pair_survive<1>), reinterpret_cast<const void*>(sigmo_k::k_pair_survive<2>),
                          reinterpret_cast<const void*>(sigmo_k::k_pair_survive<4>)};
-  for (int c = 0; c < 3; ++c) L.lds_bytes[c] = sigmo_k::survive_lds_bytes(L.R, c == 0 ? 1 : (c == 1 ? 2 : 4));
+  const void* kern_list[3] = {reinterpret_cast<const void*>(sigmo_k::k_pair_eval<1>), reinterpret_cast<const void*>(sigmo_k::k_pair_eval<2>),
+                              reinterpret_cast<const void*>(sigmo_k::k_pair_eval<4>)};
+  for (int c = 0; c < 3; ++c) L.lds_bytes[c] = L.list_mode ? 0u : sigmo_k::survive_lds_bytes(L.R, c == 0 ? 1 : (c == 1 ? 2 : 4));
   for (int c = 0; c < 3; ++c) {
     if (!db->cls_count[c]) continue;
+    if (L.list_mode) {
+      int resident = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern_list[c], 256, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        resident = 4;
+      }
+      uint64_t per_cu = std::max<uint64_t>(1, std::min<uint64_t>(8, static_cast<uint64_t>(resident)));
+      if (b->tune.filter_blocks_per_cu) per_cu = std::min<uint64_t>(per_cu, b->tune.filter_blocks_per_cu);
+      L.cls_blocks[c] = uint64_t(cus) * per_cu;
+      L.waves_total += L.cls_blocks[c] * sigmo_k::kSurviveWaves;
+      continue;
+    }
     if (L.lds_bytes[c] > 64 * 1024) return sigmo_err::set(SIGMO_ERR_UNSUPPORTED, "molecule slots exceed 64 KB of LDS per block");
     int resident = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern[c], 256, L.lds_bytes[c]) != hipSuccess) {
@@ -1270,44 +1718,100 @@ int shape_run(const sigmo_batch* b, const sigmo_graph* query, int cus, RunShape&
   return SIGMO_OK;
 }
 
-// Pair evaluation: counters and survival bits cleared by kernels, one launch per width class.
+// Pair evaluation: the pre-test over every pair (its bits in surv), the scan of its per-molecule
+// counts (total in counters[7]) and, per width class, the evaluation of the passing pairs --
+// molecule by molecule from LDS-staged planes (k_pair_survive) or, in list mode, pair by pair
+// from the pre-tested pair list (k_pair_expand, then k_pair_eval).  Counters are cleared by a
+// kernel.  Deaths clear their pre-test bit, so surv ends as the GMCR survival bitmask.
 hipError_t launch_filter(sigmo_batch* b, const RunShape& L, hipStream_t s) {
   const sigmo_db* db = b->db;
-  hipError_t e = sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(b->counters), 8, 0);
-  if (e == hipSuccess) e = sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(b->surv), uint64_t(L.Gd) * L.W, 0);
-  sigmo_k::SurviveArgs a;
-  a.q = L.qd;
-  a.d = L.dd;
-  a.R = L.R;
-  a.Gq = L.Gq;
-  a.W = L.W;
-  a.planes32 = db->planes32;
-  a.planes64 = db->planes64;
-  a.planes_hi = db->planes_hi;
-  a.lab_hi = db->lab_hi;
-  a.tok64 = reinterpret_cast<const uint64_t*>(b->tokens);
-  a.qinfo = b->qinfo;
-  a.qmax = b->qmax;
-  a.arena = b->arena;
-  a.arena_cap = b->cap_arena;
-  a.surv = b->surv;
-  a.pair_off = b->pair_off;
-  a.counters = b->counters;
+  hipError_t e = sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(b->counters), 16, 0);
+  if (e != hipSuccess || !L.Gd || !L.Gq) return e;
+  sigmo_k::PretestArgs p;
+  p.R = L.R;
+  p.Gq = L.Gq;
+  p.W = L.W;
+  p.G = L.Gd;
+  p.minfo = db->minfo;
+  p.dmax = db->dmax;
+  p.cmax = db->cmax;
+  p.qinfo = b->qinfo;
+  p.qmax = b->qmax;
+  p.qcmax = b->qcmax;
+  p.surv = b->surv;
+  p.pcnt = b->pcnt;
+  hipLaunchKernelGGL(sigmo_k::k_pair_pretest, dim3((L.Gd + 255) / 256), dim3(256), 0, s, p);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = sigmo_k::exclusive_scan(s, b->pcnt, L.Gd, b->poff, b->scan_tmp, reinterpret_cast<uint64_t*>(b->counters + 7));
+  if (e == hipSuccess && L.list_mode) {
+    hipLaunchKernelGGL(sigmo_k::k_pair_expand, dim3((L.Gd + 255) / 256), dim3(256), 0, s, L.Gd, L.W, db->minfo, b->surv, b->poff, b->list,
+                       b->cap_list, b->counters);
+    e = hipGetLastError();
+  }
   uint64_t first = 0;
   for (int c = 0; c < 3 && e == hipSuccess; ++c) {
-    a.minfo = db->minfo + first;
-    a.dmax = db->dmax + first * L.R;
-    a.count = db->cls_count[c];
-    first += db->cls_count[c];
-    if (!a.count || !L.Gq) continue;
-    const uint64_t auto_claim = std::max<uint64_t>(1, std::min<uint64_t>(16, a.count / (L.cls_blocks[c] * sigmo_k::kSurviveWaves * 4)));
-    a.claim = static_cast<uint32_t>(b->tune.filter_claim ? std::min<uint32_t>(b->tune.filter_claim, 64) : auto_claim);
-    e = sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(b->counters), 1, 0);  // each launch starts its own molecule cursor
+    const uint32_t count = db->cls_count[c];
+    const uint64_t at = first;
+    first += count;
+    if (!count) continue;
+    const int nw = c == 0 ? 1 : (c == 1 ? 2 : 4);
+    e = sigmo_k::fill_u64(s, reinterpret_cast<uint64_t*>(b->counters), 1, 0);  // each launch starts its own cursor
     if (e != hipSuccess) break;
     const dim3 grid(static_cast<unsigned>(L.cls_blocks[c]));
-    if (c == 0) hipLaunchKernelGGL(sigmo_k::k_pair_survive<1>, grid, dim3(256), L.lds_bytes[0], s, a);
-    else if (c == 1) hipLaunchKernelGGL(sigmo_k::k_pair_survive<2>, grid, dim3(256), L.lds_bytes[1], s, a);
-    else hipLaunchKernelGGL(sigmo_k::k_pair_survive<4>, grid, dim3(256), L.lds_bytes[2], s, a);
+    if (L.list_mode) {
+      sigmo_k::EvalArgs a;
+      a.d = L.dd;
+      a.R = L.R;
+      a.Gq = L.Gq;
+      a.W = L.W;
+      a.minfo = db->minfo;
+      a.planes32 = db->planes32;
+      a.planes64 = db->planes64;
+      a.planes_hi = db->planes_hi;
+      a.lab_hi = db->lab_hi;
+      a.tok64 = reinterpret_cast<const uint64_t*>(b->tokens);
+      a.qinfo = b->qinfo;
+      a.list = b->list;
+      a.poff = b->poff;
+      a.list_cap = b->cap_list;
+      a.first = static_cast<uint32_t>(at);
+      a.count = count;
+      a.chunk = 64u * (b->tune.filter_claim ? b->tune.filter_claim : 2u);
+      a.arena = b->arena;
+      a.arena_cap = b->cap_arena;
+      a.surv = b->surv;
+      a.pair_off = b->pair_off;
+      a.counters = b->counters;
+      if (nw == 1) hipLaunchKernelGGL(sigmo_k::k_pair_eval<1>, grid, dim3(256), 0, s, a);
+      else if (nw == 2) hipLaunchKernelGGL(sigmo_k::k_pair_eval<2>, grid, dim3(256), 0, s, a);
+      else hipLaunchKernelGGL(sigmo_k::k_pair_eval<4>, grid, dim3(256), 0, s, a);
+    } else {
+      sigmo_k::SurviveArgs a;
+      a.q = L.qd;
+      a.d = L.dd;
+      a.R = L.R;
+      a.Gq = L.Gq;
+      a.W = L.W;
+      a.count = count;
+      a.minfo = db->minfo + at;
+      a.pcnt = b->pcnt + at;
+      a.planes32 = db->planes32;
+      a.planes64 = db->planes64;
+      a.planes_hi = db->planes_hi;
+      a.lab_hi = db->lab_hi;
+      a.tok64 = reinterpret_cast<const uint64_t*>(b->tokens);
+      a.qinfo = b->qinfo;
+      a.arena = b->arena;
+      a.arena_cap = b->cap_arena;
+      a.surv = b->surv;
+      a.pair_off = b->pair_off;
+      a.counters = b->counters;
+      const uint64_t auto_claim = std::max<uint64_t>(1, std::min<uint64_t>(16, count / (L.cls_blocks[c] * sigmo_k::kSurviveWaves * 4)));
+      a.claim = static_cast<uint32_t>(b->tune.filter_claim ? std::min<uint32_t>(b->tune.filter_claim, 64) : auto_claim);
+      if (nw == 1) hipLaunchKernelGGL(sigmo_k::k_pair_survive<1>, grid, dim3(256), L.lds_bytes[0], s, a);
+      else if (nw == 2) hipLaunchKernelGGL(sigmo_k::k_pair_survive<2>, grid, dim3(256), L.lds_bytes[1], s, a);
+      else hipLaunchKernelGGL(sigmo_k::k_pair_survive<4>, grid, dim3(256), L.lds_bytes[2], s, a);
+    }
     e = hipGetLastError();
   }
   return e;
@@ -1373,6 +1877,9 @@ int size_fixed_buffers(sigmo_batch* b, const RunShape& L) {
   BATCH_TRY(b->grow(&b->tokens, b->cap_tokens, L.qd.N * seg));
   BATCH_TRY(b->grow(&b->qinfo, b->cap_qinfo, L.Gq));
   BATCH_TRY(b->grow(&b->qmax, b->cap_qmax, uint64_t(L.Gq) * L.R));
+  BATCH_TRY(b->grow(&b->qcmax, b->cap_qcmax, uint64_t(L.Gq) * L.R * sigmo_k::kClasses));
+  BATCH_TRY(b->grow(&b->pcnt, b->cap_pcnt, uint64_t(L.Gd) + 1));
+  BATCH_TRY(b->grow(&b->poff, b->cap_poff, uint64_t(L.Gd) + 1));
   BATCH_TRY(b->grow(&b->surv, b->cap_surv, uint64_t(L.Gd) * L.W));
   BATCH_TRY(b->grow(&b->pair_off, b->cap_pair_off, L.pairs));
   BATCH_TRY(b->grow(&b->mol_count, b->cap_mc, uint64_t(L.Gd) + 1));
@@ -1408,7 +1915,7 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
   {
     void* tok = nullptr;
     BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_SIG_QUERY, -1, 0.0, true, &tok));
-    BATCH_HIP_TRY(sigmo_k::query_prepare(s, L.qd, R, b->tokens, b->qinfo, b->qmax));
+    BATCH_HIP_TRY(sigmo_k::query_prepare(s, L.qd, R, b->tokens, b->qinfo, b->qmax, db->cls, b->qcmax));
     BATCH_TRY(ctx_timed(b->ctx, 0, 0, 0.0, false, &tok));
   }
   // arena capacity: start from 1/8 of the pairs (at most 2^26) plus every wavefront's open chunk.
@@ -1434,6 +1941,7 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
     BATCH_TRY(b->grow(&b->mask_off, b->cap_mo, bound));
     BATCH_TRY(b->grow(&b->counts, b->cap_counts, bound));
     BATCH_TRY(b->grow(&b->deep_pairs, b->cap_deep, bound));
+    if (L.list_mode) BATCH_TRY(b->grow(&b->list, b->cap_list, bound));
     const double reads = (1.0 + 8.0 * R) * double(L.dd.N) + double(L.qd.N) * (1.0 + 2.0 * 16.0 * R) + double(Gq) * (4.0 + 8.0 * R);
     void* tok = nullptr;
     BATCH_TRY(ctx_timed(b->ctx, SIGMO_PHASE_FILTER, SIGMO_KERNEL_PAIR_FILTER, reads, true, &tok));
@@ -1460,7 +1968,7 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
       ctx_add_bytes(b->ctx, SIGMO_KERNEL_PAIR_FILTER, double(cnt[1]) * 8.0 + double(Gd) * W * 8.0);
       res->filter_wave_steps = cnt[4];
       res->filter_lane_steps = cnt[5];
-      res->filter_scans = cnt[6];
+      res->filter_scans = cnt[7];
       b->P = P;
       b->Gq = Gq;
       b->ran = true;
@@ -1474,8 +1982,11 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
     want_arena = std::max<uint64_t>(want_arena + 1, cnt[1] + cnt[1] / 8 + slack_arena);  // overflowed: size and repeat
     res->arena_repeats = 1;
   }
-  for (int attempt = 0; attempt < 3; ++attempt) {
+  // the pre-tested pair list (list mode): a quarter of the pairs at first, then the measured demand
+  uint64_t want_list = L.list_mode ? std::max<uint64_t>(b->cap_list, std::min<uint64_t>(L.pairs, L.pairs / 4 + 65536)) : 0;
+  for (int attempt = 0; attempt < 4; ++attempt) {
     BATCH_TRY(b->grow(&b->arena, b->cap_arena, want_arena));
+    if (L.list_mode) BATCH_TRY(b->grow(&b->list, b->cap_list, want_list));
     // the survivors' masks, gathered in GMCR order, never take more words than the arena holds
     BATCH_TRY(b->grow(&b->gathered, b->cap_gathered, b->cap_arena));
     if (b->cap_arena >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "candidate arena exceeds 2^32 words");
@@ -1491,15 +2002,17 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
     BATCH_HIP_TRY(hipStreamSynchronize(s));
     res->filter_wave_steps = cnt[4];
     res->filter_lane_steps = cnt[5];
-    res->filter_scans = cnt[6];
+    res->filter_scans = cnt[7];
+    if (cnt[7] >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "pre-tested pairs of one run exceed 2^32");
     // writes: candidate masks, the survival bits and arena offsets
     ctx_add_bytes(b->ctx, SIGMO_KERNEL_PAIR_FILTER, double(cnt[1]) * 8.0 + double(Gd) * W * 8.0);
     if (!cnt[3]) break;
-    if (attempt == 2) return sigmo_err::set(SIGMO_ERR_NOMEM, "candidate arena overflowed repeatedly");
+    if (attempt == 3) return sigmo_err::set(SIGMO_ERR_NOMEM, "candidate arena or pair list overflowed repeatedly");
     res->arena_repeats++;
     // replay with the demand just measured; the claim order (and so the space left unused at
-    // sub-chunk ends) varies between runs, hence the 1/8 margin
-    want_arena = std::max<uint64_t>(want_arena + 1, cnt[1] + cnt[1] / 8 + slack_arena);
+    // sub-chunk ends) varies between runs, hence the 1/8 margin.  The list's demand is exact.
+    if (L.list_mode && cnt[7] > b->cap_list) want_list = cnt[7];
+    if (cnt[1] > b->cap_arena) want_arena = std::max<uint64_t>(want_arena + 1, cnt[1] + cnt[1] / 8 + slack_arena);
   }
   // canonical GMCR: count survivors per molecule, scan, scatter in (molecule, query graph) order
   void* tok = nullptr;
@@ -1547,6 +2060,7 @@ int sigmo_batch_set_tuning(sigmo_batch* b, const sigmo_batch_tuning* t) {
   if (!b || !t) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
   if (t->join_blocks_per_cu > 4) return sigmo_err::set(SIGMO_ERR_INVALID, "join_blocks_per_cu must be 0..4 (LDS keeps 4 resident)");
   if (t->filter_claim > 64) return sigmo_err::set(SIGMO_ERR_INVALID, "filter_claim must be 0..64");
+  if (t->filter_mode > 2) return sigmo_err::set(SIGMO_ERR_INVALID, "filter_mode must be 0 (automatic), 1 (molecule) or 2 (list)");
   b->tune = *t;
   return SIGMO_OK;
 }
@@ -1577,7 +2091,8 @@ int sigmo_batch_autotune(sigmo_batch* b, const sigmo_graph* query, int find_firs
   sigmo_batch_tuning best_t{};
   double best = 0;
   BATCH_TRY(time_with(best_t, &best));
-  // coordinate search: pair evaluation shape, then the join's
+  // coordinate search: pair evaluation mode and shape, then the join's
+  const uint32_t filter_modes[] = {1, 2};
   const uint32_t filter_blocks[] = {1, 2, 4, 0};
   const uint32_t filter_claims[] = {2, 8, 32, 0};
   const uint32_t join_blocks[] = {2, 3, 4};
@@ -1596,6 +2111,7 @@ int sigmo_batch_autotune(sigmo_batch* b, const sigmo_graph* query, int find_firs
     }
     return SIGMO_OK;
   };
+  BATCH_TRY(sweep(&sigmo_batch_tuning::filter_mode, filter_modes, 2));
   BATCH_TRY(sweep(&sigmo_batch_tuning::filter_blocks_per_cu, filter_blocks, 4));
   BATCH_TRY(sweep(&sigmo_batch_tuning::filter_claim, filter_claims, 4));
   BATCH_TRY(sweep(&sigmo_batch_tuning::join_blocks_per_cu, join_blocks, 3));
@@ -1625,8 +2141,8 @@ int sigmo_batch_capture(sigmo_batch* b, const sigmo_graph* query, int find_first
   hipError_t e = hipSuccess;
   if (refresh_db)
     e = sigmo_k::db_prepare(s, L.dd, db->R, db->cls_list, db->pidx, db->wide_idx, db->planes32, db->planes64, db->planes_hi, db->lab_hi,
-                            db->dmax, db->minfo);
-  if (e == hipSuccess) e = sigmo_k::query_prepare(s, L.qd, L.R, b->tokens, b->qinfo, b->qmax);
+                            db->dmax, db->minfo, db->cls, db->cmax);
+  if (e == hipSuccess) e = sigmo_k::query_prepare(s, L.qd, L.R, b->tokens, b->qinfo, b->qmax, db->cls, b->qcmax);
   if (e == hipSuccess) e = launch_filter(b, L, s);
   if (e == hipSuccess) e = launch_count_scan(b, L, s);
   if (e == hipSuccess) e = launch_scatter(b, L, s);
@@ -1686,7 +2202,7 @@ int sigmo_batch_replay(sigmo_batch* b, sigmo_batch_result* res) {
   res->join_wave_steps = h[9 + 7];
   res->filter_wave_steps = h[4];
   res->filter_lane_steps = h[5];
-  res->filter_scans = h[6];
+  res->filter_scans = h[7];
   return SIGMO_OK;
 }
 

@@ -33,10 +33,12 @@ def _case(name):
     raise KeyError(name)
 
 
-def _batched(ctx, qh, dh, R, modes=(True, False), db=None, batch=None):
+def _batched(ctx, qh, dh, R, modes=(True, False), db=None, batch=None, tuning=None):
     qd, dd = capi.DeviceGraph(ctx, qh), capi.DeviceGraph(ctx, dh)
     db = db or capi.Database(ctx, dd, R)
     batch = batch or capi.Batch(ctx, db)
+    if tuning is not None:
+        batch.tuning = tuning
     out = {}
     for mode in modes:
         r = batch.run(qd, find_first=mode)
@@ -60,11 +62,12 @@ def _assert_equal(g, o, modes=(True, False)):
         assert r["matched_pairs"] == int((o[f"counts_{key}"] > 0).sum())
 
 
+@pytest.mark.parametrize("mode", [1, 2], ids=["molecule_mode", "list_mode"])
 @pytest.mark.parametrize("name", ["cfg0", "cfg1_slice", "lognormal_aliasing", "reference_fixtures"])
-def test_batched_parity(ctx, name):
+def test_batched_parity(ctx, name, mode):
     qh, dh, R = _case(name)
-    print(f"batched case {name}: queries {qh.info()} data {dh.info()} R={R}", flush=True)
-    g = _batched(ctx, qh, dh, R)
+    print(f"batched case {name} filter_mode {mode}: queries {qh.info()} data {dh.info()} R={R}", flush=True)
+    g = _batched(ctx, qh, dh, R, tuning={"filter_mode": mode})
     o = O.pipeline(qh.arrays(), dh.arrays(), R, modes=[True, False])
     _assert_equal(g, o)
 
@@ -85,11 +88,11 @@ def test_batched_wildcards_overflow_and_replay(ctx):
     extra = capi.HostGraphs.molecules(4, 1500)
     qh = capi.HostGraphs.from_edge_lists(qs)
     dh = capi.HostGraphs.from_edge_lists(mols).concat(extra)
-    print("batched wildcard case", flush=True)
-    g = _batched(ctx, qh, dh, 4)
     o = O.pipeline(qh.arrays(), dh.arrays(), 4, modes=[True, False])
-    _assert_equal(g, o)
     assert len(o["indices"]) > 65536  # beyond the initial record capacity
+    for mode in (1, 2):
+        print(f"batched wildcard case, filter_mode {mode}", flush=True)
+        _assert_equal(_batched(ctx, qh, dh, 4, tuning={"filter_mode": mode}), o)
 
 
 def test_batched_one_sync_arena_overflow_is_repeated(ctx):
@@ -114,6 +117,22 @@ def test_batched_one_sync_arena_overflow_is_repeated(ctx):
     r = g["result_first"]
     assert r["one_sync"] == 1 and r["arena_repeats"] >= 1, r
     _assert_equal(g, o)
+
+
+def test_batched_more_than_1024_query_graphs(ctx):
+    """2,000 query graphs (config 3's set, k in 4..32, 30% with a ring) x 1,500 molecules of seed 3:
+    the query preparation, the per-class pre-test maxima and both evaluation modes past 1,024
+    query graphs (32 bitmask words per molecule) against the oracle."""
+    qh = capi.HostGraphs.queries(3, 2000, 4, 32, capi.SIZE_LOGNORMAL, ring_frac=0.3)
+    dh = capi.HostGraphs.molecules(3, 1500, capi.SIZE_LOGNORMAL)
+    o = O.pipeline(qh.arrays(), dh.arrays(), 6, modes=[True, False])
+    qd, dd = capi.DeviceGraph(ctx, qh), capi.DeviceGraph(ctx, dh)
+    db = capi.Database(ctx, dd, 6)
+    for mode in (1, 2):
+        batch = capi.Batch(ctx, db)
+        for rep in range(2):
+            print(f"2,000 query graphs x 1,500 molecules: filter_mode {mode}, run {rep}", flush=True)
+            _assert_equal(_batched(ctx, qh, dh, 6, db=db, batch=batch, tuning={"filter_mode": mode}), o)
 
 
 def test_batched_reuse_across_query_sets(ctx):
@@ -245,7 +264,9 @@ def test_batched_tuning_and_autotune_keep_results(ctx):
     base = batch.run(qd, find_first=False)
     ref = batch.pairs()
     for t in ({"filter_blocks_per_cu": 1, "filter_claim": 1, "join_blocks_per_cu": 1, "join_chunk": 1},
-              {"filter_blocks_per_cu": 8, "filter_claim": 64, "join_blocks_per_cu": 4, "join_chunk": 4096}):
+              {"filter_blocks_per_cu": 8, "filter_claim": 64, "join_blocks_per_cu": 4, "join_chunk": 4096},
+              {"filter_mode": 1}, {"filter_mode": 2}, {"filter_mode": 1, "filter_claim": 64},
+              {"filter_mode": 2, "filter_claim": 1, "filter_blocks_per_cu": 1}):
         print(f"tuning {t}", flush=True)
         batch.tuning = t
         r = batch.run(qd, find_first=False)
@@ -254,6 +275,9 @@ def test_batched_tuning_and_autotune_keep_results(ctx):
         assert r["num_matches"] == base["num_matches"]
     with pytest.raises(capi.SigmoError):
         batch.tuning = {"join_blocks_per_cu": 5}
+    with pytest.raises(capi.SigmoError):
+        batch.tuning = {"filter_mode": 3}
+    batch.tuning = {}
     chosen, ms = batch.autotune(qd, find_first=False)
     assert ms > 0 and batch.tuning == chosen
     batch.run(qd, find_first=False)
