@@ -325,44 +325,72 @@ __global__ __launch_bounds__(256) void k_query_prepare(GraphDev q, uint32_t R, u
 // token streams (k_query_prepare's output: each requirement token carries one nonzero count, of
 // label l at level r, of its node's signature).  qcmax[(g*3 + c)*R + r] is the nibble-wise max of
 // the level-(r+1) signatures of the graph's nodes labelled cl.l[c], and bit c of qinfo[g].qcls is
-// set when there is such a node.
-__global__ __launch_bounds__(256) void k_query_classes(uint32_t Gq, uint32_t R, const uint16_t* __restrict__ tokens, ClassLabels cl,
+// set when there is such a node.  Streams are read as 64-bit words, eight independent loads at a
+// time: a node's stream starts on a word boundary with its label token and ends with the end
+// token, after which the rest of that word is padding.
+__global__ __launch_bounds__(256) void k_query_classes(uint32_t Gq, uint32_t R, const uint64_t* __restrict__ tok64, ClassLabels cl,
                                                        QueryInfo* __restrict__ qinfo, uint64_t* __restrict__ qcmax) {
   const uint64_t g = blockIdx.x * 256ull + threadIdx.x;
   if (g >= Gq) return;
   const QueryInfo qi = qinfo[g];
-  const uint32_t seg = token_segment(R);
+  const uint64_t seg = token_segment(R);  // a multiple of 4 tokens
   uint64_t cm[kClasses][kMaxLevels];
 #pragma unroll
   for (uint32_t c = 0; c < kClasses; ++c)
 #pragma unroll
     for (uint32_t r = 0; r < kMaxLevels; ++r) cm[c][r] = 0;
-  uint32_t qcls = 0;
-  uint64_t pos = static_cast<uint64_t>(qi.qoff) * seg;
-  for (uint32_t i = 0; i < qi.k; ++i) {
-    const uint32_t label = tokens[pos];
-    uint32_t c = kClasses;
+  uint64_t nsig[kMaxLevels];  // the current node's signature per level, rebuilt from its tokens
 #pragma unroll
-    for (uint32_t cc = 0; cc < kClasses; ++cc)
-      if (c == kClasses && label == cl.l[cc]) c = cc;
-    uint64_t j = pos + 1;
-    for (uint32_t t = tokens[j]; t != kEndToken; t = tokens[++j]) {
-      const uint32_t r = t & 7u, l = (t >> 3) & 15u;
-      const uint64_t v = static_cast<uint64_t>((t >> 7) & 15u) << (4 * l);
+  for (uint32_t r = 0; r < kMaxLevels; ++r) nsig[r] = 0;
+  uint32_t qcls = 0, node = 0, c = kClasses;
+  bool at_start = true;
+  const uint64_t wend = (static_cast<uint64_t>(qi.qoff) + qi.k) * seg / 4;
+  for (uint64_t wpos = static_cast<uint64_t>(qi.qoff) * seg / 4; node < qi.k && wpos < wend; wpos += 8) {
+    uint64_t wv[8];
 #pragma unroll
-      for (uint32_t cc = 0; cc < kClasses; ++cc)
+    for (uint32_t u = 0; u < 8; ++u) wv[u] = wpos + u < wend ? tok64[wpos + u] : ~0ull;
 #pragma unroll
-        for (uint32_t rr = 0; rr < kMaxLevels; ++rr)
-          if (cc == c && rr == r) cm[cc][rr] = nib_max(cm[cc][rr], v);
+    for (uint32_t u = 0; u < 8; ++u) {
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        const uint32_t t = static_cast<uint32_t>(wv[u] >> (16 * j)) & 0xFFFFu;
+        if (node >= qi.k) break;
+        if (at_start) {  // j == 0: the node's label
+          if (j != 0) break;
+          c = kClasses;
+#pragma unroll
+          for (uint32_t cc = 0; cc < kClasses; ++cc)
+            if (c == kClasses && t == cl.l[cc]) c = cc;
+          if (c < kClasses) qcls |= 1u << c;
+          at_start = false;
+          continue;
+        }
+        if (t == kEndToken) {  // the node's signature joins its class's maxima; the rest of the word is padding
+#pragma unroll
+          for (uint32_t cc = 0; cc < kClasses; ++cc)
+            if (cc == c) {
+#pragma unroll
+              for (uint32_t rr = 0; rr < kMaxLevels; ++rr) cm[cc][rr] = nib_max(cm[cc][rr], nsig[rr]);
+            }
+#pragma unroll
+          for (uint32_t rr = 0; rr < kMaxLevels; ++rr) nsig[rr] = 0;
+          ++node;
+          at_start = true;
+          break;
+        }
+        // one token per nonzero (level, label) count of the node: OR it into place
+        const uint32_t r = t & 7u, l = (t >> 3) & 15u;
+        const uint64_t v = static_cast<uint64_t>((t >> 7) & 15u) << (4 * l);
+#pragma unroll
+        for (uint32_t rr = 0; rr < kMaxLevels; ++rr) nsig[rr] |= rr == r ? v : 0ull;
+      }
     }
-    if (c < kClasses) qcls |= 1u << c;
-    pos += (j - pos + 1 + 3) & ~3ull;  // the node's padded stream: label, requirements, end
   }
 #pragma unroll
-  for (uint32_t c = 0; c < kClasses; ++c)
+  for (uint32_t cc = 0; cc < kClasses; ++cc)
 #pragma unroll
     for (uint32_t r = 0; r < kMaxLevels; ++r)
-      if (r < R) qcmax[(g * kClasses + c) * R + r] = cm[c][r];
+      if (r < R) qcmax[(g * kClasses + cc) * R + r] = cm[cc][r];
   qinfo[g].qcls = qcls;
 }
 
@@ -388,67 +416,87 @@ struct PretestArgs {
   uint32_t* pcnt;                // [c] out: passing pairs of the molecule
 };
 
-constexpr uint32_t kPretestWords = 1 + (1 + kClasses) * kMaxLevels;  // per query graph in the LDS tile
+// Nibble-wise a <= b for 16 four-bit fields, as in nib_le_all, with the fields split into 8-bit
+// lanes beforehand: the even and odd nibbles of b with a guard bit set in each lane, those of a
+// without.  a <= b in every nibble exactly when both differences (b_even - a_even, b_odd - a_odd)
+// keep every guard bit, and the test of several words ANDs the differences and checks the guard
+// bits once.  Splitting each side once (a molecule's maxima per thread, a query graph's per tile)
+// leaves two 64-bit subtractions and two ANDs per word in the pair loop.
+constexpr uint64_t kNibLo = 0x0F0F0F0F0F0F0F0Full, kNibGuard = 0x1010101010101010ull;
+__device__ __forceinline__ uint64_t nib_split_b(uint64_t b, uint32_t part) { return ((part ? b >> 4 : b) & kNibLo) | kNibGuard; }
+__device__ __forceinline__ uint64_t nib_split_a(uint64_t a, uint32_t part) { return (part ? a >> 4 : a) & kNibLo; }
 
-// One thread per molecule (width-class order), the query graphs 64 at a time from an LDS tile
-// (every lane reads the same graph: broadcast reads).  A graph whose first test fails on all 64
-// molecules of a wavefront skips the rest.
+// One thread per molecule (width-class order) and the query graphs 64 at a time from an LDS tile
+// (every lane reads the same graph: broadcast reads), for RR levels.  Tile words of a graph:
+// [0] label set | classes << 32, then the split all-node maxima (2 words per level), then the
+// split class maxima (2 words per level and class).  A graph whose label test, or whose
+// all-node test, fails on all 64 molecules of a wavefront skips the rest.
+template <uint32_t RR>
 __global__ __launch_bounds__(256) void k_pair_pretest(PretestArgs a) {
-  __shared__ uint64_t tile[kWave * kPretestWords];
-  const uint32_t R = a.R, stride = 1 + (1 + kClasses) * R;
+  constexpr uint32_t S = 1 + 2 * (1 + kClasses) * RR;
+  __shared__ uint64_t tile[kWave * S];
   const uint64_t c = blockIdx.x * 256ull + threadIdx.x;
   const bool valid = c < a.G;
   uint32_t mlab = 0, m = 0;
-  uint64_t dm[kMaxLevels], cm[kClasses][kMaxLevels];
+  uint64_t md[2 * RR], mc[kClasses][2 * RR];  // the molecule's split maxima
   if (valid) {
     const MolInfo mi = a.minfo[c];
     mlab = mi.mlab;
     m = mi.m;
   }
 #pragma unroll
-  for (uint32_t r = 0; r < kMaxLevels; ++r) {
-    dm[r] = valid && r < R ? a.dmax[c * R + r] : 0ull;
+  for (uint32_t r = 0; r < RR; ++r) {
+    const uint64_t d = valid ? a.dmax[c * RR + r] : 0ull;
+    md[2 * r] = nib_split_b(d, 0);
+    md[2 * r + 1] = nib_split_b(d, 1);
 #pragma unroll
-    for (uint32_t k = 0; k < kClasses; ++k) cm[k][r] = valid && r < R ? a.cmax[(static_cast<uint64_t>(k) * R + r) * a.G + c] : 0ull;
+    for (uint32_t k = 0; k < kClasses; ++k) {
+      const uint64_t x = valid ? a.cmax[(static_cast<uint64_t>(k) * RR + r) * a.G + c] : 0ull;
+      mc[k][2 * r] = nib_split_b(x, 0);
+      mc[k][2 * r + 1] = nib_split_b(x, 1);
+    }
   }
   uint32_t cnt = 0;
   for (uint32_t w = 0; w < a.W; ++w) {
     __syncthreads();  // the previous tile has been read
-    for (uint32_t t = threadIdx.x; t < kWave * stride; t += 256) {
-      const uint32_t j = t / stride, f = t - j * stride;
+    for (uint32_t t = threadIdx.x; t < kWave * S; t += 256) {
+      const uint32_t j = t / S, f = t - j * S;
       const uint32_t g = w * kWave + j;
       uint64_t v = 0;
       if (g < a.Gq) {
         if (f == 0) {
           const QueryInfo qi = a.qinfo[g];
           v = qi.qlab | (static_cast<uint64_t>(qi.qcls) << 32);
-        } else if (f <= R) {
-          v = a.qmax[static_cast<uint64_t>(g) * R + f - 1];
+        } else if (f <= 2 * RR) {
+          v = nib_split_a(a.qmax[static_cast<uint64_t>(g) * RR + (f - 1) / 2], (f - 1) & 1u);
         } else {
-          v = a.qcmax[static_cast<uint64_t>(g) * kClasses * R + f - 1 - R];
+          const uint32_t i = f - 1 - 2 * RR;  // class i / (2RR), level (i % 2RR) / 2
+          v = nib_split_a(a.qcmax[(static_cast<uint64_t>(g) * kClasses + i / (2 * RR)) * RR + (i % (2 * RR)) / 2], i & 1u);
         }
       }
-      tile[j * stride + f] = v;
+      tile[j * S + f] = v;
     }
     __syncthreads();
     const uint32_t nj = min(static_cast<uint32_t>(kWave), a.Gq - w * kWave);
     uint64_t word = 0;
     for (uint32_t j = 0; j < nj; ++j) {
-      const uint64_t* e = tile + j * stride;
+      const uint64_t* e = tile + j * S;
       const uint64_t h = e[0];
-      bool fits = valid && (static_cast<uint32_t>(h) & ~mlab) == 0u;
+      const bool lab = valid && (static_cast<uint32_t>(h) & ~mlab) == 0u;
+      if (!__ballot(lab)) continue;  // wave-uniform
+      uint64_t acc = ~0ull;
 #pragma unroll
-      for (uint32_t r = 0; r < kMaxLevels; ++r)
-        if (r < R) fits = fits & nib_le_all(e[1 + r], dm[r]);
+      for (uint32_t i = 0; i < 2 * RR; ++i) acc &= md[i] - e[1 + i];
+      bool fits = lab && (acc & kNibGuard) == kNibGuard;
       const uint32_t qcls = static_cast<uint32_t>(h >> 32);
       if (qcls && __ballot(fits)) {  // wave-uniform
 #pragma unroll
         for (uint32_t k = 0; k < kClasses; ++k)
           if ((qcls >> k) & 1u) {
 #pragma unroll
-            for (uint32_t r = 0; r < kMaxLevels; ++r)
-              if (r < R) fits = fits & nib_le_all(e[1 + R + k * R + r], cm[k][r]);
+            for (uint32_t i = 0; i < 2 * RR; ++i) acc &= mc[k][i] - e[1 + 2 * RR + 2 * RR * k + i];
           }
+        fits = fits && (acc & kNibGuard) == kNibGuard;
       }
       word |= static_cast<uint64_t>(fits) << j;
     }
@@ -1125,6 +1173,61 @@ __global__ __launch_bounds__(256) void k_surv_gather(uint32_t Gd, const uint32_t
   }
 }
 
+// The two kernels above with one thread per molecule instead of one wavefront, for batches of
+// few query graphs (W <= 2 bitmask words): most molecules then have no or a few survivors, and a
+// wavefront per molecule would leave almost all of its lanes idle.
+__global__ __launch_bounds__(256) void k_surv_scatter_thread(uint32_t Gd, uint32_t Gq, uint32_t W, const unsigned long long* __restrict__ surv,
+                                                             const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ pair_off,
+                                                             uint32_t* __restrict__ indices, uint32_t* __restrict__ pair_graph,
+                                                             uint32_t* __restrict__ mask_off, uint64_t cap) {
+  const uint64_t m = blockIdx.x * 256ull + threadIdx.x;
+  if (m >= Gd) return;
+  uint64_t o = offsets[m];
+  if (offsets[m + 1] == o) return;
+  for (uint32_t w = 0; w < W; ++w) {
+    uint64_t x = surv[m * W + w];
+    while (x) {
+      const uint32_t g = w * 64 + static_cast<uint32_t>(__builtin_ctzll(x));
+      x &= x - 1;
+      if (o < cap) {
+        indices[o] = g;
+        pair_graph[o] = static_cast<uint32_t>(m);
+        mask_off[o] = pair_off[m * Gq + g];
+      }
+      ++o;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_surv_gather_thread(uint32_t Gd, const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ woff,
+                                                            const QueryInfo* __restrict__ qinfo, const uint32_t* __restrict__ dgo,
+                                                            const uint32_t* __restrict__ indices, const uint64_t* __restrict__ arena, uint64_t arena_cap,
+                                                            const unsigned long long* __restrict__ overflow, uint64_t* __restrict__ gathered,
+                                                            uint64_t gathered_cap, uint32_t* __restrict__ mask_off, uint64_t cap) {
+  const uint64_t m = blockIdx.x * 256ull + threadIdx.x;
+  if (m >= Gd) return;
+  if (*overflow) return;
+  const uint64_t r1 = min(static_cast<uint64_t>(offsets[m + 1]), cap);
+  uint64_t r = offsets[m];
+  if (r >= r1) return;
+  const uint32_t stride = dgo[m + 1] - dgo[m] > 64 ? 2u : 1u;
+  uint64_t wo = woff[m];
+  for (; r < r1; ++r) {
+    const uint32_t kk = qinfo[indices[r]].k * stride;
+    const uint64_t src = mask_off[r];
+    if (wo + kk <= gathered_cap && src + pair_words(kk, 1) <= arena_cap) {
+      const uint4* s4 = reinterpret_cast<const uint4*>(arena + src);
+      for (uint32_t t = 0; t < kk; t += 2) {
+        const uint4 v = s4[t / 2];
+        gathered[wo + t] = static_cast<uint64_t>(v.x) | (static_cast<uint64_t>(v.y) << 32);
+        if (t + 1 < kk) gathered[wo + t + 1] = static_cast<uint64_t>(v.z) | (static_cast<uint64_t>(v.w) << 32);
+      }
+    }
+    mask_off[r] = static_cast<uint32_t>(wo);
+    wo += kk;
+  }
+}
+
 // Per-request statistics of a batch run.  Request r owns the batch's query graphs
 // [rg[r], rg[r+1]); each GMCR pair (molecule m, query graph g, count c) adds its match flag,
 // c, and mix64(mix64(((g - rg[r]) << 32) | m) + c) to its request (sums mod 2^64, mix64 being
@@ -1183,7 +1286,7 @@ hipError_t query_prepare(hipStream_t s, const GraphDev& q, uint32_t R, uint16_t*
   k_query_prepare<<<(q.G + 3) / 4, 256, 0, s>>>(q, R, tokens, qinfo, qmax);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_query_classes<<<(q.G + 255) / 256, 256, 0, s>>>(q.G, R, tokens, cl, qinfo, qcmax);
+  k_query_classes<<<(q.G + 255) / 256, 256, 0, s>>>(q.G, R, reinterpret_cast<const uint64_t*>(tokens), cl, qinfo, qcmax);
   return hipGetLastError();
 }
 
@@ -1740,7 +1843,17 @@ hipError_t launch_filter(sigmo_batch* b, const RunShape& L, hipStream_t s) {
   p.qcmax = b->qcmax;
   p.surv = b->surv;
   p.pcnt = b->pcnt;
-  hipLaunchKernelGGL(sigmo_k::k_pair_pretest, dim3((L.Gd + 255) / 256), dim3(256), 0, s, p);
+  const dim3 pgrid((L.Gd + 255) / 256);
+  switch (L.R) {  // one instantiation per level count keeps the molecule's maxima in registers
+    case 1: hipLaunchKernelGGL(sigmo_k::k_pair_pretest<1>, pgrid, dim3(256), 0, s, p); break;
+    case 2: hipLaunchKernelGGL(sigmo_k::k_pair_pretest<2>, pgrid, dim3(256), 0, s, p); break;
+    case 3: hipLaunchKernelGGL(sigmo_k::k_pair_pretest<3>, pgrid, dim3(256), 0, s, p); break;
+    case 4: hipLaunchKernelGGL(sigmo_k::k_pair_pretest<4>, pgrid, dim3(256), 0, s, p); break;
+    case 5: hipLaunchKernelGGL(sigmo_k::k_pair_pretest<5>, pgrid, dim3(256), 0, s, p); break;
+    case 6: hipLaunchKernelGGL(sigmo_k::k_pair_pretest<6>, pgrid, dim3(256), 0, s, p); break;
+    case 7: hipLaunchKernelGGL(sigmo_k::k_pair_pretest<7>, pgrid, dim3(256), 0, s, p); break;
+    default: hipLaunchKernelGGL(sigmo_k::k_pair_pretest<8>, pgrid, dim3(256), 0, s, p); break;
+  }
   e = hipGetLastError();
   if (e == hipSuccess) e = sigmo_k::exclusive_scan(s, b->pcnt, L.Gd, b->poff, b->scan_tmp, reinterpret_cast<uint64_t*>(b->counters + 7));
   if (e == hipSuccess && L.list_mode) {
@@ -1832,8 +1945,18 @@ hipError_t launch_count_scan(sigmo_batch* b, const RunShape& L, hipStream_t s) {
 
 hipError_t launch_scatter(sigmo_batch* b, const RunShape& L, hipStream_t s) {
   if (!L.Gd) return hipSuccess;
-  const uint64_t blocks = (static_cast<uint64_t>(L.Gd) * sigmo_k::kWave + 255) / 256;  // a wavefront per molecule
   const uint64_t cap = std::min(b->cap_idx, std::min(b->cap_pg, b->cap_mo));
+  if (L.W <= 2) {  // few query graphs: a thread per molecule
+    const dim3 grid(static_cast<unsigned>((uint64_t(L.Gd) + 255) / 256));
+    hipLaunchKernelGGL(sigmo_k::k_surv_scatter_thread, grid, dim3(256), 0, s, L.Gd, L.Gq, L.W, b->surv, b->offsets, b->pair_off, b->indices,
+                       b->pair_graph, b->mask_off, cap);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sigmo_k::k_surv_gather_thread, grid, dim3(256), 0, s, L.Gd, b->offsets, b->word_off, b->qinfo, L.dd.go, b->indices,
+                       b->arena, b->cap_arena, b->counters + 3, b->gathered, b->cap_gathered, b->mask_off, cap);
+    return hipGetLastError();
+  }
+  const uint64_t blocks = (static_cast<uint64_t>(L.Gd) * sigmo_k::kWave + 255) / 256;  // a wavefront per molecule
   hipLaunchKernelGGL(sigmo_k::k_surv_scatter, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, L.Gd, L.Gq, L.W, b->surv, b->offsets,
                      b->pair_off, b->indices, b->pair_graph, b->mask_off, cap);
   hipError_t e = hipGetLastError();
