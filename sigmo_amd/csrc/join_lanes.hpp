@@ -112,6 +112,13 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
   bool have = false;
   uint64_t j = 0, visited = 0, count = 0;
   uint32_t s = 0, n = 0, qoff = 0, k = 0, base = 0, depth = 0, pos = 0;
+  // A shallow pass keeps its pair's candidate masks in registers, loaded together when the pair
+  // is taken: the DFS then reads no mask from memory (each of them used to be one dependent load
+  // per step, and the pairs' masks do not stay in L2 meanwhile).
+  constexpr bool kRegMasks = KMAX <= 16;
+  uint64_t mk[kRegMasks ? KMAX : 1];
+#pragma unroll
+  for (int t = 0; t < (kRegMasks ? KMAX : 1); ++t) mk[t] = 0;
   while (true) {
     uint64_t need = __ballot(!have);
     while (need && !exhausted) {
@@ -141,6 +148,10 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
         deep_pair = KMAX < 32 && !wide_pair && k > static_cast<uint32_t>(KMAX);
         if (!wide_pair && !deep_pair) {
           for (uint32_t t = 0; t < k; ++t) maps[t][lane] = 0;  // every mapping entry starts at local node 0
+          if (kRegMasks) {
+#pragma unroll
+            for (uint32_t t = 0; t < (kRegMasks ? KMAX : 1); ++t) mk[t] = t < k ? src.mask(base, qoff + t, t, s, n) : 0ull;
+          }
           depth = 0;
           pos = 0;
           visited = 0;
@@ -185,7 +196,14 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
         const uint32_t qad = static_cast<uint32_t>(q.fold[static_cast<uint64_t>(qoff) + depth]);
         // candidates in the row of every mapped query neighbour and in no row of another mapped
         // node: AND over t of row_t, complemented where t is not adjacent to `depth`
-        uint64_t el = src.mask(base, qoff + depth, depth, s, n) & ~visited & bits_from(pos);
+        uint64_t md = 0;
+        if (kRegMasks) {
+#pragma unroll
+          for (uint32_t t = 0; t < (kRegMasks ? KMAX : 1); ++t) md = t == depth ? mk[t] : md;
+        } else {
+          md = src.mask(base, qoff + depth, depth, s, n);
+        }
+        uint64_t el = md & ~visited & bits_from(pos);
         for (uint32_t t = 0; t < depth; ++t) el &= rows[t][lane] ^ (static_cast<uint64_t>(((qad >> t) & 1u)) - 1ull);
         if (el) {  // accept (:366-370)
           const uint32_t c = static_cast<uint32_t>(__builtin_ctzll(el));
