@@ -321,77 +321,74 @@ __global__ __launch_bounds__(256) void k_query_prepare(GraphDev q, uint32_t R, u
   }
 }
 
-// One thread per query graph: the per-class signature maxima the pre-test reads, from the graph's
-// token streams (k_query_prepare's output: each requirement token carries one nonzero count, of
-// label l at level r, of its node's signature).  qcmax[(g*3 + c)*R + r] is the nibble-wise max of
-// the level-(r+1) signatures of the graph's nodes labelled cl.l[c], and bit c of qinfo[g].qcls is
-// set when there is such a node.  Streams are read as 64-bit words, eight independent loads at a
-// time: a node's stream starts on a word boundary with its label token and ends with the end
-// token, after which the rest of that word is padding.
+// The per-class signature maxima the pre-test reads, from the query graphs' token streams
+// (k_query_prepare's output: each requirement token carries one nonzero count, of label l at
+// level r, of its node's signature, and every (r, l) occurs at most once per node, so the max over
+// a class's nodes is the max over their tokens).  qcmax[(g*3 + c)*R + r] is the nibble-wise max
+// of the level-(r+1) signatures of graph g's nodes labelled cl.l[c], and bit c of qinfo[g].qcls is
+// set when there is such a node.  One wavefront per graph reads the graph's token words 64 at a
+// time, a word per lane: a node's stream starts on a word boundary with its label token and its
+// last word holds the end token (which no label or requirement token equals), so the words that
+// hold an end token delimit the nodes and give each word its node's label.  Counts go to per-class
+// LDS maxima by atomic max; lanes then pack them into nibbles.
 __global__ __launch_bounds__(256) void k_query_classes(uint32_t Gq, uint32_t R, const uint64_t* __restrict__ tok64, ClassLabels cl,
                                                        QueryInfo* __restrict__ qinfo, uint64_t* __restrict__ qcmax) {
-  const uint64_t g = blockIdx.x * 256ull + threadIdx.x;
-  if (g >= Gq) return;
+  constexpr uint32_t kSlots = kClasses * kMaxLevels * 16;
+  __shared__ uint32_t cnt_all[4][kSlots];
+  const uint32_t lane = static_cast<uint32_t>(lane_id());
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint64_t g = static_cast<uint64_t>(blockIdx.x) * 4 + wave;
+  if (g >= Gq) return;  // wave-uniform; no block barrier follows
+  uint32_t* cnt = cnt_all[wave];
+  for (uint32_t t = lane; t < kSlots; t += kWave) cnt[t] = 0;
+  wave_sync();
   const QueryInfo qi = qinfo[g];
-  const uint64_t seg = token_segment(R);  // a multiple of 4 tokens
-  uint64_t cm[kClasses][kMaxLevels];
+  const uint64_t seg = token_segment(R);
+  const uint64_t below = (1ull << lane) - 1;
+  const uint64_t end = (static_cast<uint64_t>(qi.qoff) + qi.k) * seg / 4;
+  uint32_t nodes = 0, carry_label = 0, qcls = 0;
+  bool starts_new = true;  // the chunk's first word starts a node
+  for (uint64_t base = static_cast<uint64_t>(qi.qoff) * seg / 4; base < end && nodes < qi.k; base += kWave) {  // wave-uniform
+    const uint64_t idx = base + lane;
+    const bool in = idx < end;
+    const uint64_t w = in ? tok64[idx] : ~0ull;
+    bool has_end = false;
 #pragma unroll
-  for (uint32_t c = 0; c < kClasses; ++c)
+    for (uint32_t j = 0; j < 4; ++j) has_end |= ((w >> (16 * j)) & 0xFFFFull) == kEndToken;
+    const uint64_t ends = __ballot(in && has_end);
+    const uint64_t prev = ends & below;  // end words before this lane's in the chunk
+    const uint32_t start_lane = prev ? 64u - static_cast<uint32_t>(__builtin_clzll(prev)) : 0u;
+    const uint32_t first = static_cast<uint32_t>(w & 0xFFFFull);
+    const uint32_t lab_here = static_cast<uint32_t>(__shfl(static_cast<int>(first), static_cast<int>(start_lane)));
+    const uint32_t label = prev || starts_new ? lab_here : carry_label;
+    const bool is_start = prev ? lane == start_lane : (lane == 0 && starts_new);
+    const bool valid = in && nodes + static_cast<uint32_t>(__popcll(prev)) < qi.k;
+    uint32_t c = kClasses;
 #pragma unroll
-    for (uint32_t r = 0; r < kMaxLevels; ++r) cm[c][r] = 0;
-  uint64_t nsig[kMaxLevels];  // the current node's signature per level, rebuilt from its tokens
-#pragma unroll
-  for (uint32_t r = 0; r < kMaxLevels; ++r) nsig[r] = 0;
-  uint32_t qcls = 0, node = 0, c = kClasses;
-  bool at_start = true;
-  const uint64_t wend = (static_cast<uint64_t>(qi.qoff) + qi.k) * seg / 4;
-  for (uint64_t wpos = static_cast<uint64_t>(qi.qoff) * seg / 4; node < qi.k && wpos < wend; wpos += 8) {
-    uint64_t wv[8];
-#pragma unroll
-    for (uint32_t u = 0; u < 8; ++u) wv[u] = wpos + u < wend ? tok64[wpos + u] : ~0ull;
-#pragma unroll
-    for (uint32_t u = 0; u < 8; ++u) {
-#pragma unroll
-      for (uint32_t j = 0; j < 4; ++j) {
-        const uint32_t t = static_cast<uint32_t>(wv[u] >> (16 * j)) & 0xFFFFu;
-        if (node >= qi.k) break;
-        if (at_start) {  // j == 0: the node's label
-          if (j != 0) break;
-          c = kClasses;
-#pragma unroll
-          for (uint32_t cc = 0; cc < kClasses; ++cc)
-            if (c == kClasses && t == cl.l[cc]) c = cc;
-          if (c < kClasses) qcls |= 1u << c;
-          at_start = false;
-          continue;
-        }
-        if (t == kEndToken) {  // the node's signature joins its class's maxima; the rest of the word is padding
-#pragma unroll
-          for (uint32_t cc = 0; cc < kClasses; ++cc)
-            if (cc == c) {
-#pragma unroll
-              for (uint32_t rr = 0; rr < kMaxLevels; ++rr) cm[cc][rr] = nib_max(cm[cc][rr], nsig[rr]);
-            }
-#pragma unroll
-          for (uint32_t rr = 0; rr < kMaxLevels; ++rr) nsig[rr] = 0;
-          ++node;
-          at_start = true;
-          break;
-        }
-        // one token per nonzero (level, label) count of the node: OR it into place
-        const uint32_t r = t & 7u, l = (t >> 3) & 15u;
-        const uint64_t v = static_cast<uint64_t>((t >> 7) & 15u) << (4 * l);
-#pragma unroll
-        for (uint32_t rr = 0; rr < kMaxLevels; ++rr) nsig[rr] |= rr == r ? v : 0ull;
+    for (uint32_t cc = 0; cc < kClasses; ++cc)
+      if (c == kClasses && label == cl.l[cc]) c = cc;
+    if (valid && c < kClasses) {
+      if (is_start) qcls |= 1u << c;
+      for (uint32_t j = is_start ? 1u : 0u; j < 4; ++j) {
+        const uint32_t t = static_cast<uint32_t>(w >> (16 * j)) & 0xFFFFu;
+        if (t == kEndToken) break;
+        atomicMax(&cnt[(c * kMaxLevels + (t & 7u)) * 16 + ((t >> 3) & 15u)], (t >> 7) & 15u);
       }
     }
+    nodes += static_cast<uint32_t>(__popcll(ends));
+    starts_new = (ends >> 63) & 1ull;
+    carry_label = static_cast<uint32_t>(__shfl(static_cast<int>(label), kWave - 1));
   }
 #pragma unroll
-  for (uint32_t cc = 0; cc < kClasses; ++cc)
-#pragma unroll
-    for (uint32_t r = 0; r < kMaxLevels; ++r)
-      if (r < R) qcmax[(g * kClasses + cc) * R + r] = cm[cc][r];
-  qinfo[g].qcls = qcls;
+  for (int o = kWave / 2; o > 0; o >>= 1) qcls |= static_cast<uint32_t>(__shfl_xor(static_cast<int>(qcls), o));
+  wave_sync();
+  if (lane < kClasses * R) {
+    const uint32_t c = lane / R, r = lane - c * R;
+    uint64_t v = 0;
+    for (uint32_t l = 0; l < 16; ++l) v |= static_cast<uint64_t>(cnt[(c * kMaxLevels + r) * 16 + l]) << (4 * l);
+    qcmax[(g * kClasses + c) * R + r] = v;
+  }
+  if (lane == 0) qinfo[g].qcls = qcls;
 }
 
 // ---- pair pre-test ------------------------------------------------------------------------------
@@ -1173,9 +1170,9 @@ __global__ __launch_bounds__(256) void k_surv_gather(uint32_t Gd, const uint32_t
   }
 }
 
-// The two kernels above with one thread per molecule instead of one wavefront, for batches of
-// few query graphs (W <= 2 bitmask words): most molecules then have no or a few survivors, and a
-// wavefront per molecule would leave almost all of its lanes idle.
+// The two kernels above with one thread per molecule instead of one wavefront, for batches of at
+// most 64 query graphs (one bitmask word per molecule): most molecules then have no or a few
+// survivors, and a wavefront per molecule would leave almost all of its lanes idle.
 __global__ __launch_bounds__(256) void k_surv_scatter_thread(uint32_t Gd, uint32_t Gq, uint32_t W, const unsigned long long* __restrict__ surv,
                                                              const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ pair_off,
                                                              uint32_t* __restrict__ indices, uint32_t* __restrict__ pair_graph,
@@ -1286,7 +1283,7 @@ hipError_t query_prepare(hipStream_t s, const GraphDev& q, uint32_t R, uint16_t*
   k_query_prepare<<<(q.G + 3) / 4, 256, 0, s>>>(q, R, tokens, qinfo, qmax);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_query_classes<<<(q.G + 255) / 256, 256, 0, s>>>(q.G, R, reinterpret_cast<const uint64_t*>(tokens), cl, qinfo, qcmax);
+  k_query_classes<<<(q.G + 3) / 4, 256, 0, s>>>(q.G, R, reinterpret_cast<const uint64_t*>(tokens), cl, qinfo, qcmax);
   return hipGetLastError();
 }
 
@@ -1946,7 +1943,7 @@ hipError_t launch_count_scan(sigmo_batch* b, const RunShape& L, hipStream_t s) {
 hipError_t launch_scatter(sigmo_batch* b, const RunShape& L, hipStream_t s) {
   if (!L.Gd) return hipSuccess;
   const uint64_t cap = std::min(b->cap_idx, std::min(b->cap_pg, b->cap_mo));
-  if (L.W <= 2) {  // few query graphs: a thread per molecule
+  if (L.W == 1) {  // at most 64 query graphs: a thread per molecule
     const dim3 grid(static_cast<unsigned>((uint64_t(L.Gd) + 255) / 256));
     hipLaunchKernelGGL(sigmo_k::k_surv_scatter_thread, grid, dim3(256), 0, s, L.Gd, L.Gq, L.W, b->surv, b->offsets, b->pair_off, b->indices,
                        b->pair_graph, b->mask_off, cap);
