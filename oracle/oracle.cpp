@@ -22,6 +22,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -573,6 +574,145 @@ int64_t oracle_join_wildcard(uint32_t Gq, const uint32_t* qgo, const uint32_t* q
     }
   }
   return bad ? -1 : total;
+}
+
+// ---- adjacency-matrix (AM) query graphs -------------------------------------------------------
+// AMGraph / DeviceBatchedAMGraph, graph.hpp:20-36, 114-148, 208-214, 345-392: per graph an
+// n x n bit matrix in uint64 words, bit u*W + v set for every edge (both directions), where the
+// word count is utils.hpp:17-21's (n*n + 63) / 64 (as uint8) and the row width W is
+// utils.hpp:25-28's (uint16) sqrt(float(64 * words)).  num_nodes[] is cumulative; graph g's matrix
+// starts at word graph_offsets[g].  Graphs of more than 64 nodes are not supported.
+
+static u8 am_words(u32 n) { return static_cast<u8>((n * n + 63) / 64); }
+static u16 am_width(u8 words) { return static_cast<u16>(std::sqrt(static_cast<float>(64 * static_cast<u32>(words)))); }
+
+// utils.hpp:100-112 binaryAMSearch over the cumulative node counts.
+static u32 am_search(const u32* num_nodes, u32 total_graphs, u32 node_id) {
+  u32 low = 0, high = total_graphs - 1;
+  while (low < high) {
+    const u32 mid = (low + high) / 2;
+    if (num_nodes[mid] <= node_id) low = mid + 1;
+    else high = mid;
+  }
+  return low;
+}
+
+// utils.hpp:34-48 getNeighbors: the row's set bits in ascending order.  The reference writes
+// every neighbour into a MAX_NEIGHBORS = 4 array and its readers stop at 4 (types.hpp:28,
+// signature.hpp:110-113): the first four are the neighbours that count.  Returns how many.
+static u32 am_neighbors(const u64* adj, u32 prev, u32 n, u32 u, u32 out[4]) {
+  const u16 W = am_width(am_words(n));
+  u32 cnt = 0;
+  for (u32 v = 0; v < W && cnt < 4; ++v) {
+    const u32 idx = u * W + v;
+    if ((adj[idx / 64] >> (idx % 64)) & 1ull) out[cnt++] = v + prev;
+  }
+  return cnt;
+}
+
+// Total adjacency words of the AM form of a CSR batch (graph.hpp:208-214, 345-392); 0 if a
+// graph has more than 64 nodes.
+uint64_t oracle_am_words(uint32_t G, const uint32_t* go) {
+  u64 total = 0;
+  for (u32 g = 0; g < G; ++g) {
+    const u32 n = go[g + 1] - go[g];
+    if (n > 64) return 0;
+    total += am_words(n);
+  }
+  return total;
+}
+
+// toAMGraph (graph.hpp:208-214, setBit utils.hpp:24-32) for every graph of a CSR batch, laid
+// out as createDeviceAMGraph does: graph_offsets[g] (words), num_nodes[g] (cumulative).
+void oracle_am_build(uint32_t G, const uint32_t* go, const uint32_t* ro, const uint32_t* ci, uint32_t* graph_offsets, uint32_t* num_nodes,
+                     uint64_t* adjacency) {
+  u64 off = 0;
+  for (u32 g = 0; g < G; ++g) {
+    const u32 s = go[g], n = go[g + 1] - s;
+    const u8 words = am_words(n);
+    const u16 W = am_width(words);
+    graph_offsets[g] = static_cast<u32>(off);
+    num_nodes[g] = go[g + 1];
+    u64* m = adjacency + off;
+    std::fill(m, m + words, 0ull);
+    for (u32 v = s; v < s + n; ++v)
+      for (u32 i = ro[v]; i < ro[v + 1]; ++i) {
+        const u8 a = static_cast<u8>(v - s), b = static_cast<u8>(ci[i] - s);
+        const u32 iu = a * W + b, iv = b * W + a;
+        m[iu / 64] |= 1ull << (iu % 64);
+        m[iv / 64] |= 1ull << (iv % 64);
+      }
+    off += words;
+  }
+}
+
+// generateAMSignatures, signature.hpp:100-120: each node's (first four) neighbours' labels are
+// counted onto its signature.
+void oracle_am_generate(uint32_t G, const uint32_t* num_nodes, const uint32_t* graph_offsets, const uint64_t* adjacency,
+                        const uint8_t* labels, uint64_t total_nodes, uint64_t* sigs) {
+  for (u64 v = 0; v < total_nodes; ++v) {
+    const u32 g = am_search(num_nodes, G, static_cast<u32>(v));
+    const u32 prev = g ? num_nodes[g - 1] : 0;
+    u32 nb[4];
+    const u32 c = am_neighbors(adjacency + graph_offsets[g], prev, num_nodes[g] - prev, static_cast<u32>(v) - prev, nb);
+    for (u32 i = 0; i < c; ++i) sig_inc(sigs[v], labels[nb[i]]);
+  }
+}
+
+// refineAMSignatures<PowerGraph>(view_size), signature.hpp:250-296: a BFS of view_size layers
+// over uint32 position sets (Bitset<uint32_t>, positions through uint8, shifts masked & 31),
+// then the signature is cleared and rebuilt as the label histogram of every reached node but
+// the node itself.  mutation 1: the histogram of the last layer only (the sphere, not the ball).
+void oracle_am_refine_pg(uint32_t G, const uint32_t* num_nodes, const uint32_t* graph_offsets, const uint64_t* adjacency,
+                         const uint8_t* labels, uint64_t total_nodes, uint64_t view_size, uint64_t* sigs, int mutation) {
+  auto bit = [](u32 idx) { return 1u << (static_cast<u8>(idx) & 31u); };
+  for (u64 v = 0; v < total_nodes; ++v) {
+    const u32 g = am_search(num_nodes, G, static_cast<u32>(v));
+    const u32 prev = g ? num_nodes[g - 1] : 0, n = num_nodes[g] - prev;
+    const u64* adj = adjacency + graph_offsets[g];
+    const u32 self = static_cast<u32>(v) - prev;
+    u32 frontier = bit(self), reachable = bit(self);
+    for (u64 it = 0; it < view_size && frontier; ++it) {
+      u32 next = 0;
+      for (u32 f = frontier; f; f &= f - 1) {
+        const u32 u = static_cast<u32>(ctz32(f));
+        u32 nb[4];
+        const u32 c = am_neighbors(adj, prev, n, u, nb);
+        for (u32 i = 0; i < c; ++i) {
+          const u32 w = nb[i] - prev;
+          if (!(reachable & bit(w))) {
+            reachable |= bit(w);
+            next |= bit(w);
+          }
+        }
+      }
+      frontier = next;
+    }
+    if (mutation == 1) reachable = frontier;  // planted: the last layer only
+    reachable &= ~bit(self);
+    sigs[v] = 0;
+    for (u32 r = reachable; r; r &= r - 1) sig_inc(sigs[v], labels[prev + static_cast<u32>(ctz32(r))]);
+  }
+}
+
+// refineAMSignatures<ViewBased>(view_size), signature.hpp:214-248: as the CSR form
+// (oracle_refine_view_based) over the node's (first four) AM neighbours.
+void oracle_am_refine_vb(uint32_t G, const uint32_t* num_nodes, const uint32_t* graph_offsets, const uint64_t* adjacency,
+                         const uint8_t* labels, uint64_t total_nodes, uint64_t view_size, uint64_t* sigs) {
+  std::vector<u64> tmp(sigs, sigs + total_nodes);
+  for (u64 v = 0; v < total_nodes; ++v) {
+    const u32 g = am_search(num_nodes, G, static_cast<u32>(v));
+    const u32 prev = g ? num_nodes[g - 1] : 0;
+    u32 nb[4];
+    const u32 c = am_neighbors(adjacency + graph_offsets[g], prev, num_nodes[g] - prev, static_cast<u32>(v) - prev, nb);
+    const u8 own = labels[v];
+    for (u32 i = 0; i < c; ++i)
+      for (u8 l = 0; l < 16; l++) {
+        u8 count = sig_get(tmp[nb[i]], l);
+        if (l == own) count = static_cast<u8>(count - view_size);
+        if (count > 0) sig_inc(sigs[v], l, count);
+      }
+  }
 }
 
 }  // extern "C"

@@ -53,6 +53,11 @@ def lib() -> C.CDLL:
             "oracle_refine_view_based": ([u32, vp, vp, vp, vp, u64, vp, i32], None),
             "oracle_filter_data": ([u64, vp, u64, vp, vp], None),
             "oracle_join_wildcard": ([u32, vp, vp, vp, vp, u32, vp, vp, vp, vp, u64, vp, vp, vp, i32, u32, vp, i32], i64),
+            "oracle_am_words": ([u32, vp], u64),
+            "oracle_am_build": ([u32, vp, vp, vp, vp, vp, vp], None),
+            "oracle_am_generate": ([u32, vp, vp, vp, vp, u64, vp], None),
+            "oracle_am_refine_pg": ([u32, vp, vp, vp, vp, u64, u64, vp, i32], None),
+            "oracle_am_refine_vb": ([u32, vp, vp, vp, vp, u64, u64, vp], None),
         }
         for name, (args, res) in sigs.items():
             fn = getattr(L, name)
@@ -220,6 +225,44 @@ def join_wildcard(q: dict, d: dict, words: np.ndarray, offsets: np.ndarray, indi
     if total < 0:
         raise ValueError("wildcard join failed: query too large or per-pair count overflow")
     return int(total), counts
+
+
+def am_build(g: dict) -> dict:
+    """toAMGraph + createDeviceAMGraph (library/include/graph.hpp:208-214, 345-392) of a CSR batch:
+    per graph an n x n bit matrix in uint64 words (row width W = (uint16)sqrt(float(64*words)),
+    words = (n*n+63)/64), cumulative node counts and per-graph word offsets."""
+    G = _G(g)
+    words = lib().oracle_am_words(G, _p(g["graph_offsets"]))
+    if G and not words:
+        raise ValueError("adjacency-matrix graphs hold at most 64 nodes")
+    out = {"graph_offsets": np.zeros(G, np.uint32), "num_nodes": np.zeros(G, np.uint32), "adjacency": np.zeros(max(words, 1), np.uint64),
+           "node_labels": np.ascontiguousarray(g["node_labels"], np.uint8), "G": G, "N": _N(g)}
+    lib().oracle_am_build(G, _p(g["graph_offsets"]), _p(g["row_offsets"]), _p(g["column_indices"]), _p(out["graph_offsets"]),
+                          _p(out["num_nodes"]), _p(out["adjacency"]))
+    out["adjacency"] = out["adjacency"][:words]
+    return out
+
+
+def _am_args(am):
+    adj = am["adjacency"] if len(am["adjacency"]) else np.zeros(1, np.uint64)
+    return am["G"], _p(am["num_nodes"]), _p(am["graph_offsets"]), _p(adj), _p(am["node_labels"]), am["N"]
+
+
+def am_generate(am: dict, sigs: np.ndarray | None = None) -> np.ndarray:
+    """generateAMSignatures (library/include/signature.hpp:100-120), onto sigs."""
+    out = np.zeros(am["N"], np.uint64) if sigs is None else sigs
+    lib().oracle_am_generate(*_am_args(am), _p(out))
+    return out
+
+
+def am_refine_power_graph(am: dict, view_size: int, sigs: np.ndarray, mutation: int = 0) -> None:
+    """refineAMSignatures<PowerGraph> (library/include/signature.hpp:250-296), in place."""
+    lib().oracle_am_refine_pg(*_am_args(am), view_size, _p(sigs), mutation)
+
+
+def am_refine_view_based(am: dict, view_size: int, sigs: np.ndarray) -> None:
+    """refineAMSignatures<ViewBased> (library/include/signature.hpp:214-248), in place."""
+    lib().oracle_am_refine_vb(*_am_args(am), view_size, _p(sigs))
 
 
 def pipeline(q: dict, d: dict, refinement_steps: int, find_first: bool = True, *, keep_levels: bool = False,
