@@ -8,7 +8,9 @@
 //   device.hpp:14-21           sigmo::CandidatesDomain, sigmo::device::deviceOptions
 //   utils.hpp:127-161          sigmo::utils::BatchedEvent
 //   graph.hpp:37-112,255-432   sigmo::CSRGraph, DeviceBatchedCSRGraph, createDeviceCSRGraph, ...
-//   io.hpp:37-55               sigmo::io::loadCSRGraphsFromFile / FromLines
+//   graph.hpp:20-35,114-148,345-421  sigmo::AMGraph, DeviceBatchedAMGraph, createDeviceAMGraph, ...
+//   utils.hpp:17-62            sigmo::utils::getNumOfAdjacencyIntegers, utils::adjacency_matrix (host)
+//   io.hpp:17-55               sigmo::io::loadCSRGraphsFromFile / FromLines, loadAMGraphsFromFile / FromLines
 //   signature.hpp:17-212       sigmo::signature::Signature<PowerGraph, 4>
 //   candidates.hpp:14-204      sigmo::candidates::Candidates
 //   isomorphism.hpp:22-127     sigmo::isomorphism::filter::{filterCandidates, refineCandidates}
@@ -18,6 +20,7 @@
 #pragma once
 
 #include <chrono>
+#include <cmath>
 #include <cstdint>
 #include <fstream>
 #include <memory>
@@ -36,7 +39,9 @@ using col_index_t = uint32_t;
 using label_t = uint8_t;
 using node_t = uint32_t;
 using candidates_t = uint32_t;
+using adjacency_t = uint64_t;
 constexpr label_t WILDCARD_NODE = 255;
+constexpr size_t MAX_NEIGHBORS = 4;
 constexpr node_t NULL_NODE = 0xFFFFFFFFu;
 }  // namespace types
 
@@ -104,6 +109,42 @@ class BatchedEvent {
   uint64_t from_ = 0, to_ = 0;
 };
 
+// Words of an n-node adjacency matrix, (n*n + 63) / 64 as uint8 (utils.hpp:17-21).
+inline uint8_t getNumOfAdjacencyIntegers(uint8_t num_nodes) {
+  return static_cast<uint8_t>((uint32_t(num_nodes) * num_nodes + 63) / 64);
+}
+
+// Host forms of the matrix accessors (utils.hpp:23-62): the row width is
+// (uint16)sqrt(float(64 * words)) and bit u*W + v marks the edge u-v.
+namespace adjacency_matrix {
+inline uint16_t rowWidth(uint8_t adjacency_matrix_size) {
+  return static_cast<uint16_t>(std::sqrt(static_cast<float>(64 * uint32_t(adjacency_matrix_size))));
+}
+inline void setBit(types::adjacency_t* m, uint8_t adjacency_matrix_size, uint8_t u, uint8_t v) {
+  const uint32_t W = rowWidth(adjacency_matrix_size), iu = u * W + v, iv = v * W + u;
+  m[iu / 64] |= types::adjacency_t(1) << (iu % 64);
+  m[iv / 64] |= types::adjacency_t(1) << (iv % 64);
+}
+inline bool isNeighbor(const types::adjacency_t* m, uint8_t adjacency_matrix_size, types::node_t node_id, types::node_t neighbor_id,
+                       types::node_t previous_nodes = 0) {
+  const uint32_t W = rowWidth(adjacency_matrix_size);
+  const uint32_t idx = uint8_t(node_id - previous_nodes) * W + uint8_t(neighbor_id - previous_nodes);
+  return (m[idx / 64] >> (idx % 64)) & 1u;
+}
+// The row's neighbours in ascending order (global ids), at most MAX_NEIGHBORS of them (the
+// reference's callers read no more), NULL_NODE-terminated when fewer.
+inline void getNeighbors(const types::adjacency_t* m, uint8_t adjacency_matrix_size, types::node_t node_id, types::node_t* neighbors,
+                         types::node_t previous_nodes = 0) {
+  const uint32_t W = rowWidth(adjacency_matrix_size), u = uint8_t(node_id - previous_nodes);
+  size_t cnt = 0;
+  for (uint32_t v = 0; v < W && cnt < types::MAX_NEIGHBORS; ++v) {
+    const uint32_t idx = u * W + v;
+    if ((m[idx / 64] >> (idx % 64)) & 1u) neighbors[cnt++] = v + previous_nodes;
+  }
+  if (cnt < types::MAX_NEIGHBORS) neighbors[cnt] = types::NULL_NODE;
+}
+}  // namespace adjacency_matrix
+
 // Records the launch range of one call.
 class EventScope {
  public:
@@ -152,6 +193,30 @@ struct DeviceBatchedCSRGraph {
   size_t total_edges = 0;
 };
 
+// Host adjacency-matrix graph (graph.hpp:20-35) of at most 64 nodes.
+class AMGraph {
+ public:
+  AMGraph(std::vector<types::adjacency_t> adjacency, std::vector<types::label_t> node_labels, uint8_t num_nodes)
+      : adjacency_(std::move(adjacency)), node_labels_(std::move(node_labels)), num_nodes_(num_nodes) {}
+  types::adjacency_t* getAdjacencyMatrix() { return adjacency_.data(); }
+  const types::adjacency_t* getAdjacencyMatrix() const { return adjacency_.data(); }
+  types::label_t* getNodeLabels() { return node_labels_.data(); }
+  const types::label_t* getNodeLabels() const { return node_labels_.data(); }
+  int getNumNodes() const { return num_nodes_; }
+
+ private:
+  std::vector<types::adjacency_t> adjacency_;
+  std::vector<types::label_t> node_labels_;
+  uint8_t num_nodes_;
+};
+
+// Device batched adjacency-matrix graphs (graph.hpp:114-121); the arrays live behind `handle`.
+struct DeviceBatchedAMGraph {
+  sigmo_am_graph* handle = nullptr;
+  uint32_t num_graphs = 0;
+  size_t total_nodes = 0;
+};
+
 namespace detail {
 // Owning wrapper of a host batch.
 struct HostBatch {
@@ -181,7 +246,38 @@ inline std::vector<CSRGraph> split(const sigmo_host_graphs* b) {
 }
 }  // namespace detail
 
+namespace detail {
+// toAMGraph (graph.hpp:208-214) of every graph of a host batch.
+inline std::vector<AMGraph> to_am(const sigmo_host_graphs* b) {
+  std::vector<AMGraph> out;
+  for (const CSRGraph& g : split(b)) {
+    const size_t n = g.getNumNodes();
+    if (n > 64) throw std::runtime_error("adjacency-matrix graphs hold at most 64 nodes");
+    const uint8_t words = utils::getNumOfAdjacencyIntegers(static_cast<uint8_t>(n));
+    std::vector<types::adjacency_t> m(words, 0);
+    for (size_t u = 0; u < n; ++u)
+      for (uint32_t e = g.getRowOffsets()[u]; e < g.getRowOffsets()[u + 1]; ++e)
+        utils::adjacency_matrix::setBit(m.data(), words, static_cast<uint8_t>(u), static_cast<uint8_t>(g.getColumnIndices()[e]));
+    out.emplace_back(std::move(m), std::vector<types::label_t>(g.getNodeLabels(), g.getNodeLabels() + n), static_cast<uint8_t>(n));
+  }
+  return out;
+}
+}  // namespace detail
+
 namespace io {
+// io.hpp:17-35: query graphs in the adjacency-matrix form.
+inline std::vector<AMGraph> loadAMGraphsFromFile(const std::string& filename) {
+  detail::HostBatch b;
+  detail::check(sigmo_host_graphs_load_dat(filename.c_str(), &b.h));
+  return detail::to_am(b.h);
+}
+inline std::vector<AMGraph> loadAMGraphsFromLines(const std::vector<std::string>& lines) {
+  std::string text;
+  for (const auto& l : lines) text += l + "\n";
+  detail::HostBatch b;
+  detail::check(sigmo_host_graphs_parse_dat(text.data(), text.size(), &b.h));
+  return detail::to_am(b.h);
+}
 inline std::vector<CSRGraph> loadCSRGraphsFromFile(const std::string& filename) {
   detail::HostBatch b;
   detail::check(sigmo_host_graphs_load_dat(filename.c_str(), &b.h));
@@ -229,6 +325,44 @@ inline void destroyDeviceCSRGraph(DeviceBatchedCSRGraph& g, Queue&) {
   g.handle = nullptr;
 }
 
+// createDeviceAMGraph (graph.hpp:345-392): the matrices' edges (each row's set bits) go up as a
+// host batch and the library lays the device image out again from them; for matrices made
+// by toAMGraph (symmetric) the image holds the same bits.
+inline DeviceBatchedAMGraph createDeviceAMGraph(Queue& queue, std::vector<AMGraph>& graphs) {
+  std::vector<uint32_t> go{0}, ro{0}, ci;
+  std::vector<uint8_t> nl;
+  for (const auto& g : graphs) {
+    const uint32_t base = go.back(), n = static_cast<uint32_t>(g.getNumNodes());
+    const uint8_t words = utils::getNumOfAdjacencyIntegers(static_cast<uint8_t>(n));
+    const uint32_t W = utils::adjacency_matrix::rowWidth(words);
+    for (uint32_t u = 0; u < n; ++u) {
+      for (uint32_t v = 0; v < n; ++v) {
+        const uint32_t idx = u * W + v;
+        if ((g.getAdjacencyMatrix()[idx / 64] >> (idx % 64)) & 1u) ci.push_back(base + v);
+      }
+      ro.push_back(static_cast<uint32_t>(ci.size()));
+    }
+    nl.insert(nl.end(), g.getNodeLabels(), g.getNodeLabels() + n);
+    go.push_back(base + n);
+  }
+  std::vector<uint8_t> el(ci.size(), 0);
+  detail::HostBatch b;
+  detail::check(sigmo_host_graphs_from_arrays(static_cast<uint32_t>(graphs.size()), go.data(), ro.data(), ci.data(), nl.data(),
+                                              el.data(), &b.h));
+  DeviceBatchedAMGraph d;
+  detail::check(sigmo_am_graph_upload(queue.get(), b.h, &d.handle));
+  uint64_t N = 0;
+  detail::check(sigmo_am_graph_info(d.handle, &d.num_graphs, &N, nullptr));
+  d.total_nodes = N;
+  return d;
+}
+
+// destroyDeviceAMGraph (graph.hpp:394-399).
+inline void destroyDeviceAMGraph(DeviceBatchedAMGraph& g, Queue&) {
+  sigmo_am_graph_free(g.handle);
+  g.handle = nullptr;
+}
+
 // Bytes of the reference CSR arrays (graph.hpp:324-328).
 inline size_t getDeviceGraphAllocSize(const DeviceBatchedCSRGraph& g) {
   uint64_t csr = 0;
@@ -257,6 +391,11 @@ class Signature {
   utils::BatchedEvent generateQuerySignatures(DeviceBatchedCSRGraph& g) { return gen(g, SIGMO_SCOPE_QUERY); }
   utils::BatchedEvent refineDataSignatures(DeviceBatchedCSRGraph& g, size_t view_size = 1) { return ref(g, SIGMO_SCOPE_DATA, view_size); }
   utils::BatchedEvent refineQuerySignatures(DeviceBatchedCSRGraph& g, size_t view_size = 1) { return ref(g, SIGMO_SCOPE_QUERY, view_size); }
+  // The adjacency-matrix overloads (signature.hpp:56-96 dispatch on the graph type).
+  utils::BatchedEvent generateDataSignatures(DeviceBatchedAMGraph& g) { return gen(g, SIGMO_SCOPE_DATA); }
+  utils::BatchedEvent generateQuerySignatures(DeviceBatchedAMGraph& g) { return gen(g, SIGMO_SCOPE_QUERY); }
+  utils::BatchedEvent refineDataSignatures(DeviceBatchedAMGraph& g, size_t view_size = 1) { return ref(g, SIGMO_SCOPE_DATA, view_size); }
+  utils::BatchedEvent refineQuerySignatures(DeviceBatchedAMGraph& g, size_t view_size = 1) { return ref(g, SIGMO_SCOPE_QUERY, view_size); }
 
   std::vector<uint64_t> getDataSignatures() const { return download(SIGMO_SCOPE_DATA, data_nodes_); }
   std::vector<uint64_t> getQuerySignatures() const { return download(SIGMO_SCOPE_QUERY, query_nodes_); }
@@ -277,6 +416,19 @@ class Signature {
       detail::check(sigmo_signatures_refine(queue_.get(), s_, g.handle, scope, static_cast<uint32_t>(view)));
     else  // signature.hpp:298-334
       detail::check(sigmo_signatures_refine_view_based(queue_.get(), s_, g.handle, scope, static_cast<uint32_t>(view)));
+    return ev.done();
+  }
+  utils::BatchedEvent gen(DeviceBatchedAMGraph& g, int scope) {  // signature.hpp:100-120
+    utils::EventScope ev(queue_.get());
+    detail::check(sigmo_signatures_generate_am(queue_.get(), s_, g.handle, scope));
+    return ev.done();
+  }
+  utils::BatchedEvent ref(DeviceBatchedAMGraph& g, int scope, size_t view) {
+    utils::EventScope ev(queue_.get());
+    if constexpr (A == Algorithm::PowerGraph)  // signature.hpp:250-296
+      detail::check(sigmo_signatures_refine_am(queue_.get(), s_, g.handle, scope, static_cast<uint32_t>(view)));
+    else  // signature.hpp:214-248
+      detail::check(sigmo_signatures_refine_am_view_based(queue_.get(), s_, g.handle, scope, static_cast<uint32_t>(view)));
     return ev.done();
   }
   std::vector<uint64_t> download(int scope, size_t n) const {

@@ -60,6 +60,7 @@ extern "C" {
 typedef struct sigmo_ctx sigmo_ctx;
 typedef struct sigmo_host_graphs sigmo_host_graphs;
 typedef struct sigmo_graph sigmo_graph;
+typedef struct sigmo_am_graph sigmo_am_graph;
 typedef struct sigmo_signatures sigmo_signatures;
 typedef struct sigmo_candidates sigmo_candidates;
 typedef struct sigmo_gmcr sigmo_gmcr;
@@ -129,6 +130,22 @@ int sigmo_graph_info(const sigmo_graph* g, uint32_t* num_graphs, uint64_t* total
 /* Bytes of the reference CSR arrays (graph.hpp:324-328) and of the whole device image. */
 int sigmo_graph_alloc_size(const sigmo_graph* g, uint64_t* csr_bytes, uint64_t* device_bytes);
 
+/* ---- device batched adjacency-matrix graph (DeviceBatchedAMGraph, graph.hpp:114-148) -- */
+/* createDeviceAMGraph (graph.hpp:345-392) of a host batch, each graph's matrix built as
+ * toAMGraph does (graph.hpp:208-214): bit u*W + v of graph g's ceil(n*n/64) words (from word
+ * graph_offsets[g]) is set for every edge u-v, W = (uint16)sqrt(float(64 * words))
+ * (utils.hpp:17-28); num_nodes is cumulative.  SIGMO_ERR_INVALID for a graph of more than 64
+ * nodes (this form is for query graphs). */
+int sigmo_am_graph_upload(sigmo_ctx* ctx, const sigmo_host_graphs* host, sigmo_am_graph** out);
+int sigmo_am_graph_free(sigmo_am_graph* g); /* destroyDeviceAMGraph, graph.hpp:394-399 */
+/* adjacency_words: the sum of the graphs' word counts (the adjacency_t count of
+ * getDeviceAMGraphAllocSize(vector<AMGraph>), graph.hpp:410-421). */
+int sigmo_am_graph_info(const sigmo_am_graph* g, uint32_t* num_graphs, uint64_t* total_nodes, uint64_t* adjacency_words);
+/* Copy the device arrays back (any pointer may be null): graph_offsets[G], num_nodes[G],
+ * adjacency[adjacency_words], node_labels[total_nodes]. */
+int sigmo_am_graph_download(sigmo_ctx* ctx, const sigmo_am_graph* g, uint32_t* graph_offsets, uint32_t* num_nodes, uint64_t* adjacency,
+                            uint8_t* node_labels);
+
 /* ---- signatures (signature::Signature<PowerGraph,4>, signature.hpp:157-212) ---------- */
 int sigmo_signatures_create(sigmo_ctx* ctx, uint64_t data_nodes, uint64_t query_nodes, sigmo_signatures** out);
 int sigmo_signatures_free(sigmo_signatures* s);
@@ -142,6 +159,16 @@ int sigmo_signatures_refine(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_gra
  * view_size : 0)) through incrementLabelCount's uint8 arithmetic.  Accumulates onto the current
  * signatures, as the reference does. */
 int sigmo_signatures_refine_view_based(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_graph* g, int scope, uint32_t view_size);
+/* generateAMSignatures (signature.hpp:100-120): each node's first four neighbours (ascending;
+ * MAX_NEIGHBORS = 4, types.hpp:28) add their labels onto its signature. */
+int sigmo_signatures_generate_am(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_am_graph* g, int scope);
+/* refineAMSignatures<PowerGraph>(view_size) (signature.hpp:250-296): the signature is cleared
+ * and rebuilt as the label histogram of a view_size-layer BFS over 32-bit position sets
+ * (Bitset<uint32_t>: positions >= 32 alias mod 32), the node's own position excluded. */
+int sigmo_signatures_refine_am(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_am_graph* g, int scope, uint32_t view_size);
+/* refineAMSignatures<ViewBased>(view_size) (signature.hpp:214-248): the ViewBased rule above over
+ * each node's first four AM neighbours. */
+int sigmo_signatures_refine_am_view_based(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_am_graph* g, int scope, uint32_t view_size);
 int sigmo_signatures_download(sigmo_ctx* ctx, const sigmo_signatures* s, int scope, uint64_t* out, uint64_t count);
 /* getDataSignatureAllocationSize / getQuerySignatureAllocationSize (signature.hpp:181-198). */
 int sigmo_signatures_alloc_size(const sigmo_signatures* s, int scope, uint64_t* bytes);

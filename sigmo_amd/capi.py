@@ -48,6 +48,8 @@ EXPORTED = (
     "sigmo_gen_queries", "sigmo_gen_arrivals", "sigmo_graph_upload", "sigmo_graph_free", "sigmo_graph_info",
     "sigmo_graph_alloc_size", "sigmo_signatures_create", "sigmo_signatures_free", "sigmo_signatures_generate",
     "sigmo_signatures_refine", "sigmo_signatures_download", "sigmo_signatures_refine_view_based", "sigmo_candidates_filter_data", "sigmo_signatures_alloc_size", "sigmo_candidates_create",
+    "sigmo_am_graph_upload", "sigmo_am_graph_free", "sigmo_am_graph_info", "sigmo_am_graph_download",
+    "sigmo_signatures_generate_am", "sigmo_signatures_refine_am", "sigmo_signatures_refine_am_view_based",
     "sigmo_candidates_free", "sigmo_candidates_filter", "sigmo_candidates_refine", "sigmo_candidates_download",
     "sigmo_candidates_counts", "sigmo_candidates_alloc_size", "sigmo_gmcr_create", "sigmo_gmcr_free",
     "sigmo_gmcr_generate", "sigmo_gmcr_total", "sigmo_gmcr_download", "sigmo_join", "sigmo_join_wildcard", "sigmo_pipeline_run",
@@ -218,6 +220,13 @@ def lib() -> C.CDLL:
             "sigmo_signatures_refine": ([vp, vp, vp, i32, u32], i32),
             "sigmo_signatures_download": ([vp, vp, i32, vp, u64], i32),
             "sigmo_signatures_refine_view_based": ([vp, vp, vp, i32, u32], i32),
+            "sigmo_am_graph_upload": ([vp, vp, pvp], i32),
+            "sigmo_am_graph_free": ([vp], i32),
+            "sigmo_am_graph_info": ([vp, C.POINTER(u32), C.POINTER(u64), C.POINTER(u64)], i32),
+            "sigmo_am_graph_download": ([vp, vp, vp, vp, vp, vp], i32),
+            "sigmo_signatures_generate_am": ([vp, vp, vp, i32], i32),
+            "sigmo_signatures_refine_am": ([vp, vp, vp, i32, u32], i32),
+            "sigmo_signatures_refine_am_view_based": ([vp, vp, vp, i32, u32], i32),
             "sigmo_candidates_filter_data": ([vp, vp, vp, vp], i32),
             "sigmo_signatures_alloc_size": ([vp, i32, C.POINTER(u64)], i32),
             "sigmo_candidates_create": ([vp, u64, u64, pvp], i32),
@@ -497,6 +506,42 @@ class DeviceGraph:
         return a.value, b.value
 
 
+class DeviceAMGraph:
+    """DeviceBatchedAMGraph (graph.hpp:114-148), made by createDeviceAMGraph (graph.hpp:345-392)
+    from a host batch of graphs of at most 64 nodes."""
+
+    def __init__(self, ctx: Context, host: HostGraphs):
+        h = C.c_void_p()
+        check(lib().sigmo_am_graph_upload(ctx.handle, host.handle, C.byref(h)))
+        self._h, self.ctx = h, ctx
+        g, n, w = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        check(lib().sigmo_am_graph_info(h, C.byref(g), C.byref(n), C.byref(w)))
+        self.num_graphs, self.total_nodes, self.adjacency_words = g.value, n.value, w.value
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().sigmo_am_graph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.free()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def download(self) -> dict:
+        """The device arrays: graph_offsets and num_nodes (cumulative) per graph, adjacency words,
+        node labels."""
+        go = np.zeros(self.num_graphs, np.uint32)
+        nn = np.zeros(self.num_graphs, np.uint32)
+        adj = np.zeros(self.adjacency_words, np.uint64)
+        nl = np.zeros(self.total_nodes, np.uint8)
+        check(lib().sigmo_am_graph_download(self.ctx.handle, self._h, _ptr(go), _ptr(nn), _ptr(adj), _ptr(nl)))
+        return {"graph_offsets": go, "num_nodes": nn, "adjacency": adj, "node_labels": nl}
+
+
 class Signature:
     def __init__(self, ctx: Context, data_nodes: int, query_nodes: int):
         h = C.c_void_p()
@@ -532,6 +577,18 @@ class Signature:
     def refine_view_based(self, g: DeviceGraph, scope: int, view_size: int = 1):
         """refineCSRSignatures<Algorithm::ViewBased> on the data or query scope."""
         check(lib().sigmo_signatures_refine_view_based(self.ctx.handle, self._h, g.handle, scope, view_size))
+
+    def generate_am(self, g: DeviceAMGraph, scope: int = SCOPE_QUERY):
+        """generateAMSignatures (signature.hpp:100-120)."""
+        check(lib().sigmo_signatures_generate_am(self.ctx.handle, self._h, g.handle, scope))
+
+    def refine_am(self, g: DeviceAMGraph, view_size: int = 1, scope: int = SCOPE_QUERY):
+        """refineAMSignatures<PowerGraph> (signature.hpp:250-296)."""
+        check(lib().sigmo_signatures_refine_am(self.ctx.handle, self._h, g.handle, scope, view_size))
+
+    def refine_am_view_based(self, g: DeviceAMGraph, view_size: int = 1, scope: int = SCOPE_QUERY):
+        """refineAMSignatures<ViewBased> (signature.hpp:214-248)."""
+        check(lib().sigmo_signatures_refine_am_view_based(self.ctx.handle, self._h, g.handle, scope, view_size))
 
     def download(self, scope: int) -> np.ndarray:
         n = self.data_nodes if scope == SCOPE_DATA else self.query_nodes

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -206,6 +207,33 @@ struct sigmo_graph {
     dev_free(fold);
     dev_free(lplanes);
     dev_free(lmask);
+  }
+};
+
+// A batch of adjacency-matrix graphs (DeviceBatchedAMGraph, graph.hpp:114-148).
+struct sigmo_am_graph {
+  sigmo_ctx* ctx = nullptr;
+  uint32_t G = 0;
+  uint64_t N = 0, words = 0;
+  uint32_t *go = nullptr, *nn = nullptr;
+  uint64_t* adj = nullptr;
+  uint8_t* nl = nullptr;
+
+  sigmo_k::AMGraphDev dev() const {
+    sigmo_k::AMGraphDev g;
+    g.go = go;
+    g.nn = nn;
+    g.adj = adj;
+    g.nl = nl;
+    g.G = G;
+    g.N = N;
+    return g;
+  }
+  ~sigmo_am_graph() {
+    dev_free(go);
+    dev_free(nn);
+    dev_free(adj);
+    dev_free(nl);
   }
 };
 
@@ -701,6 +729,99 @@ int sigmo_graph_alloc_size(const sigmo_graph* g, uint64_t* csr_bytes, uint64_t* 
   return SIGMO_OK;
 }
 
+// ---- adjacency-matrix graphs ------------------------------------------------------------------
+
+namespace {
+// Word count and row width of an n-node matrix: (uint8)((n*n + 63) / 64) (utils.hpp:17-21) and
+// (uint16)sqrt(float(64 * words)) (utils.hpp:25-28).
+uint32_t am_words(uint32_t n) { return ((n * n + 63) / 64) & 255u; }
+uint32_t am_width(uint32_t words) { return static_cast<uint16_t>(std::sqrt(static_cast<float>(64 * words))); }
+}  // namespace
+
+int sigmo_am_graph_upload(sigmo_ctx* ctx, const sigmo_host_graphs* host, sigmo_am_graph** out) {
+  SIGMO_TRY(bind(ctx));
+  if (!host || !out) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  const uint32_t G = host->num_graphs();
+  const uint64_t N = host->total_nodes();
+  if (G == 0) return sigmo_err::set(SIGMO_ERR_INVALID, "empty graph batch");
+  // toAMGraph (graph.hpp:208-214) per graph, laid out as createDeviceAMGraph (graph.hpp:345-392)
+  std::vector<uint32_t> go(G), nn(G);
+  uint64_t words = 0;
+  for (uint32_t g = 0; g < G; ++g) {
+    const uint32_t n = host->graph_offsets[g + 1] - host->graph_offsets[g];
+    if (n > 64) return sigmo_err::set(SIGMO_ERR_INVALID, "adjacency-matrix graphs hold at most 64 nodes (graph " + std::to_string(g) + " has " + std::to_string(n) + ")");
+    go[g] = static_cast<uint32_t>(words);
+    nn[g] = host->graph_offsets[g + 1];
+    words += am_words(n);
+  }
+  std::vector<uint64_t> adj(words, 0);
+  for (uint32_t g = 0; g < G; ++g) {
+    const uint32_t base = host->graph_offsets[g], n = host->graph_offsets[g + 1] - base, W = am_width(am_words(n));
+    uint64_t* m = adj.data() + go[g];
+    for (uint32_t v = base; v < base + n; ++v)
+      for (uint32_t i = host->row_offsets[v]; i < host->row_offsets[v + 1]; ++i) {
+        const uint32_t a = v - base, b = host->column_indices[i] - base;
+        const uint32_t iu = a * W + b, iv = b * W + a;
+        m[iu / 64] |= 1ull << (iu % 64);
+        m[iv / 64] |= 1ull << (iv % 64);
+      }
+  }
+  auto* g = new sigmo_am_graph();
+  g->ctx = ctx;
+  g->G = G;
+  g->N = N;
+  g->words = words;
+  int st = dev_alloc(&g->go, G);
+  if (st == SIGMO_OK) st = dev_alloc(&g->nn, G);
+  if (st == SIGMO_OK) st = dev_alloc(&g->adj, words);
+  if (st == SIGMO_OK) st = dev_alloc(&g->nl, N);
+  hipError_t e = hipSuccess;
+  hipStream_t s = ctx->stream;
+  if (st == SIGMO_OK) e = hipMemcpyAsync(g->go, go.data(), uint64_t(G) * 4, hipMemcpyHostToDevice, s);
+  if (st == SIGMO_OK && e == hipSuccess) e = hipMemcpyAsync(g->nn, nn.data(), uint64_t(G) * 4, hipMemcpyHostToDevice, s);
+  if (st == SIGMO_OK && e == hipSuccess && words) e = hipMemcpyAsync(g->adj, adj.data(), words * 8, hipMemcpyHostToDevice, s);
+  if (st == SIGMO_OK && e == hipSuccess && N) e = hipMemcpyAsync(g->nl, host->node_labels.data(), N, hipMemcpyHostToDevice, s);
+  if (st == SIGMO_OK && e == hipSuccess) e = hipStreamSynchronize(s);  // the host vectors go out of scope
+  if (st == SIGMO_OK && e != hipSuccess) st = sigmo_err::set(SIGMO_ERR_HIP, std::string("AM graph upload: ") + hipGetErrorString(e));
+  if (st != SIGMO_OK) {
+    delete g;
+    return st;
+  }
+  *out = g;
+  return SIGMO_OK;
+}
+
+int sigmo_am_graph_free(sigmo_am_graph* g) {
+  if (!g) return SIGMO_OK;
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  delete g;
+  return SIGMO_OK;
+}
+
+int sigmo_am_graph_info(const sigmo_am_graph* g, uint32_t* num_graphs, uint64_t* total_nodes, uint64_t* adjacency_words) {
+  if (!g) return sigmo_err::set(SIGMO_ERR_INVALID, "null graph");
+  if (num_graphs) *num_graphs = g->G;
+  if (total_nodes) *total_nodes = g->N;
+  if (adjacency_words) *adjacency_words = g->words;
+  return SIGMO_OK;
+}
+
+int sigmo_am_graph_download(sigmo_ctx* ctx, const sigmo_am_graph* g, uint32_t* graph_offsets, uint32_t* num_nodes, uint64_t* adjacency,
+                            uint8_t* node_labels) {
+  SIGMO_TRY(bind(ctx));
+  if (!g) return sigmo_err::set(SIGMO_ERR_INVALID, "null graph");
+  hipStream_t s = ctx->stream;
+  hipError_t e = hipSuccess;
+  if (graph_offsets) e = hipMemcpyAsync(graph_offsets, g->go, uint64_t(g->G) * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && num_nodes) e = hipMemcpyAsync(num_nodes, g->nn, uint64_t(g->G) * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && adjacency && g->words) e = hipMemcpyAsync(adjacency, g->adj, g->words * 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && node_labels && g->N) e = hipMemcpyAsync(node_labels, g->nl, g->N, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return sigmo_err::set(SIGMO_ERR_HIP, std::string("AM graph download: ") + hipGetErrorString(e));
+  return SIGMO_OK;
+}
+
 // ---- signatures -------------------------------------------------------------------------------
 
 int sigmo_signatures_create(sigmo_ctx* ctx, uint64_t data_nodes, uint64_t query_nodes, sigmo_signatures** out) {
@@ -820,6 +941,45 @@ int sigmo_signatures_refine_view_based(sigmo_ctx* ctx, sigmo_signatures* s, cons
   PhaseScope ps(ctx, scope == SIGMO_SCOPE_DATA ? SIGMO_PHASE_SIG_DATA : SIGMO_PHASE_SIG_QUERY, -1, 0.0);
   if (g->N) SIGMO_HIP_TRY(hipMemcpyAsync(s->tmp, sigs, g->N * 8, hipMemcpyDeviceToDevice, ctx->stream));  // tmp_buff (:306-309)
   SIGMO_HIP_TRY(sigmo_k::sig_refine_view_based(ctx->stream, g->dev(), view_size, s->tmp, sigs, planes));
+  return ps.close();
+}
+
+static int am_sig_target(sigmo_signatures* s, const sigmo_am_graph* g, int scope, uint64_t** sigs, uint64_t** planes) {
+  SIGMO_TRY(check_scope(scope));
+  if (!s || !g) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  if (g->N != (scope == SIGMO_SCOPE_DATA ? s->D : s->Q))
+    return sigmo_err::set(SIGMO_ERR_INVALID, "graph node count does not match the signature allocation");
+  *sigs = scope == SIGMO_SCOPE_DATA ? s->dsig : s->qsig;
+  *planes = scope == SIGMO_SCOPE_DATA ? s->dplanes : nullptr;
+  return SIGMO_OK;
+}
+
+int sigmo_signatures_generate_am(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_am_graph* g, int scope) {
+  SIGMO_TRY(bind(ctx));
+  uint64_t *sigs, *planes;
+  SIGMO_TRY(am_sig_target(s, g, scope, &sigs, &planes));
+  PhaseScope ps(ctx, scope == SIGMO_SCOPE_DATA ? SIGMO_PHASE_SIG_DATA : SIGMO_PHASE_SIG_QUERY, -1, 0.0);
+  SIGMO_HIP_TRY(sigmo_k::am_generate(ctx->stream, g->dev(), sigs, planes));
+  return ps.close();
+}
+
+int sigmo_signatures_refine_am(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_am_graph* g, int scope, uint32_t view_size) {
+  SIGMO_TRY(bind(ctx));
+  uint64_t *sigs, *planes;
+  SIGMO_TRY(am_sig_target(s, g, scope, &sigs, &planes));
+  PhaseScope ps(ctx, scope == SIGMO_SCOPE_DATA ? SIGMO_PHASE_SIG_DATA : SIGMO_PHASE_SIG_QUERY, -1, 0.0);
+  SIGMO_HIP_TRY(sigmo_k::am_refine_power_graph(ctx->stream, g->dev(), view_size, sigs, planes));
+  return ps.close();
+}
+
+int sigmo_signatures_refine_am_view_based(sigmo_ctx* ctx, sigmo_signatures* s, const sigmo_am_graph* g, int scope, uint32_t view_size) {
+  SIGMO_TRY(bind(ctx));
+  uint64_t *sigs, *planes;
+  SIGMO_TRY(am_sig_target(s, g, scope, &sigs, &planes));
+  SIGMO_TRY(grow(&s->tmp, s->cap_tmp, g->N));
+  PhaseScope ps(ctx, scope == SIGMO_SCOPE_DATA ? SIGMO_PHASE_SIG_DATA : SIGMO_PHASE_SIG_QUERY, -1, 0.0);
+  if (g->N) SIGMO_HIP_TRY(hipMemcpyAsync(s->tmp, sigs, g->N * 8, hipMemcpyDeviceToDevice, ctx->stream));  // tmp_buff (signature.hpp:218-223)
+  SIGMO_HIP_TRY(sigmo_k::am_refine_view_based(ctx->stream, g->dev(), view_size, s->tmp, sigs, planes));
   return ps.close();
 }
 

@@ -239,6 +239,144 @@ __global__ __launch_bounds__(kSigBlock) void k_sig_refine_view_based(const uint3
   if (planes) write_planes<kSigBlock>(s, (N + 63) / 64, planes, tile);
 }
 
+// ---- adjacency-matrix (AM) graphs ------------------------------------------------------------
+// DeviceBatchedAMGraph, graph.hpp:114-148: per graph an n x n bit matrix, bit u*W + v set for
+// an edge u-v, starting at word go[g]; nn[] holds the cumulative node counts.
+
+// Row width of an n-node graph's matrix: words = (uint8)((n*n + 63) / 64) (utils.hpp:17-21),
+// W = (uint16)sqrt(float(64 * words)) (utils.hpp:25-28).  For 64 * words <= 16320 the float
+// square root is within 2^-16 of the real one while sqrt(k*k - 1) sits at least 1/256 below k,
+// so its floor is the integer square root computed here.
+__device__ __forceinline__ uint32_t am_width(uint32_t n) {
+  const uint32_t x = 64u * (((n * n + 63u) / 64u) & 255u);
+  uint32_t r = static_cast<uint32_t>(__builtin_sqrtf(static_cast<float>(x)));
+  while (r * r > x) --r;
+  while ((r + 1) * (r + 1) <= x) ++r;
+  return r;
+}
+
+// binaryAMSearch, utils.hpp:100-112: the graph holding node v (first g with nn[g] > v).
+__device__ __forceinline__ uint32_t am_graph_of(const uint32_t* __restrict__ nn, uint32_t G, uint32_t v) {
+  uint32_t lo = 0, hi = G - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (nn[mid] <= v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Row u (u < n) of a matrix of row width W <= 64: bits [u*W, u*W + W) as a mask.  The second
+// word is read only when the row crosses into it, which keeps the read inside the graph's
+// ceil(n*n/64) words (u*W + W <= n*W <= W*W <= 64 * words).
+__device__ __forceinline__ uint64_t am_row(const uint64_t* __restrict__ m, uint32_t u, uint32_t W) {
+  const uint32_t b = u * W, w = b / 64, sh = b % 64;
+  uint64_t r = m[w] >> sh;
+  if (sh && sh + W > 64) r |= m[w + 1] << (64 - sh);
+  return W >= 64 ? r : r & ((1ull << W) - 1);
+}
+
+struct AMNode {
+  uint32_t prev, n, W;
+  const uint64_t* m;
+};
+
+__device__ __forceinline__ AMNode am_node(const AMGraphDev& g, uint32_t v) {
+  const uint32_t gi = am_graph_of(g.nn, g.G, v);
+  AMNode a;
+  a.prev = gi ? g.nn[gi - 1] : 0;
+  a.n = g.nn[gi] - a.prev;
+  a.W = am_width(a.n);
+  a.m = g.adj + g.go[gi];
+  return a;
+}
+
+// The first four set bits of a row: getNeighbors (utils.hpp:34-48) lists the row's neighbours in
+// ascending order into a MAX_NEIGHBORS = 4 array (types.hpp:28) whose readers stop at four.
+__device__ __forceinline__ uint64_t am_first4(uint64_t row) {
+  uint64_t keep = 0;
+  for (int k = 0; k < 4 && row; ++k) {
+    keep |= row & (0 - row);
+    row &= row - 1;
+  }
+  return keep;
+}
+
+// generateAMSignatures, signature.hpp:100-120: thread per node, the labels of its first four
+// neighbours incremented onto the existing value.
+__global__ __launch_bounds__(kSigBlock) void k_am_generate(AMGraphDev g, uint64_t* __restrict__ sigs, uint64_t* __restrict__ planes) {
+  __shared__ uint64_t tile[64 * (kSigBlock / kWave)];
+  const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  uint64_t s = 0;
+  if (v < g.N) {
+    const AMNode a = am_node(g, static_cast<uint32_t>(v));
+    s = sigs[v];
+    for (uint64_t r = am_first4(am_row(a.m, static_cast<uint32_t>(v) - a.prev, a.W)); r; r &= r - 1)
+      s = sig_inc(s, g.nl[a.prev + __builtin_ctzll(r)]);
+    sigs[v] = s;
+  }
+  if (planes) write_planes<kSigBlock>(s, (g.N + 63) / 64, planes, tile);
+}
+
+// refineAMSignatures<ViewBased>(view_size), signature.hpp:214-248: the CSR ViewBased rule
+// (k_sig_refine_view_based) over the node's first four AM neighbours; `tmp` holds the
+// signatures before the pass.
+__global__ __launch_bounds__(kSigBlock) void k_am_refine_view_based(AMGraphDev g, uint32_t view_size, const uint64_t* __restrict__ tmp,
+                                                                    uint64_t* __restrict__ sigs, uint64_t* __restrict__ planes) {
+  __shared__ uint64_t tile[64 * (kSigBlock / kWave)];
+  const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  uint64_t s = 0;
+  if (v < g.N) {
+    const AMNode a = am_node(g, static_cast<uint32_t>(v));
+    s = sigs[v];
+    const uint32_t own = g.nl[v];
+    for (uint64_t r = am_first4(am_row(a.m, static_cast<uint32_t>(v) - a.prev, a.W)); r; r &= r - 1) {
+      const uint64_t t = tmp[a.prev + __builtin_ctzll(r)];
+      for (uint32_t l = 0; l < 16; ++l) {
+        uint32_t c = static_cast<uint32_t>(t >> (4 * l)) & 15u;
+        if (l == own) c = (c - view_size) & 255u;
+        if (c) s = sig_add(s, l, c);
+      }
+    }
+    sigs[v] = s;
+  }
+  if (planes) write_planes<kSigBlock>(s, (g.N + 63) / 64, planes, tile);
+}
+
+// refineAMSignatures<PowerGraph>(view_size), signature.hpp:250-296: a BFS of view_size layers
+// over 32-bit position sets (Bitset<uint32_t>: a position p is bit (uint8)p & 31, so positions
+// >= 32 alias), expanding each set position's first four neighbours; the signature is then
+// cleared and rebuilt as the label histogram of the reached positions but the node's own.
+// Every set position is below n (for n > 32 because it is below 32), so its row lies inside
+// the matrix.
+__global__ __launch_bounds__(kSigBlock) void k_am_refine_power_graph(AMGraphDev g, uint32_t view_size, uint64_t* __restrict__ sigs,
+                                                                     uint64_t* __restrict__ planes) {
+  __shared__ uint64_t tile[64 * (kSigBlock / kWave)];
+  const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  uint64_t s = 0;
+  if (v < g.N) {
+    const AMNode a = am_node(g, static_cast<uint32_t>(v));
+    const uint32_t self = 1u << ((static_cast<uint32_t>(v) - a.prev) & 31u);
+    uint32_t frontier = self, reach = self;
+    for (uint32_t it = 0; it < view_size && frontier; ++it) {
+      uint32_t next = 0;
+      for (uint32_t f = frontier; f; f &= f - 1) {
+        const uint32_t u = static_cast<uint32_t>(__builtin_ctz(f));
+        for (uint64_t r = am_first4(am_row(a.m, u, a.W)); r; r &= r - 1) {
+          const uint32_t b = 1u << (static_cast<uint32_t>(__builtin_ctzll(r)) & 31u);
+          next |= b & ~reach;
+          reach |= b;
+        }
+      }
+      frontier = next;
+    }
+    reach &= ~self;
+    for (; reach; reach &= reach - 1) s = sig_inc(s, g.nl[a.prev + __builtin_ctz(reach)]);
+    sigs[v] = s;
+  }
+  if (planes) write_planes<kSigBlock>(s, (g.N + 63) / 64, planes, tile);
+}
+
 // filterCandidates<Data>, isomorphism.hpp:49-52: row = data node, word w of the row = query
 // nodes 64w..64w+63 whose label equals the data node's or is the wildcard.
 __global__ __launch_bounds__(256) void k_filter_data(const uint8_t* __restrict__ qlabels, uint64_t Q, const uint8_t* __restrict__ dlabels,
@@ -713,6 +851,27 @@ hipError_t sig_refine_view_based(hipStream_t s, const GraphDev& g, uint32_t view
   if (!g.N) return hipSuccess;
   const unsigned blocks = static_cast<unsigned>((g.N + kSigBlock - 1) / kSigBlock);
   k_sig_refine_view_based<<<blocks, kSigBlock, 0, s>>>(g.ro, g.ci, g.nl, g.N, view_size, tmp, sigs, planes);
+  return hipGetLastError();
+}
+
+hipError_t am_generate(hipStream_t s, const AMGraphDev& g, uint64_t* sigs, uint64_t* planes) {
+  if (!g.N) return hipSuccess;
+  const unsigned blocks = static_cast<unsigned>((g.N + kSigBlock - 1) / kSigBlock);
+  k_am_generate<<<blocks, kSigBlock, 0, s>>>(g, sigs, planes);
+  return hipGetLastError();
+}
+
+hipError_t am_refine_view_based(hipStream_t s, const AMGraphDev& g, uint32_t view_size, const uint64_t* tmp, uint64_t* sigs, uint64_t* planes) {
+  if (!g.N) return hipSuccess;
+  const unsigned blocks = static_cast<unsigned>((g.N + kSigBlock - 1) / kSigBlock);
+  k_am_refine_view_based<<<blocks, kSigBlock, 0, s>>>(g, view_size, tmp, sigs, planes);
+  return hipGetLastError();
+}
+
+hipError_t am_refine_power_graph(hipStream_t s, const AMGraphDev& g, uint32_t view_size, uint64_t* sigs, uint64_t* planes) {
+  if (!g.N) return hipSuccess;
+  const unsigned blocks = static_cast<unsigned>((g.N + kSigBlock - 1) / kSigBlock);
+  k_am_refine_power_graph<<<blocks, kSigBlock, 0, s>>>(g, view_size, sigs, planes);
   return hipGetLastError();
 }
 

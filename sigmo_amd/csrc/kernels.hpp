@@ -34,6 +34,16 @@ struct GraphDev {
   uint64_t N;
 };
 
+// A batch of adjacency-matrix graphs (DeviceBatchedAMGraph, graph.hpp:114-148).
+struct AMGraphDev {
+  const uint32_t* go;   // graph_offsets [G]: first adjacency word of each graph
+  const uint32_t* nn;   // num_nodes     [G]: cumulative node counts
+  const uint64_t* adj;  // adjacency bit matrices
+  const uint8_t* nl;    // node_labels   [N]
+  uint32_t G;
+  uint64_t N;
+};
+
 // Labels < 16 that may carry a nonzero signature count, compacted into slots.
 struct LabelSlots {
   uint8_t label[16];
@@ -62,6 +72,11 @@ hipError_t sig_refine_next(hipStream_t s, const GraphDev& g, uint64_t* sigs, uin
 
 // refineCSRSignatures<ViewBased>(view_size): tmp holds the signatures before the pass.
 hipError_t sig_refine_view_based(hipStream_t s, const GraphDev& g, uint32_t view_size, const uint64_t* tmp, uint64_t* sigs, uint64_t* planes);
+// generateAMSignatures / refineAMSignatures<ViewBased> / refineAMSignatures<PowerGraph> over an
+// AM batch; `tmp` holds the signatures before a ViewBased pass.
+hipError_t am_generate(hipStream_t s, const AMGraphDev& g, uint64_t* sigs, uint64_t* planes);
+hipError_t am_refine_view_based(hipStream_t s, const AMGraphDev& g, uint32_t view_size, const uint64_t* tmp, uint64_t* sigs, uint64_t* planes);
+hipError_t am_refine_power_graph(hipStream_t s, const AMGraphDev& g, uint32_t view_size, uint64_t* sigs, uint64_t* planes);
 // filterCandidates<Data>: the D x ceil(Q/64) transposed bitset (rows = data nodes).
 hipError_t filter_data(hipStream_t s, const uint8_t* qlabels, uint64_t Q, const uint8_t* dlabels, uint64_t D, uint64_t* bits);
 
