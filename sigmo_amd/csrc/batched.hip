@@ -1064,32 +1064,21 @@ __global__ __launch_bounds__(256) void k_pair_eval(EvalArgs a) {
   }
 }
 
-// Survivors per molecule (popcount of its row of the survival bitmask) and the arena words they
-// hold (k per pair, 2k for molecules of > 64 nodes).
+// Survivors per molecule: the popcount of its row of the survival bitmask.
 __global__ __launch_bounds__(256) void k_surv_count(uint32_t Gd, uint32_t W, const unsigned long long* __restrict__ surv,
-                                                    const QueryInfo* __restrict__ qinfo, const uint32_t* __restrict__ dgo,
-                                                    uint32_t* __restrict__ counts, uint32_t* __restrict__ words) {
+                                                    uint32_t* __restrict__ counts) {
   const uint64_t m = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   if (m >= Gd) return;
-  const uint32_t stride = dgo[m + 1] - dgo[m] > 64 ? 2u : 1u;
-  uint32_t c = 0, wsum = 0;
-  for (uint32_t w = 0; w < W; ++w) {
-    uint64_t x = surv[m * W + w];
-    c += __popcll(x);
-    while (x) {
-      const uint32_t g = w * 64 + __builtin_ctzll(x);
-      x &= x - 1;
-      wsum += qinfo[g].k * stride;
-    }
-  }
+  uint32_t c = 0;
+  for (uint32_t w = 0; w < W; ++w) c += __popcll(surv[m * W + w]);
   counts[m] = c;
-  words[m] = wsum;
 }
 
 // Canonical GMCR arrays (gmcr.hpp:73-115 order: molecules ascending, query graphs ascending
 // within a molecule) from the survival bitmask, with each pair's arena offset.  One wavefront
-// per molecule: lane l takes query graph 64w+l of word w, so each word's survivors are written
-// by one coalesced store per array.
+// per molecule: lane i loads bitmask word i of each run of 64 (one coalesced load), the lanes'
+// popcounts are scanned, and the wavefront visits only the nonzero words, lane l writing query
+// graph 64w+l of word w, so each word's survivors go out in one coalesced store per array.
 __global__ __launch_bounds__(256) void k_surv_scatter(uint32_t Gd, uint32_t Gq, uint32_t W, const unsigned long long* __restrict__ surv,
                                                       const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ pair_off,
                                                       uint32_t* __restrict__ indices, uint32_t* __restrict__ pair_graph,
@@ -1101,78 +1090,37 @@ __global__ __launch_bounds__(256) void k_surv_scatter(uint32_t Gd, uint32_t Gq, 
   const uint32_t lane = static_cast<uint32_t>(lane_id());
   const uint64_t below = (1ull << lane) - 1;
   uint64_t o = offsets[m];
-  for (uint32_t w = 0; w < W; ++w) {
-    const uint64_t x = surv[static_cast<uint64_t>(m) * W + w];
-    if ((x >> lane) & 1ull) {
-      const uint64_t r = o + static_cast<uint64_t>(__popcll(x & below));
-      const uint32_t g = w * 64 + lane;
-      if (r < cap) {
-        indices[r] = g;
-        pair_graph[r] = static_cast<uint32_t>(m);
-        mask_off[r] = pair_off[static_cast<uint64_t>(m) * Gq + g];
-      }
-    }
-    o += static_cast<uint64_t>(__popcll(x));
-  }
-}
-
-// The survivors' candidate masks gathered from the arena -- where each lane of the pair
-// evaluation left them in its own sub-chunk -- into `gathered`, pair after pair in GMCR order
-// from word woff[m]: the join's neighbouring lanes, which take neighbouring pairs, then read
-// neighbouring words and share cache lines.  One wavefront per molecule walks its pairs 64 at a
-// time and rewrites each pair's mask_off from its arena offset to its gathered offset.  After an
-// arena overflow the pair evaluation handed out offsets past the arena and stored nothing there:
-// nothing is gathered (the join then does nothing and the host repeats the run), and every copy
-// is also bounded by the arena's and the gathered array's sizes.
-__global__ __launch_bounds__(256) void k_surv_gather(uint32_t Gd, const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ woff,
-                                                     const QueryInfo* __restrict__ qinfo, const uint32_t* __restrict__ dgo,
-                                                     const uint32_t* __restrict__ indices, const uint64_t* __restrict__ arena, uint64_t arena_cap,
-                                                     const unsigned long long* __restrict__ overflow, uint64_t* __restrict__ gathered,
-                                                     uint64_t gathered_cap, uint32_t* __restrict__ mask_off, uint64_t cap) {
-  const uint64_t m = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / kWave;
-  if (m >= Gd) return;  // wave-uniform
-  if (uniform32(static_cast<uint32_t>(*overflow != 0ull))) return;
-  const uint32_t lane = static_cast<uint32_t>(lane_id());
-  const uint32_t stride = dgo[m + 1] - dgo[m] > 64 ? 2u : 1u;
-  const uint64_t r0 = offsets[m], r1 = offsets[m + 1];
-  uint64_t wo = woff[m];
-  for (uint64_t rb = r0; rb < r1; rb += kWave) {  // wave-uniform
-    const uint64_t r = rb + lane;
-    const bool valid = r < r1 && r < cap;
-    const uint32_t g = valid ? indices[r] : 0u;
-    const uint64_t src = valid ? mask_off[r] : 0ull;
-    const uint32_t kk = valid ? qinfo[g].k * stride : 0u;
-    uint32_t incl = kk;  // inclusive prefix of the words over the block's pairs
+  if (offsets[m + 1] == o) return;  // no survivors (wave-uniform)
+  for (uint32_t w0 = 0; w0 < W; w0 += kWave) {  // wave-uniform
+    const uint64_t xl = w0 + lane < W ? surv[m * W + w0 + lane] : 0ull;
+    const uint32_t pc = static_cast<uint32_t>(__popcll(xl));
+    uint32_t incl = pc;
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) {
       const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), d));
       if (lane >= static_cast<uint32_t>(d)) incl += y;
     }
-    const uint64_t dst = wo + incl - kk;
-    // arena blocks start on 4-word boundaries and take whole 4-word groups, so the block is read
-    // as 16-byte loads, up to four in flight (eight words) at a time
-    if (valid && dst + kk <= gathered_cap && src + pair_words(kk, 1) <= arena_cap) {
-      const uint4* s4 = reinterpret_cast<const uint4*>(arena + src);
-      for (uint32_t t = 0; t < kk; t += 8) {
-        uint4 v[4];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u)
-          if (t + 2 * u < kk) v[u] = s4[t / 2 + u];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) {
-          if (t + 2 * u < kk) gathered[dst + t + 2 * u] = static_cast<uint64_t>(v[u].x) | (static_cast<uint64_t>(v[u].y) << 32);
-          if (t + 2 * u + 1 < kk) gathered[dst + t + 2 * u + 1] = static_cast<uint64_t>(v[u].z) | (static_cast<uint64_t>(v[u].w) << 32);
+    for (uint64_t nz = __ballot(xl != 0ull); nz; nz &= nz - 1) {  // wave-uniform
+      const int wl = static_cast<int>(__builtin_ctzll(nz));
+      const uint64_t x = __shfl(xl, wl);
+      const uint32_t ob = static_cast<uint32_t>(__shfl(static_cast<int>(incl - pc), wl));
+      if ((x >> lane) & 1ull) {
+        const uint64_t r = o + ob + static_cast<uint64_t>(__popcll(x & below));
+        const uint32_t g = (w0 + static_cast<uint32_t>(wl)) * 64 + lane;
+        if (r < cap) {
+          indices[r] = g;
+          pair_graph[r] = static_cast<uint32_t>(m);
+          mask_off[r] = pair_off[static_cast<uint64_t>(m) * Gq + g];
         }
       }
     }
-    if (valid) mask_off[r] = static_cast<uint32_t>(dst);
-    wo += static_cast<uint32_t>(__shfl(static_cast<int>(incl), kWave - 1));
+    o += static_cast<uint32_t>(__shfl(static_cast<int>(incl), kWave - 1));
   }
 }
 
-// The two kernels above with one thread per molecule instead of one wavefront, for batches of at
-// most 64 query graphs (one bitmask word per molecule): most molecules then have no or a few
-// survivors, and a wavefront per molecule would leave almost all of its lanes idle.
+// k_surv_scatter with one thread per molecule instead of one wavefront, for batches of at most 64
+// query graphs (one bitmask word per molecule): most molecules then have no or a few survivors,
+// and a wavefront per molecule would leave almost all of its lanes idle.
 __global__ __launch_bounds__(256) void k_surv_scatter_thread(uint32_t Gd, uint32_t Gq, uint32_t W, const unsigned long long* __restrict__ surv,
                                                              const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ pair_off,
                                                              uint32_t* __restrict__ indices, uint32_t* __restrict__ pair_graph,
@@ -1193,35 +1141,6 @@ __global__ __launch_bounds__(256) void k_surv_scatter_thread(uint32_t Gd, uint32
       }
       ++o;
     }
-  }
-}
-
-__global__ __launch_bounds__(256) void k_surv_gather_thread(uint32_t Gd, const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ woff,
-                                                            const QueryInfo* __restrict__ qinfo, const uint32_t* __restrict__ dgo,
-                                                            const uint32_t* __restrict__ indices, const uint64_t* __restrict__ arena, uint64_t arena_cap,
-                                                            const unsigned long long* __restrict__ overflow, uint64_t* __restrict__ gathered,
-                                                            uint64_t gathered_cap, uint32_t* __restrict__ mask_off, uint64_t cap) {
-  const uint64_t m = blockIdx.x * 256ull + threadIdx.x;
-  if (m >= Gd) return;
-  if (*overflow) return;
-  const uint64_t r1 = min(static_cast<uint64_t>(offsets[m + 1]), cap);
-  uint64_t r = offsets[m];
-  if (r >= r1) return;
-  const uint32_t stride = dgo[m + 1] - dgo[m] > 64 ? 2u : 1u;
-  uint64_t wo = woff[m];
-  for (; r < r1; ++r) {
-    const uint32_t kk = qinfo[indices[r]].k * stride;
-    const uint64_t src = mask_off[r];
-    if (wo + kk <= gathered_cap && src + pair_words(kk, 1) <= arena_cap) {
-      const uint4* s4 = reinterpret_cast<const uint4*>(arena + src);
-      for (uint32_t t = 0; t < kk; t += 2) {
-        const uint4 v = s4[t / 2];
-        gathered[wo + t] = static_cast<uint64_t>(v.x) | (static_cast<uint64_t>(v.y) << 32);
-        if (t + 1 < kk) gathered[wo + t + 1] = static_cast<uint64_t>(v.z) | (static_cast<uint64_t>(v.w) << 32);
-      }
-    }
-    mask_off[r] = static_cast<uint32_t>(wo);
-    wo += kk;
   }
 }
 
@@ -1439,11 +1358,6 @@ struct sigmo_batch {
   uint64_t cap_pair_off = 0;
   uint32_t* mol_count = nullptr;
   uint64_t cap_mc = 0;
-  uint32_t *mol_words = nullptr, *word_off = nullptr;  // survivors' arena words per molecule and their exclusive scan
-  uint64_t cap_mw = 0, cap_wo = 0;
-  uint64_t* gathered = nullptr;  // the survivors' masks in GMCR order (the join reads these)
-  uint64_t cap_gathered = 0;
-  uint64_t* total_words = nullptr;
   unsigned long long* counters = nullptr;
   // canonical GMCR and join
   uint32_t *offsets = nullptr, *indices = nullptr, *pair_graph = nullptr, *mask_off = nullptr;
@@ -1496,10 +1410,6 @@ struct sigmo_batch {
     bfree(surv);
     bfree(pair_off);
     bfree(mol_count);
-    bfree(mol_words);
-    bfree(word_off);
-    bfree(gathered);
-    bfree(total_words);
     bfree(counters);
     bfree(offsets);
     bfree(indices);
@@ -1702,7 +1612,7 @@ int sigmo_batch_workspace_bytes(const sigmo_batch* b, uint32_t num_query_graphs,
   const bool list_mode = b->tune.filter_mode == 2 || (b->tune.filter_mode == 0 && Gq < kListModeGraphs);
   const uint64_t list = !list_mode ? 0 : (one_sync ? pairs : std::min<uint64_t>(pairs, pairs / 4 + 65536));
   *bytes = query_nodes * sigmo_k::token_segment(static_cast<uint32_t>(R)) * 2 + Gq * (sizeof(sigmo_k::QueryInfo) + 8 * R * (1 + sigmo_k::kClasses)) +
-           Gd * W * 8 + pairs * 4 + arena_words * 8 + gmcr_pairs * 16 + (db->num_wide ? guess * 4 : 0) + (Gd + 1) * 16 + list * 8;
+           Gd * W * 8 + pairs * 4 + arena_words * 8 + gmcr_pairs * 16 + (db->num_wide ? guess * 4 : 0) + (Gd + 1) * 8 + list * 8;
   return SIGMO_OK;
 }
 
@@ -1930,13 +1840,10 @@ hipError_t launch_filter(sigmo_batch* b, const RunShape& L, hipStream_t s) {
 // GMCR: survivors per molecule and their exclusive scan (total in b->total).
 hipError_t launch_count_scan(sigmo_batch* b, const RunShape& L, hipStream_t s) {
   if (L.Gd) {
-    hipLaunchKernelGGL(sigmo_k::k_surv_count, dim3((L.Gd + 255) / 256), dim3(256), 0, s, L.Gd, L.W, b->surv, b->qinfo, L.dd.go, b->mol_count,
-                       b->mol_words);
+    hipLaunchKernelGGL(sigmo_k::k_surv_count, dim3((L.Gd + 255) / 256), dim3(256), 0, s, L.Gd, L.W, b->surv, b->mol_count);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipError_t e = sigmo_k::exclusive_scan(s, b->mol_words, L.Gd, b->word_off, b->scan_tmp, b->total_words);
-  if (e != hipSuccess) return e;
   return sigmo_k::exclusive_scan(s, b->mol_count, L.Gd, b->offsets, b->scan_tmp, b->total);
 }
 
@@ -1947,19 +1854,11 @@ hipError_t launch_scatter(sigmo_batch* b, const RunShape& L, hipStream_t s) {
     const dim3 grid(static_cast<unsigned>((uint64_t(L.Gd) + 255) / 256));
     hipLaunchKernelGGL(sigmo_k::k_surv_scatter_thread, grid, dim3(256), 0, s, L.Gd, L.Gq, L.W, b->surv, b->offsets, b->pair_off, b->indices,
                        b->pair_graph, b->mask_off, cap);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(sigmo_k::k_surv_gather_thread, grid, dim3(256), 0, s, L.Gd, b->offsets, b->word_off, b->qinfo, L.dd.go, b->indices,
-                       b->arena, b->cap_arena, b->counters + 3, b->gathered, b->cap_gathered, b->mask_off, cap);
     return hipGetLastError();
   }
   const uint64_t blocks = (static_cast<uint64_t>(L.Gd) * sigmo_k::kWave + 255) / 256;  // a wavefront per molecule
   hipLaunchKernelGGL(sigmo_k::k_surv_scatter, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, L.Gd, L.Gq, L.W, b->surv, b->offsets,
                      b->pair_off, b->indices, b->pair_graph, b->mask_off, cap);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(sigmo_k::k_surv_gather, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, L.Gd, b->offsets, b->word_off, b->qinfo,
-                     L.dd.go, b->indices, b->arena, b->cap_arena, b->counters + 3, b->gathered, b->cap_gathered, b->mask_off, cap);
   return hipGetLastError();
 }
 
@@ -1969,7 +1868,7 @@ hipError_t launch_join(sigmo_batch* b, const RunShape& L, hipStream_t s, unsigne
                        const unsigned long long* P_dev = nullptr) {
   if (!P) return hipSuccess;
   const uint32_t per_cu = b->tune.join_blocks_per_cu ? std::min<uint32_t>(b->tune.join_blocks_per_cu, 4) : 4;
-  return sigmo_k::launch_join_lanes(s, L.qd, L.dd, sigmo_k::ArenaMasks{b->gathered, b->mask_off, b->counters + 3}, b->indices, b->pair_graph, P,
+  return sigmo_k::launch_join_lanes(s, L.qd, L.dd, sigmo_k::ArenaMasks{b->arena, b->mask_off, b->counters + 3}, b->indices, b->pair_graph, P,
                                     find_first, b->counts, stats, b->wide_pairs, L.cus, P_dev, b->deep_pairs, per_cu, b->tune.join_chunk);
 }
 
@@ -1980,7 +1879,7 @@ hipError_t launch_join_wide(sigmo_batch* b, const RunShape& L, hipStream_t s, un
   uint64_t blocks = 0;
   uint32_t chunk = 0;
   sigmo_k::join_wide_grid(nw, L.cus, &blocks, &chunk);
-  const sigmo_k::ArenaMasks src{b->gathered, b->mask_off, b->counters + 3};
+  const sigmo_k::ArenaMasks src{b->arena, b->mask_off, b->counters + 3};
   if (find_first)
     hipLaunchKernelGGL((sigmo_k::k_join_lanes_wide<true, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, L.qd, L.dd, src,
                        b->indices, b->pair_graph, b->wide_pairs, nw, chunk, 6u, b->counts, stats, nw_dev);
@@ -2003,9 +1902,6 @@ int size_fixed_buffers(sigmo_batch* b, const RunShape& L) {
   BATCH_TRY(b->grow(&b->surv, b->cap_surv, uint64_t(L.Gd) * L.W));
   BATCH_TRY(b->grow(&b->pair_off, b->cap_pair_off, L.pairs));
   BATCH_TRY(b->grow(&b->mol_count, b->cap_mc, uint64_t(L.Gd) + 1));
-  BATCH_TRY(b->grow(&b->mol_words, b->cap_mw, uint64_t(L.Gd) + 1));
-  BATCH_TRY(b->grow(&b->word_off, b->cap_wo, uint64_t(L.Gd) + 1));
-  if (!b->total_words) BATCH_TRY(balloc(&b->total_words, 1));
   BATCH_TRY(b->grow(&b->offsets, b->cap_off, uint64_t(L.Gd) + 1));
   BATCH_TRY(b->grow(&b->scan_tmp, b->cap_scan, (uint64_t(L.Gd) + 1023) / 1024 + 1));
   return SIGMO_OK;
@@ -2053,8 +1949,7 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
   // total are read after the join.  An arena overflow (data that changed) falls back to the
   // sized path below, which repeats the run.
   const uint64_t bound = uint64_t(Gq) * Gd;
-  if (sized && !db->num_wide && bound && bound <= kOneSyncPairs && b->arena && b->cap_arena >= want_arena &&
-      b->cap_gathered >= b->cap_arena) {
+  if (sized && !db->num_wide && bound && bound <= kOneSyncPairs && b->arena && b->cap_arena >= want_arena) {
     if (!b->run_host) BATCH_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&b->run_host), 16 * sizeof(unsigned long long)));
     BATCH_TRY(b->grow(&b->indices, b->cap_idx, bound));
     BATCH_TRY(b->grow(&b->pair_graph, b->cap_pg, bound));
@@ -2107,8 +2002,6 @@ int sigmo_batch_run(sigmo_batch* b, const sigmo_graph* query, int find_first, si
   for (int attempt = 0; attempt < 4; ++attempt) {
     BATCH_TRY(b->grow(&b->arena, b->cap_arena, want_arena));
     if (L.list_mode) BATCH_TRY(b->grow(&b->list, b->cap_list, want_list));
-    // the survivors' masks, gathered in GMCR order, never take more words than the arena holds
-    BATCH_TRY(b->grow(&b->gathered, b->cap_gathered, b->cap_arena));
     if (b->cap_arena >= (1ull << 32)) return sigmo_err::set(SIGMO_ERR_OVERFLOW, "candidate arena exceeds 2^32 words");
     // algorithmic reads: the data side once per run -- a label byte and R level signatures per
     // data node, (1 + 8R) D -- plus the query side; writes are added once they are known.

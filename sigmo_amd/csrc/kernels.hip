@@ -53,18 +53,37 @@ inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 20) {
 // ---- derived graph arrays ------------------------------------------------------------------
 
 // node_graph, 128-bit local adjacency, 8 label bit-planes per 64-node strip and the
-// per-graph 64-position label masks (lmask must be zero on entry).
+// per-graph 64-position label masks (lmask must be zero on entry).  A wavefront covers 64
+// consecutive nodes: one binary search finds the graph of its first node, the next 64 graph
+// starts are loaded one per lane, and each node's graph follows from six shuffle steps over
+// them.  The first lane of each graph in the wavefront ORs that graph's label masks in with one
+// atomic per label present.
 __global__ __launch_bounds__(256) void k_derive_nodes(const uint32_t* __restrict__ go, uint32_t G, const uint32_t* __restrict__ ro,
                                                       const uint32_t* __restrict__ ci, const uint8_t* __restrict__ nl, uint64_t N,
                                                       uint32_t* __restrict__ node_graph, uint64_t* __restrict__ adj,
                                                       uint64_t* __restrict__ fold, uint64_t* __restrict__ lplanes, unsigned long long* __restrict__ lmask) {
   const uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  const uint32_t lane = static_cast<uint32_t>(lane_id());
+  const uint64_t v0 = v - lane;
+  if (v0 >= N) return;  // wave-uniform; no block barrier in this kernel
   const bool valid = v < N;
+  const uint32_t g0 = graph_of(go, G, static_cast<uint32_t>(v0));
+  // starts of graphs g0+1 .. g0+64 (go[G] = N closes the last graph; past it, none)
+  const uint32_t gi = g0 + 1 + lane;
+  const uint32_t st = gi <= G ? go[gi] : 0xFFFFFFFFu;
+  // graph of node v: g0 + (number of those starts <= v), the starts being ascending
+  uint32_t pos = 0;
+#pragma unroll
+  for (uint32_t step = 32; step; step >>= 1) {
+    const uint32_t cand = static_cast<uint32_t>(__shfl(static_cast<int>(st), static_cast<int>(pos + step - 1)));
+    if (valid && cand <= static_cast<uint32_t>(v)) pos += step;
+  }
+  const uint32_t first = go[g0];
+  const uint32_t before = static_cast<uint32_t>(__shfl(static_cast<int>(st), static_cast<int>(pos ? pos - 1 : 0)));
+  const uint32_t g = g0 + pos, prev = pos ? before : first;
   uint32_t lab = 0;
   if (valid) {
-    const uint32_t g = graph_of(go, G, static_cast<uint32_t>(v));
     node_graph[v] = g;
-    const uint32_t prev = go[g];
     uint64_t lo = 0, hi = 0;
     for (uint32_t i = ro[v]; i < ro[v + 1]; ++i) {
       const uint32_t w = ci[i] - prev;  // < 128, checked at upload
@@ -75,17 +94,31 @@ __global__ __launch_bounds__(256) void k_derive_nodes(const uint32_t* __restrict
     adj[2 * v + 1] = hi;
     fold[v] = lo | hi;
     lab = nl[v];
-    const uint32_t local = static_cast<uint32_t>(v) - prev;
-    if (local < 64 && lab < 16) atomicOr(&lmask[static_cast<uint64_t>(g) * 16 + lab], 1ull << local);
+  }
+  // label masks: lanes of one graph are consecutive; the first of them owns the graph's bits
+  const uint64_t vm = __ballot(valid);
+  const bool head = valid && (lane == 0 || static_cast<uint32_t>(v) == prev);
+  const uint64_t heads = __ballot(head);
+  const uint64_t after = lane < 63 ? heads & ~((2ull << lane) - 1) : 0ull;
+  const uint32_t stop = after ? static_cast<uint32_t>(__builtin_ctzll(after)) : 64u;
+  const uint64_t seg = (stop == 64 ? ~0ull : ((1ull << stop) - 1)) & ~((1ull << lane) - 1) & vm;
+  const uint32_t local = static_cast<uint32_t>(v) - prev;
+  uint32_t present = valid && lab < 16 ? 1u << lab : 0u;
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) present |= static_cast<uint32_t>(__shfl_xor(static_cast<int>(present), o));
+  for (uint32_t ls = present; ls; ls &= ls - 1) {  // wave-uniform
+    const uint32_t l = static_cast<uint32_t>(__builtin_ctz(ls));
+    const uint64_t b = __ballot(valid && lab == l) & seg;
+    if (head && b && local < 64) atomicOr(&lmask[static_cast<uint64_t>(g) * 16 + l], (b >> lane) << local);
   }
   uint64_t mine = 0;
 #pragma unroll
   for (int b = 0; b < 8; ++b) {
     const uint64_t m = __ballot((lab >> b) & 1u);
-    if (lane_id() == b) mine = m;
+    if (lane == static_cast<uint32_t>(b)) mine = m;
   }
   const uint64_t w = v >> 6;
-  if (lane_id() < 8 && (w << 6) < N) lplanes[w * 8 + lane_id()] = mine;
+  if (lane < 8) lplanes[w * 8 + lane] = mine;
 }
 
 // ---- signatures --------------------------------------------------------------------------
