@@ -23,6 +23,9 @@
 #include <chrono>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
 #include "../../include/sigmo_hip.h"
 #include "device_common.hpp"
 #include "host_graphs.hpp"
@@ -147,12 +150,30 @@ __device__ __forceinline__ uint64_t planes_nib_max(uint64_t col) {
 }
 
 // One wavefront per molecule: lane p is position p (and p+64 for molecules of 65..128 nodes).
+// The sort key of the width-class order: class (<= 32, 33..64, 65..128 nodes) << 23 | the
+// molecule's label set (labels < 16) << 7 | (nodes - 1).  Sorted, molecules of one label set and
+// similar size share wavefronts, so the pre-test's label test (and, mostly, its maxima tests)
+// passes or fails for whole wavefronts and the tests after a failing one are skipped.
+__global__ __launch_bounds__(256) void k_db_order_keys(uint32_t G, const uint32_t* __restrict__ go, const uint8_t* __restrict__ nl,
+                                                       uint32_t* __restrict__ keys) {
+  const uint64_t m = blockIdx.x * 256ull + threadIdx.x;
+  if (m >= G) return;
+  const uint32_t s = go[m], n = go[m + 1] - s;
+  uint32_t labels = 0;
+  for (uint32_t p = 0; p < n; ++p) {
+    const uint32_t l = nl[s + p];
+    if (l < 16) labels |= 1u << l;
+  }
+  const uint32_t cls = n <= 32 ? 0u : (n <= 64 ? 1u : 2u);
+  keys[m] = (cls << 23) | (labels << 7) | ((n - 1) & 127u);
+}
+
 // Level planes, bit p = bit i of the level-(r+1) signature of position p for plane i =
 // 4*label + bit, go to one of two arrays by the molecule's width class: planes32[(x*R + r)*64
 // + i] (u32) for molecules of <= 32 nodes, planes64[(x*R + r)*64 + i] (u64) otherwise, x =
 // pidx[m]; planes_hi holds positions 64..127 and lab_hi[w*16+l] the positions 64..127 with
 // label l (w = wide_idx[m]; positions 0..63 use the graph's label masks).  Wavefront c handles
-// molecule m = cls_list[c] (width-class order) and writes minfo[c], dmax[c*R + r], the
+// molecule m = cls_list[c] (width-class order, grouped by k_db_order_keys) and writes minfo[c], dmax[c*R + r], the
 // nibble-wise max of the level-(r+1) signatures over the molecule's positions, and
 // cmax[(k*R + r)*G + c], the same max over the positions labelled cl.l[k] (0 if none).
 __global__ __launch_bounds__(256) void k_db_prepare(GraphDev g, uint32_t R, const uint32_t* __restrict__ cls_list,
@@ -1310,7 +1331,7 @@ struct sigmo_db {
   uint32_t R = 0;
   uint32_t num_wide = 0;
   uint32_t cls_count[3] = {0, 0, 0};  // molecules of <= 32, 33..64 and 65..128 nodes
-  uint32_t* cls_list = nullptr;       // the three classes' molecules, ascending, one after another
+  uint32_t* cls_list = nullptr;       // the three classes' molecules one after another, grouped by label set and size
   uint32_t* pidx = nullptr;           // index of the molecule's planes in planes32 or planes64
   uint32_t* wide_idx = nullptr;
   uint32_t* planes32 = nullptr;
@@ -1321,7 +1342,12 @@ struct sigmo_db {
   uint64_t bytes = 0;
   uint64_t cap_G = 0, cap_32 = 0, cap_64 = 0, cap_hi = 0, cap_dmax = 0, cap_lab = 0, cap_cmax = 0;  // allocated elements
   uint64_t generation = 0;  // bumped whenever a rebind changes the data or any buffer address
+  uint32_t* order_keys = nullptr;  // 2G: the sort keys and their sorted copy
+  uint8_t* sort_tmp = nullptr;
+  uint64_t cap_keys = 0, cap_sort = 0;
   ~sigmo_db() {
+    bfree(order_keys);
+    bfree(sort_tmp);
     bfree(cls_list);
     bfree(pidx);
     bfree(wide_idx);
@@ -1463,18 +1489,15 @@ int db_fill(sigmo_db* db, const sigmo_graph* data) {
   BATCH_TRY(ctx_info(db->ctx, &device, &s, &cus, &st, &sh));
   const auto& go = graph_host_offsets(data);
   const uint32_t G = static_cast<uint32_t>(go.size() - 1);
-  std::vector<uint32_t> widx(G, sigmo_k::kNoWide), pidx(G, 0), lists[3];
-  uint32_t nw = 0, n32 = 0, n64 = 0;
+  std::vector<uint32_t> widx(G, sigmo_k::kNoWide), pidx(G, 0);
+  uint32_t nw = 0, n32 = 0, n64 = 0, count[3] = {0, 0, 0};
   for (uint32_t g = 0; g < G; ++g) {
     const uint32_t n = go[g + 1] - go[g];
     const int c = n <= 32 ? 0 : (n <= 64 ? 1 : 2);
-    lists[c].push_back(g);
+    ++count[c];
     pidx[g] = c == 0 ? n32++ : n64++;
     if (c == 2) widx[g] = nw++;
   }
-  std::vector<uint32_t> cls;
-  cls.reserve(G);
-  for (int c = 0; c < 3; ++c) cls.insert(cls.end(), lists[c].begin(), lists[c].end());
   const uint64_t R = db->R;
   uint64_t cap_g = db->cap_G;
   BATCH_TRY(bgrow(&db->cls_list, cap_g, G));
@@ -1494,13 +1517,22 @@ int db_fill(sigmo_db* db, const sigmo_graph* data) {
   db->data = data;
   db->generation++;
   db->num_wide = nw;
-  for (int c = 0; c < 3; ++c) db->cls_count[c] = static_cast<uint32_t>(lists[c].size());
+  for (int c = 0; c < 3; ++c) db->cls_count[c] = count[c];
   db->bytes = G * (12ull + sizeof(sigmo_k::MolInfo)) + (uint64_t(n32) * 256 + (uint64_t(n64) + nw) * 512) * R + nw * 128ull +
               uint64_t(G) * R * 8 * (1 + sigmo_k::kClasses);
   BATCH_TRY(choose_classes(db, data, s));
   BATCH_HIP_TRY(hipMemcpyAsync(db->wide_idx, widx.data(), G * 4ull, hipMemcpyHostToDevice, s));
   BATCH_HIP_TRY(hipMemcpyAsync(db->pidx, pidx.data(), G * 4ull, hipMemcpyHostToDevice, s));
-  BATCH_HIP_TRY(hipMemcpyAsync(db->cls_list, cls.data(), G * 4ull, hipMemcpyHostToDevice, s));
+  // the width-class order: molecules sorted by (class, label set, size), stably (by index within a key)
+  BATCH_TRY(bgrow_slack(&db->order_keys, db->cap_keys, 2ull * G));
+  const sigmo_k::GraphDev gd = graph_dev(data);
+  hipLaunchKernelGGL(sigmo_k::k_db_order_keys, dim3((G + 255) / 256), dim3(256), 0, s, G, gd.go, gd.nl, db->order_keys);
+  BATCH_HIP_TRY(hipGetLastError());
+  size_t tmp_bytes = 0;
+  const rocprim::counting_iterator<uint32_t> ids(0u);
+  BATCH_HIP_TRY(rocprim::radix_sort_pairs(nullptr, tmp_bytes, db->order_keys, db->order_keys + G, ids, db->cls_list, G, 0u, 25u, s));
+  BATCH_TRY(bgrow_slack(&db->sort_tmp, db->cap_sort, std::max<uint64_t>(tmp_bytes, 1)));
+  BATCH_HIP_TRY(rocprim::radix_sort_pairs(db->sort_tmp, tmp_bytes, db->order_keys, db->order_keys + G, ids, db->cls_list, G, 0u, 25u, s));
   BATCH_HIP_TRY(hipStreamSynchronize(s));  // the host vectors go out of scope
   BATCH_TRY(sigmo_db_refresh(db));
   BATCH_HIP_TRY(hipStreamSynchronize(s));

@@ -119,8 +119,64 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
   uint64_t mk[kRegMasks ? KMAX : 1];
 #pragma unroll
   for (int t = 0; t < (kRegMasks ? KMAX : 1); ++t) mk[t] = 0;
+  // Loads a step needs are issued a step ahead: `qad` is the query adjacency row of `depth`, and
+  // after an accept the new mapped node's row waits in `nrow` (`fresh`) until the next step, which
+  // uses it last and then stores it to LDS.
+  uint32_t qad = 0;
+  uint64_t nrow = 0;
+  bool fresh = false;
+  // Each lane prefetches its next pair while it runs the current one: nst = 1 once the pair's
+  // index is claimed and its molecule, query graph and mask base are loading; nst = 2 once its
+  // offsets and sizes are loading too.  A lane that finishes its pair takes the prefetched one
+  // without waiting on that chain of dependent loads.
+  uint32_t nst = 0, nj = 0, ndg = 0, nqg = 0, nbase = 0, ns = 0, nn = 0, nqoff = 0, nk = 0;
   while (true) {
-    uint64_t need = __ballot(!have);
+    if (!have && nst == 2) {  // take the prefetched pair
+      j = nj;
+      s = ns;
+      n = nn;
+      qoff = nqoff;
+      k = nk;
+      base = nbase;
+      for (uint32_t t = 0; t < k; ++t) maps[t][lane] = 0;  // every mapping entry starts at local node 0
+      if (kRegMasks) {
+#pragma unroll
+        for (uint32_t t = 0; t < (kRegMasks ? KMAX : 1); ++t) mk[t] = t < k ? src.mask(base, qoff + t, t, s, n) : 0ull;
+      }
+      qad = static_cast<uint32_t>(q.fold[qoff]);
+      fresh = false;
+      depth = 0;
+      pos = 0;
+      visited = 0;
+      count = 0;
+      have = true;
+      nst = 0;
+    }
+    bool wide_pair = false, deep_pair = false;
+    if (nst == 1) {  // the prefetched pair's offsets and sizes
+      ns = d.go[ndg];
+      nn = d.go[ndg + 1] - ns;
+      nqoff = q.go[nqg];
+      nk = q.go[nqg + 1] - nqoff;
+      wide_pair = nn > 64;
+      deep_pair = KMAX < 32 && !wide_pair && nk > static_cast<uint32_t>(KMAX);
+      nst = wide_pair || deep_pair ? 0u : 2u;
+    }
+    const uint64_t wm = __ballot(wide_pair);  // molecules of > 64 nodes go to the two-word kernel
+    if (wm) {
+      unsigned long long b = 0;
+      if (lane == 0) b = atomicAdd(&stats[kStatWide], static_cast<unsigned long long>(__popcll(wm)));
+      const uint64_t wb = uniform64(__shfl(b, 0));
+      if (wide_pair) wide[wb + __popcll(wm & below)] = nj;
+    }
+    const uint64_t dm = KMAX < 32 ? __ballot(deep_pair) : 0ull;  // queries deeper than KMAX go to the second pass
+    if (dm) {
+      unsigned long long b = 0;
+      if (lane == 0) b = atomicAdd(&stats[kStatDeep], static_cast<unsigned long long>(__popcll(dm)));
+      const uint64_t db = uniform64(__shfl(b, 0));
+      if (deep_pair) deep[db + __popcll(dm & below)] = nj;
+    }
+    uint64_t need = __ballot(nst == 0);  // lanes with nothing prefetched claim a pair
     while (need && !exhausted) {
       if (next >= end) {
         unsigned long long b = 0;
@@ -135,49 +191,17 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
       const uint32_t cnt = min(static_cast<uint32_t>(end - next), static_cast<uint32_t>(__popcll(need)));
       const uint32_t rn = __popcll(need & below);
       const bool take = ((need >> lane) & 1ull) && rn < cnt;
-      bool wide_pair = false, deep_pair = false;
       if (take) {
-        j = list ? list[next + rn] : next + rn;
-        const uint32_t dg = pair_graph[j], qg = indices[j];
-        base = src.base(j);
-        s = d.go[dg];
-        n = d.go[dg + 1] - s;
-        qoff = q.go[qg];
-        k = q.go[qg + 1] - qoff;
-        wide_pair = n > 64;
-        deep_pair = KMAX < 32 && !wide_pair && k > static_cast<uint32_t>(KMAX);
-        if (!wide_pair && !deep_pair) {
-          for (uint32_t t = 0; t < k; ++t) maps[t][lane] = 0;  // every mapping entry starts at local node 0
-          if (kRegMasks) {
-#pragma unroll
-            for (uint32_t t = 0; t < (kRegMasks ? KMAX : 1); ++t) mk[t] = t < k ? src.mask(base, qoff + t, t, s, n) : 0ull;
-          }
-          depth = 0;
-          pos = 0;
-          visited = 0;
-          count = 0;
-          have = true;
-        }
+        nj = list ? list[next + rn] : static_cast<uint32_t>(next + rn);
+        ndg = pair_graph[nj];
+        nqg = indices[nj];
+        nbase = src.base(nj);
+        nst = 1;
       }
       next += cnt;
       need &= ~__ballot(take);
-      const uint64_t wm = __ballot(wide_pair);  // molecules of > 64 nodes go to the two-word kernel
-      if (wm) {
-        unsigned long long b = 0;
-        if (lane == 0) b = atomicAdd(&stats[kStatWide], static_cast<unsigned long long>(__popcll(wm)));
-        const uint64_t wb = uniform64(__shfl(b, 0));
-        if (wide_pair) wide[wb + __popcll(wm & below)] = static_cast<uint32_t>(j);
-      }
-      const uint64_t dm = KMAX < 32 ? __ballot(deep_pair) : 0ull;  // queries deeper than KMAX go to the second pass
-      if (dm) {
-        unsigned long long b = 0;
-        if (lane == 0) b = atomicAdd(&stats[kStatDeep], static_cast<unsigned long long>(__popcll(dm)));
-        const uint64_t db = uniform64(__shfl(b, 0));
-        if (deep_pair) deep[db + __popcll(dm & below)] = static_cast<uint32_t>(j);
-      }
-      need |= wm | dm;  // those lanes take another pair
     }
-    if (__ballot(have) == 0) break;
+    if (__ballot(have || nst != 0) == 0) break;
     ++wave_iters;
     if (have) {
       ++steps;
@@ -187,13 +211,15 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
         if (FIND_FIRST) {
           done = true;
         } else {
+          // the pending row is the leaf's last node's, which no step reads
+          fresh = false;
           depth = k - 1;
           pos = static_cast<uint32_t>(maps[depth][lane]) + 1;
+          qad = static_cast<uint32_t>(q.fold[static_cast<uint64_t>(qoff) + depth]);
         }
       } else {
         // adjacency rows as 8-byte folded words: query graphs (<= 32 nodes) and these molecules
         // (<= 64 nodes) have no neighbour at positions >= 64, so fold = the low row word
-        const uint32_t qad = static_cast<uint32_t>(q.fold[static_cast<uint64_t>(qoff) + depth]);
         // candidates in the row of every mapped query neighbour and in no row of another mapped
         // node: AND over t of row_t, complemented where t is not adjacent to `depth`
         uint64_t md = 0;
@@ -204,14 +230,22 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
           md = src.mask(base, qoff + depth, depth, s, n);
         }
         uint64_t el = md & ~visited & bits_from(pos);
-        for (uint32_t t = 0; t < depth; ++t) el &= rows[t][lane] ^ (static_cast<uint64_t>(((qad >> t) & 1u)) - 1ull);
+        const uint32_t lim = fresh ? depth - 1 : depth;
+        for (uint32_t t = 0; t < lim; ++t) el &= rows[t][lane] ^ (static_cast<uint64_t>(((qad >> t) & 1u)) - 1ull);
+        if (fresh) {
+          el &= nrow ^ (static_cast<uint64_t>(((qad >> lim) & 1u)) - 1ull);
+          rows[lim][lane] = nrow;
+          fresh = false;
+        }
         if (el) {  // accept (:366-370)
           const uint32_t c = static_cast<uint32_t>(__builtin_ctzll(el));
           maps[depth][lane] = static_cast<uint8_t>(c);
-          rows[depth][lane] = d.fold[static_cast<uint64_t>(s) + c];
+          nrow = d.fold[static_cast<uint64_t>(s) + c];
+          fresh = true;
           visited |= 1ull << c;
           ++depth;
           pos = 0;
+          if (depth < k) qad = static_cast<uint32_t>(q.fold[static_cast<uint64_t>(qoff) + depth]);
         } else {  // exhausted frame (:346-354)
           visited &= ~(1ull << (maps[depth][lane] & 63u));
           if (depth == 0) {
@@ -220,6 +254,7 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
             if (depth == 1) visited = 0;
             --depth;
             pos = static_cast<uint32_t>(maps[depth][lane]) + 1;
+            qad = static_cast<uint32_t>(q.fold[static_cast<uint64_t>(qoff) + depth]);
           }
         }
       }
