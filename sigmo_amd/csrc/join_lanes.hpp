@@ -231,7 +231,15 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
         }
         uint64_t el = md & ~visited & bits_from(pos);
         const uint32_t lim = fresh ? depth - 1 : depth;
-        for (uint32_t t = 0; t < lim; ++t) el &= rows[t][lane] ^ (static_cast<uint64_t>(((qad >> t) & 1u)) - 1ull);
+        if (kRegMasks) {  // all KMAX rows read at once (the unused ones masked off), one LDS wait
+#pragma unroll
+          for (uint32_t t = 0; t < (kRegMasks ? KMAX : 1); ++t) {
+            const uint64_t rw = rows[t][lane];
+            if (t < lim) el &= rw ^ (static_cast<uint64_t>(((qad >> t) & 1u)) - 1ull);
+          }
+        } else {
+          for (uint32_t t = 0; t < lim; ++t) el &= rows[t][lane] ^ (static_cast<uint64_t>(((qad >> t) & 1u)) - 1ull);
+        }
         if (fresh) {
           el &= nrow ^ (static_cast<uint64_t>(((qad >> lim) & 1u)) - 1ull);
           rows[lim][lane] = nrow;
@@ -429,8 +437,9 @@ hipError_t launch_join_lanes(hipStream_t s, const GraphDev& q, const GraphDev& d
   // full-depth pass.
   if (P < kJoinTwoPassPairs) deep = nullptr;
   const uint64_t cus = static_cast<uint64_t>(num_cus > 0 ? num_cus : 256);
-  // the shallow pass keeps 2.5x the wavefronts resident: up to 10 blocks per CU
-  const uint64_t per_cu = deep ? static_cast<uint64_t>(blocks_per_cu) * 5 / 2 : blocks_per_cu;
+  // the shallow pass keeps 2.75x the wavefronts resident: up to 11 blocks per CU (13.5 KB of LDS
+  // each, 80 VGPRs), which the LDS holds at once
+  const uint64_t per_cu = deep ? static_cast<uint64_t>(blocks_per_cu) * 11 / 4 : blocks_per_cu;
   const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((P + 127) / 128, cus * per_cu));
   const uint32_t ch = chunk ? chunk
                             : static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(64, P / (blocks * kJoinLaneWaves * 16))));
