@@ -1907,18 +1907,8 @@ hipError_t launch_join(sigmo_batch* b, const RunShape& L, hipStream_t s, unsigne
 // Lane join over the nw listed pairs of 65..128-node molecules (two-word positions).
 hipError_t launch_join_wide(sigmo_batch* b, const RunShape& L, hipStream_t s, unsigned long long* stats, uint64_t nw, int find_first,
                             const unsigned long long* nw_dev = nullptr) {
-  if (!nw) return hipSuccess;
-  uint64_t blocks = 0;
-  uint32_t chunk = 0;
-  sigmo_k::join_wide_grid(nw, L.cus, &blocks, &chunk);
-  const sigmo_k::ArenaMasks src{b->arena, b->mask_off, b->counters + 3};
-  if (find_first)
-    hipLaunchKernelGGL((sigmo_k::k_join_lanes_wide<true, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, L.qd, L.dd, src,
-                       b->indices, b->pair_graph, b->wide_pairs, nw, chunk, 6u, b->counts, stats, nw_dev);
-  else
-    hipLaunchKernelGGL((sigmo_k::k_join_lanes_wide<false, sigmo_k::ArenaMasks>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, L.qd, L.dd,
-                       src, b->indices, b->pair_graph, b->wide_pairs, nw, chunk, 6u, b->counts, stats, nw_dev);
-  return hipGetLastError();
+  return sigmo_k::launch_join_wide(s, L.qd, L.dd, sigmo_k::ArenaMasks{b->arena, b->mask_off, b->counters + 3}, b->indices, b->pair_graph,
+                                   b->wide_pairs, nw, find_first, b->counts, stats, L.cus, nw_dev, b->deep_pairs);
 }
 
 // Buffers of a run that do not depend on its results.

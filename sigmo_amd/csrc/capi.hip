@@ -1261,9 +1261,9 @@ int sigmo_join(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph* data
       mm->last_wide = nw;
     }
     if (nw)
-      SIGMO_HIP_TRY(sigmo_k::join_lanes_wide(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, mm->wide_pairs, nw, 6u,
+      SIGMO_HIP_TRY(sigmo_k::join_lanes_wide(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, mm->wide_pairs, nw,
                                              find_first, mm->pair_counts, ctx->stats, ctx->num_cus,
-                                             ctx->capturing ? ctx->stats + 5 : nullptr));
+                                             ctx->capturing ? ctx->stats + 5 : nullptr, mm->deep_pairs));
   }
   SIGMO_TRY(ps.close());
   SIGMO_HIP_TRY(hipMemcpyAsync(ctx->stats_host, ctx->stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
@@ -1329,7 +1329,7 @@ int sigmo_join_noalloc(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_gra
     SIGMO_HIP_TRY(hipStreamSynchronize(s));
     const uint64_t nw = ctx->stats_host[5];
     if (nw)
-      SIGMO_HIP_TRY(sigmo_k::join_lanes_wide(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, wide, nw, 6u,
+      SIGMO_HIP_TRY(sigmo_k::join_lanes_wide(s, query->dev(), data->dev(), c->bits, c->W64, m->indices, m->pair_graph, wide, nw,
                                              find_first, counts, stats, ctx->num_cus));
   }
   SIGMO_HIP_TRY(hipMemcpyAsync(ctx->stats_host, stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));

@@ -295,17 +295,22 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
 // positions span two words: candidates and the adjacency rows of mapped positions are (lo, hi)
 // pairs, while `visited` stays one word addressed by position mod 64, the reference's Bitset
 // aliasing (utils.hpp:71-73), exactly as the wavefront join does for these molecules.  LDS holds
-// two row words per depth (34 KB), so a block is one wavefront.
-
-template <bool FIND_FIRST, class Src>
+// two row words per depth and lane: with KMAX = kJoinShallow (12.75 KB per wavefront) the pass
+// lists the pairs of deeper queries in `deep` (count in stats[kStatWideDeep]) for a second pass
+// with KMAX = 32 (34.8 KB).  The shallow pass keeps its pair's masks in registers and, as the
+// narrow lane join, prefetches its next pair, issues each step's loads a step ahead and reads all
+// row words at once.
+template <bool FIND_FIRST, class Src, int KMAX = 32>
 __global__ __launch_bounds__(64) void k_join_lanes_wide(GraphDev q, GraphDev d, Src src, const uint32_t* __restrict__ indices,
                                                         const uint32_t* __restrict__ pair_graph, const uint32_t* __restrict__ list, uint64_t P,
                                                         uint32_t chunk, uint32_t cursor, int32_t* __restrict__ counts,
                                                         unsigned long long* __restrict__ stats,
-                                                        const unsigned long long* __restrict__ P_dev = nullptr) {
-  __shared__ uint64_t rows_lo[32][kWave];
-  __shared__ uint64_t rows_hi[32][kWave];
-  __shared__ uint8_t maps[32][kWave];
+                                                        const unsigned long long* __restrict__ P_dev = nullptr,
+                                                        uint32_t* __restrict__ deep = nullptr) {
+  static_assert(KMAX >= 2 && KMAX <= 32, "query depth");
+  __shared__ uint64_t rows_lo[KMAX][kWave];
+  __shared__ uint64_t rows_hi[KMAX][kWave];
+  __shared__ uint8_t maps[KMAX][kWave];
   if (src.aborted()) return;
   // the list's length as the lane join left it on the device, bounded by what the caller sized
   if (P_dev) P = min(P, static_cast<uint64_t>(*P_dev));
@@ -318,8 +323,56 @@ __global__ __launch_bounds__(64) void k_join_lanes_wide(GraphDev q, GraphDev d, 
   bool have = false;
   uint64_t j = 0, visited = 0, count = 0;
   uint32_t s = 0, n = 0, qoff = 0, k = 0, base = 0, depth = 0, pos = 0;
+  constexpr bool kReg = KMAX <= 16;
+  uint64_t mlo[kReg ? KMAX : 1], mhi[kReg ? KMAX : 1];
+#pragma unroll
+  for (int t = 0; t < (kReg ? KMAX : 1); ++t) mlo[t] = mhi[t] = 0;
+  uint32_t qad = 0;
+  uint64_t nlo = 0, nhi = 0;  // the accepted node's row words, stored to LDS by the next step
+  bool fresh = false;
+  uint32_t nst = 0, nj = 0, ndg = 0, nqg = 0, nbase = 0, ns = 0, nn = 0, nqoff = 0, nk = 0;
   while (true) {
-    uint64_t need = __ballot(!have);
+    if (!have && nst == 2) {  // take the prefetched pair
+      j = nj;
+      s = ns;
+      n = nn;
+      qoff = nqoff;
+      k = nk;
+      base = nbase;
+      for (uint32_t t = 0; t < k; ++t) maps[t][lane] = 0;  // every mapping entry starts at local node 0
+      if (kReg) {
+#pragma unroll
+        for (uint32_t t = 0; t < (kReg ? KMAX : 1); ++t) {
+          mlo[t] = mhi[t] = 0;
+          if (t < k) src.mask2(base, qoff + t, t, s, n, mlo[t], mhi[t]);
+        }
+      }
+      qad = static_cast<uint32_t>(q.adj[2 * static_cast<uint64_t>(qoff)]);
+      fresh = false;
+      depth = 0;
+      pos = 0;
+      visited = 0;
+      count = 0;
+      have = true;
+      nst = 0;
+    }
+    bool deep_pair = false;
+    if (nst == 1) {  // the prefetched pair's offsets and sizes
+      ns = d.go[ndg];
+      nn = d.go[ndg + 1] - ns;
+      nqoff = q.go[nqg];
+      nk = q.go[nqg + 1] - nqoff;
+      deep_pair = KMAX < 32 && nk > static_cast<uint32_t>(KMAX);
+      nst = deep_pair ? 0u : 2u;
+    }
+    const uint64_t dm = KMAX < 32 ? __ballot(deep_pair) : 0ull;  // queries deeper than KMAX go to the second pass
+    if (dm) {
+      unsigned long long b = 0;
+      if (lane == 0) b = atomicAdd(&stats[kStatWideDeep], static_cast<unsigned long long>(__popcll(dm)));
+      const uint64_t db = uniform64(__shfl(b, 0));
+      if (deep_pair) deep[db + __popcll(dm & below)] = nj;
+    }
+    uint64_t need = __ballot(nst == 0);  // lanes with nothing prefetched claim a pair
     while (need && !exhausted) {
       if (next >= end) {
         unsigned long long b = 0;
@@ -335,24 +388,16 @@ __global__ __launch_bounds__(64) void k_join_lanes_wide(GraphDev q, GraphDev d, 
       const uint32_t rn = __popcll(need & below);
       const bool take = ((need >> lane) & 1ull) && rn < cnt;
       if (take) {
-        j = list[next + rn];
-        const uint32_t dg = pair_graph[j], qg = indices[j];
-        base = src.base(j);
-        s = d.go[dg];
-        n = d.go[dg + 1] - s;
-        qoff = q.go[qg];
-        k = q.go[qg + 1] - qoff;
-        for (uint32_t t = 0; t < k; ++t) maps[t][lane] = 0;  // every mapping entry starts at local node 0
-        depth = 0;
-        pos = 0;
-        visited = 0;
-        count = 0;
-        have = true;
+        nj = list[next + rn];
+        ndg = pair_graph[nj];
+        nqg = indices[nj];
+        nbase = src.base(nj);
+        nst = 1;
       }
       next += cnt;
       need &= ~__ballot(take);
     }
-    if (__ballot(have) == 0) break;
+    if (__ballot(have || nst != 0) == 0) break;
     ++wave_iters;
     if (have) {
       ++steps;
@@ -362,29 +407,61 @@ __global__ __launch_bounds__(64) void k_join_lanes_wide(GraphDev q, GraphDev d, 
         if (FIND_FIRST) {
           done = true;
         } else {
+          fresh = false;  // the pending rows are the leaf's last node's, which no step reads
           depth = k - 1;
           pos = static_cast<uint32_t>(maps[depth][lane]) + 1;
+          qad = static_cast<uint32_t>(q.adj[2 * (static_cast<uint64_t>(qoff) + depth)]);
         }
       } else {
-        uint64_t clo, chi;
-        src.mask2(base, qoff + depth, depth, s, n, clo, chi);
+        uint64_t clo = 0, chi = 0;
+        if (kReg) {
+#pragma unroll
+          for (uint32_t t = 0; t < (kReg ? KMAX : 1); ++t) {
+            clo = t == depth ? mlo[t] : clo;
+            chi = t == depth ? mhi[t] : chi;
+          }
+        } else {
+          src.mask2(base, qoff + depth, depth, s, n, clo, chi);
+        }
         clo &= ~visited & bits_from(pos);
         chi &= ~visited & (pos <= 64 ? ~0ull : bits_from(pos - 64));
-        const uint32_t qad = static_cast<uint32_t>(q.adj[2 * (static_cast<uint64_t>(qoff) + depth)]);
         uint64_t el = clo, eh = chi;
-        for (uint32_t t = 0; t < depth; ++t) {
-          const uint64_t flip = static_cast<uint64_t>(((qad >> t) & 1u)) - 1ull;  // all ones where t is not adjacent
-          el &= rows_lo[t][lane] ^ flip;
-          eh &= rows_hi[t][lane] ^ flip;
+        const uint32_t lim = fresh ? depth - 1 : depth;
+        if (kReg) {  // all KMAX rows read at once (the unused ones masked off), one LDS wait
+#pragma unroll
+          for (uint32_t t = 0; t < (kReg ? KMAX : 1); ++t) {
+            const uint64_t rl = rows_lo[t][lane], rh = rows_hi[t][lane];
+            if (t < lim) {
+              const uint64_t flip = static_cast<uint64_t>(((qad >> t) & 1u)) - 1ull;  // all ones where t is not adjacent
+              el &= rl ^ flip;
+              eh &= rh ^ flip;
+            }
+          }
+        } else {
+          for (uint32_t t = 0; t < lim; ++t) {
+            const uint64_t flip = static_cast<uint64_t>(((qad >> t) & 1u)) - 1ull;
+            el &= rows_lo[t][lane] ^ flip;
+            eh &= rows_hi[t][lane] ^ flip;
+          }
+        }
+        if (fresh) {
+          const uint64_t flip = static_cast<uint64_t>(((qad >> lim) & 1u)) - 1ull;
+          el &= nlo ^ flip;
+          eh &= nhi ^ flip;
+          rows_lo[lim][lane] = nlo;
+          rows_hi[lim][lane] = nhi;
+          fresh = false;
         }
         if (el | eh) {  // accept (:366-370)
           const uint32_t c = el ? static_cast<uint32_t>(__builtin_ctzll(el)) : 64u + static_cast<uint32_t>(__builtin_ctzll(eh));
           maps[depth][lane] = static_cast<uint8_t>(c);
-          rows_lo[depth][lane] = d.adj[2 * (static_cast<uint64_t>(s) + c)];
-          rows_hi[depth][lane] = d.adj[2 * (static_cast<uint64_t>(s) + c) + 1];
+          nlo = d.adj[2 * (static_cast<uint64_t>(s) + c)];
+          nhi = d.adj[2 * (static_cast<uint64_t>(s) + c) + 1];
+          fresh = true;
           visited |= 1ull << (c & 63);
           ++depth;
           pos = 0;
+          if (depth < k) qad = static_cast<uint32_t>(q.adj[2 * (static_cast<uint64_t>(qoff) + depth)]);
         } else {  // exhausted frame (:346-354)
           visited &= ~(1ull << (maps[depth][lane] & 63u));
           if (depth == 0) {
@@ -393,6 +470,7 @@ __global__ __launch_bounds__(64) void k_join_lanes_wide(GraphDev q, GraphDev d, 
             if (depth == 1) visited = 0;
             --depth;
             pos = static_cast<uint32_t>(maps[depth][lane]) + 1;
+            qad = static_cast<uint32_t>(q.adj[2 * (static_cast<uint64_t>(qoff) + depth)]);
           }
         }
       }
@@ -473,11 +551,44 @@ hipError_t launch_join_lanes(hipStream_t s, const GraphDev& q, const GraphDev& d
   return hipGetLastError();
 }
 
-// Grid of k_join_lanes_wide for `nw` listed pairs: resident one-wavefront blocks, ~16 claims each.
-inline void join_wide_grid(uint64_t nw, int num_cus, uint64_t* blocks, uint32_t* chunk) {
+// The two-word lane join over the nw pairs listed in `list` (their count also at nw_dev when
+// given, nw then only bounding the grid).  With `deep` (capacity nw) a kJoinShallow-level pass
+// (12 resident one-wavefront blocks per CU) lists the deeper queries for a full-depth pass over
+// that list (4 blocks per CU); without it, one full-depth pass.
+template <class Src>
+hipError_t launch_join_wide(hipStream_t s, const GraphDev& q, const GraphDev& d, const Src& src, const uint32_t* indices,
+                            const uint32_t* pair_graph, const uint32_t* list, uint64_t nw, int find_first, int32_t* counts,
+                            unsigned long long* stats, int num_cus, const unsigned long long* nw_dev, uint32_t* deep) {
+  if (!nw) return hipSuccess;
   const uint64_t cus = static_cast<uint64_t>(num_cus > 0 ? num_cus : 256);
-  *blocks = std::max<uint64_t>(1, std::min<uint64_t>((nw + 63) / 64, cus * 4));
-  *chunk = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(64, nw / (*blocks * 16))));
+  const uint64_t per_cu = deep ? 12 : 4;
+  const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((nw + 63) / 64, cus * per_cu));
+  const uint32_t chunk = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(64, nw / (blocks * 16))));
+  if (!deep) {
+    if (find_first)
+      hipLaunchKernelGGL((k_join_lanes_wide<true, Src, 32>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, q, d, src, indices, pair_graph,
+                         list, nw, chunk, static_cast<uint32_t>(kStatWideCursor), counts, stats, nw_dev, nullptr);
+    else
+      hipLaunchKernelGGL((k_join_lanes_wide<false, Src, 32>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, q, d, src, indices, pair_graph,
+                         list, nw, chunk, static_cast<uint32_t>(kStatWideCursor), counts, stats, nw_dev, nullptr);
+    return hipGetLastError();
+  }
+  if (find_first)
+    hipLaunchKernelGGL((k_join_lanes_wide<true, Src, kJoinShallow>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, q, d, src, indices,
+                       pair_graph, list, nw, chunk, static_cast<uint32_t>(kStatWideCursor), counts, stats, nw_dev, deep);
+  else
+    hipLaunchKernelGGL((k_join_lanes_wide<false, Src, kJoinShallow>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, q, d, src, indices,
+                       pair_graph, list, nw, chunk, static_cast<uint32_t>(kStatWideCursor), counts, stats, nw_dev, deep);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const uint64_t dblocks = std::max<uint64_t>(1, std::min<uint64_t>((nw + 63) / 64, cus * 4));
+  if (find_first)
+    hipLaunchKernelGGL((k_join_lanes_wide<true, Src, 32>), dim3(static_cast<unsigned>(dblocks)), dim3(64), 0, s, q, d, src, indices, pair_graph,
+                       deep, nw, 16u, static_cast<uint32_t>(kStatWideDeepCursor), counts, stats, stats + kStatWideDeep, nullptr);
+  else
+    hipLaunchKernelGGL((k_join_lanes_wide<false, Src, 32>), dim3(static_cast<unsigned>(dblocks)), dim3(64), 0, s, q, d, src, indices, pair_graph,
+                       deep, nw, 16u, static_cast<uint32_t>(kStatWideDeepCursor), counts, stats, stats + kStatWideDeep, nullptr);
+  return hipGetLastError();
 }
 
 }  // namespace

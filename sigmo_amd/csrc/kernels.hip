@@ -506,6 +506,19 @@ constexpr int kBitsBlock = 256;
 // Rows per block for the word-strip kernels.  Each thread loads its strip's planes once per
 // block, so a block should cover at least `min_rows` rows (planes then cost <= ~10% of the
 // row traffic); beyond that, enough row chunks to give every CU ~4 blocks of 4 waves.
+// Blocks of a (column strip group, row chunk) grid are placed so that the row chunks of one strip
+// group run on one XCD: the hardware hands blocks to its 8 XCDs round-robin (block b to XCD
+// b mod 8), so XCD x walks the strip groups x, x+8, ... each with its row chunks in a row, and a
+// strip's planes come from HBM into that XCD's L2 once instead of once per chunk.
+constexpr uint64_t kXcds = 8;
+__device__ __forceinline__ bool xcd_tile(uint64_t b, uint64_t cols, uint64_t chunks, uint64_t& col, uint64_t& chunk) {
+  const uint64_t x = b % kXcds, t = b / kXcds;
+  col = x + kXcds * (t / chunks);
+  chunk = t % chunks;
+  return col < cols;
+}
+inline uint64_t xcd_grid(uint64_t cols, uint64_t chunks) { return kXcds * ((cols + kXcds - 1) / kXcds) * chunks; }
+
 inline unsigned rows_per_block(uint64_t Q, uint64_t col_blocks, int num_cus, uint64_t min_rows, uint64_t blocks_per_cu = 4) {
   const uint64_t target_blocks = static_cast<uint64_t>(num_cus > 0 ? num_cus : 256) * blocks_per_cu;
   uint64_t chunks = (target_blocks + col_blocks - 1) / col_blocks;
@@ -550,14 +563,16 @@ __global__ __launch_bounds__(kBitsBlock) void k_refine(const uint64_t* __restric
                                                        uint64_t D, LabelSlots slots, uint64_t* __restrict__ bits, unsigned rpb,
                                                        unsigned chunks) {
   const uint64_t W64 = (D + 63) / 64;
-  const uint64_t w = (blockIdx.x / chunks) * static_cast<uint64_t>(kBitsBlock) + threadIdx.x;
+  uint64_t col, chunk;
+  if (!xcd_tile(blockIdx.x, (W64 + kBitsBlock - 1) / kBitsBlock, chunks, col, chunk)) return;
+  const uint64_t w = col * kBitsBlock + threadIdx.x;
   if (w >= W64) return;
   uint64_t P[U][4];
 #pragma unroll
   for (int u = 0; u < U; ++u)
 #pragma unroll
     for (int b = 0; b < 4; ++b) P[u][b] = planes[(static_cast<uint64_t>(slots.label[u]) * 4 + b) * W64 + w];
-  const uint64_t q0 = static_cast<uint64_t>(blockIdx.x % chunks) * rpb;
+  const uint64_t q0 = chunk * rpb;
   const uint64_t q1 = q0 + rpb < Q ? q0 + rpb : Q;
   auto keep_of = [&](uint64_t qs) {
     uint64_t keep = ~0ull;
@@ -598,7 +613,9 @@ __global__ __launch_bounds__(kBitsBlock) void k_filter_refine(const uint8_t* __r
                                                               const uint64_t* __restrict__ planes, uint64_t D, LabelSlots slots,
                                                               uint64_t* __restrict__ bits, unsigned rpb, unsigned chunks) {
   const uint64_t W64 = (D + 63) / 64;
-  const uint64_t w = (blockIdx.x / chunks) * static_cast<uint64_t>(kBitsBlock) + threadIdx.x;
+  uint64_t col, chunk;
+  if (!xcd_tile(blockIdx.x, (W64 + kBitsBlock - 1) / kBitsBlock, chunks, col, chunk)) return;
+  const uint64_t w = col * kBitsBlock + threadIdx.x;
   if (w >= W64) return;
   uint64_t L[8];
 #pragma unroll
@@ -609,7 +626,7 @@ __global__ __launch_bounds__(kBitsBlock) void k_filter_refine(const uint8_t* __r
 #pragma unroll
     for (int b = 0; b < 4; ++b) P[u][b] = planes[(static_cast<uint64_t>(slots.label[u]) * 4 + b) * W64 + w];
   const uint64_t valid = (w == W64 - 1 && (D & 63)) ? (~0ull >> (64 - (D & 63))) : ~0ull;
-  const uint64_t q0 = static_cast<uint64_t>(blockIdx.x % chunks) * rpb;
+  const uint64_t q0 = chunk * rpb;
   const uint64_t q1 = q0 + rpb < Q ? q0 + rpb : Q;
   constexpr int kRows = 4;  // the rows' label and signature loads go out together
   for (uint64_t q = q0; q < q1; q += kRows) {
@@ -958,12 +975,11 @@ hipError_t refine(hipStream_t s, const uint64_t* qsig, uint64_t Q, const uint64_
   // ms per pass)
   const uint64_t cus = static_cast<uint64_t>(num_cus > 0 ? num_cus : 256);
   const unsigned rpb = rows_per_block(Q, cols, num_cus, 16 * static_cast<uint64_t>(slots.count), cols >= cus * 4 ? 8 : 4);
-  // one block per (word strip, row chunk) in a 1-D grid, row chunk fastest: a strip's chunks are
-  // dispatched together and can share its planes through cache (measured no faster or slower
-  // than the strip-fastest 2-D grid, and without its 65,535-chunk limit)
+  // one block per (word strip, row chunk) in a 1-D grid placed by xcd_tile: a strip's chunks run
+  // on one XCD, one after another, and share its planes through that XCD's L2
   const unsigned chunks = static_cast<unsigned>((Q + rpb - 1) / rpb);
-  if (cols * chunks > 0x7fffffffull) return hipErrorInvalidValue;
-  const dim3 grid(static_cast<unsigned>(cols * chunks));
+  if (xcd_grid(cols, chunks) > 0x7fffffffull) return hipErrorInvalidValue;
+  const dim3 grid(static_cast<unsigned>(xcd_grid(cols, chunks)));
   switch (slots.count) {
 #define SIGMO_REFINE_CASE(u) \
   case u: return launch_refine<u>(s, grid, qsig, Q, planes, D, slots, bits, rpb, chunks);
@@ -995,8 +1011,8 @@ hipError_t filter_refine(hipStream_t s, const uint8_t* qlabels, const uint64_t* 
   // 8 blocks 140 us, 41 rows 135 us
   const unsigned rpb = rows_per_block(Q, cols, num_cus, std::max<uint64_t>(64, 8 * static_cast<uint64_t>(slots.count)), 16);
   const unsigned chunks = static_cast<unsigned>((Q + rpb - 1) / rpb);  // row chunk fastest, as in refine
-  if (cols * chunks > 0x7fffffffull) return hipErrorInvalidValue;
-  const dim3 grid(static_cast<unsigned>(cols * chunks));
+  if (xcd_grid(cols, chunks) > 0x7fffffffull) return hipErrorInvalidValue;
+  const dim3 grid(static_cast<unsigned>(xcd_grid(cols, chunks)));
   switch (slots.count) {
 #define SIGMO_FILTER_REFINE_CASE(u) \
   case u: return launch_filter_refine<u>(s, grid, qlabels, qsig, Q, lplanes, planes, D, slots, bits, rpb, chunks);
@@ -1070,20 +1086,9 @@ hipError_t join_wildcard(hipStream_t s, const GraphDev& q, const GraphDev& d, co
 }
 
 hipError_t join_lanes_wide(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
-                           const uint32_t* pair_graph, const uint32_t* list, uint64_t nw, uint32_t cursor, int find_first, int32_t* counts,
-                           unsigned long long* stats, int num_cus, const unsigned long long* nw_dev) {
-  if (!nw) return hipSuccess;
-  uint64_t blocks = 0;
-  uint32_t chunk = 0;
-  join_wide_grid(nw, num_cus, &blocks, &chunk);
-  const BitsetMasks src{bits, W64};
-  if (find_first)
-    k_join_lanes_wide<true, BitsetMasks><<<static_cast<unsigned>(blocks), 64, 0, s>>>(q, d, src, indices, pair_graph, list, nw, chunk, cursor,
-                                                                                     counts, stats, nw_dev);
-  else
-    k_join_lanes_wide<false, BitsetMasks><<<static_cast<unsigned>(blocks), 64, 0, s>>>(q, d, src, indices, pair_graph, list, nw, chunk, cursor,
-                                                                                      counts, stats, nw_dev);
-  return hipGetLastError();
+                           const uint32_t* pair_graph, const uint32_t* list, uint64_t nw, int find_first, int32_t* counts,
+                           unsigned long long* stats, int num_cus, const unsigned long long* nw_dev, uint32_t* deep) {
+  return launch_join_wide(s, q, d, BitsetMasks{bits, W64}, indices, pair_graph, list, nw, find_first, counts, stats, num_cus, nw_dev, deep);
 }
 
 }  // namespace sigmo_k

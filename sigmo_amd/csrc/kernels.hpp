@@ -116,7 +116,9 @@ enum : uint32_t {
   kStatWideCursor = 6, // its work cursor
   kStatWaveIters = 7,  // loop iterations summed over wavefronts
   kStatDeep = 8,       // pairs listed for the full-depth pass of the lane join
-  kStatDeepCursor = 9  // its work cursor
+  kStatDeepCursor = 9, // its work cursor
+  kStatWideDeep = 10,  // pairs listed for the full-depth pass of the two-word join
+  kStatWideDeepCursor = 11  // its work cursor
 };
 constexpr uint32_t kJoinStatWords = 16;
 
@@ -136,11 +138,11 @@ hipError_t join_lanes(hipStream_t s, const GraphDev& q, const GraphDev& d, const
 hipError_t join_wildcard(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint8_t* qel, const uint8_t* del, const uint64_t* bits,
                          uint64_t W64, const uint32_t* indices, const uint32_t* pair_graph, uint64_t P, int find_first, int32_t* counts,
                          unsigned long long* stats, int num_cus);
-// The lane-per-pair join over the `nw` listed pairs of molecules with 65..128 nodes;
-// stats[cursor] is the work counter.  With nw_dev the kernel reads the list's length on the
-// device, bounded by nw.
+// The lane-per-pair join over the `nw` listed pairs of molecules with 65..128 nodes
+// (stats[kStatWideCursor] is the work counter).  With nw_dev the kernel reads the list's length on
+// the device, bounded by nw.  With `deep` (capacity nw) it runs in two passes as join_lanes does.
 hipError_t join_lanes_wide(hipStream_t s, const GraphDev& q, const GraphDev& d, const uint64_t* bits, uint64_t W64, const uint32_t* indices,
-                           const uint32_t* pair_graph, const uint32_t* list, uint64_t nw, uint32_t cursor, int find_first, int32_t* counts,
-                           unsigned long long* stats, int num_cus, const unsigned long long* nw_dev = nullptr);
+                           const uint32_t* pair_graph, const uint32_t* list, uint64_t nw, int find_first, int32_t* counts,
+                           unsigned long long* stats, int num_cus, const unsigned long long* nw_dev = nullptr, uint32_t* deep = nullptr);
 
 }  // namespace sigmo_k
