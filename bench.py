@@ -428,6 +428,9 @@ def main() -> int:
     ap.add_argument("--batched-chunk", type=int, default=1 << 20, help="config 3: most molecules per chunk of the batched form")
     ap.add_argument("--no-overlap", action="store_true", help="config 3: load chunks inline instead of on a side stream")
     ap.add_argument("--autotune", action="store_true", help="batched form: pick launch shapes with the autotuner before timing")
+    ap.add_argument("--filter-mode", type=int, default=0, choices=(0, 1, 2),
+                    help="configs 1-2, batched form: pair evaluation by molecule (1) or from the pre-tested pair list (2); "
+                         "0 (default): the library's choice (list mode below 512 query graphs)")
     ap.add_argument("--graph", action="store_true",
                     help="configs 1-2: capture one whole step (the same kernel launches and work; the batched form's "
                          "includes the database refresh) as a hipGraph and time its replays; per-kernel figures then "
@@ -507,6 +510,9 @@ def main() -> int:
             else:
                 db = capi.Database(ctx, dd, args.refinement_steps)
                 batch = capi.Batch(ctx, db)
+                if args.filter_mode:
+                    batch.tuning = {"filter_mode": args.filter_mode}
+                    args.tuning = batch.tuning
                 if args.autotune:
                     print("[bench] autotuning", file=sys.stderr, flush=True)
                     args.tuning, _ = batch.autotune(qd, find_first)
