@@ -31,6 +31,30 @@ struct ArenaMasks {
     lo = arena[static_cast<uint64_t>(base) + 2 * depth];
     hi = arena[static_cast<uint64_t>(base) + 2 * depth + 1];
   }
+  // The masks of query nodes 0..k-1 (0 from k on), two per 16-byte load: a pair's block starts on a
+  // 4-word boundary and spans a multiple of 4 words, so a load that starts below k stays inside it.
+  template <int N>
+  __device__ __forceinline__ void masks(uint32_t base, uint32_t, uint32_t k, uint32_t, uint32_t, uint64_t (&m)[N]) const {
+    static_assert(N % 2 == 0, "masks come in pairs");
+#pragma unroll
+    for (uint32_t t = 0; t < static_cast<uint32_t>(N); t += 2) {
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (t < k) v = *reinterpret_cast<const uint4*>(arena + static_cast<uint64_t>(base) + t);
+      m[t] = static_cast<uint64_t>(v.x) | (static_cast<uint64_t>(v.y) << 32);
+      m[t + 1] = t + 1 < k ? (static_cast<uint64_t>(v.z) | (static_cast<uint64_t>(v.w) << 32)) : 0ull;
+    }
+  }
+  // The two-word masks of query nodes 0..k-1, one 16-byte load per node.
+  template <int N>
+  __device__ __forceinline__ void masks2(uint32_t base, uint32_t, uint32_t k, uint32_t, uint32_t, uint64_t (&lo)[N], uint64_t (&hi)[N]) const {
+#pragma unroll
+    for (uint32_t t = 0; t < static_cast<uint32_t>(N); ++t) {
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (t < k) v = *reinterpret_cast<const uint4*>(arena + static_cast<uint64_t>(base) + 2 * t);
+      lo[t] = static_cast<uint64_t>(v.x) | (static_cast<uint64_t>(v.y) << 32);
+      hi[t] = static_cast<uint64_t>(v.z) | (static_cast<uint64_t>(v.w) << 32);
+    }
+  }
 };
 
 // Candidate masks cut from the bitset row of the query node: positions s..s+n-1 (n <= 64 spans
@@ -63,6 +87,19 @@ struct BitsetMasks {
     lo = sh ? (x0 >> sh) | (x1 << (64 - sh)) : x0;
     hi = sh ? (x1 >> sh) | (x2 << (64 - sh)) : x1;
     if (n < 128) hi &= (1ull << (n - 64)) - 1;
+  }
+  template <int N>
+  __device__ __forceinline__ void masks(uint32_t base, uint32_t qoff, uint32_t k, uint32_t s, uint32_t n, uint64_t (&m)[N]) const {
+#pragma unroll
+    for (uint32_t t = 0; t < static_cast<uint32_t>(N); ++t) m[t] = t < k ? mask(base, qoff + t, t, s, n) : 0ull;
+  }
+  template <int N>
+  __device__ __forceinline__ void masks2(uint32_t base, uint32_t qoff, uint32_t k, uint32_t s, uint32_t n, uint64_t (&lo)[N], uint64_t (&hi)[N]) const {
+#pragma unroll
+    for (uint32_t t = 0; t < static_cast<uint32_t>(N); ++t) {
+      lo[t] = hi[t] = 0;
+      if (t < k) mask2(base, qoff + t, t, s, n, lo[t], hi[t]);
+    }
   }
 };
 
@@ -139,10 +176,7 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
       k = nk;
       base = nbase;
       for (uint32_t t = 0; t < k; ++t) maps[t][lane] = 0;  // every mapping entry starts at local node 0
-      if (kRegMasks) {
-#pragma unroll
-        for (uint32_t t = 0; t < (kRegMasks ? KMAX : 1); ++t) mk[t] = t < k ? src.mask(base, qoff + t, t, s, n) : 0ull;
-      }
+      if constexpr (kRegMasks) src.masks(base, qoff, k, s, n, mk);
       qad = static_cast<uint32_t>(q.fold[qoff]);
       fresh = false;
       depth = 0;
@@ -340,13 +374,7 @@ __global__ __launch_bounds__(64) void k_join_lanes_wide(GraphDev q, GraphDev d, 
       k = nk;
       base = nbase;
       for (uint32_t t = 0; t < k; ++t) maps[t][lane] = 0;  // every mapping entry starts at local node 0
-      if (kReg) {
-#pragma unroll
-        for (uint32_t t = 0; t < (kReg ? KMAX : 1); ++t) {
-          mlo[t] = mhi[t] = 0;
-          if (t < k) src.mask2(base, qoff + t, t, s, n, mlo[t], mhi[t]);
-        }
-      }
+      if constexpr (kReg) src.masks2(base, qoff, k, s, n, mlo, mhi);
       qad = static_cast<uint32_t>(q.adj[2 * static_cast<uint64_t>(qoff)]);
       fresh = false;
       depth = 0;
