@@ -833,6 +833,10 @@ __global__ __launch_bounds__(256) void k_pair_survive(SurviveArgs a) {
   uint32_t pending[S];                        // listed pairs of the slot's molecule not yet finished
   MolInfo smol[S];                            // the slot's molecule
   uint32_t open = S, scan_qg = 0;             // slot being listed (S: none) and its next query graph
+  // With at most 64 bitmask words per molecule the open molecule's words sit in registers (lane i:
+  // word i, listed bits cleared), loaded with one coalesced load when it is opened.
+  const bool reg_words = a.W <= kWave;
+  uint64_t mw = 0;
   uint32_t om = 0;                            // the open molecule
   bool exhausted = false;
   uint64_t mc_next = 0, mc_end = 0;           // claimed molecules [mc_next, mc_end)
@@ -869,7 +873,48 @@ __global__ __launch_bounds__(256) void k_pair_survive(SurviveArgs a) {
         if (take) pw.base = ac.lc_next;
         continue;
       }
-      if (open < S && scan_qg < Gq) {  // list the open molecule's pre-tested pairs of the next 64 query graphs
+      if (reg_words && open < S && __ballot(mw != 0ull)) {
+        // list the open molecule's next pre-tested pairs: the words from the lowest unlisted one up
+        // whose pairs fit the 64-entry list (the first always does), then each entry's query graph
+        // fields, one load per entry, all in flight together
+        const uint32_t pc = static_cast<uint32_t>(__popcll(mw));
+        uint32_t incl = pc;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+          const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), d));
+          if (lane >= static_cast<uint32_t>(d)) incl += y;
+        }
+        const bool sel = pc != 0 && incl <= kWave;
+        const uint64_t selm = __ballot(sel);
+        for (uint64_t f = selm; f; f &= f - 1) {  // wave-uniform
+          const int w = static_cast<int>(__builtin_ctzll(f));
+          const uint64_t x = __shfl(mw, w);
+          const uint32_t at = static_cast<uint32_t>(__shfl(static_cast<int>(incl - pc), w));
+          if ((x >> lane) & 1ull) {
+            PairEntry& e = plist[at + __popcll(x & below)];
+            e.slot = open;
+            e.qg = static_cast<uint32_t>(w) * kWave + lane;
+          }
+        }
+        const uint32_t cnt = static_cast<uint32_t>(__shfl(static_cast<int>(incl), 63 - static_cast<int>(__builtin_clzll(selm))));
+        if (sel) mw = 0;
+        wave_sync();
+        if (lane < cnt) {
+          PairEntry& e = plist[lane];
+          const QueryInfo qi = a.qinfo[e.qg];
+          e.qoff = qi.qoff;
+          e.k = qi.k;
+          e.tw = qi.first;
+        }
+        wave_sync();
+        lhead = 0;
+        lcount = cnt;
+#pragma unroll
+        for (uint32_t sl = 0; sl < S; ++sl)
+          if (sl == open) pending[sl] += lcount;
+        continue;
+      }
+      if (!reg_words && open < S && scan_qg < Gq) {  // list the open molecule's pre-tested pairs of the next 64 query graphs
         const uint64_t word = a.surv[static_cast<uint64_t>(om) * a.W + scan_qg / kWave];
         const uint32_t g = scan_qg + lane;
         const bool ok = (word >> lane) & 1ull;
@@ -935,6 +980,7 @@ __global__ __launch_bounds__(256) void k_pair_survive(SurviveArgs a) {
       open = free_slot;
       om = mi.m;
       scan_qg = 0;
+      if (reg_words) mw = lane < a.W ? a.surv[static_cast<uint64_t>(om) * a.W + lane] : 0ull;
     }
     const uint64_t active = __ballot(have);
     if (active == 0) break;
