@@ -181,10 +181,10 @@ __global__ __launch_bounds__(256) void k_db_prepare(GraphDev g, uint32_t R, cons
                                                     uint32_t* __restrict__ planes32, uint64_t* __restrict__ planes64,
                                                     uint64_t* __restrict__ planes_hi, uint64_t* __restrict__ lab_hi,
                                                     uint64_t* __restrict__ dmax, MolInfo* __restrict__ minfo, ClassLabels cl,
-                                                    uint64_t* __restrict__ cmax) {
+                                                    uint64_t* __restrict__ cmax, uint32_t c0) {
   __shared__ uint64_t fold_all[4][kWave];
   const uint32_t lane = static_cast<uint32_t>(lane_id());
-  const uint64_t c = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const uint64_t c = c0 + static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (c >= g.G) return;  // wave-uniform
   const uint64_t m = cls_list[c];
   const uint32_t s = g.go[m], n = g.go[m + 1] - s;
@@ -265,6 +265,121 @@ __global__ __launch_bounds__(256) void k_db_prepare(GraphDev g, uint32_t R, cons
     info.n = n;
     info.x = static_cast<uint32_t>(x);
     info.wi = wi;
+    info.mlab = labels;
+    minfo[c] = info;
+  }
+}
+
+// planes_nib_max of the two 32-position halves of a plane column at once (positions 0..31 and
+// 32..63 of `col`): the nibble-wise maxima over each half's positions.
+__device__ __forceinline__ void planes_nib_max2(uint64_t col, uint64_t& lo, uint64_t& hi) {
+  const uint32_t lane = static_cast<uint32_t>(lane_id());
+  const uint64_t p1 = static_cast<uint64_t>(__shfl_down(static_cast<unsigned long long>(col), 1));
+  const uint64_t p2 = static_cast<uint64_t>(__shfl_down(static_cast<unsigned long long>(col), 2));
+  const uint64_t p3 = static_cast<uint64_t>(__shfl_down(static_cast<unsigned long long>(col), 3));
+  const uint64_t planes[4] = {col, p1, p2, p3};
+  uint32_t ca = ~0u, cb = ~0u, va = 0, vb = 0;
+#pragma unroll
+  for (int b = 3; b >= 0; --b) {
+    const uint32_t ta = ca & static_cast<uint32_t>(planes[b]), tb = cb & static_cast<uint32_t>(planes[b] >> 32);
+    if (ta) {
+      va |= 1u << b;
+      ca = ta;
+    }
+    if (tb) {
+      vb |= 1u << b;
+      cb = tb;
+    }
+  }
+  const bool leader = (lane & 3u) == 0;
+  lo = hi = 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    lo |= __ballot(leader && ((va >> b) & 1u)) << b;
+    hi |= __ballot(leader && ((vb >> b) & 1u)) << b;
+  }
+}
+
+// k_db_prepare for molecules of <= 32 nodes, two per wavefront: lanes 0..31 hold the positions of
+// width-class entry 2v, lanes 32..63 those of 2v+1 (none past `count`).  Each lane's BFS runs over
+// its own molecule's positions; one 64 x 64 transpose gives both molecules' level planes (low and
+// high 32 bits of each plane word), so the transposes, and the lanes, serve two molecules.
+__global__ __launch_bounds__(256) void k_db_prepare_pairs(GraphDev g, uint32_t R, uint32_t count, const uint32_t* __restrict__ cls_list,
+                                                          const uint32_t* __restrict__ pidx, uint32_t* __restrict__ planes32,
+                                                          uint64_t* __restrict__ dmax, MolInfo* __restrict__ minfo, ClassLabels cl,
+                                                          uint64_t* __restrict__ cmax) {
+  __shared__ uint64_t fold_all[4][kWave];
+  const uint32_t lane = static_cast<uint32_t>(lane_id());
+  const uint64_t v = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const uint64_t cA = 2 * v, cB = cA + 1;
+  if (cA >= count) return;  // wave-uniform
+  const bool hasB = cB < count;
+  const uint32_t half = lane >> 5, p = lane & 31u;
+  const uint64_t c = half ? cB : cA;
+  const bool live = !half || hasB;
+  const uint64_t m = live ? cls_list[c] : 0;
+  const uint32_t s = live ? g.go[m] : 0, n = live ? g.go[m + 1] - s : 0;
+  const uint64_t* lm = g.lmask + m * 16;
+  uint64_t lmv[16];
+  uint32_t present = 0;
+#pragma unroll
+  for (int l = 0; l < 16; ++l) {
+    lmv[l] = live ? lm[l] : 0ull;
+    if (lmv[l]) present |= 1u << l;
+  }
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) present |= static_cast<uint32_t>(__shfl_xor(static_cast<int>(present), o));
+  uint64_t* fold = fold_all[threadIdx.x / kWave];
+  fold[lane] = p < n ? g.fold[static_cast<uint64_t>(s) + p] : 0ull;
+  wave_sync();
+  const uint64_t* myfold = fold + 32 * half;
+  const bool valid = p < n;
+  const uint32_t mylab = valid ? g.nl[s + p] : 0x100u;
+  uint32_t labels = mylab < 16 ? 1u << mylab : 0u;
+  uint64_t cpos[kClasses];
+#pragma unroll
+  for (uint32_t k = 0; k < kClasses; ++k) cpos[k] = __ballot(mylab == cl.l[k]);
+  const uint64_t xA = pidx[cls_list[cA]], xB = hasB ? pidx[cls_list[cB]] : 0;
+  uint64_t self = 1ull << p, ball = self, frontier = self;
+  for (uint32_t r = 0; r < R; ++r) {
+    uint64_t sig = 0;
+    if (valid) {
+      uint64_t next = 0;
+      for (uint64_t f = frontier; f; f &= f - 1) next |= myfold[__builtin_ctzll(f)];
+      next &= ~ball;
+      ball |= next;
+      frontier = next;
+#pragma unroll
+      for (int l = 0; l < 16; ++l)
+        if ((present >> l) & 1u) {  // counts saturate at 15 (signature.hpp:44-51)
+          uint32_t cnt = __popcll(next & lmv[l]);
+          cnt = cnt < 15 ? cnt : 15;
+          sig |= static_cast<uint64_t>(cnt) << (4 * l);
+        }
+    }
+    const uint64_t col = transpose64(sig);
+    planes32[(xA * R + r) * 64 + lane] = static_cast<uint32_t>(col);
+    if (hasB) planes32[(xB * R + r) * 64 + lane] = static_cast<uint32_t>(col >> 32);
+    uint64_t ma, mb;
+    planes_nib_max2(col, ma, mb);
+    if (lane == 0) dmax[cA * R + r] = ma;
+    if (lane == 1 && hasB) dmax[cB * R + r] = mb;
+#pragma unroll
+    for (uint32_t k = 0; k < kClasses; ++k) {
+      planes_nib_max2(col & cpos[k], ma, mb);
+      if (lane == 0) cmax[(static_cast<uint64_t>(k) * R + r) * g.G + cA] = ma;
+      if (lane == 1 && hasB) cmax[(static_cast<uint64_t>(k) * R + r) * g.G + cB] = mb;
+    }
+  }
+#pragma unroll
+  for (int o = 16; o > 0; o >>= 1) labels |= static_cast<uint32_t>(__shfl_xor(static_cast<int>(labels), o));  // within each half
+  if (p == 0 && live) {
+    MolInfo info;
+    info.m = static_cast<uint32_t>(m);
+    info.s = s;
+    info.n = n;
+    info.x = static_cast<uint32_t>(half ? xB : xA);
+    info.wi = kNoWide;
     info.mlab = labels;
     minfo[c] = info;
   }
@@ -1255,11 +1370,21 @@ __global__ __launch_bounds__(256) void k_request_stats(uint64_t P, const uint32_
 
 }  // namespace
 
+// The first `small` width-class entries (molecules of <= 32 nodes) two per wavefront, the rest one
+// per wavefront.
 hipError_t db_prepare(hipStream_t s, const GraphDev& g, uint32_t R, const uint32_t* cls_list, const uint32_t* pidx, const uint32_t* wide_idx,
                       uint32_t* planes32, uint64_t* planes64, uint64_t* planes_hi, uint64_t* lab_hi, uint64_t* dmax, MolInfo* minfo,
-                      const ClassLabels& cl, uint64_t* cmax) {
+                      const ClassLabels& cl, uint64_t* cmax, uint32_t small) {
   if (!g.G) return hipSuccess;
-  k_db_prepare<<<(g.G + 3) / 4, 256, 0, s>>>(g, R, cls_list, pidx, wide_idx, planes32, planes64, planes_hi, lab_hi, dmax, minfo, cl, cmax);
+  if (small) {
+    const uint32_t waves = (small + 1) / 2;
+    k_db_prepare_pairs<<<(waves + 3) / 4, 256, 0, s>>>(g, R, small, cls_list, pidx, planes32, dmax, minfo, cl, cmax);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (g.G > small)
+    k_db_prepare<<<(g.G - small + 3) / 4, 256, 0, s>>>(g, R, cls_list, pidx, wide_idx, planes32, planes64, planes_hi, lab_hi, dmax, minfo, cl,
+                                                       cmax, small);
   return hipGetLastError();
 }
 
@@ -1628,7 +1753,7 @@ int sigmo_db_refresh(sigmo_db* db) {
   void* tok = nullptr;
   BATCH_TRY(ctx_timed(db->ctx, SIGMO_PHASE_SIG_DATA, SIGMO_KERNEL_DB_PREPARE, bytes, true, &tok));
   BATCH_HIP_TRY(sigmo_k::db_prepare(s, g, db->R, db->cls_list, db->pidx, db->wide_idx, db->planes32, db->planes64, db->planes_hi, db->lab_hi,
-                                    db->dmax, db->minfo, db->cls, db->cmax));
+                                    db->dmax, db->minfo, db->cls, db->cmax, db->cls_count[0]));
   BATCH_TRY(ctx_timed(db->ctx, 0, 0, 0.0, false, &tok));
   return SIGMO_OK;
 }
@@ -2222,7 +2347,7 @@ int sigmo_batch_capture(sigmo_batch* b, const sigmo_graph* query, int find_first
   hipError_t e = hipSuccess;
   if (refresh_db)
     e = sigmo_k::db_prepare(s, L.dd, db->R, db->cls_list, db->pidx, db->wide_idx, db->planes32, db->planes64, db->planes_hi, db->lab_hi,
-                            db->dmax, db->minfo, db->cls, db->cmax);
+                            db->dmax, db->minfo, db->cls, db->cmax, db->cls_count[0]);
   if (e == hipSuccess) e = sigmo_k::query_prepare(s, L.qd, L.R, b->tokens, b->qinfo, b->qmax, db->cls, b->qcmax);
   if (e == hipSuccess) e = launch_filter(b, L, s);
   if (e == hipSuccess) e = launch_count_scan(b, L, s);
