@@ -453,6 +453,9 @@ typedef struct sigmo_stream_options {
   int find_first;
   int form;                 /* SIGMO_FORM_PLAIN: per-phase kernels over the chunk's bitset; SIGMO_FORM_BATCHED */
   int overlap;              /* nonzero: load the next chunk while the current one runs */
+  uint32_t workers;         /* batched form with overlap: chunks run on this many contexts at once (each
+                             * its own stream, database and batch; at most 4; 0 or 1: one).  Kernel
+                             * timings of the extra contexts are added to the caller's after a pass. */
 } sigmo_stream_options;
 typedef struct sigmo_stream_result {
   uint64_t num_matches, matched_pairs, gmcr_pairs, dfs_steps;

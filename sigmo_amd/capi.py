@@ -157,7 +157,7 @@ SERVE_DONE, SERVE_REJECTED, SERVE_EXPIRED, SERVE_PENDING = 0, 1, 2, 3
 
 class StreamOptions(C.Structure):
     _fields_ = [("chunk_molecules", C.c_uint32), ("refinement_steps", C.c_uint32), ("find_first", C.c_int), ("form", C.c_int),
-                ("overlap", C.c_int)]
+                ("overlap", C.c_int), ("workers", C.c_uint32)]
 
 
 class StreamResult(C.Structure):
@@ -1017,14 +1017,14 @@ class Server:
 
 def stream_run(ctx: Context, query: DeviceGraph, data: HostGraphs, chunk_molecules: int = 0, form: str = "batched",
                refinement_steps: int = 6, find_first: bool = True, overlap: bool = True, collect: bool = False,
-               digest: bool = False) -> dict:
+               digest: bool = False, workers: int = 0) -> dict:
     """Run a host-resident molecule shard in chunks (sigmo_stream_run: one pass of a session).  With
     collect, also returns the whole shard's GMCR (offsets over all molecules, indices) and per-pair
     counts, assembled from the chunks.  With digest, returns instead, per chunk in order, (first
     molecule, molecules, pairs, a 128-bit xxh3 digest of the chunk's offsets, indices and counts)
     under 'chunk_digests': equal chunk plans give equal digests exactly when the results are equal."""
     opt = StreamOptions(chunk_molecules, refinement_steps, int(find_first), FORM_PLAIN if form == "plain" else FORM_BATCHED,
-                        int(overlap))
+                        int(overlap), workers)
     return _stream_call(lambda cb, r: lib().sigmo_stream_run(ctx.handle, query.handle, data.handle, C.byref(opt), cb, None,
                                                              C.byref(r)), collect, digest)
 
@@ -1035,9 +1035,9 @@ class Stream:
     as :func:`stream_run`."""
 
     def __init__(self, ctx: Context, query: DeviceGraph, data: HostGraphs, chunk_molecules: int = 0, form: str = "batched",
-                 refinement_steps: int = 6, find_first: bool = True, overlap: bool = True):
+                 refinement_steps: int = 6, find_first: bool = True, overlap: bool = True, workers: int = 0):
         opt = StreamOptions(chunk_molecules, refinement_steps, int(find_first), FORM_PLAIN if form == "plain" else FORM_BATCHED,
-                            int(overlap))
+                            int(overlap), workers)
         h = C.c_void_p()
         check(lib().sigmo_stream_open(ctx.handle, query.handle, data.handle, C.byref(opt), C.byref(h)))
         self._h, self.ctx, self.query, self.data = h, ctx, query, data

@@ -1598,6 +1598,28 @@ int pipeline_rebind(sigmo_ctx* ctx, const sigmo_graph* query, const sigmo_graph*
   return SIGMO_OK;
 }
 int ctx_device(const sigmo_ctx* ctx) { return ctx->device; }
+
+// Add src's phase and kernel timings to dst's and clear them in src (a streaming pass whose chunks
+// ran on several contexts reports them on the caller's).  Both contexts are synchronised first.
+int ctx_merge_timings(sigmo_ctx* dst, sigmo_ctx* src) {
+  SIGMO_TRY(bind(src));
+  SIGMO_TRY(src->collect());
+  SIGMO_TRY(bind(dst));
+  SIGMO_TRY(dst->collect());
+  for (int p = 0; p < SIGMO_PHASE_COUNT; ++p) {
+    dst->accum_ms[p] += src->accum_ms[p];
+    src->accum_ms[p] = 0;
+  }
+  for (int k = 0; k < SIGMO_KERNEL_COUNT; ++k) {
+    dst->k_launches[k] += src->k_launches[k];
+    dst->k_ms[k] += src->k_ms[k];
+    dst->k_bytes[k] += src->k_bytes[k];
+    src->k_launches[k] = 0;
+    src->k_ms[k] = 0;
+    src->k_bytes[k] = 0;
+  }
+  return SIGMO_OK;
+}
 }  // namespace sigmo_internal
 
 extern "C" {

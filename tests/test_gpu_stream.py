@@ -70,3 +70,23 @@ def test_stream_session_passes_repeat_the_result(ctx, shard, form):
         np.testing.assert_array_equal(r["counts"], o["counts_all"], err_msg="counts")
         assert r["chunks"] == 4
     sess.close()
+
+
+@pytest.mark.parametrize("workers", [2, 3])
+def test_stream_workers_equal_one_run(ctx, shard, workers):
+    # chunks on several contexts at once (batched form): results in chunk order, equal to the oracle
+    dh, qh, qd, o = shard
+    for find_first in (True, False):
+        print(f"stream workers={workers} find_first={find_first}", flush=True)
+        r = capi.stream_run(ctx, qd, dh, chunk_molecules=400, form="batched", find_first=find_first, collect=True, workers=workers)
+        key = "first" if find_first else "all"
+        np.testing.assert_array_equal(r["offsets"], o["offsets"], err_msg="GMCR offsets")
+        np.testing.assert_array_equal(r["indices"], o["indices"], err_msg="GMCR indices")
+        np.testing.assert_array_equal(r["counts"], o[f"counts_{key}"], err_msg=f"counts ({key})")
+        assert r["num_matches"] == o[f"total_{key}"]
+        assert r["chunks"] == -(-dh.num_graphs // 400)
+    # a session keeps its workers' buffers: a second pass gives the same digests
+    s = capi.Stream(ctx, qd, dh, chunk_molecules=400, form="batched", workers=workers)
+    a, b = s.pass_(digest=True), s.pass_(digest=True)
+    s.close()
+    assert a["chunk_digests"] == b["chunk_digests"]
