@@ -436,7 +436,7 @@ def main() -> int:
     ap.add_argument("--autotune", action="store_true", help="batched form: pick launch shapes with the autotuner before timing")
     ap.add_argument("--filter-mode", type=int, default=0, choices=(0, 1, 2),
                     help="configs 1-2, batched form: pair evaluation by molecule (1) or from the pre-tested pair list (2); "
-                         "0 (default): the library's choice (list mode below 512 query graphs)")
+                         "0 (default): the library's choice (list mode below 256 query graphs)")
     ap.add_argument("--graph", action="store_true",
                     help="configs 1-2: capture one whole step (the same kernel launches and work; the batched form's "
                          "includes the database refresh) as a hipGraph and time its replays; per-kernel figures then "

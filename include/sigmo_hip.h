@@ -315,7 +315,7 @@ typedef struct sigmo_batch_tuning {
   uint32_t join_blocks_per_cu;   /* lane join: 2-wavefront blocks per CU (1..4); 0: 4 */
   uint32_t join_chunk;           /* GMCR pairs per claim of the lane join; 0: ~P / (16 x wavefronts), 64..1024 */
   uint32_t filter_mode;          /* pair evaluation: 1 molecule by molecule from LDS-staged planes, 2 pair by pair from the
-                                    pre-tested pair list; 0: list mode below 512 query graphs */
+                                    pre-tested pair list; 0: list mode below 256 query graphs */
 } sigmo_batch_tuning;
 int sigmo_db_prepare(sigmo_ctx* ctx, const sigmo_graph* data, uint32_t refinement_steps, sigmo_db** out);
 /* Recompute the level planes from the data graph's current device contents (same result as a

@@ -1489,7 +1489,7 @@ constexpr uint64_t kOneSyncPairs = 1ull << 25;
 // Query graphs per batch below which the pair evaluation runs pair by pair from the list: with
 // fewer, a molecule has too few pre-tested pairs to keep a wavefront's lanes busy from the few
 // molecules its LDS ring holds.
-constexpr uint32_t kListModeGraphs = 512;
+constexpr uint32_t kListModeGraphs = 256;
 
 struct sigmo_batch;
 namespace sigmo_internal {
@@ -1806,7 +1806,7 @@ int sigmo_batch_workspace_bytes(const sigmo_batch* b, uint32_t num_query_graphs,
   // 4 B per pair for the list of wide-molecule pairs).  A run whose survivors exceed the guess
   // grows the arena and those arrays to the measured demand; a batch keeps its buffers at their
   // largest size, so a caller admitting by this figure should leave room for that growth.  In
-  // list mode (below 512 query graphs by default) the pre-tested pair list takes 8 B per listed
+  // list mode (below 256 query graphs by default) the pre-tested pair list takes 8 B per listed
   // pair, priced at its first sizing (a quarter of the pairs; every pair on the one-sync path).
   const uint64_t guess = std::min<uint64_t>(std::max<uint64_t>(pairs / 8, 1 << 16), 1ull << 26);
   const uint64_t arena_words = static_cast<uint64_t>(double(guess) * (mean_k + 1.0) * (db->num_wide ? 2.0 : 1.0));
