@@ -290,6 +290,28 @@ inline std::vector<CSRGraph> loadCSRGraphsFromLines(const std::vector<std::strin
   detail::check(sigmo_host_graphs_parse_dat(text.data(), text.size(), &b.h));
   return detail::split(b.h);
 }
+// loadFileLinesMPI (src/sigmo_mpi.cpp:18-80) with the rank and the rank count passed in instead
+// of read from MPI_COMM_WORLD: the lines this rank's byte range of the file yields, filtered as
+// the reference filters them (sigmo_dat_shard_lines).
+inline std::vector<std::string> loadFileLinesShard(const std::string& filename, int max_lines, int rank, int nprocs) {
+  if (max_lines < 0 || rank < 0 || nprocs <= 0) throw std::runtime_error("negative line count, rank or rank count");
+  size_t len = 0;
+  uint32_t kept = 0;
+  detail::check(sigmo_dat_shard_lines(filename.c_str(), static_cast<uint32_t>(rank), static_cast<uint32_t>(nprocs),
+                                      static_cast<uint32_t>(max_lines), nullptr, 0, &len, &kept));
+  std::string text(len, '\0');
+  detail::check(sigmo_dat_shard_lines(filename.c_str(), static_cast<uint32_t>(rank), static_cast<uint32_t>(nprocs),
+                                      static_cast<uint32_t>(max_lines), text.empty() ? nullptr : &text[0], text.size(), &len, &kept));
+  if (len != text.size()) throw std::runtime_error(filename + " changed between the two reads of its shard");
+  std::vector<std::string> lines;
+  size_t at = 0;
+  while (at < text.size()) {
+    const size_t e = text.find('\n', at);
+    lines.push_back(text.substr(at, e - at));
+    at = e + 1;
+  }
+  return lines;
+}
 }  // namespace io
 
 // createDeviceCSRGraph (graph.hpp:255-314): concatenate with global ids, upload.

@@ -96,6 +96,17 @@ int sigmo_host_graphs_from_arrays(uint32_t num_graphs, const uint32_t* graph_off
 int sigmo_host_graphs_load_dat(const char* path, sigmo_host_graphs** out);
 /* Same parser over an in-memory buffer of '\n'-separated lines (io.hpp:37-44). */
 int sigmo_host_graphs_parse_dat(const char* text, size_t len, sigmo_host_graphs** out);
+/* The .dat lines that rank `rank` of `nprocs` reads in the reference's multi-node driver
+ * (library/src/sigmo_mpi.cpp:18-80, loadFileLinesMPI), with plain file reads in place of MPI-IO:
+ * the bytes [rank * (size / nprocs), + 2048 * max_lines + 1024) (past the end of the file: NUL),
+ * the partial first line dropped on ranks > 0 and the partial last line on ranks < nprocs - 1, at
+ * most max_lines lines, then only lines with exactly one 'n', 'e' and 'l' kept (:72-77).  The
+ * kept lines, each ended by '\n', go to text[0..*len) when cap >= *len (call with text = NULL to
+ * size it); *num_lines = lines kept.  A NUL run past the end of the file is returned as one NUL. */
+int sigmo_dat_shard_lines(const char* path, uint32_t rank, uint32_t nprocs, uint32_t max_lines, char* text, size_t cap, size_t* len,
+                          uint32_t* num_lines);
+/* Those lines parsed as io::loadCSRGraphsFromLines does (io.hpp:37-44; sigmo_mpi.cpp:110-114). */
+int sigmo_host_graphs_load_dat_shard(const char* path, uint32_t rank, uint32_t nprocs, uint32_t max_lines, sigmo_host_graphs** out);
 int sigmo_host_graphs_save_dat(const sigmo_host_graphs* g, const char* path);
 int sigmo_host_graphs_info(const sigmo_host_graphs* g, uint32_t* num_graphs, uint64_t* total_nodes, uint64_t* total_edges);
 int sigmo_host_graphs_arrays(const sigmo_host_graphs* g, const uint32_t** graph_offsets, const uint32_t** row_offsets,

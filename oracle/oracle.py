@@ -303,3 +303,39 @@ def pair_table(offsets: np.ndarray, indices: np.ndarray, counts: np.ndarray) -> 
     """(query graph, data graph, count) rows in GMCR order."""
     dg = np.repeat(np.arange(len(offsets) - 1, dtype=np.uint32), np.diff(offsets.astype(np.int64)))
     return np.stack([indices.astype(np.int64), dg.astype(np.int64), counts.astype(np.int64)], axis=1)
+
+
+# ---- one rank's share of a .dat file (pure Python) --------------------------------------------
+
+
+def dat_shard_lines(data: bytes, rank: int, nprocs: int, max_lines: int) -> list[bytes]:
+    """loadFileLinesMPI (library/src/sigmo_mpi.cpp:18-80) over the file's bytes, step by step:
+    the rank's range starts at rank * (size // nprocs) (:32-33) and ends 2048 * max_lines bytes
+    later (:34); the buffer has 1024 more bytes and is zero-filled, so what lies past the end of
+    the file reads as NUL (:37-44); ranks > 0 drop through their first newline (:49-54), ranks <
+    nprocs - 1 cut at their last newline (:57-62); std::getline splitting with at most max_lines
+    lines (:65-70); a line stays if it holds exactly one 'n', one 'e' and one 'l' (:72-77).  The
+    NUL run is kept at its full length."""
+    size = len(data)
+    start = rank * (size // nprocs)
+    buf = bytearray(2048 * max_lines + 1024)
+    chunk = data[start:start + len(buf)]
+    buf[: len(chunk)] = chunk
+    s = bytes(buf)
+    if rank != 0:
+        i = s.find(b"\n")
+        if i >= 0:
+            s = s[i + 1:]
+    if rank != nprocs - 1:
+        i = s.rfind(b"\n")
+        if i >= 0:
+            s = s[:i]
+    lines = []
+    pos = 0
+    while pos < len(s) and len(lines) < max_lines:  # std::getline: a final "\n" yields no empty line
+        i = s.find(b"\n", pos)
+        if i < 0:
+            i = len(s)
+        lines.append(s[pos:i])
+        pos = i + 1
+    return [ln for ln in lines if ln.count(b"n") == 1 and ln.count(b"e") == 1 and ln.count(b"l") == 1]

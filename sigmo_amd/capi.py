@@ -7,6 +7,8 @@ the reference classes (paths relative to the SIGMo repository):
 Python                                 reference
 =====================================  ==========================================================
 ``HostGraphs.load_dat``                ``io::loadCSRGraphsFromFile`` (library/include/io.hpp:46-55)
+``dat_shard_lines``                    ``loadFileLinesMPI`` (library/src/sigmo_mpi.cpp:18-80)
+``HostGraphs.load_dat_shard``          that, then ``io::loadCSRGraphsFromLines`` (io.hpp:37-44)
 ``Context``                            the ``sycl::queue`` (library/src/sigmo.cpp:19)
 ``DeviceGraph``                        ``createDeviceCSRGraph`` (library/include/graph.hpp:255-314)
 ``Signature``                          ``signature::Signature<>`` (library/include/signature.hpp:20-212)
@@ -43,7 +45,7 @@ SIZE_NORMAL, SIZE_LOGNORMAL = 0, 1
 EXPORTED = (
     "sigmo_last_error", "sigmo_ctx_create", "sigmo_ctx_destroy", "sigmo_ctx_sync", "sigmo_phase_ms",
     "sigmo_phase_reset", "sigmo_device_info", "sigmo_host_graphs_from_arrays", "sigmo_host_graphs_load_dat",
-    "sigmo_host_graphs_parse_dat", "sigmo_host_graphs_save_dat", "sigmo_host_graphs_info", "sigmo_host_graphs_arrays",
+    "sigmo_host_graphs_parse_dat", "sigmo_dat_shard_lines", "sigmo_host_graphs_load_dat_shard", "sigmo_host_graphs_save_dat", "sigmo_host_graphs_info", "sigmo_host_graphs_arrays",
     "sigmo_host_graphs_slice", "sigmo_host_graphs_concat", "sigmo_host_graphs_free", "sigmo_gen_molecules",
     "sigmo_gen_queries", "sigmo_gen_arrivals", "sigmo_graph_upload", "sigmo_graph_free", "sigmo_graph_info",
     "sigmo_graph_alloc_size", "sigmo_signatures_create", "sigmo_signatures_free", "sigmo_signatures_generate",
@@ -201,6 +203,8 @@ def lib() -> C.CDLL:
             "sigmo_host_graphs_from_arrays": ([u32, vp, vp, vp, vp, vp, pvp], i32),
             "sigmo_host_graphs_load_dat": ([C.c_char_p, pvp], i32),
             "sigmo_host_graphs_parse_dat": ([C.c_char_p, C.c_size_t, pvp], i32),
+            "sigmo_dat_shard_lines": ([C.c_char_p, u32, u32, u32, vp, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(u32)], i32),
+            "sigmo_host_graphs_load_dat_shard": ([C.c_char_p, u32, u32, u32, pvp], i32),
             "sigmo_host_graphs_save_dat": ([vp, C.c_char_p], i32),
             "sigmo_host_graphs_info": ([vp, C.POINTER(u32), C.POINTER(u64), C.POINTER(u64)], i32),
             "sigmo_host_graphs_arrays": ([vp, pvp, pvp, pvp, pvp, pvp], i32),
@@ -351,6 +355,14 @@ class HostGraphs:
         return cls(h)
 
     @classmethod
+    def load_dat_shard(cls, path, rank: int, nprocs: int, max_lines: int) -> "HostGraphs":
+        """Rank `rank` of `nprocs`'s share of a .dat file, read as the reference's multi-node
+        driver reads it (sigmo_mpi.cpp:18-80 then io.hpp:37-44); see dat_shard_lines."""
+        h = C.c_void_p()
+        check(lib().sigmo_host_graphs_load_dat_shard(str(path).encode(), rank, nprocs, max_lines, C.byref(h)))
+        return cls(h)
+
+    @classmethod
     def parse_dat(cls, text: str) -> "HostGraphs":
         b = text.encode()
         h = C.c_void_p()
@@ -385,6 +397,10 @@ class HostGraphs:
     @property
     def total_nodes(self) -> int:
         return self.info()[1]
+
+    @property
+    def total_edges(self) -> int:
+        return self.info()[2]
 
     def arrays(self) -> dict:
         """Copies of the five batched-CSR arrays as numpy arrays."""
@@ -422,6 +438,24 @@ def arrivals(seed: int, count: int, qps: float, qmin: int = 1, qmax: int = 50):
     q = np.zeros(count, np.uint32)
     check(lib().sigmo_gen_arrivals(seed, count, qps, qmin, qmax, _ptr(t), _ptr(q)))
     return t, q
+
+
+def dat_shard_lines(path, rank: int, nprocs: int, max_lines: int) -> list[bytes]:
+    """The .dat lines rank `rank` of `nprocs` reads in the reference's multi-node driver
+    (loadFileLinesMPI, sigmo_mpi.cpp:18-80): its byte range [rank * (size // nprocs), + 2048 *
+    max_lines + 1024), the partial first line dropped on ranks > 0 and the partial last line on
+    ranks < nprocs - 1, at most max_lines lines, then only those with exactly one 'n', 'e' and
+    'l' (:72-77).  A NUL run past the end of the file comes back as one NUL byte."""
+    n = C.c_size_t()
+    k = C.c_uint32()
+    check(lib().sigmo_dat_shard_lines(str(path).encode(), rank, nprocs, max_lines, None, 0, C.byref(n), C.byref(k)))
+    buf = C.create_string_buffer(max(1, n.value))
+    check(lib().sigmo_dat_shard_lines(str(path).encode(), rank, nprocs, max_lines, buf, n.value, C.byref(n), C.byref(k)))
+    text = buf.raw[: n.value]
+    lines = text.split(b"\n")[:-1] if text else []
+    if len(lines) != k.value:
+        raise SigmoError(ERR_IO, f"{path} changed between the two reads of its shard")
+    return lines
 
 
 # ---- device objects ---------------------------------------------------------------------------------

@@ -192,6 +192,66 @@ static int parse_stream(std::istream& in, sigmo_host_graphs** out) {
   return SIGMO_OK;
 }
 
+// ---- one rank's share of a .dat file (library/src/sigmo_mpi.cpp:18-80) ------------------
+//
+// Rank r of p reads the bytes [r * (size / p), r * (size / p) + 2048 * max_lines + 1024) of the
+// file, the bytes past its end reading as NUL (:32-44).  A rank other than the first drops
+// everything up to and including its first '\n' (:49-54), even when its range starts on a line;
+// a rank other than the last drops its last '\n' and what follows (:57-62).  The rest splits at
+// '\n' as std::getline does (a final '\n' ends the last line, an empty segment between two '\n'
+// is a line) into at most max_lines lines (:65-70), and only then is a line kept if it holds
+// exactly one 'n', one 'e' and one 'l' character (:72-77).  A run of NULs past the end of the
+// file is kept as one NUL: the parser splits tokens at whitespace and a number's digits end at
+// a NUL, so a run of any length yields the same tokens and values.  The byte count is computed
+// in 64 bits (the reference multiplies max_lines by 2048 in int).
+static int shard_lines(const char* path, uint32_t rank, uint32_t nprocs, uint32_t max_lines, std::string& text, uint32_t& kept) {
+  if (!path) return sigmo_err::set(SIGMO_ERR_INVALID, "null path");
+  if (nprocs == 0 || rank >= nprocs) return sigmo_err::set(SIGMO_ERR_INVALID, "rank must be below nprocs");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return sigmo_err::set(SIGMO_ERR_IO, std::string("cannot open ") + path);
+  if (fseeko(f, 0, SEEK_END) != 0) {
+    std::fclose(f);
+    return sigmo_err::set(SIGMO_ERR_IO, std::string("cannot size ") + path);
+  }
+  const uint64_t size = static_cast<uint64_t>(ftello(f));
+  const uint64_t start = static_cast<uint64_t>(rank) * (size / nprocs);
+  const uint64_t span = 2048ull * max_lines + 1024ull;
+  const uint64_t n = start < size ? std::min(span, size - start) : 0;
+  std::string data(n, '\0');
+  if (n && (fseeko(f, static_cast<off_t>(start), SEEK_SET) != 0 || std::fread(&data[0], 1, n, f) != n)) {
+    std::fclose(f);
+    return sigmo_err::set(SIGMO_ERR_IO, std::string("short read from ") + path);
+  }
+  std::fclose(f);
+  if (n < span) data.push_back('\0');  // the NUL run past the end of the file
+  if (rank != 0) {
+    const size_t p = data.find('\n');
+    if (p != std::string::npos) data.erase(0, p + 1);
+  }
+  if (rank != nprocs - 1) {
+    const size_t p = data.rfind('\n');
+    if (p != std::string::npos) data.resize(p);
+  }
+  text.clear();
+  kept = 0;
+  uint32_t taken = 0;
+  size_t at = 0;
+  while (at < data.size() && taken < max_lines) {
+    size_t e = data.find('\n', at);
+    if (e == std::string::npos) e = data.size();
+    ++taken;
+    const char* b = data.data() + at;
+    const char* z = data.data() + e;
+    if (std::count(b, z, 'n') == 1 && std::count(b, z, 'e') == 1 && std::count(b, z, 'l') == 1) {
+      text.append(b, z);
+      text.push_back('\n');
+      ++kept;
+    }
+    at = e + 1;
+  }
+  return SIGMO_OK;
+}
+
 }  // namespace sigmo_host
 
 using namespace sigmo_host;
@@ -233,6 +293,29 @@ int sigmo_host_graphs_load_dat(const char* path, sigmo_host_graphs** out) {
 int sigmo_host_graphs_parse_dat(const char* text, size_t len, sigmo_host_graphs** out) {
   if (!out || (!text && len)) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
   std::istringstream s(std::string(text ? text : "", len));
+  return parse_stream(s, out);
+}
+
+int sigmo_dat_shard_lines(const char* path, uint32_t rank, uint32_t nprocs, uint32_t max_lines, char* text, size_t cap, size_t* len,
+                          uint32_t* num_lines) {
+  if (!len) return sigmo_err::set(SIGMO_ERR_INVALID, "null length");
+  std::string t;
+  uint32_t kept = 0;
+  const int st = shard_lines(path, rank, nprocs, max_lines, t, kept);
+  if (st != SIGMO_OK) return st;
+  *len = t.size();
+  if (num_lines) *num_lines = kept;
+  if (text && cap >= t.size()) std::memcpy(text, t.data(), t.size());
+  return SIGMO_OK;
+}
+
+int sigmo_host_graphs_load_dat_shard(const char* path, uint32_t rank, uint32_t nprocs, uint32_t max_lines, sigmo_host_graphs** out) {
+  if (!out) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
+  std::string t;
+  uint32_t kept = 0;
+  const int st = shard_lines(path, rank, nprocs, max_lines, t, kept);
+  if (st != SIGMO_OK) return st;
+  std::istringstream s(t);
   return parse_stream(s, out);
 }
 

@@ -23,7 +23,8 @@ def digest(*arrays) -> str:
 
 
 def witness_graphs():
-    """Appendix-B style witnesses: (queries, molecules, expected find-all per pair)."""
+    """Hand-traced witnesses of the reference DFS's stale-state quirks (isomorphism.hpp:319-371):
+    (queries, molecules, expected find-all per pair)."""
     # 1. stale sibling in visited: molecule path 0-2-1, query star centred on q0.
     mol1 = ([8, 8, 8], [(0, 2), (1, 2)])
     q1 = ([8, 8, 8], [(0, 1), (0, 2)])

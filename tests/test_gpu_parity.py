@@ -49,7 +49,7 @@ def test_aliasing_case_has_large_molecules():
 
 
 @pytest.mark.parametrize("idx", [0, 1])
-def test_appendix_b_witnesses(ctx, idx):
+def test_dfs_quirk_witnesses(ctx, idx):
     q, mol, expected = witness_graphs()[idx]
     qh, dh = capi.HostGraphs.from_edge_lists([q]), capi.HostGraphs.from_edge_lists([mol])
     g = gpu_phases(ctx, qh, dh, 0)

@@ -18,9 +18,12 @@ filter_candidates        numpy label equality / wildcard closed form; the supers
 refine_candidates        brute force: label match and sphere-histogram dominance at every
                          level, from networkx distances
 gmcr                     brute force over all (query graph, molecule) pairs
-join                     Appendix-B witnesses (hand-traced: 1 and 4 embeddings); networkx
+join                     hand-traced DFS-quirk witnesses (1 and 4 embeddings); networkx
                          induced-subgraph counts where the DFS quirks cannot fire (k = 2,
                          unique-label molecules); the oracle's textbook mode equals networkx
+dat_shard_lines          byte ranges worked out by hand on files of 40-byte lines (a rank that
+                         starts on a line still drops it, the line limit counts lines before
+                         the n/e/l filter, the NUL run past the end): tests/test_shard.py
 ======================  ===================================================================
 
 Planted mistakes (the oracle's `mutation` switches) must make the matching check fail.
@@ -387,7 +390,7 @@ def nx_induced_count(qgraph, mgraph):
 
 
 @pytest.mark.parametrize("idx", [0, 1])
-def test_appendix_b_witnesses(idx):
+def test_dfs_quirk_witnesses(idx):
     q, mol, expected = witness_graphs()[idx]
     assert join_counts([q], [mol], False)[0, 2] == expected
     assert join_counts([q], [mol], True)[0, 2] == 1
