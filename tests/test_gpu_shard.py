@@ -40,3 +40,50 @@ def test_ranks_equal_the_oracle_on_their_shards(ctx, files, nprocs, max_lines, f
     if nprocs == 1:
         whole = capi.HostGraphs.load_dat(data_file)
         assert got["data_graphs"] == whole.num_graphs
+
+
+def _rank_main(rank, world, port, query, data_file, max_lines, out_dir):
+    import json
+
+    import torch.distributed as dist
+
+    dist.init_process_group("gloo", init_method=f"tcp://127.0.0.1:{port}", rank=rank, world_size=world)
+    try:
+        ctx = capi.Context(0)
+        got = shard.run_rank(ctx, query, data_file, max_lines, refinement_steps=R)
+        got.pop("result")
+        (out_dir / f"rank{rank}.json").write_text(json.dumps(got))
+    finally:
+        dist.destroy_process_group()
+
+
+@pytest.mark.timeout(600)  # the first import of torch on a fresh machine can take minutes
+def test_two_gloo_ranks_on_one_gpu(files, tmp_path):
+    # Two processes share the GPU, each a rank of a gloo group: their sums and gathered times
+    # are the same on both and equal the oracle's per-shard results added up.
+    import json
+    import socket
+
+    import torch.multiprocessing as mp
+
+    query, data_file = files
+    with socket.socket() as s:
+        s.bind(("127.0.0.1", 0))
+        port = s.getsockname()[1]
+    world, max_lines = 2, 800
+    print("two gloo ranks", flush=True)
+    mp.spawn(_rank_main, args=(world, port, query, data_file, max_lines, tmp_path), nprocs=world, join=True)
+    qa = capi.HostGraphs.load_dat(query).arrays()
+    data = data_file.read_bytes()
+    graphs = matches = 0
+    out = [json.loads((tmp_path / f"rank{r}.json").read_text()) for r in range(world)]
+    for r in range(world):
+        lines = O.dat_shard_lines(data, r, world, max_lines)
+        sh = capi.HostGraphs.parse_dat(b"\n".join(lines).decode() + "\n")
+        m = O.pipeline(qa, sh.arrays(), R)["total_first"]
+        assert (out[r]["rank"], out[r]["data_graphs"], out[r]["num_matches"]) == (r, sh.num_graphs, m)
+        graphs += sh.num_graphs
+        matches += m
+    for r in range(world):
+        assert (out[r]["total_data_graphs"], out[r]["total_matches"], out[r]["average_matches"]) == (graphs, matches, matches // world)
+        assert out[r]["rank_ms"] == out[0]["rank_ms"] and len(out[r]["rank_ms"]) == world
