@@ -71,17 +71,44 @@ __device__ __forceinline__ uint64_t transpose64(uint64_t x) {
   return x;
 }
 
+// One stage of ge4_bits on 32 positions: m ? (x & ge) : (x | ge) (the majority of x, ge and ~m)
+// as one three-input bit operation, v_bitop3_b32.  Its truth table 0xB2 puts the select m in
+// the middle operand, whose table weight (0xCC) is the same whichever of the outer operands is
+// weighted 0xF0; x and ge enter the function symmetrically, so the table does not depend on
+// that order either.
+__device__ __forceinline__ uint32_t ge_stage(uint32_t x, uint32_t m, uint32_t ge) { return __builtin_amdgcn_bitop3_b32(x, m, ge, 0xB2); }
+
+// x_0 >= c_0 on 32 positions: x | ~m, written (x & x) | ~m so that the table (0xB3) is
+// symmetric in the outer operands as well.
+__device__ __forceinline__ uint32_t ge_first(uint32_t x, uint32_t m) { return __builtin_amdgcn_bitop3_b32(x, m, x, 0xB3); }
+
+// bit b of c spread over 32 bits (0 or ~0): one signed bit-field extract
+__device__ __forceinline__ uint32_t bit_mask(uint32_t c, uint32_t b) {
+  return static_cast<uint32_t>(static_cast<int32_t>(c << (31u - b)) >> 31);
+}
+
 // x >= c for four-bit values held as bit planes x0..x3 and a per-lane c, from the lowest bit
 // up: ge = c_b ? (x_b & ge) : (x_b | ge), starting from x_0 >= c_0.  Each stage is one
-// bit-select of an AND and an OR.
+// ge_stage per 32 positions (the compiler's form was an AND, an OR, a compare and a select).
 template <typename T>
 __device__ __forceinline__ T ge4_bits(T x0, T x1, T x2, T x3, uint32_t c) {
-  const T m0 = T(0) - T(c & 1u), m1 = T(0) - T((c >> 1) & 1u), m2 = T(0) - T((c >> 2) & 1u), m3 = T(0) - T((c >> 3) & 1u);
-  T ge = x0 | ~m0;
-  ge = (m1 & (x1 & ge)) | (~m1 & (x1 | ge));
-  ge = (m2 & (x2 & ge)) | (~m2 & (x2 | ge));
-  ge = (m3 & (x3 & ge)) | (~m3 & (x3 | ge));
-  return ge;
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32 or 64 positions");
+  const uint32_t m0 = bit_mask(c, 0), m1 = bit_mask(c, 1), m2 = bit_mask(c, 2), m3 = bit_mask(c, 3);
+  if constexpr (sizeof(T) == 4) {
+    uint32_t ge = ge_first(x0, m0);
+    ge = ge_stage(x1, m1, ge);
+    ge = ge_stage(x2, m2, ge);
+    return ge_stage(x3, m3, ge);
+  } else {
+    uint32_t lo = ge_first(static_cast<uint32_t>(x0), m0), hi = ge_first(static_cast<uint32_t>(x0 >> 32), m0);
+    lo = ge_stage(static_cast<uint32_t>(x1), m1, lo);
+    hi = ge_stage(static_cast<uint32_t>(x1 >> 32), m1, hi);
+    lo = ge_stage(static_cast<uint32_t>(x2), m2, lo);
+    hi = ge_stage(static_cast<uint32_t>(x2 >> 32), m2, hi);
+    lo = ge_stage(static_cast<uint32_t>(x3), m3, lo);
+    hi = ge_stage(static_cast<uint32_t>(x3 >> 32), m3, hi);
+    return static_cast<T>(lo) | (static_cast<T>(hi) << 32);
+  }
 }
 
 }  // namespace sigmo_k
