@@ -549,29 +549,51 @@ struct PretestArgs {
   uint32_t* pcnt;                // [c] out: passing pairs of the molecule
 };
 
-// Nibble-wise a <= b for 16 four-bit fields, as in nib_le_all, with the fields split into 8-bit
-// lanes beforehand: the even and odd nibbles of b with a guard bit set in each lane, those of a
-// without.  a <= b in every nibble exactly when both differences (b_even - a_even, b_odd - a_odd)
-// keep every guard bit, and the test of several words ANDs the differences and checks the guard
-// bits once.  Splitting each side once (a molecule's maxima per thread, a query graph's per tile)
-// leaves two 64-bit subtractions and two ANDs per word in the pair loop.
-constexpr uint64_t kNibLo = 0x0F0F0F0F0F0F0F0Full, kNibGuard = 0x1010101010101010ull;
-__device__ __forceinline__ uint64_t nib_split_b(uint64_t b, uint32_t part) { return ((part ? b >> 4 : b) & kNibLo) | kNibGuard; }
-__device__ __forceinline__ uint64_t nib_split_a(uint64_t a, uint32_t part) { return (part ? a >> 4 : a) & kNibLo; }
+// Nibble-wise a <= b for the 16 four-bit fields of several words, bit-sliced: bit b of every
+// nibble of a word is one plane (bits 4l), and four words share a set of planes, word j's
+// fields at bits 4l + j.  Over the planes, b >= a is the lowest-bit-up chain of ge4_bits with the
+// query's planes as per-position thresholds, one v_bitop3_b32 per plane and 32 positions: a
+// group of four words costs 8 operations, where splitting each word into even and odd nibbles
+// took two 64-bit subtractions and two ANDs per word.  Positions of an unused word hold 0 on both
+// sides and pass.
+constexpr uint64_t kNibBit = 0x1111111111111111ull;
+__device__ __forceinline__ uint64_t nib_plane(uint64_t w, uint32_t b, uint32_t j) { return ((w >> b) & kNibBit) << j; }
+
+// acc &= (x >= m) position-wise over four planes (lo and hi halves of 64 positions)
+__device__ __forceinline__ void planes_ge(const uint64_t x[4], const uint64_t* m, uint32_t& acc_lo, uint32_t& acc_hi) {
+  uint32_t lo = ge_first(static_cast<uint32_t>(x[0]), static_cast<uint32_t>(m[0]));
+  uint32_t hi = ge_first(static_cast<uint32_t>(x[0] >> 32), static_cast<uint32_t>(m[0] >> 32));
+#pragma unroll
+  for (int b = 1; b < 4; ++b) {
+    lo = ge_stage(static_cast<uint32_t>(x[b]), static_cast<uint32_t>(m[b]), lo);
+    hi = ge_stage(static_cast<uint32_t>(x[b] >> 32), static_cast<uint32_t>(m[b] >> 32), hi);
+  }
+  acc_lo &= lo;
+  acc_hi &= hi;
+}
 
 // One thread per molecule (width-class order) and the query graphs 64 at a time from an LDS tile
-// (every lane reads the same graph: broadcast reads), for RR levels.  Tile words of a graph:
-// [0] label set | classes << 32, then the split all-node maxima (2 words per level), then the
-// split class maxima (2 words per level and class).  A graph whose label test, or whose
-// all-node test, fails on all 64 molecules of a wavefront skips the rest.
+// (every lane reads the same graph: broadcast reads), for RR levels in GA groups of four.  Tile
+// words of a graph: [0] label set | classes << 32, then the all-node maxima's planes (4 words per
+// group), then each class's (4 words per group).  A graph whose label test, or whose all-node
+// test, fails on all 64 molecules of a wavefront skips the rest.
 template <uint32_t RR>
 __global__ __launch_bounds__(256) void k_pair_pretest(PretestArgs a) {
-  constexpr uint32_t S = 1 + 2 * (1 + kClasses) * RR;
+  constexpr uint32_t GA = (RR + 3) / 4;
+  constexpr uint32_t S = 1 + 4 * GA * (1 + kClasses);
   __shared__ uint64_t tile[kWave * S];
   const uint64_t c = blockIdx.x * 256ull + threadIdx.x;
   const bool valid = c < a.G;
   uint32_t mlab = 0, m = 0;
-  uint64_t md[2 * RR], mc[kClasses][2 * RR];  // the molecule's split maxima
+  uint64_t md[GA][4], mc[kClasses][GA][4];  // the molecule's maxima as planes
+#pragma unroll
+  for (uint32_t g = 0; g < GA; ++g)
+#pragma unroll
+    for (uint32_t b = 0; b < 4; ++b) {
+      md[g][b] = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < kClasses; ++k) mc[k][g][b] = 0;
+    }
   if (valid) {
     const MolInfo mi = a.minfo[c];
     mlab = mi.mlab;
@@ -580,13 +602,13 @@ __global__ __launch_bounds__(256) void k_pair_pretest(PretestArgs a) {
 #pragma unroll
   for (uint32_t r = 0; r < RR; ++r) {
     const uint64_t d = valid ? a.dmax[c * RR + r] : 0ull;
-    md[2 * r] = nib_split_b(d, 0);
-    md[2 * r + 1] = nib_split_b(d, 1);
+#pragma unroll
+    for (uint32_t b = 0; b < 4; ++b) md[r / 4][b] |= nib_plane(d, b, r % 4);
 #pragma unroll
     for (uint32_t k = 0; k < kClasses; ++k) {
       const uint64_t x = valid ? a.cmax[(static_cast<uint64_t>(k) * RR + r) * a.G + c] : 0ull;
-      mc[k][2 * r] = nib_split_b(x, 0);
-      mc[k][2 * r + 1] = nib_split_b(x, 1);
+#pragma unroll
+      for (uint32_t b = 0; b < 4; ++b) mc[k][r / 4][b] |= nib_plane(x, b, r % 4);
     }
   }
   uint32_t cnt = 0;
@@ -600,11 +622,12 @@ __global__ __launch_bounds__(256) void k_pair_pretest(PretestArgs a) {
         if (f == 0) {
           const QueryInfo qi = a.qinfo[g];
           v = qi.qlab | (static_cast<uint64_t>(qi.qcls) << 32);
-        } else if (f <= 2 * RR) {
-          v = nib_split_a(a.qmax[static_cast<uint64_t>(g) * RR + (f - 1) / 2], (f - 1) & 1u);
         } else {
-          const uint32_t i = f - 1 - 2 * RR;  // class i / (2RR), level (i % 2RR) / 2
-          v = nib_split_a(a.qcmax[(static_cast<uint64_t>(g) * kClasses + i / (2 * RR)) * RR + (i % (2 * RR)) / 2], i & 1u);
+          const uint32_t i = f - 1;                   // (class slot, group, plane)
+          const uint32_t cs = i / (4 * GA), grp = (i / 4) % GA, b = i % 4;  // cs 0: all nodes, 1 + k: class k
+          const uint64_t* src = cs == 0 ? a.qmax + static_cast<uint64_t>(g) * RR
+                                        : a.qcmax + (static_cast<uint64_t>(g) * kClasses + (cs - 1)) * RR;
+          for (uint32_t jj = 0; jj < 4 && 4 * grp + jj < RR; ++jj) v |= nib_plane(src[4 * grp + jj], b, jj);
         }
       }
       tile[j * S + f] = v;
@@ -617,19 +640,19 @@ __global__ __launch_bounds__(256) void k_pair_pretest(PretestArgs a) {
       const uint64_t h = e[0];
       const bool lab = valid && (static_cast<uint32_t>(h) & ~mlab) == 0u;
       if (!__ballot(lab)) continue;  // wave-uniform
-      uint64_t acc = ~0ull;
+      uint32_t acc_lo = ~0u, acc_hi = ~0u;
 #pragma unroll
-      for (uint32_t i = 0; i < 2 * RR; ++i) acc &= md[i] - e[1 + i];
-      bool fits = lab && (acc & kNibGuard) == kNibGuard;
+      for (uint32_t g = 0; g < GA; ++g) planes_ge(md[g], e + 1 + 4 * g, acc_lo, acc_hi);
+      bool fits = lab && (acc_lo & acc_hi) == ~0u;
       const uint32_t qcls = static_cast<uint32_t>(h >> 32);
       if (qcls && __ballot(fits)) {  // wave-uniform
 #pragma unroll
         for (uint32_t k = 0; k < kClasses; ++k)
           if ((qcls >> k) & 1u) {
 #pragma unroll
-            for (uint32_t i = 0; i < 2 * RR; ++i) acc &= mc[k][i] - e[1 + 2 * RR + 2 * RR * k + i];
+            for (uint32_t g = 0; g < GA; ++g) planes_ge(mc[k][g], e + 1 + 4 * GA * (1 + k) + 4 * g, acc_lo, acc_hi);
           }
-        fits = fits && (acc & kNibGuard) == kNibGuard;
+        fits = fits && (acc_lo & acc_hi) == ~0u;
       }
       word |= static_cast<uint64_t>(fits) << j;
     }
