@@ -265,7 +265,7 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
         }
         uint64_t el = md & ~visited & bits_from(pos);
         const uint32_t lim = fresh ? depth - 1 : depth;
-        if (kRegMasks) {  // all KMAX rows read at once (the unused ones masked off), one LDS wait
+        if (kRegMasks) {  // unrolled over KMAX; the compiler branches around the rows at t >= lim
 #pragma unroll
           for (uint32_t t = 0; t < (kRegMasks ? KMAX : 1); ++t) {
             const uint64_t rw = rows[t][lane];
@@ -332,8 +332,8 @@ __global__ __launch_bounds__(kJoinLaneWaves * 64) void k_join_lanes(GraphDev q, 
 // two row words per depth and lane: with KMAX = kJoinShallow (12.75 KB per wavefront) the pass
 // lists the pairs of deeper queries in `deep` (count in stats[kStatWideDeep]) for a second pass
 // with KMAX = 32 (34.8 KB).  The shallow pass keeps its pair's masks in registers and, as the
-// narrow lane join, prefetches its next pair, issues each step's loads a step ahead and reads all
-// row words at once.
+// narrow lane join, prefetches its next pair, issues each step's loads a step ahead and unrolls
+// its row loop.
 template <bool FIND_FIRST, class Src, int KMAX = 32>
 __global__ __launch_bounds__(64) void k_join_lanes_wide(GraphDev q, GraphDev d, Src src, const uint32_t* __restrict__ indices,
                                                         const uint32_t* __restrict__ pair_graph, const uint32_t* __restrict__ list, uint64_t P,
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(64) void k_join_lanes_wide(GraphDev q, GraphDev d, 
         chi &= ~visited & (pos <= 64 ? ~0ull : bits_from(pos - 64));
         uint64_t el = clo, eh = chi;
         const uint32_t lim = fresh ? depth - 1 : depth;
-        if (kReg) {  // all KMAX rows read at once (the unused ones masked off), one LDS wait
+        if (kReg) {  // unrolled over KMAX; the compiler branches around the rows at t >= lim
 #pragma unroll
           for (uint32_t t = 0; t < (kReg ? KMAX : 1); ++t) {
             const uint64_t rl = rows_lo[t][lane], rh = rows_hi[t][lane];
