@@ -26,19 +26,48 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
 // Bits at positions >= pos of a 64-bit word (0 when pos >= 64).
 __device__ __forceinline__ uint64_t bits_from(uint32_t pos) { return pos >= 64 ? 0ull : (~0ull << pos); }
 
+// bit b of c spread over 32 bits (0 or ~0): one signed bit-field extract
+__device__ __forceinline__ uint32_t bit_mask(uint32_t c, uint32_t b) {
+  return static_cast<uint32_t>(static_cast<int32_t>(c << (31u - b)) >> 31);
+}
+
+// The hardware's three-input bit operation, v_bitop3_b32, weighs its operands 0xF0, 0xCC and
+// 0xAA in order; the truth tables below follow from that.
+// Majority of a, b, c on 32 positions (table 0xE8, the same for any operand order).
+__device__ __forceinline__ uint32_t maj3(uint32_t a, uint32_t b, uint32_t c) { return __builtin_amdgcn_bitop3_b32(a, b, c, 0xE8); }
+// m & (l ^ k) on 32 positions (table 0x60): with k all ones where a label bit is 0, the
+// positions whose label plane matches that bit.
+__device__ __forceinline__ uint32_t and_xor(uint32_t m, uint32_t l, uint32_t k) { return __builtin_amdgcn_bitop3_b32(m, l, k, 0x60); }
+
 // x >= c for 64 four-bit values held as bit planes x[0..3] and a wave-uniform c in 1..15:
 // x >= c exactly when the 4-bit sum x + (16 - c) carries out.  The carry planes are built from
-// the lowest bit up, carry' = (x_b & carry) | (k_b & (x_b | carry)) with k_b a uniform all-ones
-// or all-zero mask: straight-line code, no selects.
+// the lowest bit up, carry' = maj(x_b, carry, k_b) with k_b a uniform all-ones or all-zero mask:
+// one maj3 per plane and 32 positions (the compiler's form of (x & carry) | (k & (x | carry))
+// took an AND, an XOR and a bit-select).
 __device__ __forceinline__ uint64_t ge4(const uint64_t x[4], uint32_t c) {
   const uint32_t k = 16u - c;
-  uint64_t carry = x[0] & (0ull - static_cast<uint64_t>(k & 1u));
+  const uint32_t k0 = bit_mask(k, 0);
+  uint32_t lo = static_cast<uint32_t>(x[0]) & k0, hi = static_cast<uint32_t>(x[0] >> 32) & k0;
 #pragma unroll
   for (int b = 1; b < 4; ++b) {
-    const uint64_t km = 0ull - static_cast<uint64_t>((k >> b) & 1u);
-    carry = (x[b] & carry) | (km & (x[b] | carry));
+    const uint32_t km = bit_mask(k, static_cast<uint32_t>(b));
+    lo = maj3(static_cast<uint32_t>(x[b]), lo, km);
+    hi = maj3(static_cast<uint32_t>(x[b] >> 32), hi, km);
   }
-  return carry;
+  return lo | (static_cast<uint64_t>(hi) << 32);
+}
+
+// The positions of a 64-bit word whose 8-bit label (bit planes L[0..7]) equals a wave-uniform
+// ql, within `valid`: one and_xor per plane and 32 positions.
+__device__ __forceinline__ uint64_t label_match(const uint64_t L[8], uint32_t ql, uint64_t valid) {
+  uint32_t lo = static_cast<uint32_t>(valid), hi = static_cast<uint32_t>(valid >> 32);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const uint32_t k = bit_mask(~ql, static_cast<uint32_t>(b));
+    lo = and_xor(lo, static_cast<uint32_t>(L[b]), k);
+    hi = and_xor(hi, static_cast<uint32_t>(L[b] >> 32), k);
+  }
+  return lo | (static_cast<uint64_t>(hi) << 32);
 }
 
 // The same with a per-lane c (no branches).
@@ -81,11 +110,6 @@ __device__ __forceinline__ uint32_t ge_stage(uint32_t x, uint32_t m, uint32_t ge
 // x_0 >= c_0 on 32 positions: x | ~m, written (x & x) | ~m so that the table (0xB3) is
 // symmetric in the outer operands as well.
 __device__ __forceinline__ uint32_t ge_first(uint32_t x, uint32_t m) { return __builtin_amdgcn_bitop3_b32(x, m, x, 0xB3); }
-
-// bit b of c spread over 32 bits (0 or ~0): one signed bit-field extract
-__device__ __forceinline__ uint32_t bit_mask(uint32_t c, uint32_t b) {
-  return static_cast<uint32_t>(static_cast<int32_t>(c << (31u - b)) >> 31);
-}
 
 // x >= c for four-bit values held as bit planes x0..x3 and a per-lane c, from the lowest bit
 // up: ge = c_b ? (x_b & ge) : (x_b | ge), starting from x_0 >= c_0.  Each stage is one

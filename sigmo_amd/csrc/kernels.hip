@@ -545,12 +545,7 @@ __global__ __launch_bounds__(kBitsBlock) void k_filter(const uint8_t* __restrict
   const uint64_t q1 = q0 + rpb < Q ? q0 + rpb : Q;
   for (uint64_t q = q0; q < q1; ++q) {
     const uint32_t ql = qlabels[q];
-    uint64_t m = valid;
-    if (ql != 255u) {
-#pragma unroll
-      for (int b = 0; b < 8; ++b) m &= ((ql >> b) & 1u) ? P[b] : ~P[b];
-    }
-    bits[q * W64 + w] = m;
+    bits[q * W64 + w] = ql != 255u ? label_match(P, ql, valid) : valid;
   }
 }
 
@@ -640,11 +635,7 @@ __global__ __launch_bounds__(kBitsBlock) void k_filter_refine(const uint8_t* __r
     }
 #pragma unroll
     for (int i = 0; i < kRows; ++i) {
-      uint64_t m = valid;
-      if (ql[i] != 255u) {
-#pragma unroll
-        for (int b = 0; b < 8; ++b) m &= ((ql[i] >> b) & 1u) ? L[b] : ~L[b];
-      }
+      uint64_t m = ql[i] != 255u ? label_match(L, ql[i], valid) : valid;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const uint32_t c = static_cast<uint32_t>(qs[i] >> (4 * slots.label[u])) & 15u;
