@@ -80,14 +80,6 @@ __device__ __forceinline__ uint64_t histogram16(uint64_t r, const uint64_t* __re
   return s;
 }
 
-// Nibble-wise a <= b for all 16 nibbles (even and odd nibbles in 8-bit fields, borrow-free).
-__device__ __forceinline__ bool nib_le_all(uint64_t a, uint64_t b) {
-  constexpr uint64_t L = 0x0F0F0F0F0F0F0F0Full, H = 0x1010101010101010ull;
-  const uint64_t te = ((b & L) | H) - (a & L);
-  const uint64_t to = (((b >> 4) & L) | H) - ((a >> 4) & L);
-  return (te & to & H) == H;
-}
-
 // Nibble-wise max.
 __device__ __forceinline__ uint64_t nib_max(uint64_t a, uint64_t b) {
   constexpr uint64_t L = 0x0F0F0F0F0F0F0F0Full, H = 0x1010101010101010ull;

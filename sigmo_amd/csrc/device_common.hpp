@@ -70,19 +70,6 @@ __device__ __forceinline__ uint64_t label_match(const uint64_t L[8], uint32_t ql
   return lo | (static_cast<uint64_t>(hi) << 32);
 }
 
-// The same with a per-lane c (no branches).
-__device__ __forceinline__ uint64_t ge4_lane(uint64_t x0, uint64_t x1, uint64_t x2, uint64_t x3, uint32_t c) {
-  const uint64_t x[4] = {x0, x1, x2, x3};
-  uint64_t gt = 0, eq = ~0ull;
-#pragma unroll
-  for (int b = 3; b >= 0; --b) {
-    const uint64_t m = 0ull - static_cast<uint64_t>((c >> b) & 1u);  // all ones if bit b of c is set
-    gt |= eq & x[b] & ~m;
-    eq &= ~(x[b] ^ m);
-  }
-  return gt | eq;
-}
-
 // 64 x 64 bit-matrix transpose across the wavefront: lane p holds row p on entry, lane i
 // holds column i (bit p = bit i of lane p's row) on exit.  Six butterfly stages, each
 // exchanging the off-diagonal j x j blocks between lanes p and p ^ j.
