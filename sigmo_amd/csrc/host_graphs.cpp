@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <new>
 #include <sstream>
 #include <stdexcept>
 
@@ -213,7 +214,12 @@ static int shard_lines(const char* path, uint32_t rank, uint32_t nprocs, uint32_
     std::fclose(f);
     return sigmo_err::set(SIGMO_ERR_IO, std::string("cannot size ") + path);
   }
-  const uint64_t size = static_cast<uint64_t>(ftello(f));
+  const off_t end = ftello(f);
+  if (end < 0) {
+    std::fclose(f);
+    return sigmo_err::set(SIGMO_ERR_IO, std::string("cannot size ") + path);
+  }
+  const uint64_t size = static_cast<uint64_t>(end);
   const uint64_t start = static_cast<uint64_t>(rank) * (size / nprocs);
   const uint64_t span = 2048ull * max_lines + 1024ull;
   const uint64_t n = start < size ? std::min(span, size - start) : 0;
@@ -301,7 +307,12 @@ int sigmo_dat_shard_lines(const char* path, uint32_t rank, uint32_t nprocs, uint
   if (!len) return sigmo_err::set(SIGMO_ERR_INVALID, "null length");
   std::string t;
   uint32_t kept = 0;
-  const int st = shard_lines(path, rank, nprocs, max_lines, t, kept);
+  int st;
+  try {
+    st = shard_lines(path, rank, nprocs, max_lines, t, kept);
+  } catch (const std::bad_alloc&) {
+    return sigmo_err::set(SIGMO_ERR_NOMEM, "shard does not fit host memory");
+  }
   if (st != SIGMO_OK) return st;
   *len = t.size();
   if (num_lines) *num_lines = kept;
@@ -313,7 +324,12 @@ int sigmo_host_graphs_load_dat_shard(const char* path, uint32_t rank, uint32_t n
   if (!out) return sigmo_err::set(SIGMO_ERR_INVALID, "null argument");
   std::string t;
   uint32_t kept = 0;
-  const int st = shard_lines(path, rank, nprocs, max_lines, t, kept);
+  int st;
+  try {
+    st = shard_lines(path, rank, nprocs, max_lines, t, kept);
+  } catch (const std::bad_alloc&) {
+    return sigmo_err::set(SIGMO_ERR_NOMEM, "shard does not fit host memory");
+  }
   if (st != SIGMO_OK) return st;
   std::istringstream s(t);
   return parse_stream(s, out);
